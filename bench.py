@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark: OS-EM reconstructions/s at 128x128x80, 128 views, 37x37 PSF.
+
+One step = one OS-EM reconstruction (8 subsets x 4 iterations, i.e. 5
+back-projections and 4 projections of 128 views, plus the EM updates) on
+device-resident inputs.  Prints one JSON line with the metric, per-step
+latency percentiles, proj+backproj milliseconds, the roofline of the
+dominant kernel (HIP-event timed inside the timed region) and a CPU baseline
+(the oracle's Float32 restatement of the reference, threaded over views).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "spectrecon.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+NX, NZ, NVIEW, PSIZE = 128, 80, 128, 37
+NBLOCKS, NITER = 8, 4
+DY = 0.48  # cm: mu is in 1/cm and the reference multiplies mu*dy directly
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload():
+    import spectrecon_amd as S
+
+    image = S.phantom(NX, NX, NZ, seed=0)
+    mumap = S.mumap_phantom(NX, NX, NZ)
+    psf = S.psf_gauss(ny=NX, px=PSIZE, fwhm_start=1.5, fwhm_end=6.0)
+    psfs = np.asfortranarray(np.repeat(psf[:, :, :, None], NVIEW, 3))
+    return image, mumap, psfs
+
+
+def algorithmic(cls, nb, g):
+    """(bytes, flops) of one launch of a kernel class over nb views."""
+    nx, ny, nz, p = g
+    V, P = nx * ny * nz, nx * nz
+    if cls == "psf_fwd":  # separable clamped conv: 2 passes x p MACs per pixel per plane; read work, write partials
+        return 4 * V * nb, 2 * 2 * p * nx * nz * ny * nb
+    if cls == "psf_adj":  # read view, write attenuated plane stack, read attenuation
+        return 4 * (P + 2 * V) * nb, 2 * 2 * p * nx * nz * ny * nb
+    if cls == "rot_fwd":  # read image, read attenuation, write rotated volume
+        return 4 * 3 * V * nb, 0
+    if cls == "rot_adj":  # read work, write partial images (groups)
+        return 4 * 2 * V * nb, 0
+    if cls == "view_reduce":
+        return 4 * 2 * P * nb, 0
+    if cls == "img_reduce":
+        return 4 * 3 * V, 0
+    return 0, 0
+
+
+def cpu_baseline(image, mumap, psfs, budget_s):
+    """The oracle's Float32 restatement (FFT per plane, O(ny^2) attenuation,
+    views chunked over all host cores like spawner) on one OS-EM subset."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    import spectrecon_amd as S
+
+    cores = os.cpu_count() or 1
+    nsub = NVIEW // NBLOCKS
+    ang = S.default_viewangle(NVIEW).astype(np.float64)
+    P = O.Problem(mumap, psfs, DY, ang, "two", np.float32, "fft", nthread=min(cores, nsub))
+    idx = np.arange(0, NVIEW, NBLOCKS, dtype=np.int32)
+    t0 = time.perf_counter()
+    v = P.project(image, index=idx)
+    tp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    P.backproject(v, index=idx)
+    tb = time.perf_counter() - t0
+    # one reconstruction = 4 x 8 subset projections + 5 x 8 subset back-projections
+    t_recon = NITER * NBLOCKS * tp + (NITER + 1) * NBLOCKS * tb
+    return {"value": 1.0 / t_recon, "unit": "recons/s", "cores": min(cores, nsub), "kind": "port",
+            "sample": (f"project + backproject of one {nsub}-view subset at {NX}x{NX}x{NZ}, {PSIZE}x{PSIZE} PSF "
+                       f"({tp:.2f} s + {tb:.2f} s), extrapolated to 32 subset projections + 40 subset "
+                       f"back-projections per reconstruction")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding over views is described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+
+    import spectrecon_amd as S
+    from spectrecon_amd import _lib as L
+
+    if S.device_count() == 0:
+        raise SystemExit("no HIP device")
+    lib = S.lib()
+    image, mumap, psfs = workload()
+    t0 = time.perf_counter()
+    plan = S.SPECTplan(mumap, psfs, DY)
+    plan_ms = 1e3 * (time.perf_counter() - t0)
+    info = plan.info()
+    log(f"plan: {plan_ms:.1f} ms, {info}")
+
+    # measurements: y ~ Poisson(s*A x + b), s = 5e6 / sum(A x), b = 0.1 mean(s A x)
+    ytrue = S.project(image, plan)
+    s = 5e6 / float(ytrue.sum())
+    bval = 0.1 * float((s * ytrue).mean())
+    b = np.full_like(ytrue, bval)
+    y = S.poisson(s * ytrue + b, seed=0)
+    d_y = S.DeviceArray.from_host(y)
+    d_b = S.DeviceArray.from_host(b)
+    d_x0 = S.DeviceArray.from_host(np.ones(image.shape, np.float32))
+    d_out = S.DeviceArray(image.shape)
+
+    def recon():
+        L.check(lib.sr_osem(plan.handle, d_out.ptr, d_x0.ptr, d_y.ptr, d_b.ptr, NBLOCKS, NITER, L.SR_MEM_DEVICE))
+
+    for w in range(args.warmup):
+        recon()
+        log(f"warmup {w + 1}/{args.warmup} done")
+    plan.set_timing(True)
+    L.check(lib.sr_device_synchronize())
+    lat = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        t1 = time.perf_counter()
+        recon()  # returns after the plan's stream finished
+        lat.append(time.perf_counter() - t1)
+        log(f"step {k + 1}/{args.steps}: {1e3 * lat[-1]:.2f} ms")
+    L.check(lib.sr_device_synchronize())
+    t_total = time.perf_counter() - t_start
+    timing = plan.timing()
+    plan.set_timing(False)
+    xhat = d_out.to_host()
+    if not np.all(np.isfinite(xhat)):
+        raise SystemExit("non-finite reconstruction")
+
+    # proj + backproj of all 128 views, device resident, timed separately
+    d_img = S.DeviceArray.from_host(image)
+    d_views = S.DeviceArray((NX, NZ, NVIEW))
+    reps = 5
+    L.check(lib.sr_project(plan.handle, d_views.ptr, d_img.ptr, None, NVIEW, L.SR_MEM_DEVICE))
+    L.check(lib.sr_backproject(plan.handle, d_img.ptr, d_views.ptr, None, NVIEW, L.SR_MEM_DEVICE))
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        L.check(lib.sr_project(plan.handle, d_views.ptr, d_img.ptr, None, NVIEW, L.SR_MEM_DEVICE))
+    proj_ms = 1e3 * (time.perf_counter() - t1) / reps
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        L.check(lib.sr_backproject(plan.handle, d_img.ptr, d_views.ptr, None, NVIEW, L.SR_MEM_DEVICE))
+    back_ms = 1e3 * (time.perf_counter() - t1) / reps
+
+    # roofline of the dominant kernel over the timed region
+    nsub = NVIEW // NBLOCKS
+    g = (NX, NX, NZ, PSIZE)
+    dom = max(timing, key=lambda k: timing[k][0])
+    tot_ms, nl = timing[dom]
+    # launches in OS-EM are one subset (nsub views) each
+    by, fl = algorithmic(dom, nsub, g)
+    avg_s = tot_ms / 1e3 / max(nl, 1)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(dom)
+    if fl > 0:
+        roof = {"bound": "mfma", "achieved": fl / avg_s / 1e12, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": by / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = traffic
+    roof["kernel"] = dom
+    roof["avg_launch_ms"] = 1e3 * avg_s
+    kernels = {k: {"total_ms": round(v[0], 4), "launches": v[1],
+                   "avg_ms": round(v[0] / max(v[1], 1), 5)} for k, v in timing.items()}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(image, mumap, psfs, args.cpu_budget)
+
+    lat_ms = sorted(1e3 * t for t in lat)
+    value = args.steps / t_total
+    out = {
+        "metric": "OS-EM recons/sec + p50/p99 latency (128²×80, 128 views); proj+backproj ms",
+        "value": value,
+        "unit": "recons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, mu-map, Gaussian PSF, Poisson counts)",
+        "config": {"workload": "OS-EM 8 subsets x 4 iterations, 128x128x80 image, 128 views over 360 deg, "
+                               "37x37 depth-varying Gaussian PSF (FWHM 1.5->6 px), dy=0.48",
+                   "nx": NX, "ny": NX, "nz": NZ, "nview": NVIEW, "psf": PSIZE, "nblocks": NBLOCKS,
+                   "niter": NITER, "interpmeth": "two", "conv": info["conv"]},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
+                       "min": lat_ms[0], "max": lat_ms[-1]},
+        "proj_ms": proj_ms,
+        "backproj_ms": back_ms,
+        "proj_backproj_ms": proj_ms + back_ms,
+        "plan_ms": plan_ms,
+        "kernels": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

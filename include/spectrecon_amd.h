@@ -1,0 +1,155 @@
+/*
+ * spectrecon_amd.h -- C-ABI of the MI355X SPECT projector / reconstructor.
+ *
+ * A drop-in for the hot path of SPECTrecon.jl (SPECTplan -> project! /
+ * backproject! -> mlem / osem).  Every entry point takes plain pointers and
+ * sizes, FP32 data in the reference's column-major (Julia) layout:
+ *   image, mumap : [nx, ny, nz]      (x fastest)
+ *   views        : [nx, nz, nview]
+ *   psfs         : [px, pz, ny, nview]
+ * View indices are 0-based int32.  `mem` says where the array pointers live:
+ * SR_MEM_HOST (the library stages them through device memory and returns
+ * when the results are back on the host) or SR_MEM_DEVICE (pointers from
+ * sr_malloc; the call runs on the plan's stream and returns after it
+ * finished unless SR_ASYNC is or-ed into `mem`).
+ *
+ * Every function returns SR_OK (0) or an error code; sr_last_error() gives
+ * the calling thread's last message.  The messages of argument checks equal
+ * the strings the reference throws, so a host wrapper can rethrow them.
+ */
+#ifndef SPECTRECON_AMD_H
+#define SPECTRECON_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SR_OK 0
+#define SR_ERR_ARG 1   /* reference: throw(String) from an argument check   */
+#define SR_ERR_DIM 2   /* reference: DimensionMismatch                      */
+#define SR_ERR_HIP 3   /* HIP runtime or kernel launch failure              */
+#define SR_ERR_ALLOC 4 /* device memory could not be allocated              */
+#define SR_ERR_UNSUPPORTED 5 /* valid for the reference, not built here     */
+
+#define SR_MEM_HOST 0
+#define SR_MEM_DEVICE 1
+#define SR_ASYNC 0x100
+
+#define SR_INTERP_ONE 1 /* :one, 3-pass 1-D linear interpolation          */
+#define SR_INTERP_TWO 2 /* :two, single-pass bilinear (reference default) */
+
+#define SR_MODE_FAST 0 /* :fast */
+#define SR_MODE_MEM 1  /* :mem  */
+
+#define SR_CONV_AUTO 0     /* pick per plan (separable when every PSF is rank 1) */
+#define SR_CONV_DIRECT 1   /* direct 2-D clamped convolution                     */
+#define SR_CONV_SEPARABLE 2
+
+typedef struct sr_plan_s* sr_plan;
+
+/* Plan geometry and options: SPECTplan(mumap, psfs, dy; T, viewangle,
+ * interpmeth, nthread, mode), src/spectplan.jl:58-67.  T is Float32 here;
+ * nthread is a Julia-side check and has no counterpart. */
+typedef struct sr_plan_desc {
+  int32_t nx, ny, nz; /* image size; nx == ny and even (src/spectplan.jl:76-77) */
+  int32_t px, pz;     /* PSF size, both odd (src/spectplan.jl:82)               */
+  int32_t nview;      /* size(psfs, 4)                                          */
+  float dy;           /* voxel size used in exp(-mu*dy) (src/project.jl:111)    */
+  int32_t interpmeth; /* SR_INTERP_ONE / SR_INTERP_TWO (src/spectplan.jl:87)    */
+  int32_t mode;       /* SR_MODE_FAST / SR_MODE_MEM (src/spectplan.jl:93)       */
+  int32_t conv;       /* SR_CONV_*: PSF method choice (no reference counterpart) */
+} sr_plan_desc;
+
+/* ---- errors, devices, memory --------------------------------------------- */
+const char* sr_last_error(void);
+int sr_version(void);
+/* number of visible GPUs (0 when there is none); never fails */
+int sr_device_count(void);
+int sr_malloc(void** dptr, size_t bytes);
+int sr_free(void* dptr);
+int sr_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int sr_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int sr_memset(void* dptr, int value, size_t bytes);
+int sr_device_synchronize(void);
+
+/* ---- SPECTplan ------------------------------------------------------------
+ * Replaces SPECTplan(...) (src/spectplan.jl:58-137) with device-resident
+ * state: mu-map, per-view rotation tables, per-view attenuation factors
+ * exp(-dy*(sum_{j<=y} mu_r[j] - mu_r[y]/2)) and PSF taps.  `viewangle` has
+ * nview radians (the reference default is (0:nview-1)/nview*T(2pi),
+ * src/spectplan.jl:63, computed by the caller).  `mumap`, `psfs` and
+ * `viewangle` are host pointers. */
+int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* psfs, const float* viewangle,
+                   sr_plan* plan);
+int sr_plan_destroy(sr_plan plan);
+/* resident device bytes and the PSF method the plan chose (SR_CONV_*) */
+int sr_plan_info(sr_plan plan, size_t* device_bytes, int32_t* conv_method, int32_t* max_batch);
+/* cap on views processed per launch batch (memory/parallelism trade-off) */
+int sr_plan_set_batch(sr_plan plan, int32_t max_batch);
+/* wait for all work queued on the plan's stream */
+int sr_plan_synchronize(sr_plan plan);
+/* Per-kernel timing with HIP events on the plan's stream (no reference
+ * counterpart; for measurement).  While enabled, every kernel launch of the
+ * plan is bracketed by events; sr_plan_get_timing returns the summed device
+ * time and launch count of one kernel class since timing was (re)enabled:
+ * 0 rotate, 1 PSF forward, 2 view reduce, 3 PSF adjoint, 4 rotate adjoint,
+ * 5 image reduce / EM update. */
+#define SR_K_ROT_FWD 0
+#define SR_K_PSF_FWD 1
+#define SR_K_VIEW_REDUCE 2
+#define SR_K_PSF_ADJ 3
+#define SR_K_ROT_ADJ 4
+#define SR_K_IMG_REDUCE 5
+int sr_plan_set_timing(sr_plan plan, int32_t enable);
+int sr_plan_get_timing(sr_plan plan, int32_t kernel, double* total_ms, int64_t* launches);
+
+/* ---- project! / backproject! ----------------------------------------------
+ * sr_project replaces project!(views, image, plan; index) (src/project.jl:143-163):
+ *   views[:,:,ii] = A_{index[ii]} image, ii = 0..nindex-1.
+ * sr_backproject replaces backproject!(image, views, plan; index)
+ * (src/backproject.jl:340-370): image = sum_ii A_{index[ii]}' views[:,:,ii]
+ * (image is overwritten, as the reference zeroes it).
+ * index == NULL means all views 0..nview-1 (nindex must then be nview). */
+int sr_project(sr_plan plan, float* views, const float* image, const int32_t* index, int32_t nindex, int32_t mem);
+int sr_backproject(sr_plan plan, float* image, const float* views, const int32_t* index, int32_t nindex,
+                   int32_t mem);
+
+/* ---- mlem! / osem! ---------------------------------------------------------
+ * sr_mlem replaces mlem!(out, x0, ynoisy, background, A; niter) with A the
+ * plan's operator (src/ml-os-em.jl:21-49); sr_osem replaces
+ * osem!(out, x0, ynoisy, background, Ablock(plan, nblocks); niter)
+ * (src/ml-os-em.jl:70-131).  y, b: [nx,nz,nview]; b must be > 0 and
+ * nview % nblocks == 0, with the reference's messages otherwise. */
+int sr_mlem(sr_plan plan, float* out, const float* x0, const float* y, const float* b, int32_t niter, int32_t mem);
+int sr_osem(sr_plan plan, float* out, const float* x0, const float* y, const float* b, int32_t nblocks,
+            int32_t niter, int32_t mem);
+
+/* ---- unit operators ----------------------------------------------------------
+ * sr_imrotate replaces imrotate!/imrotate_adj! on a stack of nz square n x n
+ * slices (src/rotatez.jl:240-416, 3-D forms :472-495). */
+int sr_imrotate(float* out, const float* img, int32_t n, int32_t nz, float theta, int32_t interpmeth, int32_t adj,
+                int32_t mem);
+/* sr_fft_conv replaces fft_conv!/fft_conv_adj! (src/fft_convolve.jl:33-130):
+ * replicate-padded convolution of an nx x nz image with a px x pz kernel. */
+int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const float* ker, int32_t px, int32_t pz,
+                int32_t adj, int32_t mem);
+
+/* ---- input generators (host only) --------------------------------------------
+ * sr_psf_gauss replaces psf_gauss(; ny, px, pz, fwhm_x, fwhm_z)
+ * (src/psf-gauss.jl:24-55); psf is [px, pz, ny]. */
+int sr_psf_gauss(float* psf, int32_t ny, int32_t px, int32_t pz, const double* fwhm_x, const double* fwhm_z);
+/* Seeded synthetic inputs (no reference counterpart; SplitMix64): a sum of
+ * 6 random ellipsoids with intensities U(0.5,4), and the attenuation map of
+ * an outer 0.15 ellipsoid with two 0.05 inner ellipsoids. */
+int sr_phantom(float* image, int32_t nx, int32_t ny, int32_t nz, uint64_t seed);
+int sr_mumap(float* mumap, int32_t nx, int32_t ny, int32_t nz);
+/* out[i] ~ Poisson(mean[i]), seeded */
+int sr_poisson(float* out, const float* mean, size_t n, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPECTRECON_AMD_H */

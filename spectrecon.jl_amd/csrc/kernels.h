@@ -1,0 +1,91 @@
+// kernels.h -- launch interface of the HIP kernels (kernels.hip) used by the
+// plan runtime (runtime.cpp).  Data is FP32, column-major as in the
+// reference: a volume [x,y,z] has x fastest; the per-batch work volume is
+// [b][z][y][x] (same as [x,y,z,b]).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace sr {
+
+// Per-view rotation parameters.
+//  :two -- kind 0: forward copy (theta == 0 shortcut, src/rotatez.jl:341-344),
+//          kind 2: bilinear through the affine map (xx..y0); the adjoint
+//          always uses the map (src/rotatez.jl:390-416 has no shortcut).
+//  :one -- kind 0: copy, 1: quadrant turn m only, 2: quadrant turn then the
+//          three shears whose per-row tables live in the plan's tab arrays.
+struct RotView {
+  int kind;
+  int m;
+  float xx, xy, x0, yx, yy, y0;
+};
+
+struct Geo {
+  int nx, ny, nz;  // image (nx == ny)
+  int pad, Np;     // rotation pad and padded size, src/plan-rotate.jl:56
+  int px, pz;      // PSF size
+};
+
+// PSF taps for a plan, padded with zeros to the template tap counts.
+//  separable: u [planes][ntx], w [planes][ntz]; plane = v*vstride + y
+//  direct:    h [planes][pz][ntx]
+struct Taps {
+  const float* u;
+  const float* w;
+  const float* h;
+  int ntx, ntz;
+  int vstride;  // 0 when every view shares one PSF stack, else ny
+};
+
+enum FwdEpilogue { EPI_VIEWS = 0, EPI_RATIO = 1 };
+enum ImgMode { IMG_WRITE = 0, IMG_ACCUM = 1, IMG_EM = 2 };
+
+// ---- rotation -----------------------------------------------------------
+// out[b] (batch work volume) = rot_{vids[b]}(img) .* atten[vids[b]]
+// (atten may be null).  img is one volume (shared by all b).
+hipError_t launch_rot_fwd(int interp, const float* img, float* out, const float* atten, long atten_vstride,
+                          const RotView* rv, const int* vids, int nb, const int* tab_b, const float* tab_w,
+                          const Geo& g, hipStream_t s);
+// part[grp] = sum over b in group of rot_{vids[b]}^T(work[b]); groups of vpg
+// views; returns the number of groups through *ngroups.
+hipError_t launch_rot_adj(int interp, const float* work, float* part, const RotView* rv, const int* vids, int nb,
+                          int vpg, const int* tab_b, const float* tab_w, const Geo& g, hipStream_t s);
+// image-reduce: out = combine(sum_g part[g]) per ImgMode
+hipError_t launch_img_reduce(const float* part, int ngroups, float* out, const float* asum, long n, int mode,
+                             hipStream_t s);
+
+// attenuation factors, in place over the rotated mu volumes of the batch:
+// a[b][z][y][x] = exp(-dy * (sum_{j<=y} mu_r[j] - mu_r[y]/2)) in reference order
+hipError_t launch_atten_exact(float* mur, int nb, const Geo& g, float dy, hipStream_t s);
+
+// ---- PSF ----------------------------------------------------------------
+// per-chunk partial views of the batch from the work volume; returns the
+// chunk count actually used through *nchunk_used
+hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
+                          int nchunk, int* nchunk_used, const Geo& g, hipStream_t s);
+// views (or the ML-EM ratio y/(view + b) when epi == EPI_RATIO) = sum of partials
+hipError_t launch_view_reduce(const float* part, int nchunk, const int* vids, int nb, float* out, const float* yobs,
+                              const float* bkg, int epi, const Geo& g, hipStream_t s);
+// work[b][:,y,:] = atten[v][:,y,:] .* conv_adj(views[b], psf[v,y])
+hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, const int* vids, int nb, float* work,
+                          const float* atten, long atten_vstride, int nchunk, const Geo& g, hipStream_t s);
+
+// ---- elementwise --------------------------------------------------------
+hipError_t launch_fill(float* p, float v, long n, hipStream_t s);
+hipError_t launch_zero_to_inf(float* p, long n, hipStream_t s);
+
+// psf kernel limits: the fast separable / direct kernels handle tap counts
+// up to this many (padded to a multiple of 8)
+constexpr int kMaxTaps = 64;
+// LDS per workgroup on gfx950
+constexpr size_t kMaxLds = 160 * 1024;
+// largest padded rotation size the :one LDS kernel handles
+constexpr int kMaxNpOne = 202;
+// largest plane (nx * nz) the PSF kernels handle (one strip of <= 32 outputs
+// per thread and phase, <= 1024 threads)
+constexpr long kMaxPlane = 32768;
+
+// kernel classes for per-kernel event timing
+enum KernelClass { K_ROT_FWD = 0, K_PSF_FWD, K_VIEW_REDUCE, K_PSF_ADJ, K_ROT_ADJ, K_IMG_REDUCE, K_OTHER, K_NCLASS };
+
+}  // namespace sr

@@ -1,0 +1,873 @@
+// runtime.cpp -- device-resident SPECT plans and the C-ABI (spectrecon_amd.h).
+//
+// A plan owns: the mu-map, the per-view rotation parameters, the per-view
+// attenuation factors (the reference recomputes them for every view of every
+// call, src/project.jl:86-113; they depend only on mu, the angle and dy, so
+// they are built once here), the PSF taps, and the batch work buffers.  All
+// work of a plan runs on its own HIP stream; calls on one plan must not
+// overlap (the reference's plans are not re-entrant either).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/spectrecon_amd.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess)                                                               \
+      return fail(SR_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+  } while (0)
+
+constexpr double kPi = 3.14159265358979323846;
+
+long jmod(long x, long y) {
+  long r = x % y;
+  return r < 0 ? r + y : r;
+}
+
+// src/plan-rotate.jl:56
+int rot_padsize(int nx) { return (int)std::ceil(1.0 + nx * std::sqrt(2.0) / 2.0 - nx / 2.0); }
+
+// A device buffer that frees itself.
+struct DBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int ensure(size_t bytes) {
+    if (bytes <= n) return SR_OK;
+    release();
+    if (bytes == 0) return SR_OK;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return fail(SR_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    n = bytes;
+    return SR_OK;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Per-view rotation parameters from the angle (host, Float64 where the
+// reference computes in Float64).
+//  :one -- src/rotatez.jl:249-265 and linearinterp! (:21-51), per-row tables
+//          (b, C_odd, C_even) for the x pass (tan(θm/2)) and the y pass (-sin θm)
+//  :two -- the affine map of src/rotatez.jl:348-353, coefficients rounded to
+//          Float32 as AffineTransform2D{Float32} holds them.
+void rot_params(double theta, int interp, int Np, sr::RotView& rv, int* tb, float* tw) {
+  rv = sr::RotView{};
+  if (interp == SR_INTERP_TWO) {
+    rv.kind = (std::fmod(theta, 2 * kPi) == 0.0) ? 0 : 2;
+    const double c = (1.0 + Np) / 2.0;
+    const double phi = 2 * kPi - theta;
+    const double cs = std::cos(phi), sn = std::sin(phi);
+    const float cT = (float)c;
+    rv.xx = (float)(cs * 1.0 - sn * 0.0);
+    rv.xy = (float)(cs * 0.0 - sn * 1.0);
+    const float xr = (float)(cs * (double)(-cT) - sn * (double)(-cT));
+    rv.yx = (float)(cs * 0.0 + sn * 1.0);
+    rv.yy = (float)(cs * 1.0 + sn * 0.0);
+    const float yr = (float)(cs * (double)(-cT) + sn * (double)(-cT));
+    rv.x0 = (float)((double)xr + c);
+    rv.y0 = (float)((double)yr + c);
+    return;
+  }
+  if (std::fmod(theta, 2 * kPi) == 0.0) {
+    rv.kind = 0;
+    return;
+  }
+  const long m = jmod((long)std::floor(0.5 + theta / (kPi / 2)), 4);
+  rv.m = (int)m;
+  const double rtol = std::sqrt((double)std::numeric_limits<float>::epsilon());
+  const double q = m * (kPi / 2);
+  if (theta == q || std::fabs(theta - q) <= rtol * std::max(std::fabs(theta), std::fabs(q))) {
+    rv.kind = 1;
+    return;
+  }
+  rv.kind = 2;
+  const double mod_theta = theta - m * (kPi / 2);
+  const double coef[2] = {std::tan(mod_theta / 2), -std::sin(mod_theta)};
+  const double c = (Np + 1) / 2.0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 1; i <= Np; ++i) {
+      const double s = (i - c) * coef[pass] + 1;
+      const double e = (i - c) * coef[pass] + Np;
+      const double dec = std::ceil(s) - s;
+      int b;
+      if (e <= (double)Np) b = (int)std::ceil(e) - Np - 1;  // J = max(1, ceil(e) - Np + k [-1])
+      else b = (int)std::floor(s) - 1;                     // J = min(Np, floor(s) + k [-1])
+      tb[pass * Np + (i - 1)] = b;
+      tw[pass * 2 * Np + 2 * (i - 1)] = (float)dec;
+      tw[pass * 2 * Np + 2 * (i - 1) + 1] = (float)(1.0 - dec);
+    }
+}
+
+int round_taps(int p) { return p <= 8 ? 8 : p <= 16 ? 16 : p <= 24 ? 24 : p <= 32 ? 32 : p <= 40 ? 40 : p <= 48 ? 48 : 64; }
+
+// Rank-1 factorisation h[a,b] ~ u[a] w[b] of a px x pz plane, accepted when
+// every entry agrees to 1e-6 of max|h| (psf_gauss planes are outer products
+// rounded to Float32, src/psf-gauss.jl:40).
+bool factor_rank1(const float* h, int px, int pz, std::vector<double>& u, std::vector<double>& w) {
+  int ia = 0, ib = 0;
+  double hm = 0;
+  for (int b = 0; b < pz; ++b)
+    for (int a = 0; a < px; ++a)
+      if (std::fabs(h[a + px * b]) > hm) {
+        hm = std::fabs(h[a + px * b]);
+        ia = a;
+        ib = b;
+      }
+  u.assign(px, 0.0);
+  w.assign(pz, 0.0);
+  if (hm == 0) return true;
+  const double piv = h[ia + px * ib];
+  for (int a = 0; a < px; ++a) u[a] = h[a + px * ib];
+  for (int b = 0; b < pz; ++b) w[b] = h[ia + px * b] / piv;
+  for (int b = 0; b < pz; ++b)
+    for (int a = 0; a < px; ++a)
+      if (std::fabs(u[a] * w[b] - (double)h[a + px * b]) > 1e-6 * hm) return false;
+  return true;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Plan
+// ===========================================================================
+struct sr_plan_s {
+  sr_plan_desc d{};
+  sr::Geo g{};
+  long V = 0, P = 0;  // voxels, pixels per view
+  hipStream_t stream = nullptr;
+  int separable = 0;
+  int nt = 8;
+  int vstride = 0;
+  int max_batch = 32;
+  size_t bytes = 0;
+
+  DBuf mumap, atten, taps_u, taps_w, taps_h, rotv, tab_b, tab_w;
+  DBuf work, fpart, ipart, vids, ones;
+  DBuf st_img, st_views, st_aux1, st_aux2, st_x0;  // staging for host-pointer calls and EM temporaries
+  DBuf em_ratio, em_back, em_asum;
+  std::vector<int> h_vids;
+
+  // per-kernel event timing (sr_plan_set_timing)
+  bool timing = false;
+  struct Pending {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  double t_ms[sr::K_NCLASS] = {};
+  long t_n[sr::K_NCLASS] = {};
+
+  sr::Taps taps() const {
+    sr::Taps t;
+    t.u = taps_u.as<float>();
+    t.w = taps_w.as<float>();
+    t.h = taps_h.as<float>();
+    t.ntx = nt;
+    t.ntz = nt;
+    t.vstride = vstride;
+    return t;
+  }
+  long atten_vstride() const { return V; }
+  int nchunk_fwd(int nb) const {
+    const int target = std::max(1, (int)std::ceil(1024.0 / std::max(1, nb)));
+    return std::min(d.ny, target);
+  }
+  int groups_adj(int nb, int* vpg) const {
+    // views per rotation-adjoint group: enough blocks (nz * ngroups) to fill
+    // the device; the :one kernel takes one view per block
+    if (d.interpmeth == SR_INTERP_ONE) {
+      *vpg = 1;
+      return nb;
+    }
+    int v = nb;
+    while (v > 1 && (long)d.nz * ((nb + v - 1) / v) < 1024) v = (v + 1) / 2;
+    *vpg = v;
+    return (nb + v - 1) / v;
+  }
+};
+
+namespace {
+
+hipEvent_t pool_event(sr_plan p) {
+  if (!p->event_pool.empty()) {
+    hipEvent_t e = p->event_pool.back();
+    p->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return e;
+}
+
+// Brackets one kernel launch with events on the plan's stream when timing is on.
+struct KTimer {
+  sr_plan p;
+  int cls;
+  hipEvent_t a = nullptr;
+  KTimer(sr_plan p_, int cls_) : p(p_), cls(cls_) {
+    if (p->timing && (a = pool_event(p))) (void)hipEventRecord(a, p->stream);
+  }
+  ~KTimer() {
+    if (!a) return;
+    hipEvent_t b = pool_event(p);
+    if (!b) {
+      p->event_pool.push_back(a);
+      return;
+    }
+    (void)hipEventRecord(b, p->stream);
+    p->pending.push_back({cls, a, b});
+  }
+};
+
+// fold completed event pairs into the per-class totals (after a stream sync)
+void resolve_timing(sr_plan p) {
+  for (auto& e : p->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+      p->t_ms[e.cls] += ms;
+      p->t_n[e.cls] += 1;
+    } else {
+      (void)hipGetLastError();
+    }
+    p->event_pool.push_back(e.a);
+    p->event_pool.push_back(e.b);
+  }
+  p->pending.clear();
+}
+
+int ensure_batch_buffers(sr_plan p, int nb) {
+  int rc;
+  if ((rc = p->work.ensure(sizeof(float) * p->V * nb))) return rc;
+  if ((rc = p->fpart.ensure(sizeof(float) * p->P * (size_t)nb * p->d.ny))) return rc;
+  int vpg;
+  const int ng = p->groups_adj(nb, &vpg);
+  if ((rc = p->ipart.ensure(sizeof(float) * p->V * (size_t)ng))) return rc;
+  return SR_OK;
+}
+
+int upload_vids(sr_plan p, const int32_t* index, int32_t nindex) {
+  int rc;
+  if ((rc = p->vids.ensure(sizeof(int) * std::max(1, nindex)))) return rc;
+  p->h_vids.resize(nindex);
+  for (int i = 0; i < nindex; ++i) {
+    const int v = index ? index[i] : i;
+    if (v < 0 || v >= p->d.nview) return fail(SR_ERR_ARG, "view index %d out of range 0..%d", v, p->d.nview - 1);
+    p->h_vids[i] = v;
+  }
+  HIP_TRY(hipMemcpyAsync(p->vids.p, p->h_vids.data(), sizeof(int) * nindex, hipMemcpyHostToDevice, p->stream));
+  return SR_OK;
+}
+
+// views[ii] = A_{vids[ii]} image (or the ML-EM ratio y/(A x + b) when yobs)
+int run_project(sr_plan p, float* views, const float* image, const int* d_vids, int n, const float* yobs,
+                const float* bkg) {
+  const sr::Taps t = p->taps();
+  for (int s0 = 0; s0 < n; s0 += p->max_batch) {
+    const int nb = std::min(p->max_batch, n - s0);
+    int rc;
+    if ((rc = ensure_batch_buffers(p, nb))) return rc;
+    {
+      KTimer kt(p, sr::K_ROT_FWD);
+      HIP_TRY(sr::launch_rot_fwd(p->d.interpmeth, image, p->work.as<float>(), p->atten.as<float>(),
+                                 p->atten_vstride(), p->rotv.as<sr::RotView>(), d_vids + s0, nb, p->tab_b.as<int>(),
+                                 p->tab_w.as<float>(), p->g, p->stream));
+    }
+    int nchunk = 0;
+    {
+      KTimer kt(p, sr::K_PSF_FWD);
+      HIP_TRY(sr::launch_psf_fwd(p->separable, p->work.as<float>(), t, d_vids + s0, nb, p->fpart.as<float>(),
+                                 p->nchunk_fwd(nb), &nchunk, p->g, p->stream));
+    }
+    {
+      KTimer kt(p, sr::K_VIEW_REDUCE);
+      HIP_TRY(sr::launch_view_reduce(p->fpart.as<float>(), nchunk, d_vids + s0, nb, views + (long)s0 * p->P, yobs,
+                                     bkg, yobs ? sr::EPI_RATIO : sr::EPI_VIEWS, p->g, p->stream));
+    }
+  }
+  return SR_OK;
+}
+
+// image = sum_ii A_{vids[ii]}' views[ii]  (mode IMG_WRITE), or with IMG_EM:
+// image .*= (A' views) ./ asum
+int run_backproject(sr_plan p, float* image, const float* views, const int* d_vids, int n, int mode,
+                    const float* asum) {
+  const sr::Taps t = p->taps();
+  const bool single = n <= p->max_batch;
+  float* target = image;
+  if (mode == sr::IMG_EM && !single) {
+    int rc;
+    if ((rc = p->em_back.ensure(sizeof(float) * p->V))) return rc;
+    target = p->em_back.as<float>();
+  }
+  if (n == 0) {
+    if (mode == sr::IMG_WRITE) HIP_TRY(sr::launch_fill(image, 0.f, p->V, p->stream));
+    return SR_OK;
+  }
+  for (int s0 = 0; s0 < n; s0 += p->max_batch) {
+    const int nb = std::min(p->max_batch, n - s0);
+    int rc;
+    if ((rc = ensure_batch_buffers(p, nb))) return rc;
+    {
+      KTimer kt(p, sr::K_PSF_ADJ);
+      HIP_TRY(sr::launch_psf_adj(p->separable, views + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(),
+                                 p->atten.as<float>(), p->atten_vstride(), p->nchunk_fwd(nb), p->g, p->stream));
+    }
+    int vpg;
+    const int ng = p->groups_adj(nb, &vpg);
+    {
+      KTimer kt(p, sr::K_ROT_ADJ);
+      HIP_TRY(sr::launch_rot_adj(p->d.interpmeth, p->work.as<float>(), p->ipart.as<float>(),
+                                 p->rotv.as<sr::RotView>(), d_vids + s0, nb, vpg, p->tab_b.as<int>(),
+                                 p->tab_w.as<float>(), p->g, p->stream));
+    }
+    int m = (s0 == 0) ? sr::IMG_WRITE : sr::IMG_ACCUM;
+    if (mode == sr::IMG_EM && single) m = sr::IMG_EM;
+    KTimer kt(p, sr::K_IMG_REDUCE);
+    HIP_TRY(sr::launch_img_reduce(p->ipart.as<float>(), ng, target, asum, p->V, m, p->stream));
+  }
+  if (mode == sr::IMG_EM && !single) {
+    KTimer kt(p, sr::K_IMG_REDUCE);
+    HIP_TRY(sr::launch_img_reduce(target, 1, image, asum, p->V, sr::IMG_EM, p->stream));
+  }
+  return SR_OK;
+}
+
+int finish(sr_plan p, int32_t mem) {
+  if (mem & SR_ASYNC) return SR_OK;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  resolve_timing(p);
+  return SR_OK;
+}
+
+// Host-pointer staging: copies `bytes` from host to a device buffer.
+int stage_in(sr_plan p, DBuf& buf, const void* host, size_t bytes) {
+  int rc;
+  if ((rc = buf.ensure(bytes))) return rc;
+  HIP_TRY(hipMemcpyAsync(buf.p, host, bytes, hipMemcpyHostToDevice, p->stream));
+  return SR_OK;
+}
+
+int check_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return fail(SR_ERR_HIP, "no HIP device is available");
+  }
+  return SR_OK;
+}
+
+int check_background(const float* b, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!(b[i] > 0.f)) return fail(SR_ERR_ARG, "need background > 0");
+  return SR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sr_last_error(void) { return g_err.c_str(); }
+int sr_version(void) { return 10000; }
+
+int sr_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int sr_malloc(void** dptr, size_t bytes) {
+  if (!dptr) return fail(SR_ERR_ARG, "null pointer");
+  hipError_t e = hipMalloc(dptr, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(SR_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  return SR_OK;
+}
+int sr_free(void* dptr) {
+  HIP_TRY(hipFree(dptr));
+  return SR_OK;
+}
+int sr_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return SR_OK;
+}
+int sr_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return SR_OK;
+}
+int sr_memset(void* dptr, int value, size_t bytes) {
+  HIP_TRY(hipMemset(dptr, value, bytes));
+  return SR_OK;
+}
+int sr_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return SR_OK;
+}
+
+// src/spectplan.jl:58-137
+int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* psfs, const float* viewangle,
+                   sr_plan* out) {
+  if (!desc || !mumap || !psfs || !viewangle || !out) return fail(SR_ERR_ARG, "null argument");
+  *out = nullptr;
+  const sr_plan_desc& d = *desc;
+  // checks in the reference's order and with its messages (src/spectplan.jl:76-93)
+  if (d.nx != d.ny) return fail(SR_ERR_ARG, "nx != ny");
+  if (d.nx % 2 != 0 || d.ny % 2 != 0) return fail(SR_ERR_ARG, "nx odd");
+  if (d.px % 2 == 0 || d.pz % 2 == 0) return fail(SR_ERR_ARG, "non-odd size psfs");
+  if (d.nx <= 0 || d.nz <= 0 || d.px <= 0 || d.pz <= 0 || d.nview <= 0) return fail(SR_ERR_ARG, "size");
+  const long pxz = (long)d.px * d.pz;
+  const long nplane = (long)d.ny * d.nview;
+  for (long pl = 0; pl < nplane; ++pl) {
+    const float* h = psfs + pl * pxz;
+    for (long k = 0; k < pxz; ++k)
+      if (!(h[k] == h[pxz - 1 - k])) return fail(SR_ERR_ARG, "asym. psf");
+  }
+  if (d.interpmeth != SR_INTERP_ONE && d.interpmeth != SR_INTERP_TWO) return fail(SR_ERR_ARG, "bad interpmeth");
+  if (d.mode != SR_MODE_FAST && d.mode != SR_MODE_MEM) return fail(SR_ERR_ARG, "bad mode");
+  if (d.px > sr::kMaxTaps || d.pz > sr::kMaxTaps)
+    return fail(SR_ERR_UNSUPPORTED, "PSF size %dx%d exceeds the %d-tap kernels", d.px, d.pz, sr::kMaxTaps);
+  if ((long)d.nx * d.nz > sr::kMaxPlane) return fail(SR_ERR_UNSUPPORTED, "plane %dx%d too large", d.nx, d.nz);
+  int rc;
+  if ((rc = check_device())) return rc;
+
+  std::unique_ptr<sr_plan_s> guard(new sr_plan_s());
+  sr_plan p = guard.get();
+  p->d = d;
+  const int pad = rot_padsize(d.nx);
+  p->g = sr::Geo{d.nx, d.ny, d.nz, pad, d.nx + 2 * pad, d.px, d.pz};
+  if (d.interpmeth == SR_INTERP_ONE && p->g.Np > sr::kMaxNpOne)
+    return fail(SR_ERR_UNSUPPORTED, ":one rotation supports nx up to 140 (padded %d > %d)", d.nx + 2 * pad,
+                sr::kMaxNpOne);
+  p->V = (long)d.nx * d.ny * d.nz;
+  p->P = (long)d.nx * d.nz;
+  p->max_batch = std::max(1, std::min(d.nview, 64));
+  auto bail = [](int code) { return code; };
+  if (hipStreamCreate(&p->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return bail(fail(SR_ERR_HIP, "hipStreamCreate failed"));
+  }
+
+  // --- PSF taps: dedupe views, choose separable vs direct ---
+  bool shared = true;
+  for (int v = 1; v < d.nview && shared; ++v)
+    shared = std::memcmp(psfs, psfs + (long)v * d.ny * pxz, sizeof(float) * d.ny * pxz) == 0;
+  p->vstride = shared ? 0 : d.ny;
+  const int nstore = shared ? d.ny : d.ny * d.nview;
+  bool sep = d.conv != SR_CONV_DIRECT;
+  std::vector<double> u, w;
+  if (sep)
+    for (int pl = 0; pl < nstore && sep; ++pl) sep = factor_rank1(psfs + (long)pl * pxz, d.px, d.pz, u, w);
+  if (d.conv == SR_CONV_SEPARABLE && !sep) return bail(fail(SR_ERR_ARG, "PSF planes are not rank 1"));
+  p->separable = sep ? 1 : 0;
+  p->nt = round_taps(std::max(d.px, d.pz));
+  if (sep) {
+    std::vector<float> hu((size_t)nstore * p->nt, 0.f), hw((size_t)nstore * p->nt, 0.f);
+    for (int pl = 0; pl < nstore; ++pl) {
+      factor_rank1(psfs + (long)pl * pxz, d.px, d.pz, u, w);
+      for (int a = 0; a < d.px; ++a) hu[(size_t)pl * p->nt + a] = (float)u[a];
+      for (int b = 0; b < d.pz; ++b) hw[(size_t)pl * p->nt + b] = (float)w[b];
+    }
+    if ((rc = p->taps_u.ensure(sizeof(float) * hu.size())) || (rc = p->taps_w.ensure(sizeof(float) * hw.size())))
+      return bail(rc);
+    HIP_TRY(hipMemcpy(p->taps_u.p, hu.data(), sizeof(float) * hu.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->taps_w.p, hw.data(), sizeof(float) * hw.size(), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> hh((size_t)nstore * d.pz * p->nt, 0.f);
+    for (int pl = 0; pl < nstore; ++pl)
+      for (int b = 0; b < d.pz; ++b)
+        for (int a = 0; a < d.px; ++a)
+          hh[((size_t)pl * d.pz + b) * p->nt + a] = psfs[(long)pl * pxz + a + (long)d.px * b];
+    if ((rc = p->taps_h.ensure(sizeof(float) * hh.size()))) return bail(rc);
+    HIP_TRY(hipMemcpy(p->taps_h.p, hh.data(), sizeof(float) * hh.size(), hipMemcpyHostToDevice));
+  }
+
+  // --- rotation parameters ---
+  const int Np = p->g.Np;
+  std::vector<sr::RotView> rv(d.nview);
+  std::vector<int> tb((size_t)d.nview * 2 * Np, 0);
+  std::vector<float> tw((size_t)d.nview * 4 * Np, 0.f);
+  for (int v = 0; v < d.nview; ++v)
+    rot_params((double)viewangle[v], d.interpmeth, Np, rv[v], tb.data() + (size_t)v * 2 * Np,
+               tw.data() + (size_t)v * 4 * Np);
+  if ((rc = p->rotv.ensure(sizeof(sr::RotView) * d.nview)) || (rc = p->tab_b.ensure(sizeof(int) * tb.size())) ||
+      (rc = p->tab_w.ensure(sizeof(float) * tw.size())))
+    return bail(rc);
+  HIP_TRY(hipMemcpy(p->rotv.p, rv.data(), sizeof(sr::RotView) * d.nview, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->tab_b.p, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->tab_w.p, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice));
+
+  // --- mu-map and attenuation factors for every view ---
+  if ((rc = p->mumap.ensure(sizeof(float) * p->V))) return bail(rc);
+  HIP_TRY(hipMemcpy(p->mumap.p, mumap, sizeof(float) * p->V, hipMemcpyHostToDevice));
+  if ((rc = p->atten.ensure(sizeof(float) * p->V * d.nview))) return bail(rc);
+  std::vector<int> all(d.nview);
+  for (int v = 0; v < d.nview; ++v) all[v] = v;
+  if ((rc = p->vids.ensure(sizeof(int) * d.nview))) return bail(rc);
+  HIP_TRY(hipMemcpy(p->vids.p, all.data(), sizeof(int) * d.nview, hipMemcpyHostToDevice));
+  const int chunk = 32;
+  for (int v0 = 0; v0 < d.nview; v0 += chunk) {
+    const int nb = std::min(chunk, d.nview - v0);
+    float* dst = p->atten.as<float>() + p->V * v0;
+    HIP_TRY(sr::launch_rot_fwd(d.interpmeth, p->mumap.as<float>(), dst, nullptr, 0, p->rotv.as<sr::RotView>(),
+                               p->vids.as<int>() + v0, nb, p->tab_b.as<int>(), p->tab_w.as<float>(), p->g,
+                               p->stream));
+    HIP_TRY(sr::launch_atten_exact(dst, nb, p->g, d.dy, p->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->rotv.n + p->tab_b.n + p->tab_w.n;
+  *out = guard.release();
+  return SR_OK;
+}
+
+int sr_plan_destroy(sr_plan p) {
+  if (!p) return SR_OK;
+  if (p->stream) {
+    (void)hipStreamSynchronize(p->stream);
+    resolve_timing(p);
+    for (hipEvent_t e : p->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(p->stream);
+  }
+  delete p;
+  return SR_OK;
+}
+
+int sr_plan_set_timing(sr_plan p, int32_t enable) {
+  if (!p) return fail(SR_ERR_ARG, "null plan");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  resolve_timing(p);
+  p->timing = enable != 0;
+  for (int k = 0; k < sr::K_NCLASS; ++k) {
+    p->t_ms[k] = 0;
+    p->t_n[k] = 0;
+  }
+  return SR_OK;
+}
+
+int sr_plan_get_timing(sr_plan p, int32_t kernel, double* total_ms, int64_t* launches) {
+  if (!p) return fail(SR_ERR_ARG, "null plan");
+  if (kernel < 0 || kernel >= sr::K_NCLASS) return fail(SR_ERR_ARG, "kernel class out of range");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  resolve_timing(p);
+  if (total_ms) *total_ms = p->t_ms[kernel];
+  if (launches) *launches = p->t_n[kernel];
+  return SR_OK;
+}
+
+int sr_plan_info(sr_plan p, size_t* device_bytes, int32_t* conv_method, int32_t* max_batch) {
+  if (!p) return fail(SR_ERR_ARG, "null plan");
+  if (device_bytes)
+    *device_bytes = p->bytes + p->work.n + p->fpart.n + p->ipart.n + p->st_img.n + p->st_views.n + p->st_aux1.n +
+                    p->st_aux2.n + p->st_x0.n + p->em_ratio.n + p->em_back.n + p->em_asum.n + p->ones.n;
+  if (conv_method) *conv_method = p->separable ? SR_CONV_SEPARABLE : SR_CONV_DIRECT;
+  if (max_batch) *max_batch = p->max_batch;
+  return SR_OK;
+}
+
+int sr_plan_set_batch(sr_plan p, int32_t max_batch) {
+  if (!p) return fail(SR_ERR_ARG, "null plan");
+  if (max_batch < 1) return fail(SR_ERR_ARG, "max_batch must be >= 1");
+  p->max_batch = max_batch;
+  return SR_OK;
+}
+
+int sr_plan_synchronize(sr_plan p) {
+  if (!p) return fail(SR_ERR_ARG, "null plan");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return SR_OK;
+}
+
+// src/project.jl:143-163
+int sr_project(sr_plan p, float* views, const float* image, const int32_t* index, int32_t nindex, int32_t mem) {
+  if (!p || !views || !image) return fail(SR_ERR_ARG, "null argument");
+  if (!index && nindex != p->d.nview) return fail(SR_ERR_DIM, "nindex must equal nview when index is NULL");
+  int rc;
+  if ((rc = upload_vids(p, index, nindex))) return rc;
+  const bool host = (mem & 0xff) == SR_MEM_HOST;
+  const float* d_img = image;
+  float* d_views = views;
+  if (host) {
+    if ((rc = stage_in(p, p->st_img, image, sizeof(float) * p->V))) return rc;
+    if ((rc = p->st_views.ensure(sizeof(float) * p->P * std::max(1, nindex)))) return rc;
+    d_img = p->st_img.as<float>();
+    d_views = p->st_views.as<float>();
+  }
+  if ((rc = run_project(p, d_views, d_img, p->vids.as<int>(), nindex, nullptr, nullptr))) return rc;
+  if (host) {
+    HIP_TRY(hipMemcpyAsync(views, d_views, sizeof(float) * p->P * nindex, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    resolve_timing(p);
+    return SR_OK;
+  }
+  return finish(p, mem);
+}
+
+// src/backproject.jl:340-370
+int sr_backproject(sr_plan p, float* image, const float* views, const int32_t* index, int32_t nindex, int32_t mem) {
+  if (!p || !views || !image) return fail(SR_ERR_ARG, "null argument");
+  if (!index && nindex != p->d.nview) return fail(SR_ERR_DIM, "nindex must equal nview when index is NULL");
+  int rc;
+  if ((rc = upload_vids(p, index, nindex))) return rc;
+  const bool host = (mem & 0xff) == SR_MEM_HOST;
+  float* d_img = image;
+  const float* d_views = views;
+  if (host) {
+    if ((rc = stage_in(p, p->st_views, views, sizeof(float) * p->P * std::max(1, nindex)))) return rc;
+    if ((rc = p->st_img.ensure(sizeof(float) * p->V))) return rc;
+    d_img = p->st_img.as<float>();
+    d_views = p->st_views.as<float>();
+  }
+  if ((rc = run_backproject(p, d_img, d_views, p->vids.as<int>(), nindex, sr::IMG_WRITE, nullptr))) return rc;
+  if (host) {
+    HIP_TRY(hipMemcpyAsync(image, d_img, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    resolve_timing(p);
+    return SR_OK;
+  }
+  return finish(p, mem);
+}
+
+namespace {
+
+// shared body of mlem!/osem!: blocks of views nb:nblocks:nview
+// (src/ml-os-em.jl:21-49 with nblocks == 1; :97-131 otherwise)
+int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* b, int nblocks, int niter,
+           int32_t mem) {
+  const int nview = p->d.nview;
+  const long M = p->P * nview;
+  const bool host = (mem & 0xff) == SR_MEM_HOST;
+  int rc;
+  // background > 0 (src/ml-os-em.jl:30, :106), checked on the host
+  if (host) {
+    if ((rc = check_background(b, (size_t)M))) return rc;
+  } else {
+    std::vector<float> hb(M);
+    HIP_TRY(hipMemcpy(hb.data(), b, sizeof(float) * M, hipMemcpyDeviceToHost));
+    if ((rc = check_background(hb.data(), (size_t)M))) return rc;
+  }
+  const float *d_x0 = x0, *d_y = y, *d_b = b;
+  float* d_out = out;
+  if (host) {
+    if ((rc = stage_in(p, p->st_x0, x0, sizeof(float) * p->V)) || (rc = stage_in(p, p->st_aux1, y, sizeof(float) * M)) ||
+        (rc = stage_in(p, p->st_aux2, b, sizeof(float) * M)) || (rc = p->st_img.ensure(sizeof(float) * p->V)))
+      return rc;
+    d_x0 = p->st_x0.as<float>();
+    d_y = p->st_aux1.as<float>();
+    d_b = p->st_aux2.as<float>();
+    d_out = p->st_img.as<float>();
+  }
+  // subset view lists, uploaded once
+  std::vector<int> order;
+  for (int nb = 0; nb < nblocks; ++nb)
+    for (int v = nb; v < nview; v += nblocks) order.push_back(v);
+  const int nsub = nview / nblocks;
+  if ((rc = p->vids.ensure(sizeof(int) * nview))) return rc;
+  HIP_TRY(hipMemcpyAsync(p->vids.p, order.data(), sizeof(int) * nview, hipMemcpyHostToDevice, p->stream));
+  const int* d_vids = p->vids.as<int>();
+  // asum per block = A_b' 1, zeros -> Inf
+  if ((rc = p->ones.ensure(sizeof(float) * p->P * nsub)) || (rc = p->em_asum.ensure(sizeof(float) * p->V * nblocks)) ||
+      (rc = p->em_ratio.ensure(sizeof(float) * p->P * nsub)))
+    return rc;
+  HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P * nsub, p->stream));
+  for (int nb = 0; nb < nblocks; ++nb) {
+    float* as = p->em_asum.as<float>() + p->V * nb;
+    if ((rc = run_backproject(p, as, p->ones.as<float>(), d_vids + nb * nsub, nsub, sr::IMG_WRITE, nullptr)))
+      return rc;
+    HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(d_out, d_x0, sizeof(float) * p->V, hipMemcpyDeviceToDevice, p->stream));
+  for (int it = 0; it < niter; ++it)
+    for (int nb = 0; nb < nblocks; ++nb) {
+      const int* sv = d_vids + nb * nsub;
+      if ((rc = run_project(p, p->em_ratio.as<float>(), d_out, sv, nsub, d_y, d_b))) return rc;
+      if ((rc = run_backproject(p, d_out, p->em_ratio.as<float>(), sv, nsub, sr::IMG_EM,
+                                p->em_asum.as<float>() + p->V * nb)))
+        return rc;
+    }
+  if (host) {
+    HIP_TRY(hipMemcpyAsync(out, d_out, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    resolve_timing(p);
+    return SR_OK;
+  }
+  return finish(p, mem);
+}
+
+}  // namespace
+
+int sr_mlem(sr_plan p, float* out, const float* x0, const float* y, const float* b, int32_t niter, int32_t mem) {
+  if (!p || !out || !x0 || !y || !b) return fail(SR_ERR_ARG, "null argument");
+  if (niter < 0) return fail(SR_ERR_ARG, "niter < 0");
+  return run_em(p, out, x0, y, b, 1, niter, mem);
+}
+
+int sr_osem(sr_plan p, float* out, const float* x0, const float* y, const float* b, int32_t nblocks, int32_t niter,
+            int32_t mem) {
+  if (!p || !out || !x0 || !y || !b) return fail(SR_ERR_ARG, "null argument");
+  if (nblocks < 1) return fail(SR_ERR_ARG, "nblocks < 1");
+  if (p->d.nview % nblocks != 0) return fail(SR_ERR_ARG, "nview must be divisible by nblocks!");
+  if (niter < 0) return fail(SR_ERR_ARG, "niter < 0");
+  return run_em(p, out, x0, y, b, nblocks, niter, mem);
+}
+
+// ---------------------------------------------------------------------------
+// Unit operators
+// ---------------------------------------------------------------------------
+
+// src/rotatez.jl:240-416 and the 3-D forms :472-495
+int sr_imrotate(float* out, const float* img, int32_t n, int32_t nz, float theta, int32_t interpmeth, int32_t adj,
+                int32_t mem) {
+  if (!out || !img) return fail(SR_ERR_ARG, "null argument");
+  if (n <= 0 || nz <= 0) return fail(SR_ERR_ARG, "size");
+  if (interpmeth != SR_INTERP_ONE && interpmeth != SR_INTERP_TWO) return fail(SR_ERR_ARG, "bad interpmeth");
+  int rc;
+  if ((rc = check_device())) return rc;
+  const int pad = rot_padsize(n);
+  sr::Geo g{n, n, nz, pad, n + 2 * pad, 1, 1};
+  if (interpmeth == SR_INTERP_ONE && g.Np > sr::kMaxNpOne)
+    return fail(SR_ERR_UNSUPPORTED, ":one rotation supports n up to 140");
+  sr::RotView rv;
+  std::vector<int> tb(2 * g.Np);
+  std::vector<float> tw(4 * g.Np);
+  rot_params((double)theta, interpmeth, g.Np, rv, tb.data(), tw.data());
+  const long V = (long)n * n * nz;
+  DBuf d_rv, d_tb, d_tw, d_vid, d_in, d_out;
+  if ((rc = d_rv.ensure(sizeof rv)) || (rc = d_tb.ensure(sizeof(int) * tb.size())) ||
+      (rc = d_tw.ensure(sizeof(float) * tw.size())) || (rc = d_vid.ensure(sizeof(int))))
+    return rc;
+  const int zero = 0;
+  HIP_TRY(hipMemcpy(d_rv.p, &rv, sizeof rv, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_tb.p, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_tw.p, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_vid.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+  const bool host = (mem & 0xff) == SR_MEM_HOST;
+  const float* din = img;
+  float* dout = out;
+  if (host) {
+    if ((rc = d_in.ensure(sizeof(float) * V)) || (rc = d_out.ensure(sizeof(float) * V))) return rc;
+    HIP_TRY(hipMemcpy(d_in.p, img, sizeof(float) * V, hipMemcpyHostToDevice));
+    din = d_in.as<float>();
+    dout = d_out.as<float>();
+  }
+  if (adj)
+    HIP_TRY(sr::launch_rot_adj(interpmeth, din, dout, d_rv.as<sr::RotView>(), d_vid.as<int>(), 1, 1, d_tb.as<int>(),
+                               d_tw.as<float>(), g, nullptr));
+  else
+    HIP_TRY(sr::launch_rot_fwd(interpmeth, din, dout, nullptr, 0, d_rv.as<sr::RotView>(), d_vid.as<int>(), 1,
+                               d_tb.as<int>(), d_tw.as<float>(), g, nullptr));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  if (host) HIP_TRY(hipMemcpy(out, dout, sizeof(float) * V, hipMemcpyDeviceToHost));
+  return SR_OK;
+}
+
+// src/fft_convolve.jl:33-130 (fft_conv!, fft_conv_adj!), one plane
+int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const float* ker, int32_t px, int32_t pz,
+                int32_t adj, int32_t mem) {
+  if (!out || !img || !ker) return fail(SR_ERR_ARG, "null argument");
+  if (nx <= 0 || nz <= 0 || px <= 0 || pz <= 0) return fail(SR_ERR_ARG, "size");
+  if (px % 2 == 0 || pz % 2 == 0) return fail(SR_ERR_ARG, "non-odd size psfs");
+  if (px > sr::kMaxTaps || pz > sr::kMaxTaps) return fail(SR_ERR_UNSUPPORTED, "kernel larger than %d taps", sr::kMaxTaps);
+  if ((long)nx * nz > sr::kMaxPlane) return fail(SR_ERR_UNSUPPORTED, "plane too large");
+  int rc;
+  if ((rc = check_device())) return rc;
+  const bool host = (mem & 0xff) == SR_MEM_HOST;
+  std::vector<float> hk((size_t)px * pz);
+  if (host) std::memcpy(hk.data(), ker, sizeof(float) * hk.size());
+  else HIP_TRY(hipMemcpy(hk.data(), ker, sizeof(float) * hk.size(), hipMemcpyDeviceToHost));
+  std::vector<double> u, w;
+  const bool sep = factor_rank1(hk.data(), px, pz, u, w);
+  const int nt = round_taps(std::max(px, pz));
+  sr::Geo g{nx, 1, nz, 0, 0, px, pz};
+  DBuf d_u, d_w, d_h, d_vid, d_in, d_out, d_part;
+  sr::Taps t{};
+  t.ntx = t.ntz = nt;
+  t.vstride = 0;
+  if (sep) {
+    std::vector<float> hu(nt, 0.f), hw(nt, 0.f);
+    for (int a = 0; a < px; ++a) hu[a] = (float)u[a];
+    for (int b = 0; b < pz; ++b) hw[b] = (float)w[b];
+    if ((rc = d_u.ensure(sizeof(float) * nt)) || (rc = d_w.ensure(sizeof(float) * nt))) return rc;
+    HIP_TRY(hipMemcpy(d_u.p, hu.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_w.p, hw.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> hh((size_t)pz * nt, 0.f);
+    for (int b = 0; b < pz; ++b)
+      for (int a = 0; a < px; ++a) hh[(size_t)b * nt + a] = hk[a + (size_t)px * b];
+    if ((rc = d_h.ensure(sizeof(float) * hh.size()))) return rc;
+    HIP_TRY(hipMemcpy(d_h.p, hh.data(), sizeof(float) * hh.size(), hipMemcpyHostToDevice));
+  }
+  t.u = d_u.as<float>();
+  t.w = d_w.as<float>();
+  t.h = d_h.as<float>();
+  const int zero = 0;
+  if ((rc = d_vid.ensure(sizeof(int)))) return rc;
+  HIP_TRY(hipMemcpy(d_vid.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+  const long P = (long)nx * nz;
+  const float* din = img;
+  float* dout = out;
+  if (host) {
+    if ((rc = d_in.ensure(sizeof(float) * P)) || (rc = d_out.ensure(sizeof(float) * P))) return rc;
+    HIP_TRY(hipMemcpy(d_in.p, img, sizeof(float) * P, hipMemcpyHostToDevice));
+    din = d_in.as<float>();
+    dout = d_out.as<float>();
+  }
+  if (adj) {
+    HIP_TRY(sr::launch_psf_adj(sep, din, t, d_vid.as<int>(), 1, dout, nullptr, 0, 1, g, nullptr));
+  } else {
+    if ((rc = d_part.ensure(sizeof(float) * P))) return rc;
+    int nchunk = 0;
+    HIP_TRY(sr::launch_psf_fwd(sep, din, t, d_vid.as<int>(), 1, d_part.as<float>(), 1, &nchunk, g, nullptr));
+    HIP_TRY(sr::launch_view_reduce(d_part.as<float>(), nchunk, d_vid.as<int>(), 1, dout, nullptr, nullptr,
+                                   sr::EPI_VIEWS, g, nullptr));
+  }
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  if (host) HIP_TRY(hipMemcpy(out, dout, sizeof(float) * P, hipMemcpyDeviceToHost));
+  return SR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,255 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Tolerances (floating point, FP32 on the device):
+  * rotations, convolutions: max|g-r|/max|r| <= 2e-6 vs the FP32 oracle
+  * projections: per-view max-relative <= 2e-5 vs the FP32 oracle
+  * adjoint identity <y, Ax> = <A'y, x>: relative <= 1e-5
+  * ML-EM / OS-EM: relative L2 <= 1e-4 vs the FP64 oracle
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import maxrel, per_view_maxrel, rand_sym_psfs
+
+pytestmark = pytest.mark.gpu
+
+ROT_TOL = 2e-6
+PROJ_TOL = 2e-5
+
+
+def angles32(nview):
+    twopi = float(np.float32(2 * math.pi))
+    return np.array([np.float32(k * twopi / nview) for k in range(nview)], np.float32)
+
+
+# ---------------------------------------------------------------------------
+# rotation
+# ---------------------------------------------------------------------------
+ANGLES = [0.0, 0.3, float(np.float32(math.pi / 6)), float(np.float32(math.pi / 2)), 2.0,
+          float(np.float32(math.pi)), 4.0, float(np.float32(3 * math.pi / 2)), 5.9, float(np.float32(2 * math.pi))]
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+@pytest.mark.parametrize("n", [16, 20, 64, 128])
+def test_imrotate_vs_oracle(gpu, O, method, n):
+    rng = np.random.default_rng(n)
+    img = np.asfortranarray(rng.random((n, n, 3), dtype=np.float32))
+    for th in ANGLES:
+        th = float(np.float32(th))
+        g = gpu.imrotate(img, th, method=method)
+        r = O.imrotate(img, th, method)
+        assert maxrel(g, r) <= ROT_TOL, (method, n, th, maxrel(g, r))
+        ga = gpu.imrotate_adj(img, th, method=method)
+        ra = O.imrotate(img, th, method, adj=True)
+        assert maxrel(ga, ra) <= ROT_TOL, ("adj", method, n, th, maxrel(ga, ra))
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+def test_imrotate_dense_adjoint(gpu, method):
+    """Matrix(A') == Matrix(A)' at 16x16, 7 angles (test/adjoint-rotate.jl:43-58):
+    all 256 unit images rotated in one 3-D call."""
+    M = 16
+    eye = np.asfortranarray(np.eye(M * M, dtype=np.float32).reshape(M, M, M * M, order="F"))
+    for k in range(7):
+        th = float(np.float32(k / 7 * 2 * math.pi))
+        A = gpu.imrotate(eye, th, method=method).reshape(M * M, M * M, order="F")
+        At = gpu.imrotate_adj(eye, th, method=method).reshape(M * M, M * M, order="F")
+        assert np.allclose(A.T, At, rtol=1e-6, atol=1e-7), (method, th)
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+def test_imrotate_bounds(gpu, method):
+    """test/adjoint-rotate.jl:28-37: rotated rand has max <= 1.01, adjoint <= 1.5."""
+    rng = np.random.default_rng(7)
+    img = np.asfortranarray(rng.random((20, 20, 4), dtype=np.float32))
+    th = math.pi / 6
+    assert gpu.imrotate(img, th, method=method).max() <= 1.01
+    assert gpu.imrotate_adj(img, th, method=method).max() <= 1.5
+
+
+# ---------------------------------------------------------------------------
+# PSF convolution
+# ---------------------------------------------------------------------------
+CONV_CASES = [(20, 14, 5, 5), (12, 8, 7, 7), (12, 10, 5, 3), (64, 32, 15, 15), (128, 80, 37, 37), (7, 1, 3, 1),
+              (33, 17, 9, 1), (16, 16, 1, 1)]
+
+
+@pytest.mark.parametrize("nx,nz,px,pz", CONV_CASES)
+@pytest.mark.parametrize("kind", ["separable", "general"])
+def test_fft_conv_vs_oracle(gpu, O, nx, nz, px, pz, kind):
+    rng = np.random.default_rng(nx * 100 + nz)
+    img = np.asfortranarray(rng.standard_normal((nx, nz)).astype(np.float32))
+    if kind == "separable":
+        u = rng.random(px) + 0.1
+        w = rng.random(pz) + 0.1
+        u = u + u[::-1]
+        w = w + w[::-1]
+        ker = np.outer(u, w)
+    else:
+        ker = rng.random((px, pz))
+        ker = ker + ker[::-1, ::-1]
+    ker = np.asfortranarray((ker / ker.sum()).astype(np.float32))
+    for adj in (False, True):
+        r = O.fft_conv(img.astype(np.float64), ker.astype(np.float64), adj, "direct", np.float64)
+        g = gpu.fft_conv_adj(img, ker) if adj else gpu.fft_conv(img, ker)
+        assert maxrel(g, r) <= ROT_TOL, (adj, maxrel(g, r))
+
+
+def test_fft_conv_bounds(gpu):
+    """test/fftconv.jl:198-214: box kernel keeps rand <= 1, adjoint <= 1.5."""
+    rng = np.random.default_rng(3)
+    nx, nz, px, pz = 12, 10, 5, 3
+    plan = gpu.plan_psf(nx=nx, nz=nz, px=px, pz=pz)
+    image3 = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    ker3 = np.ones((px, pz, nx), np.float32) / (px * pz)
+    result = np.zeros_like(image3)
+    gpu.fft_conv_(result, image3, ker3, plan)
+    assert result.max() <= 1
+    gpu.fft_conv_adj_(result, image3, ker3, plan)
+    assert result.max() <= 1.5
+    gpu.fft_conv_adj2_(result, image3[:, 2, :], ker3, plan)
+    assert result.max() <= 1.5
+
+
+# ---------------------------------------------------------------------------
+# projector / back-projector
+# ---------------------------------------------------------------------------
+def small_problem(seed, nx=16, nz=8, nview=8, px=5, pz=3, separable=False):
+    rng = np.random.default_rng(seed)
+    mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
+    if separable:
+        psfs = np.asfortranarray(np.repeat(
+            np.asarray(__import__("oracle").psf_gauss(nx, px, pz, 1.0, 3.0), np.float32)[:, :, :, None], nview, 3))
+    else:
+        psfs = rand_sym_psfs(rng, px, pz, nx, nview)
+    image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    return mumap, psfs, image
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+@pytest.mark.parametrize("mode", ["fast", "mem"])
+@pytest.mark.parametrize("separable", [False, True])
+def test_project_vs_oracle(gpu, O, method, mode, separable):
+    nx, nz, nview = 16, 8, 8
+    mumap, psfs, image = small_problem(11, nx, nz, nview, 5, 3 if not separable else 5, separable)
+    dy = 0.48
+    plan = gpu.SPECTplan(mumap, psfs, dy, interpmeth=method, mode=mode)
+    assert plan.info()["conv"] == ("separable" if separable else "direct")
+    ang = angles32(nview).astype(np.float64)
+    P32 = O.Problem(mumap, psfs, dy, ang, method, np.float32, "fft")
+    P64 = O.Problem(mumap, psfs, dy, ang, method, np.float64, "direct")
+    g = gpu.project(image, plan)
+    r = P32.project(image)
+    r64 = P64.project(image.astype(np.float64))
+    assert per_view_maxrel(g, r) <= PROJ_TOL, per_view_maxrel(g, r)
+    assert per_view_maxrel(g, r64) <= PROJ_TOL
+    rng = np.random.default_rng(5)
+    views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    gb = gpu.backproject(views, plan)
+    rb = P32.backproject(views)
+    assert maxrel(gb, rb) <= PROJ_TOL, maxrel(gb, rb)
+    # subsets of views, in arbitrary order
+    idx = [5, 0, 3]
+    gs = gpu.project(image, plan, index=idx)
+    assert per_view_maxrel(gs, g[:, :, idx]) <= 1e-6
+    gbs = gpu.backproject(views[:, :, idx], plan, index=idx)
+    rbs = P32.backproject(views[:, :, idx], index=np.array(idx, np.int32))
+    assert maxrel(gbs, rbs) <= PROJ_TOL
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+def test_adjoint_dot(gpu, method):
+    """<y, Ax> = <A'y, x> at 32x32x20, 30 views, 5x3 PSF (test/adjoint-project.jl:52-107)."""
+    nx, nz, nview = 32, 20, 30
+    rng = np.random.default_rng(17)
+    mumap = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    psfs = rand_sym_psfs(rng, 5, 3, nx, nview)
+    plan = gpu.SPECTplan(mumap, psfs, 4.7952, interpmeth=method)
+    x = np.asfortranarray(rng.standard_normal((nx, nx, nz)).astype(np.float32))
+    y = np.asfortranarray(rng.standard_normal((nx, nz, nview)).astype(np.float32))
+    Ax = gpu.project(x, plan)
+    Aty = gpu.backproject(y, plan)
+    a = float(np.vdot(y.astype(np.float64), Ax.astype(np.float64)))
+    b = float(np.vdot(x.astype(np.float64), Aty.astype(np.float64)))
+    assert abs(a - b) <= 1e-5 * abs(a), (a, b)
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+def test_adjoint_dense(gpu, method):
+    """Matrix(A)' == Matrix(A') at 8x8x6, 7 views, 5x3 PSF (test/adjoint-project.jl:12-49)."""
+    nx, nz, nview = 8, 6, 7
+    rng = np.random.default_rng(23)
+    mumap = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    psfs = rand_sym_psfs(rng, 5, 3, nx, nview)
+    plan = gpu.SPECTplan(mumap, psfs, 4.7952, interpmeth=method)
+    N, M = nx * nx * nz, nx * nz * nview
+    A = np.zeros((M, N))
+    for k in range(N):
+        e = np.zeros(N, np.float32)
+        e[k] = 1
+        A[:, k] = gpu.project(e.reshape(nx, nx, nz, order="F"), plan).ravel(order="F")
+    At = np.zeros((N, M))
+    for k in range(M):
+        e = np.zeros(M, np.float32)
+        e[k] = 1
+        At[:, k] = gpu.backproject(e.reshape(nx, nz, nview, order="F"), plan).ravel(order="F")
+    assert np.allclose(A.T, At, rtol=1e-5, atol=1e-6 * np.abs(A).max())
+
+
+# ---------------------------------------------------------------------------
+# ML-EM / OS-EM
+# ---------------------------------------------------------------------------
+def em_problem(nx=32, nz=16, nview=24, px=5):
+    import oracle as O
+
+    rng = np.random.default_rng(2)
+    x = np.zeros((nx, nx, nz), np.float32)
+    x[nx // 4:2 * nx // 3, nx // 5:3 * nx // 5, 2 * nz // 6:3 * nz // 6] = 1
+    x[2 * nx // 5:3 * nx // 5, nx // 5:2 * nx // 5, 4 * nz // 6:5 * nz // 6] = 2
+    psf = O.psf_gauss(nx, px, px, 1.0, 4.0)
+    psfs = np.asfortranarray(np.repeat(psf[:, :, :, None], nview, 3))
+    mumap = np.asfortranarray((0.02 * rng.random((nx, nx, nz))).astype(np.float32))
+    return np.asfortranarray(x), mumap, psfs
+
+
+def test_mlem_osem_vs_oracle(gpu, O):
+    nx, nz, nview = 32, 16, 24
+    xtrue, mumap, psfs = em_problem(nx, nz, nview)
+    dy = 0.8
+    plan = gpu.SPECTplan(mumap, psfs, dy)
+    ang = angles32(nview).astype(np.float64)
+    P64 = O.Problem(mumap, psfs, dy, ang, "two", np.float64, "direct")
+    ytrue = gpu.project(xtrue, plan)
+    b = np.full_like(ytrue, 0.1 * ytrue.mean())
+    y = gpu.poisson(20 * (ytrue + b), seed=4) / 20
+    x0 = np.ones_like(xtrue)
+    A = gpu.spect_operator(plan)
+    xg = gpu.mlem(x0, y, b, A, niter=5)
+    xr = P64.mlem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), 5)
+    rel = np.linalg.norm(xg - xr) / np.linalg.norm(xr)
+    assert rel <= 1e-4, rel
+    xg2 = np.zeros_like(x0)
+    gpu.mlem_(xg2, x0, y, b, A, niter=5)
+    # xhat1 ≈ xhat2 (test/ml-os-em.jl:45): isapprox with rtol sqrt(eps(Float32));
+    # the rotation adjoint accumulates with LDS atomics, so runs agree to rounding
+    assert np.linalg.norm(xg - xg2) <= np.sqrt(np.finfo(np.float32).eps) * np.linalg.norm(xg)
+    Ab = gpu.Ablock(plan, 4)
+    og = gpu.osem(x0, y, b, Ab, niter=3)
+    orr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), 4, 3)
+    rel = np.linalg.norm(og - orr) / np.linalg.norm(orr)
+    assert rel <= 1e-4, rel
+
+
+def test_errors_on_device(gpu):
+    mumap, psfs, image = small_problem(1)
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    with pytest.raises(gpu.SPECTError):
+        gpu.project(image, plan, index=[0, 99])
+    Ab = gpu.Ablock(plan, 2)
+    y = np.ones((16, 8, 8), np.float32)
+    with pytest.raises(gpu.SPECTError, match="need background > 0"):
+        gpu.osem(np.ones_like(image), y, np.zeros_like(y), Ab, niter=1)
+    with pytest.raises(gpu.SPECTError, match="nview must be divisible by nblocks!"):
+        gpu.Ablock(plan, 3)
