@@ -33,6 +33,10 @@ struct Taps {
   const float* u;
   const float* w;
   const float* h;
+  // separable only: per plane, tail sums sum_{a>=k} and head sums sum_{a<=k}
+  // of u and w (the replicate-pad fold of the adjoint), [planes][4][ntx]
+  // ordered (u tail, u head, w tail, w head)
+  const float* cum;
   int ntx, ntz;
   int vstride;  // 0 when every view shares one PSF stack, else ny
 };
@@ -59,8 +63,9 @@ hipError_t launch_img_reduce(const float* part, int ngroups, float* out, const f
 hipError_t launch_atten_exact(float* mur, int nb, const Geo& g, float dy, hipStream_t s);
 
 // ---- PSF ----------------------------------------------------------------
-// per-chunk partial views of the batch from the work volume; returns the
-// chunk count actually used through *nchunk_used
+// per-chunk partial views of the batch from the work volume (rotated and
+// attenuated already); returns the chunk count actually used through
+// *nchunk_used
 hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
                           int nchunk, int* nchunk_used, const Geo& g, hipStream_t s);
 // views (or the ML-EM ratio y/(view + b) when epi == EPI_RATIO) = sum of partials
@@ -84,6 +89,10 @@ constexpr int kMaxNpOne = 202;
 // largest plane (nx * nz) the PSF kernels handle (one strip of <= 32 outputs
 // per thread and phase, <= 1024 threads)
 constexpr long kMaxPlane = 32768;
+
+// whether the separable PSF kernels handle this geometry (nx % 4 == 0 and a
+// plane small enough for one strip per thread); otherwise the direct kernel
+bool psf_sep_supported(const Geo& g);
 
 // kernel classes for per-kernel event timing
 enum KernelClass { K_ROT_FWD = 0, K_PSF_FWD, K_VIEW_REDUCE, K_PSF_ADJ, K_ROT_ADJ, K_IMG_REDUCE, K_OTHER, K_NCLASS };

@@ -5,9 +5,32 @@
 // PSF convolutions, whose reference arithmetic is an FFT, use fmaf).
 #include "kernels.h"
 
+#include <mutex>
+#include <unordered_map>
+
 namespace sr {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// Walks the elements k = start, start+step, ... of a row-major (rows x ncol)
+// array as (row, col) without a division per element.
+struct RowIter {
+  int row, col, drow, dcol, ncol;
+  __device__ __forceinline__ RowIter(int start, int step, int ncol_) : ncol(ncol_) {
+    row = start / ncol;
+    col = start - row * ncol;
+    drow = step / ncol;
+    dcol = step - drow * ncol;
+  }
+  __device__ __forceinline__ void next() {
+    col += dcol;
+    row += drow;
+    if (col >= ncol) {
+      col -= ncol;
+      ++row;
+    }
+  }
+};
 
 // ===========================================================================
 // Rotation, single-pass bilinear (:two)
@@ -82,36 +105,169 @@ __global__ void __launch_bounds__(256) k_rot2_fwd(const float* __restrict__ img,
   }
 }
 
-// Adjoint: scatter of each work pixel into an LDS accumulator of the image
-// slice (ds_add_f32), over the vpg views of the group.
-__global__ void __launch_bounds__(256) k_rot2_adj(const float* __restrict__ work, float* __restrict__ part,
-                                                  const RotView* __restrict__ rv, const int* __restrict__ vids,
-                                                  int nb, int vpg, Geo g) {
-  extern __shared__ float acc[];
-  const int N = g.nx, z = blockIdx.x, grp = blockIdx.y;
-  const long S = (long)N * N, V = S * g.nz;
-  for (long k = threadIdx.x; k < S; k += blockDim.x) acc[k] = 0.f;
-  __syncthreads();
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int b0 = grp * vpg, b1 = min(nb, b0 + vpg);
-  for (int b = b0; b < b1; ++b) {
-    const RotView r = rv[vids[b]];
-    const float* src = work + V * b + S * z;
-    for (int y = ty; y < N; y += 4)
-      for (int x = tx; x < N; x += 64) {
-        const float p = src[x + (long)N * y];
-        const BiCoef c = bilinear_coef(r, g.pad, x, y);
-        const bool in1a = c.j1 >= 0 && c.j1 < N, in1b = c.j1 + 1 >= 0 && c.j1 + 1 < N;
-        const bool in2a = c.j2 >= 0 && c.j2 < N, in2b = c.j2 + 1 >= 0 && c.j2 + 1 < N;
-        if (in1a && in2a) atomicAdd(&acc[c.j1 + N * c.j2], (c.w11 * c.w21) * p);
-        if (in1b && in2a) atomicAdd(&acc[c.j1 + 1 + N * c.j2], (c.w12 * c.w21) * p);
-        if (in1a && in2b) atomicAdd(&acc[c.j1 + N * (c.j2 + 1)], (c.w11 * c.w22) * p);
-        if (in1b && in2b) atomicAdd(&acc[c.j1 + 1 + N * (c.j2 + 1)], (c.w12 * c.w22) * p);
-      }
+// Vectorized form for N % 4 == 0: block 1024, the slice staged with float4
+// loads, each thread producing float4 groups of 4 consecutive x; the
+// attenuation float4 loads of a view are issued before the LDS gathers.
+constexpr int R2F_THREADS = 1024;
+constexpr int R2F_ITEMS = 4;  // float4 groups per thread and pass
+__global__ void __launch_bounds__(R2F_THREADS) k_rot2_fwd4(const float* __restrict__ img, float* __restrict__ out,
+                                                          const float* __restrict__ atten, long atten_vstride,
+                                                          const RotView* __restrict__ rv,
+                                                          const int* __restrict__ vids, int nb, int vpg, Geo g) {
+  extern __shared__ float s_img[];
+  const int N = g.nx, z = blockIdx.x, tid = threadIdx.x;
+  const int NN = N * N, n4 = NN >> 2, q = N >> 2;
+  const long S = (long)NN, V = S * g.nz;
+  const float4* g4 = reinterpret_cast<const float4*>(img + S * z);
+  float4* s4 = reinterpret_cast<float4*>(s_img);
+  for (int i0 = 0; i0 < n4; i0 += R2F_THREADS * R2F_ITEMS) {
+    float4 t4[R2F_ITEMS];
+#pragma unroll
+    for (int k = 0; k < R2F_ITEMS; ++k) {
+      const int i = i0 + tid + k * R2F_THREADS;
+      if (i < n4) t4[k] = g4[i];
+    }
+#pragma unroll
+    for (int k = 0; k < R2F_ITEMS; ++k) {
+      const int i = i0 + tid + k * R2F_THREADS;
+      if (i < n4) s4[i] = t4[k];
+    }
   }
   __syncthreads();
+  const int b0 = blockIdx.y * vpg, b1 = min(nb, b0 + vpg);
+  for (int b = b0; b < b1; ++b) {
+    const int v = vids[b];
+    const RotView r = rv[v];
+    float4* o4 = reinterpret_cast<float4*>(out + V * b + S * z);
+    const float4* a4 = atten ? reinterpret_cast<const float4*>(atten + atten_vstride * v + S * z) : nullptr;
+    for (int i0 = 0; i0 < n4; i0 += R2F_THREADS * R2F_ITEMS) {
+      float4 av[R2F_ITEMS];
+#pragma unroll
+      for (int k = 0; k < R2F_ITEMS; ++k) {
+        const int i = i0 + tid + k * R2F_THREADS;
+        av[k] = (a4 && i < n4) ? a4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
+      }
+#pragma unroll
+      for (int k = 0; k < R2F_ITEMS; ++k) {
+        const int i = i0 + tid + k * R2F_THREADS;
+        if (i < n4) {
+          const int y = i / q, x = 4 * (i - y * q);
+          float vv[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            vv[e] = (r.kind == 0) ? s_img[x + e + N * y] : bilinear_two(s_img, N, g.pad, r, x + e, y);
+          float4 res;
+          if (a4) {
+            res.x = vv[0] * av[k].x;
+            res.y = vv[1] * av[k].y;
+            res.z = vv[2] * av[k].z;
+            res.w = vv[3] * av[k].w;
+          } else {
+            res = make_float4(vv[0], vv[1], vv[2], vv[3]);
+          }
+          o4[i] = res;
+        }
+      }
+    }
+  }
+}
+
+// Adjoint as a gather: image pixel j collects w(i,j) * p[i] from every work
+// pixel i whose forward sample lands within one pixel of j.  Since the map is
+// a rotation, those i lie within sqrt(2) of q = R^-1(j), i.e. in a 3x3 window.
+// Each candidate's sample x(i) is recomputed exactly as the forward does
+// ((xx*I1 + xy*I2) + x0 in Float32), and its weight max(0, 1-|x1-J1|) *
+// max(0, 1-|x2-J2|) equals the forward's (1-t or t) bit for bit (x - J is
+// exact by Sterbenz), so this is the exact transpose, without atomics and
+// deterministic.  grid (nz, ngroups), block 1024; the views of a group are
+// summed in registers; part[grp] receives the group's sum.
+constexpr int RA_THREADS = 1024;
+constexpr int RA_MAXP = 16;  // pixels per thread and pass (one pass covers N <= 128)
+
+__global__ void __launch_bounds__(RA_THREADS) k_rot2_adj(const float* __restrict__ work, float* __restrict__ part,
+                                                         const RotView* __restrict__ rv,
+                                                         const int* __restrict__ vids, int nb, int vpg, Geo g) {
+  extern __shared__ float sp[];  // the work slice, row stride N+1 (odd: rows fall in different banks)
+  const int N = g.nx, pad = g.pad, z = blockIdx.x, grp = blockIdx.y;
+  const int NN = N * N, SP = N + 1;
+  const long S = (long)NN, V = S * g.nz;
+  const int tid = threadIdx.x;
+  const int b0 = grp * vpg, b1 = min(nb, b0 + vpg);
   float* o = part + V * grp + S * z;
-  for (long k = threadIdx.x; k < S; k += blockDim.x) o[k] = acc[k];
+  for (int p0 = 0; p0 < NN; p0 += RA_THREADS * RA_MAXP) {
+    float acc[RA_MAXP];
+#pragma unroll
+    for (int k = 0; k < RA_MAXP; ++k) acc[k] = 0.f;
+    for (int b = b0; b < b1; ++b) {
+      const RotView r = rv[vids[b]];
+      const float* src = work + V * b + S * z;
+      __syncthreads();  // previous view's reads of sp are done
+      for (int k0 = 0; k0 < NN; k0 += RA_THREADS * 8) {
+        float tv[8];
+        int dst[8];
+        RowIter it(k0 + tid, RA_THREADS, N);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int k = k0 + tid + e * RA_THREADS;
+          tv[e] = k < NN ? src[k] : 0.f;
+          dst[e] = k < NN ? it.row * SP + it.col : -1;
+          it.next();
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (dst[e] >= 0) sp[dst[e]] = tv[e];
+      }
+      __syncthreads();
+      RowIter it(p0 + tid, RA_THREADS, N);
+#pragma unroll
+      for (int k = 0; k < RA_MAXP; ++k) {
+        const int jj = p0 + tid + k * RA_THREADS;
+        const int j2 = it.row, j1 = it.col;
+        it.next();
+        if (jj < NN) {
+          const float J1 = (float)(j1 + pad + 1), J2 = (float)(j2 + pad + 1);
+          // source position of J (approximate inverse; only picks the window)
+          const float u1 = J1 - r.x0, u2 = J2 - r.y0;
+          const float q1 = r.xx * u1 + r.yx * u2, q2 = r.xy * u1 + r.yy * u2;
+          const int lo1 = (int)floorf(q1 - 1.41422f) + 1, lo2 = (int)floorf(q2 - 1.41422f) + 1;
+          float A[3], B[3], C[3], D[3];
+          int ci[3], cj[3];
+          bool vi[3], vj[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const float F1 = (float)(lo1 + a), F2 = (float)(lo2 + a);
+            A[a] = r.xx * F1;
+            C[a] = r.yx * F1;
+            B[a] = r.xy * F2;
+            D[a] = r.yy * F2;
+            const int i1 = lo1 + a - pad - 1, i2 = lo2 + a - pad - 1;
+            vi[a] = (unsigned)i1 < (unsigned)N;
+            vj[a] = (unsigned)i2 < (unsigned)N;
+            ci[a] = clampi(i1, 0, N - 1);
+            cj[a] = __umul24(clampi(i2, 0, N - 1), SP);
+          }
+          float sum = acc[k];
+#pragma unroll
+          for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const float x1 = (A[a] + B[bb]) + r.x0;
+              const float x2 = (C[a] + D[bb]) + r.y0;
+              const float w1 = fmaxf(0.f, 1.f - fabsf(x1 - J1));
+              const float w2 = fmaxf(0.f, 1.f - fabsf(x2 - J2));
+              const float pv = (vi[a] && vj[bb]) ? sp[ci[a] + cj[bb]] : 0.f;
+              sum = sum + (w1 * w2) * pv;
+            }
+          acc[k] = sum;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < RA_MAXP; ++k) {
+      const int jj = p0 + tid + k * RA_THREADS;
+      if (jj < NN) o[jj] = acc[k];
+    }
+  }
 }
 
 // ===========================================================================
@@ -385,34 +541,58 @@ __device__ __forceinline__ void conv_strip_adj(Ld ld, const float* __restrict__ 
 
 __host__ __device__ __forceinline__ int odd_up(int v) { return v | 1; }
 
-// --- forward, separable: work -> per-chunk partial views -------------------
+// --- forward, separable: per-chunk partial views -------------------------------
 // grid (nchunk, nb), block >= max(#x strips, #z strips) (one strip of SL
 // outputs per thread per phase).  Each block convolves the planes y of its
-// chunk and keeps the running sum over y in registers; partials are reduced
-// by k_view_reduce.  LDS: the plane, replicate-padded along x (the x-pass
-// result is written back over it), and the plane's taps.
+// chunk (read from the batch work volume, rotated and attenuated already) and
+// keeps the running sum over y in registers; partials are reduced by
+// k_view_reduce.  The next plane is prefetched into registers while the
+// current one is convolved.  LDS holds the plane replicate-padded along x (x
+// pass), then, in the same space, the x-pass result transposed to
+// z-contiguous columns Qt[x][PT + z] with replicate pads along z (z pass), so
+// every window read of both passes is an immediate offset; then the taps.
+constexpr int PF_MAX = 4;  // prefetched float4 per thread (plane <= 16*nthreads floats)
+
 template <int NT, int SL>
-__global__ void __launch_bounds__(1024) k_psf_fwd_sep(const float* __restrict__ work, Taps t,
-                                                      const int* __restrict__ vids, int Yc,
-                                                      float* __restrict__ part, Geo g) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
+k_psf_fwd_sep(const float* __restrict__ work, Taps t, const int* __restrict__ vids, int Yc,
+              float* __restrict__ part, Geo g) {
   extern __shared__ float L[];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
   const int nxs = (nx + SL - 1) / SL, nzs = (nz + SL - 1) / SL;
-  const int WX = odd_up(SL * nxs + NT - 1);
-  const int LP = NT - 1 - hx;
+  const int WX = odd_up(SL * nxs + NT - 1);   // x-pass row stride
+  const int LP = NT - 1 - hx;                 // replicate columns left of x = 0
+  const int PT = NT - 1 - hz;                 // replicate rows above z = 0
+  const int WZ = odd_up(SL * nzs + NT - 1);   // z-pass column stride
+  const int area = max(nz * WX, SL * nxs * WZ);
   const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
   const int v = vids[b];
   const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
-  const long S = (long)nx * ny, V = S * nz;
-  const float* wb = work + V * b;
-  float* s_tu = L + nz * WX;
+  float* s_tu = L + area;
   float* s_tw = s_tu + NT;
   const int tid = threadIdx.x, nthr = blockDim.x;
+  const long S = (long)nx * ny;
+  const float* wb = work + S * nz * b;
   // x strip (z fastest) and z strip (x fastest) of this thread
   const bool hasx = tid < nz * nxs, hasz = tid < nx * nzs;
   const int xs_z = tid % nz, xs_x0 = (tid / nz) * SL;
   const int zs_x = tid % nx, zs_z0 = (tid / nx) * SL;
+  // plane elements of this thread as float4 along x: i = tid + k*nthr over (nz x nx/4)
+  const int nq = nx >> 2, nv = nz * nq;
+
+  float4 pf[PF_MAX];
+  auto prefetch = [&](int y) {
+#pragma unroll
+    for (int k = 0; k < PF_MAX; ++k) {
+      const int i = tid + k * nthr;
+      if (i < nv) {
+        const int z = i / nq, x4 = i - z * nq;
+        pf[k] = *reinterpret_cast<const float4*>(wb + S * z + (long)nx * y + 4 * x4);
+      }
+    }
+  };
+  if (y0 < y1) prefetch(y0);
 
   float acc[SL];
 #pragma unroll
@@ -424,9 +604,25 @@ __global__ void __launch_bounds__(1024) k_psf_fwd_sep(const float* __restrict__ 
       s_tu[tid] = t.u[(long)plane * NT + tid];
       s_tw[tid] = t.w[(long)plane * NT + tid];
     }
-    for (int k = tid; k < nz * WX; k += nthr) {
-      const int z = k / WX, col = k - z * WX;
-      L[k] = wb[clampi(col - LP, 0, nx - 1) + (long)nx * y + S * z];
+#pragma unroll
+    for (int k = 0; k < PF_MAX; ++k) {
+      const int i = tid + k * nthr;
+      if (i < nv) {
+        const int z = i / nq, x4 = i - z * nq;
+        float* d = L + z * WX + LP + 4 * x4;
+        d[0] = pf[k].x;
+        d[1] = pf[k].y;
+        d[2] = pf[k].z;
+        d[3] = pf[k].w;
+      }
+    }
+    if (y + 1 < y1) prefetch(y + 1);
+    __syncthreads();
+    // replicate pad columns, all threads
+    const int rp = WX - LP - nx;
+    for (RowIter it(tid, nthr, LP + rp); it.row < nz; it.next()) {
+      float* row = L + it.row * WX;
+      row[it.col < LP ? it.col : LP + nx + (it.col - LP)] = row[it.col < LP ? LP : LP + nx - 1];
     }
     __syncthreads();
     float q[SL];
@@ -438,13 +634,22 @@ __global__ void __launch_bounds__(1024) k_psf_fwd_sep(const float* __restrict__ 
     }
     __syncthreads();
     if (hasx) {
+      float* col = L + xs_x0 * WZ + PT + xs_z;  // Qt[x0 + k][PT + z] = col[k*WZ]
 #pragma unroll
-      for (int k = 0; k < SL; ++k) L[xs_z * WX + xs_x0 + k] = q[k];
+      for (int k = 0; k < SL; ++k) col[k * WZ] = q[k];
+    }
+    __syncthreads();
+    // replicate pad rows (z) of Qt, all threads
+    const int nlow = PT, nhigh = WZ - PT - nz;
+    for (RowIter it(tid, nthr, nlow + nhigh); it.row < nx; it.next()) {
+      float* colx = L + it.row * WZ;
+      colx[it.col < nlow ? it.col : PT + nz + (it.col - nlow)] = colx[it.col < nlow ? PT : PT + nz - 1];
     }
     __syncthreads();
     if (hasz) {
-      const int zb = zs_z0 + hz - (NT - 1);
-      conv_strip_fwd<NT, SL>([&](int j) { return L[clampi(zb + j, 0, nz - 1) * WX + zs_x]; }, s_tw, acc);
+      // output z reads rows z + hz - a (a < NT), i.e. Qt[x][z0 + j], j < NT+SL-1
+      const float* col = L + zs_x * WZ + zs_z0;
+      conv_strip_fwd<NT, SL>([&](int j) { return col[j]; }, s_tw, acc);
     }
     __syncthreads();
   }
@@ -457,6 +662,7 @@ __global__ void __launch_bounds__(1024) k_psf_fwd_sep(const float* __restrict__ 
 }
 
 // --- forward, direct 2-D ----------------------------------------------------
+// general (non-separable) PSF: strips of 8 x-outputs, pz rows of NT taps each
 template <int NT, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_psf_fwd_2d(const float* __restrict__ work, Taps t,
                                                       const int* __restrict__ vids, int Yc,
@@ -471,8 +677,8 @@ __global__ void __launch_bounds__(BLOCK) k_psf_fwd_2d(const float* __restrict__ 
   const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
   const int v = vids[b];
   const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
-  const long S = (long)nx * ny, V = S * nz;
-  const float* wb = work + V * b;
+  const long S = (long)nx * ny;
+  const float* wb = work + S * nz * b;
   const int nsx = nz * nxs;
 
   float acc[PSF_MAXS][8];
@@ -483,11 +689,10 @@ __global__ void __launch_bounds__(BLOCK) k_psf_fwd_2d(const float* __restrict__ 
 
   for (int y = y0; y < y1; ++y) {
     const float* th = t.h + (long)(v * t.vstride + y) * pz * NT;
-    // L[r][col] = P[clamp(col-LP), clamp(r-hz)]
-    for (int k = threadIdx.x; k < NR * WX; k += BLOCK) {
-      const int r = k / WX, col = k - r * WX;
-      L[k] = wb[clampi(col - LP, 0, nx - 1) + (long)nx * y + S * clampi(r - hz, 0, nz - 1)];
-    }
+    // L[r][col] = P[clamp(col-LP), clamp(r-hz)] (small kernels: direct clamped loads)
+    for (RowIter it(threadIdx.x, BLOCK, WX); it.row < NR; it.next())
+      L[it.row * WX + it.col] =
+          wb[clampi(it.col - LP, 0, nx - 1) + (long)nx * y + S * clampi(it.row - hz, 0, nz - 1)];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < PSF_MAXS; ++i) {
@@ -557,16 +762,16 @@ __device__ __forceinline__ float adj_edge_hi(const float* in, int stride, int n,
   return s;
 }
 
-// grid (nchunk, nb), one strip of SL outputs per thread per phase.  LDS: the
-// view V[z][x] (stride nx), Qp = the z-adjoint of V with zero columns around
-// it (stride WQ, data at column offset hx), and the plane's taps.
-// kUniformRows: nx is a multiple of 64, so every wave's z strips share z0 and
-// the row bounds of the z pass are wave-uniform (scalar).
-template <int NT, int SL, bool kUniformRows>
-__global__ void __launch_bounds__(1024) k_psf_adj_sep(const float* __restrict__ views, Taps t,
-                                                      const int* __restrict__ vids, int Yc,
-                                                      float* __restrict__ work, const float* __restrict__ atten,
-                                                      long atten_vstride, Geo g) {
+// grid (nchunk, nb), one strip of SL outputs per thread per phase.  LDS: Vz =
+// the view with zero rows around it (z-pass windows are immediate offsets),
+// Qp = the z-adjoint of the view with zero columns around it (stride WQ, data
+// at column offset hx), the taps and their tail/head sums.  The x-pass strip
+// (z, x0..x0+SL-1) is written straight to the work volume, times the
+// attenuation factor (mul3dj!, src/backproject.jl:316).
+template <int NT, int SL>
+__global__ void __launch_bounds__(1024)
+k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ vids, int Yc, float* __restrict__ work,
+              const float* __restrict__ atten, long atten_vstride, Geo g) {
   extern __shared__ float L[];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
@@ -578,8 +783,11 @@ __global__ void __launch_bounds__(1024) k_psf_adj_sep(const float* __restrict__ 
   const long S = (long)nx * ny, V = S * nz;
   const int P = nx * nz;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  float* Vs = L;
-  float* Qp = L + P;
+  // Vt: the view transposed to z-contiguous columns Vt[x][hz + z] with zero
+  // pads (column stride VZ), so the z-pass windows are immediate offsets
+  const int VZ = odd_up(SL * nzs + NT - 1);
+  float* Vt = L;
+  float* Qp = L + (long)nx * VZ;
   float* s_tu = Qp + nz * WQ;  // taps u, w and their tail/head sums
   float* s_tw = s_tu + NT;
   float* s_ut = s_tw + NT;
@@ -587,34 +795,29 @@ __global__ void __launch_bounds__(1024) k_psf_adj_sep(const float* __restrict__ 
   float* s_wt = s_uh + NT;
   float* s_wh = s_wt + NT;
   const float* vin = views + (long)b * P;
-  for (int k = tid; k < P; k += nthr) Vs[k] = vin[k];
+  for (int k = tid; k < nx * VZ; k += nthr) Vt[k] = 0.f;
   for (int k = tid; k < nz * WQ; k += nthr) Qp[k] = 0.f;
+  __syncthreads();
+  for (RowIter it(tid, nthr, nx); it.row < nz; it.next()) Vt[it.col * VZ + hz + it.row] = vin[it.row * nx + it.col];
   const bool hasx = tid < nz * nxs, hasz = tid < nx * nzs;
   const int xs_z = tid % nz, xs_x0 = (tid / nz) * SL;
   const int zs_x = tid % nx, zs_z0 = (tid / nx) * SL;
   float* wout = work + V * b;
   const float* av = atten ? atten + atten_vstride * v : nullptr;
+  const bool vec4 = (nx % 4 == 0) && (SL % 4 == 0);
 
   for (int y = y0; y < y1; ++y) {
     const int plane = v * t.vstride + y;
     if (tid < NT) {
-      const float* gu = t.u + (long)plane * NT;
-      const float* gw = t.w + (long)plane * NT;
-      s_tu[tid] = gu[tid];
-      s_tw[tid] = gw[tid];
-      float ut = 0.f, uh = 0.f, wt = 0.f, wh = 0.f;
-      for (int a = tid; a < NT; ++a) {
-        ut += gu[a];
-        wt += gw[a];
-      }
-      for (int a = 0; a <= tid; ++a) {
-        uh += gu[a];
-        wh += gw[a];
-      }
-      s_ut[tid] = ut;
-      s_uh[tid] = uh;
-      s_wt[tid] = wt;
-      s_wh[tid] = wh;
+      const float* cum = t.cum + (long)plane * 4 * NT;
+      const float u0 = t.u[(long)plane * NT + tid], w0 = t.w[(long)plane * NT + tid];
+      const float c0 = cum[tid], c1 = cum[NT + tid], c2 = cum[2 * NT + tid], c3 = cum[3 * NT + tid];
+      s_tu[tid] = u0;
+      s_tw[tid] = w0;
+      s_ut[tid] = c0;
+      s_uh[tid] = c1;
+      s_wt[tid] = c2;
+      s_wh[tid] = c3;
     }
     __syncthreads();
     float q[SL];
@@ -622,22 +825,19 @@ __global__ void __launch_bounds__(1024) k_psf_adj_sep(const float* __restrict__ 
     if (hasz) {
 #pragma unroll
       for (int k = 0; k < SL; ++k) q[k] = 0.f;
-      const int zb = kUniformRows ? __builtin_amdgcn_readfirstlane(zs_z0 - hz) : zs_z0 - hz;
-      conv_strip_adj<NT, SL>(
-          [&](int j) {
-            const int zr = zb + j;
-            return (zr >= 0 && zr < nz) ? Vs[zr * nx + zs_x] : 0.f;
-          },
-          s_tw, q);
+      // row zs_z0 - hz + j of the view is Vt[x][zs_z0 + j]
+      const float* col = Vt + zs_x * VZ + zs_z0;
+      conv_strip_adj<NT, SL>([&](int j) { return col[j]; }, s_tw, q);
       // replicate-pad fold at z = 0 and z = nz-1 (one evaluation each)
       const int khi = nz - 1 - zs_z0;
       const bool lo = zs_z0 == 0, hi = khi < SL;
       float elo = 0.f, ehi = 0.f;
+      const float* vcol = Vt + zs_x * VZ + hz;  // the view's column x, z contiguous
       if (nz == 1) {
-        elo = Vs[zs_x] * s_wt[0];
+        elo = vcol[0] * s_wt[0];
       } else {
-        if (lo) elo = adj_edge_lo(Vs + zs_x, nx, nz, hz, s_wt);
-        if (hi) ehi = adj_edge_hi(Vs + zs_x, nx, nz, hz, s_wh);
+        if (lo) elo = adj_edge_lo(vcol, 1, nz, hz, s_wt);
+        if (hi) ehi = adj_edge_hi(vcol, 1, nz, hz, s_wh);
       }
 #pragma unroll
       for (int k = 0; k < SL; ++k)
@@ -648,8 +848,15 @@ __global__ void __launch_bounds__(1024) k_psf_adj_sep(const float* __restrict__ 
         if (zs_z0 + k < nz) Qp[(zs_z0 + k) * WQ + hx + zs_x] = q[k];
     }
     __syncthreads();
-    // x adjoint: strip (z, x0..x0+SL-1)
+    // x adjoint: strip (z, x0..x0+SL-1), then out to the work volume; the
+    // strip's attenuation factors are loaded before the convolution
     if (hasx) {
+      const long o = xs_x0 + (long)nx * y + S * xs_z;
+      const bool v4 = vec4 && xs_x0 + SL <= nx;
+      float4 a4[SL / 4];
+#pragma unroll
+      for (int k = 0; k < SL / 4; ++k)
+        a4[k] = (av && v4) ? *reinterpret_cast<const float4*>(av + o + 4 * k) : make_float4(1.f, 1.f, 1.f, 1.f);
 #pragma unroll
       for (int k = 0; k < SL; ++k) q[k] = 0.f;
       const float* row = Qp + xs_z * WQ + xs_x0;
@@ -668,21 +875,23 @@ __global__ void __launch_bounds__(1024) k_psf_adj_sep(const float* __restrict__ 
       for (int k = 0; k < SL; ++k)
         if (hi && k == khi && nx > 1) q[k] = ehi;
       if (lo) q[0] = elo;
-    }
-    __syncthreads();
-    // stage the plane through Qp's data columns, then write it coalesced,
-    // times the attenuation factor (mul3dj!, src/backproject.jl:316)
-    if (hasx) {
+      if (v4) {
 #pragma unroll
-      for (int k = 0; k < SL; ++k)
-        if (xs_x0 + k < nx) Qp[xs_z * WQ + hx + xs_x0 + k] = q[k];
-    }
-    __syncthreads();
-    for (int k = tid; k < P; k += nthr) {
-      const int z = k / nx, x = k - z * nx;
-      const long o = x + (long)nx * y + S * z;
-      const float val = Qp[z * WQ + hx + x];
-      wout[o] = av ? val * av[o] : val;
+        for (int k = 0; k < SL; k += 4) {
+          float4 val = make_float4(q[k], q[k + 1], q[k + 2], q[k + 3]);
+          if (av) {
+            val.x = val.x * a4[k / 4].x;
+            val.y = val.y * a4[k / 4].y;
+            val.z = val.z * a4[k / 4].z;
+            val.w = val.w * a4[k / 4].w;
+          }
+          *reinterpret_cast<float4*>(wout + o + k) = val;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+          if (xs_x0 + k < nx) wout[o + k] = av ? q[k] * av[o + k] : q[k];
+      }
     }
     __syncthreads();
   }
@@ -797,12 +1006,18 @@ static int sep_strip_len(const Geo& g) {
   return 0;
 }
 
-// kernels whose dynamic LDS exceeds 64 KiB must opt in
+// kernels whose dynamic LDS exceeds 64 KiB must opt in (once per kernel and size)
+static std::mutex g_lds_mu;
+static std::unordered_map<const void*, size_t> g_lds_done;
 template <typename K>
 static void allow_lds(K kernel, size_t bytes) {
-  if (bytes > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes);
+  if (bytes <= 65536) return;
+  const void* f = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lk(g_lds_mu);
+  size_t& done = g_lds_done[f];
+  if (bytes <= done) return;
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess) done = bytes;
+  else (void)hipGetLastError();
 }
 
 #define SR_SL_SWITCH(SLV, ...)                              \
@@ -828,7 +1043,13 @@ hipError_t launch_rot_fwd(int interp, const float* img, float* out, const float*
     int vpg = 1;
     while (vpg < 8 && (long)g.nz * cdiv(nb, vpg * 2) >= 1024) vpg *= 2;
     const size_t lds = sizeof(float) * S;
-    if (lds <= kMaxLds) {
+    if (g.nx % 4 == 0 && lds <= kMaxLds) {
+      int vpg4 = 1;
+      while (vpg4 < 8 && (long)g.nz * cdiv(nb, vpg4 * 2) >= 512) vpg4 *= 2;
+      allow_lds(k_rot2_fwd4, lds);
+      k_rot2_fwd4<<<dim3(g.nz, cdiv(nb, vpg4)), R2F_THREADS, lds, s>>>(img, out, atten, atten_vstride, rv, vids, nb,
+                                                                       vpg4, g);
+    } else if (lds <= kMaxLds) {
       allow_lds(k_rot2_fwd<true>, lds);
       k_rot2_fwd<true><<<dim3(g.nz, cdiv(nb, vpg)), 256, lds, s>>>(img, out, atten, atten_vstride, rv, vids, nb,
                                                                   vpg, g);
@@ -849,10 +1070,10 @@ hipError_t launch_rot_adj(int interp, const float* work, float* part, const RotV
     k_rot1_adj<<<dim3(g.nz, nb), R1_THREADS, lds, s>>>(work, part, rv, vids, tab_b, tab_w, g);
   } else {
     const int ng = (nb + vpg - 1) / vpg;
-    const size_t lds = sizeof(float) * g.nx * g.nx;
+    const size_t lds = sizeof(float) * g.nx * (g.nx + 1);
     if (lds > kMaxLds) return hipErrorInvalidConfiguration;
     allow_lds(k_rot2_adj, lds);
-    k_rot2_adj<<<dim3(g.nz, ng), 256, lds, s>>>(work, part, rv, vids, nb, vpg, g);
+    k_rot2_adj<<<dim3(g.nz, ng), RA_THREADS, lds, s>>>(work, part, rv, vids, nb, vpg, g);
   }
   return hipGetLastError();
 }
@@ -898,35 +1119,48 @@ static int pick_block(long strips) {
   }
 
 
+bool psf_sep_supported(const Geo& g) {
+  const int SLv = sep_strip_len(g);
+  if (SLv == 0 || g.nx % 4 != 0) return false;
+  const int sx = (g.nx + SLv - 1) / SLv, sz = (g.nz + SLv - 1) / SLv;
+  const long need = std::max((long)std::max(g.nz * sx, g.nx * sz), (long)cdiv((long)g.nx * g.nz, 4L * PF_MAX));
+  return need <= 1024;
+}
+
 hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
                           int nchunk, int* nchunk_used, const Geo& g, hipStream_t s) {
   const int Yc = (g.ny + nchunk - 1) / nchunk;
   nchunk = (g.ny + Yc - 1) / Yc;
   if (nchunk_used) *nchunk_used = nchunk;
   if (nb <= 0) return hipSuccess;
-  if (separable) {
-    const int SLv = sep_strip_len(g);
-    if (SLv == 0) return hipErrorInvalidValue;
+  const int SLv = separable ? sep_strip_len(g) : 0;
+  if (separable && SLv > 0 && g.nx % 4 == 0) {
     const int sx = (g.nx + SLv - 1) / SLv, sz = (g.nz + SLv - 1) / SLv;
-    const int block = std::max(64, (int)cdiv(std::max(g.nz * sx, g.nx * sz), 64) * 64);
-    const size_t lds = sizeof(float) * ((size_t)g.nz * odd_up(SLv * sx + t.ntx - 1) + 2 * t.ntx);
+    // one strip per thread per pass, and enough threads to prefetch the plane
+    const long need = std::max((long)std::max(g.nz * sx, g.nx * sz), (long)cdiv((long)g.nx * g.nz, 4L * PF_MAX));
+    const int block = std::max(64, (int)cdiv(need, 64) * 64);
+    if (block > 1024) return hipErrorInvalidConfiguration;
+    const size_t area = std::max((size_t)g.nz * odd_up(SLv * sx + t.ntx - 1),
+                                 (size_t)SLv * sx * odd_up(SLv * sz + t.ntx - 1));
+    const size_t lds = sizeof(float) * (area + 2 * t.ntx);
     if (lds > kMaxLds) return hipErrorInvalidConfiguration;
     SR_NT_SWITCH(t.ntx, SR_SL_SWITCH(SLv, {
       allow_lds(k_psf_fwd_sep<NT, SL>, lds);
       k_psf_fwd_sep<NT, SL><<<dim3(nchunk, nb), block, lds, s>>>(work, t, vids, Yc, part, g);
     }));
-  } else {
-    const int nxs = (g.nx + 7) / 8;
-    const int WX = odd_up(8 * nxs + t.ntx - 1);
-    const long strips = (long)g.nz * nxs;
-    const int block = pick_block(strips);
-    const size_t lds = sizeof(float) * (g.nz + g.pz - 1) * WX;
-    if (strips > (long)PSF_MAXS * 1024 || lds > kMaxLds) return hipErrorInvalidConfiguration;
-    SR_NT_SWITCH(t.ntx, SR_BLOCK_SWITCH(block, {
-      allow_lds(k_psf_fwd_2d<NT, BLOCK>, lds);
-      k_psf_fwd_2d<NT, BLOCK><<<dim3(nchunk, nb), BLOCK, lds, s>>>(work, t, vids, Yc, part, g);
-    }));
+    return hipGetLastError();
   }
+  if (separable) return hipErrorInvalidConfiguration;
+  const int nxs = (g.nx + 7) / 8;
+  const int WX = odd_up(8 * nxs + t.ntx - 1);
+  const long strips = (long)g.nz * nxs;
+  const int block = pick_block(strips);
+  const size_t lds = sizeof(float) * (g.nz + g.pz - 1) * WX;
+  if (strips > (long)PSF_MAXS * 1024 || lds > kMaxLds) return hipErrorInvalidConfiguration;
+  SR_NT_SWITCH(t.ntx, SR_BLOCK_SWITCH(block, {
+    allow_lds(k_psf_fwd_2d<NT, BLOCK>, lds);
+    k_psf_fwd_2d<NT, BLOCK><<<dim3(nchunk, nb), BLOCK, lds, s>>>(work, t, vids, Yc, part, g);
+  }));
   return hipGetLastError();
 }
 
@@ -948,22 +1182,13 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
     if (SLv == 0) return hipErrorInvalidValue;
     const int sx = (g.nx + SLv - 1) / SLv, sz = (g.nz + SLv - 1) / SLv;
     const int block = std::max(64, (int)cdiv(std::max(g.nz * sx, g.nx * sz), 64) * 64);
-    const size_t lds =
-        sizeof(float) * ((size_t)g.nx * g.nz + (size_t)g.nz * odd_up(SLv * sx + t.ntx - 1) + 6 * t.ntx);
+    const size_t lds = sizeof(float) * ((size_t)g.nx * odd_up(SLv * sz + t.ntx - 1) +
+                                        (size_t)g.nz * odd_up(SLv * sx + t.ntx - 1) + 6 * t.ntx);
     if (lds > kMaxLds) return hipErrorInvalidConfiguration;
-    if (g.nx % 64 == 0) {
-      SR_NT_SWITCH(t.ntx, SR_SL_SWITCH(SLv, {
-        allow_lds(k_psf_adj_sep<NT, SL, true>, lds);
-        k_psf_adj_sep<NT, SL, true>
-            <<<dim3(nchunk, nb), block, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride, g);
-      }));
-    } else {
-      SR_NT_SWITCH(t.ntx, SR_SL_SWITCH(SLv, {
-        allow_lds(k_psf_adj_sep<NT, SL, false>, lds);
-        k_psf_adj_sep<NT, SL, false>
-            <<<dim3(nchunk, nb), block, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride, g);
-      }));
-    }
+    SR_NT_SWITCH(t.ntx, SR_SL_SWITCH(SLv, {
+      allow_lds(k_psf_adj_sep<NT, SL>, lds);
+      k_psf_adj_sep<NT, SL><<<dim3(nchunk, nb), block, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride, g);
+    }));
   } else {
     const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
     const size_t lds = sizeof(float) * (size_t)odd_up(g.nx + 2 * hx) * (g.nz + 2 * hz);

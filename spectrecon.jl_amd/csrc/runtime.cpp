@@ -136,6 +136,23 @@ void rot_params(double theta, int interp, int Np, sr::RotView& rv, int* tb, floa
     }
 }
 
+// Per plane tail/head sums of the separable taps (for the adjoint's
+// replicate-pad fold): [plane][u tail, u head, w tail, w head][nt], in double.
+std::vector<float> cum_sums(const std::vector<float>& hu, const std::vector<float>& hw, int nplane, int nt) {
+  std::vector<float> c((size_t)nplane * 4 * nt);
+  for (int pl = 0; pl < nplane; ++pl)
+    for (int which = 0; which < 2; ++which) {
+      const float* t = (which == 0 ? hu : hw).data() + (size_t)pl * nt;
+      float* tail = c.data() + ((size_t)pl * 4 + 2 * which) * nt;
+      float* head = tail + nt;
+      double acc = 0;
+      for (int k = nt - 1; k >= 0; --k) tail[k] = (float)(acc += t[k]);
+      acc = 0;
+      for (int k = 0; k < nt; ++k) head[k] = (float)(acc += t[k]);
+    }
+  return c;
+}
+
 int round_taps(int p) { return p <= 8 ? 8 : p <= 16 ? 16 : p <= 24 ? 24 : p <= 32 ? 32 : p <= 40 ? 40 : p <= 48 ? 48 : 64; }
 
 // Rank-1 factorisation h[a,b] ~ u[a] w[b] of a px x pz plane, accepted when
@@ -179,7 +196,7 @@ struct sr_plan_s {
   int max_batch = 32;
   size_t bytes = 0;
 
-  DBuf mumap, atten, taps_u, taps_w, taps_h, rotv, tab_b, tab_w;
+  DBuf mumap, atten, taps_u, taps_w, taps_h, taps_c, rotv, tab_b, tab_w;
   DBuf work, fpart, ipart, vids, ones;
   DBuf st_img, st_views, st_aux1, st_aux2, st_x0;  // staging for host-pointer calls and EM temporaries
   DBuf em_ratio, em_back, em_asum;
@@ -201,14 +218,16 @@ struct sr_plan_s {
     t.u = taps_u.as<float>();
     t.w = taps_w.as<float>();
     t.h = taps_h.as<float>();
+    t.cum = taps_c.as<float>();
     t.ntx = nt;
     t.ntz = nt;
     t.vstride = vstride;
     return t;
   }
   long atten_vstride() const { return V; }
+  // planes are split into chunks so that a launch has about 512 blocks
   int nchunk_fwd(int nb) const {
-    const int target = std::max(1, (int)std::ceil(1024.0 / std::max(1, nb)));
+    const int target = std::max(1, (int)std::ceil(512.0 / std::max(1, nb)));
     return std::min(d.ny, target);
   }
   int groups_adj(int nb, int* vpg) const {
@@ -219,7 +238,7 @@ struct sr_plan_s {
       return nb;
     }
     int v = nb;
-    while (v > 1 && (long)d.nz * ((nb + v - 1) / v) < 1024) v = (v + 1) / 2;
+    while (v > 1 && (long)d.nz * ((nb + v - 1) / v) < 512) v = (v + 1) / 2;
     *vpg = v;
     return (nb + v - 1) / v;
   }
@@ -499,7 +518,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
     shared = std::memcmp(psfs, psfs + (long)v * d.ny * pxz, sizeof(float) * d.ny * pxz) == 0;
   p->vstride = shared ? 0 : d.ny;
   const int nstore = shared ? d.ny : d.ny * d.nview;
-  bool sep = d.conv != SR_CONV_DIRECT;
+  bool sep = d.conv != SR_CONV_DIRECT && sr::psf_sep_supported(p->g);
   std::vector<double> u, w;
   if (sep)
     for (int pl = 0; pl < nstore && sep; ++pl) sep = factor_rank1(psfs + (long)pl * pxz, d.px, d.pz, u, w);
@@ -513,10 +532,13 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
       for (int a = 0; a < d.px; ++a) hu[(size_t)pl * p->nt + a] = (float)u[a];
       for (int b = 0; b < d.pz; ++b) hw[(size_t)pl * p->nt + b] = (float)w[b];
     }
-    if ((rc = p->taps_u.ensure(sizeof(float) * hu.size())) || (rc = p->taps_w.ensure(sizeof(float) * hw.size())))
+    const std::vector<float> hc = cum_sums(hu, hw, nstore, p->nt);
+    if ((rc = p->taps_u.ensure(sizeof(float) * hu.size())) || (rc = p->taps_w.ensure(sizeof(float) * hw.size())) ||
+        (rc = p->taps_c.ensure(sizeof(float) * hc.size())))
       return bail(rc);
     HIP_TRY(hipMemcpy(p->taps_u.p, hu.data(), sizeof(float) * hu.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->taps_w.p, hw.data(), sizeof(float) * hw.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->taps_c.p, hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice));
   } else {
     std::vector<float> hh((size_t)nstore * d.pz * p->nt, 0.f);
     for (int pl = 0; pl < nstore; ++pl)
@@ -560,7 +582,8 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
     HIP_TRY(sr::launch_atten_exact(dst, nb, p->g, d.dy, p->stream));
   }
   HIP_TRY(hipStreamSynchronize(p->stream));
-  p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->rotv.n + p->tab_b.n + p->tab_w.n;
+  p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->taps_c.n + p->rotv.n + p->tab_b.n +
+             p->tab_w.n;
   *out = guard.release();
   return SR_OK;
 }
@@ -820,10 +843,10 @@ int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const floa
   if (host) std::memcpy(hk.data(), ker, sizeof(float) * hk.size());
   else HIP_TRY(hipMemcpy(hk.data(), ker, sizeof(float) * hk.size(), hipMemcpyDeviceToHost));
   std::vector<double> u, w;
-  const bool sep = factor_rank1(hk.data(), px, pz, u, w);
-  const int nt = round_taps(std::max(px, pz));
   sr::Geo g{nx, 1, nz, 0, 0, px, pz};
-  DBuf d_u, d_w, d_h, d_vid, d_in, d_out, d_part;
+  const bool sep = sr::psf_sep_supported(g) && factor_rank1(hk.data(), px, pz, u, w);
+  const int nt = round_taps(std::max(px, pz));
+  DBuf d_u, d_w, d_h, d_c, d_vid, d_in, d_out, d_part;
   sr::Taps t{};
   t.ntx = t.ntz = nt;
   t.vstride = 0;
@@ -831,9 +854,13 @@ int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const floa
     std::vector<float> hu(nt, 0.f), hw(nt, 0.f);
     for (int a = 0; a < px; ++a) hu[a] = (float)u[a];
     for (int b = 0; b < pz; ++b) hw[b] = (float)w[b];
-    if ((rc = d_u.ensure(sizeof(float) * nt)) || (rc = d_w.ensure(sizeof(float) * nt))) return rc;
+    const std::vector<float> hc = cum_sums(hu, hw, 1, nt);
+    if ((rc = d_u.ensure(sizeof(float) * nt)) || (rc = d_w.ensure(sizeof(float) * nt)) ||
+        (rc = d_c.ensure(sizeof(float) * hc.size())))
+      return rc;
     HIP_TRY(hipMemcpy(d_u.p, hu.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_w.p, hw.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_c.p, hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice));
   } else {
     std::vector<float> hh((size_t)pz * nt, 0.f);
     for (int b = 0; b < pz; ++b)
@@ -844,6 +871,7 @@ int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const floa
   t.u = d_u.as<float>();
   t.w = d_w.as<float>();
   t.h = d_h.as<float>();
+  t.cum = d_c.as<float>();
   const int zero = 0;
   if ((rc = d_vid.ensure(sizeof(int)))) return rc;
   HIP_TRY(hipMemcpy(d_vid.p, &zero, sizeof(int), hipMemcpyHostToDevice));
