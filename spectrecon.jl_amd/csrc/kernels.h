@@ -93,6 +93,8 @@ constexpr long kMaxPlane = 32768;
 // whether the separable PSF kernels handle this geometry (nx % 4 == 0 and a
 // plane small enough for one strip per thread); otherwise the direct kernel
 bool psf_sep_supported(const Geo& g);
+// blocks of the :two rotation adjoint that one CU holds at once (LDS-bound)
+int rot2_adj_blocks_per_cu(const Geo& g);
 
 // kernel classes for per-kernel event timing
 enum KernelClass { K_ROT_FWD = 0, K_PSF_FWD, K_VIEW_REDUCE, K_PSF_ADJ, K_ROT_ADJ, K_IMG_REDUCE, K_OTHER, K_NCLASS };

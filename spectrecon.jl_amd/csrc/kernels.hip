@@ -270,6 +270,113 @@ __global__ void __launch_bounds__(RA_THREADS) k_rot2_adj(const float* __restrict
   }
 }
 
+// Adjoint as a collision-free scatter (N <= 128).  Work pixel i contributes
+// w(i,j) p[i] to the 2x2 image pixels around its forward sample x(i) (the
+// forward's own Float32 coordinates and weights: the exact transpose).  The
+// pixels are processed in 9 phases by (i1 mod 3, i2 mod 3): two pixels of one
+// class are >= 3 apart, their samples (a rotation is an isometry) >= 3/sqrt2
+// > 2 apart in one coordinate, so their 2x2 footprints are disjoint and the
+// LDS read-modify-writes of a phase never touch one address twice: plain
+// (non-atomic) updates are exact and deterministic, in a fixed phase order.
+// The accumulator has a one-pixel guard ring ((N+2) rows of stride N+8, image
+// pixel (j1,j2) at (j1+1,j2+1)), so a footprint that overlaps the image is
+// updated unconditionally with two ds_read2/ds_write2 pairs; the ring is
+// dropped at the end.  Each wave takes 8x8 tiles of a class (its lanes hit a
+// rotated lattice of spacing 3, which the stride N+8 spreads over the banks),
+// and the next phase's pixels are loaded while the current one is scattered.
+// grid (nz, ngroups), block 1024.
+constexpr int RS_THREADS = 1024;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_TILES = 3;  // tiles per wave and phase: 16 x 3 = 48 >= ceil(43/8)^2 = 36
+
+struct ScatterPix {
+  int i1[RS_TILES], i2[RS_TILES];
+  float v[RS_TILES];
+  int ntl;  // wave-uniform: tiles of this wave in the phase
+};
+
+// pixels of phase c (class (c mod 3, c div 3)); i1 = -1 marks an idle lane
+__device__ __forceinline__ void scatter_load(ScatterPix& P, const float* __restrict__ src, int N, int c, int wave,
+                                             int lane) {
+  const int c1 = c % 3, c2 = c / 3;
+  const int na = (N - c1 + 2) / 3, nbq = (N - c2 + 2) / 3;
+  const int ta = (na + 7) >> 3, ntile = ta * ((nbq + 7) >> 3);
+  P.ntl = ntile > wave ? (ntile - wave + RS_WAVES - 1) / RS_WAVES : 0;
+#pragma unroll
+  for (int k = 0; k < RS_TILES; ++k) {
+    const int t = wave + RS_WAVES * k;  // wave-uniform
+    const int tb = t / ta, tt = t - tb * ta;
+    const int a = tt * 8 + (lane & 7), bq = tb * 8 + (lane >> 3);
+    const bool ok = a < na && bq < nbq;
+    P.i1[k] = ok ? c1 + 3 * a : -1;
+    P.i2[k] = ok ? c2 + 3 * bq : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < RS_TILES; ++k)
+    if (k < P.ntl) P.v[k] = src[max(P.i1[k], 0) + N * P.i2[k]];
+}
+
+// one phase: scatter the pixels of P into the accumulator
+__device__ __forceinline__ void scatter_phase(float* acc, const ScatterPix& P, const RotView& r, int N, int pad,
+                                              int SP) {
+#pragma unroll
+  for (int k = 0; k < RS_TILES; ++k) {
+    if (k >= P.ntl) break;
+    const BiCoef cf = bilinear_coef(r, pad, P.i1[k], P.i2[k]);
+    if (P.i1[k] >= 0 && (unsigned)(cf.j1 + 1) <= (unsigned)N && (unsigned)(cf.j2 + 1) <= (unsigned)N) {
+      const float p1 = P.v[k] * cf.w21, p2 = P.v[k] * cf.w22;
+      float* q = acc + (cf.j1 + 1) + SP * (cf.j2 + 1);
+      const float q0 = q[0], q1 = q[1], q2 = q[SP], q3 = q[SP + 1];
+      q[0] = q0 + cf.w11 * p1;
+      q[1] = q1 + cf.w12 * p1;
+      q[SP] = q2 + cf.w11 * p2;
+      q[SP + 1] = q3 + cf.w12 * p2;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_rot2_adj_scatter(const float* __restrict__ work,
+                                                                 float* __restrict__ part,
+                                                                 const RotView* __restrict__ rv,
+                                                                 const int* __restrict__ vids, int nb, int vpg,
+                                                                 Geo g) {
+  extern __shared__ float acc[];
+  const int N = g.nx, pad = g.pad, z = blockIdx.x, grp = blockIdx.y, tid = threadIdx.x;
+  const int NN = N * N, SP = N + 8;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const long S = (long)NN, V = S * g.nz;
+  for (int k = tid; k < (N + 2) * SP; k += RS_THREADS) acc[k] = 0.f;
+  const int b0 = grp * vpg, b1 = min(nb, b0 + vpg);
+  const int st0 = 9 * b0, st1 = 9 * b1;
+  const float* zsrc = work + S * z;
+  // (view, phase) steps in ping-pong buffers: the next step's pixels load
+  // while this one scatters
+  ScatterPix A, B;
+  if (st0 < st1) scatter_load(A, zsrc + V * b0, N, 0, wave, lane);
+  __syncthreads();  // acc zeroed
+#pragma unroll 1
+  for (int st = st0; st < st1; st += 2) {
+    const int b = st / 9, c = st - 9 * b;
+    const bool more = st + 1 < st1;
+    const int bn = (st + 1) / 9;
+    if (more) scatter_load(B, zsrc + V * bn, N, st + 1 - 9 * bn, wave, lane);
+    scatter_phase(acc, A, rv[vids[b]], N, pad, SP);
+    __syncthreads();  // the phase's updates land before the next phase reads
+    if (!more) break;
+    const int bm = (st + 2) / 9;
+    if (st + 2 < st1) scatter_load(A, zsrc + V * bm, N, st + 2 - 9 * bm, wave, lane);
+    scatter_phase(acc, B, rv[vids[bn]], N, pad, SP);
+    __syncthreads();
+    (void)c;
+  }
+  float* o = part + V * grp + S * z;
+  RowIter it(tid, RS_THREADS, N);
+  for (int k = tid; k < NN; k += RS_THREADS) {
+    o[k] = acc[(it.row + 1) * SP + it.col + 1];
+    it.next();
+  }
+}
+
 // ===========================================================================
 // Rotation, 3-pass 1-D linear interpolation (:one), one slice per block in LDS
 // ===========================================================================
@@ -1060,6 +1167,11 @@ hipError_t launch_rot_fwd(int interp, const float* img, float* out, const float*
   return hipGetLastError();
 }
 
+int rot2_adj_blocks_per_cu(const Geo& g) {
+  const size_t lds = g.nx <= 128 ? sizeof(float) * (g.nx + 2) * (g.nx + 8) : sizeof(float) * g.nx * (g.nx + 1);
+  return std::max(1, std::min(2, (int)(kMaxLds / lds)));  // 1024-thread blocks: at most 2 per CU
+}
+
 hipError_t launch_rot_adj(int interp, const float* work, float* part, const RotView* rv, const int* vids, int nb,
                           int vpg, const int* tab_b, const float* tab_w, const Geo& g, hipStream_t s) {
   if (nb <= 0) return hipSuccess;
@@ -1070,10 +1182,16 @@ hipError_t launch_rot_adj(int interp, const float* work, float* part, const RotV
     k_rot1_adj<<<dim3(g.nz, nb), R1_THREADS, lds, s>>>(work, part, rv, vids, tab_b, tab_w, g);
   } else {
     const int ng = (nb + vpg - 1) / vpg;
-    const size_t lds = sizeof(float) * g.nx * (g.nx + 1);
-    if (lds > kMaxLds) return hipErrorInvalidConfiguration;
-    allow_lds(k_rot2_adj, lds);
-    k_rot2_adj<<<dim3(g.nz, ng), RA_THREADS, lds, s>>>(work, part, rv, vids, nb, vpg, g);
+    if (g.nx <= 128) {
+      const size_t lds = sizeof(float) * (g.nx + 2) * (g.nx + 8);
+      allow_lds(k_rot2_adj_scatter, lds);
+      k_rot2_adj_scatter<<<dim3(g.nz, ng), RS_THREADS, lds, s>>>(work, part, rv, vids, nb, vpg, g);
+    } else {
+      const size_t lds = sizeof(float) * g.nx * (g.nx + 1);
+      if (lds > kMaxLds) return hipErrorInvalidConfiguration;
+      allow_lds(k_rot2_adj, lds);
+      k_rot2_adj<<<dim3(g.nz, ng), RA_THREADS, lds, s>>>(work, part, rv, vids, nb, vpg, g);
+    }
   }
   return hipGetLastError();
 }

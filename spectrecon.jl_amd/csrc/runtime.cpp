@@ -194,6 +194,7 @@ struct sr_plan_s {
   int nt = 8;
   int vstride = 0;
   int max_batch = 32;
+  int ncu = 256;  // compute units of the plan's device
   size_t bytes = 0;
 
   DBuf mumap, atten, taps_u, taps_w, taps_h, taps_c, rotv, tab_b, tab_w;
@@ -237,10 +238,21 @@ struct sr_plan_s {
       *vpg = 1;
       return nb;
     }
-    int v = nb;
-    while (v > 1 && (long)d.nz * ((nb + v - 1) / v) < 512) v = (v + 1) / 2;
-    *vpg = v;
-    return (nb + v - 1) / v;
+    // time ~ (rounds of blocks over the CUs) x (views per block); each group
+    // also writes and re-reads one partial image (about 1/16 of a view's time)
+    const long slots = (long)ncu * sr::rot2_adj_blocks_per_cu(g);
+    long best = -1;
+    int bv = nb;
+    for (int v = 1; v <= nb; ++v) {
+      const long ng = (nb + v - 1) / v;
+      const long cost = 16 * v * (((long)d.nz * ng + slots - 1) / slots) + ng;
+      if (best < 0 || cost < best) {
+        best = cost;
+        bv = v;
+      }
+    }
+    *vpg = bv;
+    return (nb + bv - 1) / bv;
   }
 };
 
@@ -506,6 +518,13 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   p->V = (long)d.nx * d.ny * d.nz;
   p->P = (long)d.nx * d.nz;
   p->max_batch = std::max(1, std::min(d.nview, 64));
+  {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      p->ncu = n;
+    (void)hipGetLastError();
+  }
   auto bail = [](int code) { return code; };
   if (hipStreamCreate(&p->stream) != hipSuccess) {
     (void)hipGetLastError();
