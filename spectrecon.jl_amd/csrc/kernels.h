@@ -78,6 +78,8 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
 // ---- elementwise --------------------------------------------------------
 hipError_t launch_fill(float* p, float v, long n, hipStream_t s);
 hipError_t launch_zero_to_inf(float* p, long n, hipStream_t s);
+// *flag = 0, then 1 if some p[k] is not > 0
+hipError_t launch_flag_nonpositive(const float* p, long n, int* flag, hipStream_t s);
 
 // psf kernel limits: the fast separable / direct kernels handle tap counts
 // up to this many (padded to a multiple of 8)

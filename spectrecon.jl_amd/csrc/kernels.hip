@@ -631,7 +631,7 @@ __device__ __forceinline__ void conv_strip_fwd(Ld ld, const float* __restrict__ 
 }
 template <int NT, int SL, class Ld>
 __device__ __forceinline__ void conv_strip_adj(Ld ld, const float* __restrict__ t, float (&o)[SL]) {
-#pragma unroll
+#pragma unroll 1
   for (int g = 0; g < NT; g += 8) {
     float w[SL + 7];
 #pragma unroll
@@ -857,15 +857,20 @@ __global__ void __launch_bounds__(256) k_view_reduce(const float* __restrict__ p
 //   n == 1:   in[0] * Tail(0)
 // (the fold of the replicate pad, src/fft_convolve.jl:99-122).  Tail/Head
 // are per-plane tables in LDS.
-__device__ __forceinline__ float adj_edge_lo(const float* in, int stride, int n, int h, const float* tail) {
+template <int NT>
+__device__ __forceinline__ float adj_edge_lo(const float* in, int n, int h, const float* tail) {
   float s = 0.f;
   const int i1 = min(n - 1, h);
-  for (int i = 0; i <= i1; ++i) s = fmaf(in[i * stride], tail[i + h], s);
+#pragma unroll 8
+  for (int i = 0; i <= i1; ++i) s = fmaf(in[i], tail[i + h], s);
   return s;
 }
-__device__ __forceinline__ float adj_edge_hi(const float* in, int stride, int n, int h, const float* head) {
+template <int NT>
+__device__ __forceinline__ float adj_edge_hi(const float* in, int n, int h, const float* head) {
   float s = 0.f;
-  for (int i = max(0, n - 1 - h); i < n; ++i) s = fmaf(in[i * stride], head[i + h - n + 1], s);
+  const int i0 = max(0, n - 1 - h);
+#pragma unroll 8
+  for (int i = i0; i < n; ++i) s = fmaf(in[i], head[i + h - n + 1], s);
   return s;
 }
 
@@ -895,12 +900,7 @@ k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ v
   const int VZ = odd_up(SL * nzs + NT - 1);
   float* Vt = L;
   float* Qp = L + (long)nx * VZ;
-  float* s_tu = Qp + nz * WQ;  // taps u, w and their tail/head sums
-  float* s_tw = s_tu + NT;
-  float* s_ut = s_tw + NT;
-  float* s_uh = s_ut + NT;
-  float* s_wt = s_uh + NT;
-  float* s_wh = s_wt + NT;
+  float* s_taps = Qp + nz * WQ;  // 2 buffers of: taps u, w and their tail/head sums
   const float* vin = views + (long)b * P;
   for (int k = tid; k < nx * VZ; k += nthr) Vt[k] = 0.f;
   for (int k = tid; k < nz * WQ; k += nthr) Qp[k] = 0.f;
@@ -913,20 +913,24 @@ k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ v
   const float* av = atten ? atten + atten_vstride * v : nullptr;
   const bool vec4 = (nx % 4 == 0) && (SL % 4 == 0);
 
+  // thread tid < 6 NT holds entry tid of the (u, w, u tail, u head, w tail,
+  // w head) table of the next plane, loaded one plane ahead
+  const int tab = tid / NT, tk = tid - tab * NT;
+  auto tap_load = [&](int y) {
+    const long plane = (long)v * t.vstride + y;
+    return tab == 0 ? t.u[plane * NT + tk] : tab == 1 ? t.w[plane * NT + tk] : t.cum[(plane * 4 + tab - 2) * NT + tk];
+  };
+  float tv = (tid < 6 * NT && y0 < y1) ? tap_load(y0) : 0.f;
   for (int y = y0; y < y1; ++y) {
-    const int plane = v * t.vstride + y;
-    if (tid < NT) {
-      const float* cum = t.cum + (long)plane * 4 * NT;
-      const float u0 = t.u[(long)plane * NT + tid], w0 = t.w[(long)plane * NT + tid];
-      const float c0 = cum[tid], c1 = cum[NT + tid], c2 = cum[2 * NT + tid], c3 = cum[3 * NT + tid];
-      s_tu[tid] = u0;
-      s_tw[tid] = w0;
-      s_ut[tid] = c0;
-      s_uh[tid] = c1;
-      s_wt[tid] = c2;
-      s_wh[tid] = c3;
-    }
+    float* s_tu = s_taps + (y & 1) * 6 * NT;
+    float* s_tw = s_tu + NT;
+    float* s_ut = s_tw + NT;
+    float* s_uh = s_ut + NT;
+    float* s_wt = s_uh + NT;
+    float* s_wh = s_wt + NT;
+    if (tid < 6 * NT) s_tu[tid] = tv;
     __syncthreads();
+    if (tid < 6 * NT && y + 1 < y1) tv = tap_load(y + 1);
     float q[SL];
     // z adjoint: strip (x, z0..z0+SL-1) -> Qp[z][hx + x]
     if (hasz) {
@@ -943,8 +947,8 @@ k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ v
       if (nz == 1) {
         elo = vcol[0] * s_wt[0];
       } else {
-        if (lo) elo = adj_edge_lo(vcol, 1, nz, hz, s_wt);
-        if (hi) ehi = adj_edge_hi(vcol, 1, nz, hz, s_wh);
+        if (lo) elo = adj_edge_lo<NT>(vcol, nz, hz, s_wt);
+        if (hi) ehi = adj_edge_hi<NT>(vcol, nz, hz, s_wh);
       }
 #pragma unroll
       for (int k = 0; k < SL; ++k)
@@ -975,8 +979,8 @@ k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ v
       if (nx == 1) {
         elo = qrow[0] * s_ut[0];
       } else {
-        if (lo) elo = adj_edge_lo(qrow, 1, nx, hx, s_ut);
-        if (hi) ehi = adj_edge_hi(qrow, 1, nx, hx, s_uh);
+        if (lo) elo = adj_edge_lo<NT>(qrow, nx, hx, s_ut);
+        if (hi) ehi = adj_edge_hi<NT>(qrow, nx, hx, s_uh);
       }
 #pragma unroll
       for (int k = 0; k < SL; ++k)
@@ -1096,6 +1100,13 @@ __global__ void __launch_bounds__(256) k_fill(float* __restrict__ p, float v, lo
 __global__ void __launch_bounds__(256) k_zero_to_inf(float* __restrict__ p, long n) {
   const long k = (long)blockIdx.x * 256 + threadIdx.x;
   if (k < n && p[k] == 0.f) p[k] = __builtin_inff();  // asum[asum .== 0] .= Inf
+}
+
+// *flag = 1 when some entry is not > 0 (NaN included); the caller clears it
+__global__ void __launch_bounds__(256) k_flag_nonpositive(const float* __restrict__ p, long n, int* __restrict__ flag) {
+  bool bad = false;
+  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256) bad |= !(p[k] > 0.f);
+  if (bad) *flag = 1;
 }
 
 // ===========================================================================
@@ -1299,9 +1310,10 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
     const int SLv = sep_strip_len(g);
     if (SLv == 0) return hipErrorInvalidValue;
     const int sx = (g.nx + SLv - 1) / SLv, sz = (g.nz + SLv - 1) / SLv;
-    const int block = std::max(64, (int)cdiv(std::max(g.nz * sx, g.nx * sz), 64) * 64);
+    // one strip per thread and pass; at least one thread per tap-table entry
+    const int block = (int)cdiv(std::max(std::max(g.nz * sx, g.nx * sz), 6 * t.ntx), 64) * 64;
     const size_t lds = sizeof(float) * ((size_t)g.nx * odd_up(SLv * sz + t.ntx - 1) +
-                                        (size_t)g.nz * odd_up(SLv * sx + t.ntx - 1) + 6 * t.ntx);
+                                        (size_t)g.nz * odd_up(SLv * sx + t.ntx - 1) + 12 * t.ntx);
     if (lds > kMaxLds) return hipErrorInvalidConfiguration;
     SR_NT_SWITCH(t.ntx, SR_SL_SWITCH(SLv, {
       allow_lds(k_psf_adj_sep<NT, SL>, lds);
@@ -1322,6 +1334,12 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
 hipError_t launch_fill(float* p, float v, long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   k_fill<<<cdiv(n, 256), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_flag_nonpositive(const float* p, long n, int* flag, hipStream_t s) {
+  k_fill<<<1, 256, 0, s>>>(reinterpret_cast<float*>(flag), 0.f, 1);  // bit pattern of int 0
+  if (n > 0) k_flag_nonpositive<<<std::min(cdiv(n, 256), 1024u), 256, 0, s>>>(p, n, flag);
   return hipGetLastError();
 }
 

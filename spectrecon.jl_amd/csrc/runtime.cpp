@@ -201,6 +201,7 @@ struct sr_plan_s {
   DBuf work, fpart, ipart, vids, ones;
   DBuf st_img, st_views, st_aux1, st_aux2, st_x0;  // staging for host-pointer calls and EM temporaries
   DBuf em_ratio, em_back, em_asum;
+  DBuf flag;  // one int: device-side argument checks
   std::vector<int> h_vids;
 
   // per-kernel event timing (sr_plan_set_timing)
@@ -231,6 +232,9 @@ struct sr_plan_s {
     const int target = std::max(1, (int)std::ceil(512.0 / std::max(1, nb)));
     return std::min(d.ny, target);
   }
+  // PSF adjoint: one block per CU (its LDS holds the padded view and the
+  // intermediate plane), the planes of a view split over ceil(ncu/nb) chunks
+  int nchunk_adj(int nb) const { return std::min(d.ny, std::max(1, (ncu + nb - 1) / std::max(1, nb))); }
   int groups_adj(int nb, int* vpg) const {
     // views per rotation-adjoint group: enough blocks (nz * ngroups) to fill
     // the device; the :one kernel takes one view per block
@@ -383,7 +387,7 @@ int run_backproject(sr_plan p, float* image, const float* views, const int* d_vi
     {
       KTimer kt(p, sr::K_PSF_ADJ);
       HIP_TRY(sr::launch_psf_adj(p->separable, views + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(),
-                                 p->atten.as<float>(), p->atten_vstride(), p->nchunk_fwd(nb), p->g, p->stream));
+                                 p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(nb), p->g, p->stream));
     }
     int vpg;
     const int ng = p->groups_adj(nb, &vpg);
@@ -724,13 +728,17 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
   const long M = p->P * nview;
   const bool host = (mem & 0xff) == SR_MEM_HOST;
   int rc;
-  // background > 0 (src/ml-os-em.jl:30, :106), checked on the host
+  // background > 0 (src/ml-os-em.jl:30, :106), checked before any update:
+  // on the host for host arrays, else by a device scan that returns one flag
   if (host) {
     if ((rc = check_background(b, (size_t)M))) return rc;
   } else {
-    std::vector<float> hb(M);
-    HIP_TRY(hipMemcpy(hb.data(), b, sizeof(float) * M, hipMemcpyDeviceToHost));
-    if ((rc = check_background(hb.data(), (size_t)M))) return rc;
+    if ((rc = p->flag.ensure(sizeof(int)))) return rc;
+    HIP_TRY(sr::launch_flag_nonpositive(b, M, p->flag.as<int>(), p->stream));
+    int bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, p->flag.p, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    if (bad) return fail(SR_ERR_ARG, "need background > 0");
   }
   const float *d_x0 = x0, *d_y = y, *d_b = b;
   float* d_out = out;

@@ -232,8 +232,7 @@ def test_mlem_osem_vs_oracle(gpu, O):
     assert rel <= 1e-4, rel
     xg2 = np.zeros_like(x0)
     gpu.mlem_(xg2, x0, y, b, A, niter=5)
-    # xhat1 ≈ xhat2 (test/ml-os-em.jl:45): isapprox with rtol sqrt(eps(Float32));
-    # the rotation adjoint accumulates with LDS atomics, so runs agree to rounding
+    # xhat1 ≈ xhat2 (test/ml-os-em.jl:45): isapprox with rtol sqrt(eps(Float32))
     assert np.linalg.norm(xg - xg2) <= np.sqrt(np.finfo(np.float32).eps) * np.linalg.norm(xg)
     Ab = gpu.Ablock(plan, 4)
     og = gpu.osem(x0, y, b, Ab, niter=3)
@@ -253,3 +252,29 @@ def test_errors_on_device(gpu):
         gpu.osem(np.ones_like(image), y, np.zeros_like(y), Ab, niter=1)
     with pytest.raises(gpu.SPECTError, match="nview must be divisible by nblocks!"):
         gpu.Ablock(plan, 3)
+
+
+def test_background_check_device_arrays(gpu):
+    """b > 0 (src/ml-os-em.jl:30, :106) is checked on the device for device
+    arrays, before any update: the output stays untouched."""
+    from spectrecon_amd import _lib as L
+
+    mumap, psfs, image = small_problem(1)
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    nx, ny, nz = image.shape
+    nview = psfs.shape[3]
+    y = np.ones((nx, nz, nview), np.float32)
+    x0 = np.ones_like(image)
+    sentinel = np.full_like(image, 7.0)
+    for bad in (0.0, -1.0, np.nan):
+        b = np.ones_like(y)
+        b[3, 2, nview - 1] = bad
+        d = [gpu.DeviceArray.from_host(a) for a in (sentinel, x0, y, b)]
+        rc = gpu.lib().sr_osem(plan.handle, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, 2, 1, L.SR_MEM_DEVICE)
+        assert rc == L.SR_ERR_ARG
+        assert gpu.lib().sr_last_error().decode() == "need background > 0"
+        assert np.array_equal(d[0].to_host(), sentinel)
+    b = np.ones_like(y)
+    d = [gpu.DeviceArray.from_host(a) for a in (sentinel, x0, y, b)]
+    L.check(gpu.lib().sr_osem(plan.handle, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, 2, 1, L.SR_MEM_DEVICE))
+    assert np.all(np.isfinite(d[0].to_host()))
