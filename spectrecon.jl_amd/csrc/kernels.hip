@@ -608,43 +608,66 @@ __device__ __forceinline__ void conv8_adj(const float (&w)[NT + 7], const float*
 }
 
 constexpr int PSF_MAXS = 4;  // strips per thread per phase
+// separable PSF kernels: one strip per thread and pass, at most this many
+// threads (3 waves per SIMD: 168 VGPRs for a fully unrolled strip)
+constexpr int kSepThreads = 768;
 
-// Strip of SL outputs, taps processed in groups of 8 so that only SL+7
-// inputs are live at a time.  ld(i) returns input i of the strip's window.
+// Strip of SL outputs, taps processed in groups of 8 so that only SL+8
+// inputs are live at a time.  row[i] is input i of the strip's window; row,
+// the taps t and NT are multiples of 16 bytes / 4 floats, so windows and taps
+// load as float4 (the window's last float is loaded but unused).
 //   forward (convolution): o[k] += sum_a t[a] * in[k + NT-1-a]
 //   transpose (correlation): o[k] += sum_a t[a] * in[k + a]
-template <int NT, int SL, class Ld>
-__device__ __forceinline__ void conv_strip_fwd(Ld ld, const float* __restrict__ t, float (&o)[SL]) {
+template <int N>
+__device__ __forceinline__ void lds_load4(const float* __restrict__ p, float (&w)[N]) {
+  const float4* p4 = reinterpret_cast<const float4*>(__builtin_assume_aligned(p, 16));
 #pragma unroll
-  for (int g = 0; g < NT; g += 8) {
-    const int base = NT - 8 - g;
-    float w[SL + 7];
-#pragma unroll
-    for (int j = 0; j < SL + 7; ++j) w[j] = ld(base + j);
-#pragma unroll
-    for (int aa = 0; aa < 8; ++aa) {
-      const float ta = t[g + aa];
-#pragma unroll
-      for (int k = 0; k < SL; ++k) o[k] = fmaf(ta, w[k + 7 - aa], o[k]);
-    }
+  for (int j = 0; j < N / 4; ++j) {
+    const float4 q = p4[j];
+    w[4 * j] = q.x;
+    w[4 * j + 1] = q.y;
+    w[4 * j + 2] = q.z;
+    w[4 * j + 3] = q.w;
   }
 }
-template <int NT, int SL, class Ld>
-__device__ __forceinline__ void conv_strip_adj(Ld ld, const float* __restrict__ t, float (&o)[SL]) {
+template <int NT, int SL>
+__device__ __forceinline__ void conv_strip_fwd(const float* __restrict__ row, const float* __restrict__ t,
+                                               float (&o)[SL]) {
+#pragma unroll
+  for (int g = 0; g < NT; g += 8) {
+    float w[SL + 8], ta[8];
+    lds_load4(row + NT - 8 - g, w);
+    lds_load4(t + g, ta);
+#pragma unroll
+    for (int aa = 0; aa < 8; ++aa)
+#pragma unroll
+      for (int k = 0; k < SL; ++k) o[k] = fmaf(ta[aa], w[k + 7 - aa], o[k]);
+  }
+}
+template <int NT, int SL>
+__device__ __forceinline__ void conv_strip_adj(const float* __restrict__ row, const float* __restrict__ t,
+                                               float (&o)[SL]) {
 #pragma unroll 1
   for (int g = 0; g < NT; g += 8) {
-    float w[SL + 7];
+    float w[SL + 8], ta[8];
+    lds_load4(row + g, w);
+    lds_load4(t + g, ta);
 #pragma unroll
-    for (int j = 0; j < SL + 7; ++j) w[j] = ld(g + j);
+    for (int aa = 0; aa < 8; ++aa)
 #pragma unroll
-    for (int aa = 0; aa < 8; ++aa) {
-      const float ta = t[g + aa];
-#pragma unroll
-      for (int k = 0; k < SL; ++k) o[k] = fmaf(ta, w[k + aa], o[k]);
-    }
+      for (int k = 0; k < SL; ++k) o[k] = fmaf(ta[aa], w[k + aa], o[k]);
   }
 }
 
+
+// Row stride of the strip layouts: room for v floats plus the one extra float
+// a window's float4 load reads, a multiple of 4 floats (16-byte rows) and an
+// odd number of float4s, so lanes that read rows r, r+1, ... hit distinct
+// bank quads.
+__host__ __device__ __forceinline__ int quad_stride(int v) {
+  const int q = (v + 4) / 4;  // float4s holding v + 1 floats
+  return 4 * (q | 1);
+}
 
 __host__ __device__ __forceinline__ int odd_up(int v) { return v | 1; }
 
@@ -661,17 +684,17 @@ __host__ __device__ __forceinline__ int odd_up(int v) { return v | 1; }
 constexpr int PF_MAX = 4;  // prefetched float4 per thread (plane <= 16*nthreads floats)
 
 template <int NT, int SL>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(kSepThreads)
 k_psf_fwd_sep(const float* __restrict__ work, Taps t, const int* __restrict__ vids, int Yc,
               float* __restrict__ part, Geo g) {
   extern __shared__ float L[];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
   const int nxs = (nx + SL - 1) / SL, nzs = (nz + SL - 1) / SL;
-  const int WX = odd_up(SL * nxs + NT - 1);   // x-pass row stride
+  const int WX = quad_stride(SL * nxs + NT - 1);  // x-pass row stride
   const int LP = NT - 1 - hx;                 // replicate columns left of x = 0
   const int PT = NT - 1 - hz;                 // replicate rows above z = 0
-  const int WZ = odd_up(SL * nzs + NT - 1);   // z-pass column stride
+  const int WZ = quad_stride(SL * nzs + NT - 1);  // z-pass column stride
   const int area = max(nz * WX, SL * nxs * WZ);
   const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
   const int v = vids[b];
@@ -737,7 +760,7 @@ k_psf_fwd_sep(const float* __restrict__ work, Taps t, const int* __restrict__ vi
     for (int k = 0; k < SL; ++k) q[k] = 0.f;
     if (hasx) {
       const float* row = L + xs_z * WX + xs_x0;
-      conv_strip_fwd<NT, SL>([&](int j) { return row[j]; }, s_tu, q);
+      conv_strip_fwd<NT, SL>(row, s_tu, q);
     }
     __syncthreads();
     if (hasx) {
@@ -756,7 +779,7 @@ k_psf_fwd_sep(const float* __restrict__ work, Taps t, const int* __restrict__ vi
     if (hasz) {
       // output z reads rows z + hz - a (a < NT), i.e. Qt[x][z0 + j], j < NT+SL-1
       const float* col = L + zs_x * WZ + zs_z0;
-      conv_strip_fwd<NT, SL>([&](int j) { return col[j]; }, s_tw, acc);
+      conv_strip_fwd<NT, SL>(col, s_tw, acc);
     }
     __syncthreads();
   }
@@ -881,14 +904,14 @@ __device__ __forceinline__ float adj_edge_hi(const float* in, int n, int h, cons
 // (z, x0..x0+SL-1) is written straight to the work volume, times the
 // attenuation factor (mul3dj!, src/backproject.jl:316).
 template <int NT, int SL>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kSepThreads)
 k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ vids, int Yc, float* __restrict__ work,
               const float* __restrict__ atten, long atten_vstride, Geo g) {
   extern __shared__ float L[];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
   const int nxs = (nx + SL - 1) / SL, nzs = (nz + SL - 1) / SL;
-  const int WQ = odd_up(SL * nxs + NT - 1);
+  const int WQ = quad_stride(SL * nxs + NT - 1);
   const int b = blockIdx.y, c = blockIdx.x;
   const int v = vids[b];
   const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
@@ -897,7 +920,7 @@ k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ v
   const int tid = threadIdx.x, nthr = blockDim.x;
   // Vt: the view transposed to z-contiguous columns Vt[x][hz + z] with zero
   // pads (column stride VZ), so the z-pass windows are immediate offsets
-  const int VZ = odd_up(SL * nzs + NT - 1);
+  const int VZ = quad_stride(SL * nzs + NT - 1);
   float* Vt = L;
   float* Qp = L + (long)nx * VZ;
   float* s_taps = Qp + nz * WQ;  // 2 buffers of: taps u, w and their tail/head sums
@@ -938,7 +961,7 @@ k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ v
       for (int k = 0; k < SL; ++k) q[k] = 0.f;
       // row zs_z0 - hz + j of the view is Vt[x][zs_z0 + j]
       const float* col = Vt + zs_x * VZ + zs_z0;
-      conv_strip_adj<NT, SL>([&](int j) { return col[j]; }, s_tw, q);
+      conv_strip_adj<NT, SL>(col, s_tw, q);
       // replicate-pad fold at z = 0 and z = nz-1 (one evaluation each)
       const int khi = nz - 1 - zs_z0;
       const bool lo = zs_z0 == 0, hi = khi < SL;
@@ -971,7 +994,7 @@ k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ v
 #pragma unroll
       for (int k = 0; k < SL; ++k) q[k] = 0.f;
       const float* row = Qp + xs_z * WQ + xs_x0;
-      conv_strip_adj<NT, SL>([&](int j) { return row[j]; }, s_tu, q);
+      conv_strip_adj<NT, SL>(row, s_tu, q);
       const float* qrow = Qp + xs_z * WQ + hx;
       const int khi = nx - 1 - xs_x0;
       const bool lo = xs_x0 == 0, hi = khi < SL;
@@ -1119,7 +1142,7 @@ static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b)
 static int sep_strip_len(const Geo& g) {
   for (int sl : {8, 16, 32}) {
     const long sx = (long)g.nz * ((g.nx + sl - 1) / sl), sz = (long)g.nx * ((g.nz + sl - 1) / sl);
-    if (std::max(sx, sz) <= 1024) return sl;
+    if (std::max(sx, sz) <= kSepThreads) return sl;
   }
   return 0;
 }
@@ -1269,8 +1292,8 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
     const long need = std::max((long)std::max(g.nz * sx, g.nx * sz), (long)cdiv((long)g.nx * g.nz, 4L * PF_MAX));
     const int block = std::max(64, (int)cdiv(need, 64) * 64);
     if (block > 1024) return hipErrorInvalidConfiguration;
-    const size_t area = std::max((size_t)g.nz * odd_up(SLv * sx + t.ntx - 1),
-                                 (size_t)SLv * sx * odd_up(SLv * sz + t.ntx - 1));
+    const size_t area = std::max((size_t)g.nz * quad_stride(SLv * sx + t.ntx - 1),
+                                 (size_t)SLv * sx * quad_stride(SLv * sz + t.ntx - 1));
     const size_t lds = sizeof(float) * (area + 2 * t.ntx);
     if (lds > kMaxLds) return hipErrorInvalidConfiguration;
     SR_NT_SWITCH(t.ntx, SR_SL_SWITCH(SLv, {
@@ -1312,8 +1335,8 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
     const int sx = (g.nx + SLv - 1) / SLv, sz = (g.nz + SLv - 1) / SLv;
     // one strip per thread and pass; at least one thread per tap-table entry
     const int block = (int)cdiv(std::max(std::max(g.nz * sx, g.nx * sz), 6 * t.ntx), 64) * 64;
-    const size_t lds = sizeof(float) * ((size_t)g.nx * odd_up(SLv * sz + t.ntx - 1) +
-                                        (size_t)g.nz * odd_up(SLv * sx + t.ntx - 1) + 12 * t.ntx);
+    const size_t lds = sizeof(float) * ((size_t)g.nx * quad_stride(SLv * sz + t.ntx - 1) +
+                                        (size_t)g.nz * quad_stride(SLv * sx + t.ntx - 1) + 12 * t.ntx);
     if (lds > kMaxLds) return hipErrorInvalidConfiguration;
     SR_NT_SWITCH(t.ntx, SR_SL_SWITCH(SLv, {
       allow_lds(k_psf_adj_sep<NT, SL>, lds);
