@@ -73,19 +73,24 @@ def cpu_baseline(image, mumap, psfs, budget_s):
     nsub = NVIEW // NBLOCKS
     ang = S.default_viewangle(NVIEW).astype(np.float64)
     P = O.Problem(mumap, psfs, DY, ang, "two", np.float32, "fft", nthread=min(cores, nsub))
-    idx = np.arange(0, NVIEW, NBLOCKS, dtype=np.int32)
-    t0 = time.perf_counter()
-    v = P.project(image, index=idx)
-    tp = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    P.backproject(v, index=idx)
-    tb = time.perf_counter() - t0
+    # subset projection + back-projection, repeated for about budget_s / 3 (at least twice): median times
+    tps, tbs = [], []
+    t_start = time.perf_counter()
+    while len(tps) < 2 or time.perf_counter() - t_start < budget_s / 3:
+        idx = np.arange(len(tps) % NBLOCKS, NVIEW, NBLOCKS, dtype=np.int32)
+        t0 = time.perf_counter()
+        v = P.project(image, index=idx)
+        tps.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        P.backproject(v, index=idx)
+        tbs.append(time.perf_counter() - t0)
+    tp, tb = float(np.median(tps)), float(np.median(tbs))
     # one reconstruction = 4 x 8 subset projections + 5 x 8 subset back-projections
     t_recon = NITER * NBLOCKS * tp + (NITER + 1) * NBLOCKS * tb
     return {"value": 1.0 / t_recon, "unit": "recons/s", "cores": min(cores, nsub), "kind": "port",
-            "sample": (f"project + backproject of one {nsub}-view subset at {NX}x{NX}x{NZ}, {PSIZE}x{PSIZE} PSF "
-                       f"({tp:.2f} s + {tb:.2f} s), extrapolated to 32 subset projections + 40 subset "
-                       f"back-projections per reconstruction")}
+            "sample": (f"{len(tps)} x (project + backproject of one {nsub}-view subset) at {NX}x{NX}x{NZ}, "
+                       f"{PSIZE}x{PSIZE} PSF, median {tp:.2f} s + {tb:.2f} s, extrapolated to 32 subset "
+                       f"projections + 40 subset back-projections per reconstruction")}
 
 
 def main():
@@ -95,7 +100,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the benchmark, copy this many MiB 3 times with the library's copy kernel "
+                         "(the known-size reference for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="no benchmark: turn the counter_collection.csv files of a FETCH_SIZE and a WRITE_SIZE "
+                         "rocprofv3 pass (run with --copy-mib) into profiles/traffic.json")
     args = ap.parse_args()
+    if args.traffic_from:
+        from spectrecon_amd import profiling
+
+        if args.copy_mib <= 0:
+            raise SystemExit("--traffic-from needs the --copy-mib of the profiled run")
+        t = profiling.write_traffic(*args.traffic_from, args.copy_mib << 20,
+                                    os.path.join(ROOT, "profiles", "traffic.json"))
+        print(json.dumps(t, indent=1))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding over views is described in DESIGN.md but not built", file=sys.stderr)
@@ -163,6 +183,16 @@ def main():
         L.check(lib.sr_backproject(plan.handle, d_img.ptr, d_views.ptr, None, NVIEW, L.SR_MEM_DEVICE))
     back_ms = 1e3 * (time.perf_counter() - t1) / reps
 
+    # PCIe-inclusive: the same reconstruction from host arrays (staged in and out by the library)
+    hx0, hout = np.ones(image.shape, np.float32, order="F"), np.empty(image.shape, np.float32, order="F")
+    hy, hb = np.asfortranarray(y), np.asfortranarray(b)
+    host_ms = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        L.check(lib.sr_osem(plan.handle, hout.ctypes.data, hx0.ctypes.data, hy.ctypes.data, hb.ctypes.data,
+                            NBLOCKS, NITER, L.SR_MEM_HOST))
+        host_ms.append(1e3 * (time.perf_counter() - t1))
+
     # roofline of the dominant kernel over the timed region
     nsub = NVIEW // NBLOCKS
     g = (NX, NX, NZ, PSIZE)
@@ -174,7 +204,7 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(dom)
+        traffic = json.load(open(tpath)).get("kernels", {}).get(dom, {}).get("traffic_bytes")
     if fl > 0:
         roof = {"bound": "mfma", "achieved": fl / avg_s / 1e12, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}
     else:
@@ -190,6 +220,13 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         cpu = cpu_baseline(image, mumap, psfs, args.cpu_budget)
+
+    if args.copy_mib > 0:
+        n = args.copy_mib << 20
+        src, dst = S.DeviceArray((n // 4,)), S.DeviceArray((n // 4,))
+        for _ in range(3):
+            L.check(lib.sr_memcpy_d2d(dst.ptr, src.ptr, n))
+        log(f"copied {args.copy_mib} MiB x 3 (counter calibration)")
 
     lat_ms = sorted(1e3 * t for t in lat)
     value = args.steps / t_total
@@ -216,6 +253,7 @@ def main():
         "backproj_ms": back_ms,
         "proj_backproj_ms": proj_ms + back_ms,
         "plan_ms": plan_ms,
+        "osem_host_ms": float(np.median(host_ms)),
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,

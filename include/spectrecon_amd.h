@@ -72,6 +72,10 @@ int sr_malloc(void** dptr, size_t bytes);
 int sr_free(void* dptr);
 int sr_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int sr_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* device to device through the library's own streaming copy kernel (16-byte
+ * vectors when both pointers are 16-byte aligned): also the plain copy of
+ * known size that calibrates the HBM byte counters of a profile */
+int sr_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int sr_memset(void* dptr, int value, size_t bytes);
 int sr_device_synchronize(void);
 

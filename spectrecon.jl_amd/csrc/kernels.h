@@ -78,6 +78,8 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
 // ---- elementwise --------------------------------------------------------
 hipError_t launch_fill(float* p, float v, long n, hipStream_t s);
 hipError_t launch_zero_to_inf(float* p, long n, hipStream_t s);
+// dst[i] = src[i] for n 16-byte vectors (grid-stride streaming copy)
+hipError_t launch_copy16(void* dst, const void* src, long n, hipStream_t s);
 // *flag = 0, then 1 if some p[k] is not > 0
 hipError_t launch_flag_nonpositive(const float* p, long n, int* flag, hipStream_t s);
 

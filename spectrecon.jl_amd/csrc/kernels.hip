@@ -1125,6 +1125,10 @@ __global__ void __launch_bounds__(256) k_zero_to_inf(float* __restrict__ p, long
   if (k < n && p[k] == 0.f) p[k] = __builtin_inff();  // asum[asum .== 0] .= Inf
 }
 
+__global__ void __launch_bounds__(256) k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, long n) {
+  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256) dst[k] = src[k];
+}
+
 // *flag = 1 when some entry is not > 0 (NaN included); the caller clears it
 __global__ void __launch_bounds__(256) k_flag_nonpositive(const float* __restrict__ p, long n, int* __restrict__ flag) {
   bool bad = false;
@@ -1357,6 +1361,12 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
 hipError_t launch_fill(float* p, float v, long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   k_fill<<<cdiv(n, 256), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy16(void* dst, const void* src, long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_copy16<<<std::min(cdiv(n, 256), 8192u), 256, 0, s>>>(static_cast<uint4*>(dst), static_cast<const uint4*>(src), n);
   return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -474,6 +475,18 @@ int sr_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 }
 int sr_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return SR_OK;
+}
+int sr_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return SR_OK;
+  if (!dst || !src) return fail(SR_ERR_ARG, "null argument");
+  const bool al = ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15) == 0;
+  const size_t n16 = al ? bytes / 16 : 0;
+  if (n16) HIP_TRY(sr::launch_copy16(dst, src, (long)n16, nullptr));
+  if (bytes > 16 * n16)
+    HIP_TRY(hipMemcpy(static_cast<char*>(dst) + 16 * n16, static_cast<const char*>(src) + 16 * n16, bytes - 16 * n16,
+                      hipMemcpyDeviceToDevice));
+  HIP_TRY(hipDeviceSynchronize());
   return SR_OK;
 }
 int sr_memset(void* dptr, int value, size_t bytes) {

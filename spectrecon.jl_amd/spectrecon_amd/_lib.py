@@ -51,6 +51,7 @@ SIGNATURES = {
     "sr_free": (C.c_int, [_vp]),
     "sr_memcpy_h2d": (C.c_int, [_vp, _vp, C.c_size_t]),
     "sr_memcpy_d2h": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "sr_memcpy_d2d": (C.c_int, [_vp, _vp, C.c_size_t]),
     "sr_memset": (C.c_int, [_vp, C.c_int, C.c_size_t]),
     "sr_device_synchronize": (C.c_int, []),
     "sr_plan_create": (C.c_int, [C.POINTER(PlanDesc), _vp, _vp, _vp, C.POINTER(C.c_void_p)]),

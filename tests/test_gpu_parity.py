@@ -278,3 +278,67 @@ def test_background_check_device_arrays(gpu):
     d = [gpu.DeviceArray.from_host(a) for a in (sentinel, x0, y, b)]
     L.check(gpu.lib().sr_osem(plan.handle, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, 2, 1, L.SR_MEM_DEVICE))
     assert np.all(np.isfinite(d[0].to_host()))
+
+
+def test_memcpy_d2d(gpu):
+    """sr_memcpy_d2d: the library's streaming copy (16-byte vectors) plus the
+    unaligned remainder, bit-exact."""
+    from spectrecon_amd import _lib as L
+
+    rng = np.random.default_rng(5)
+    for n in (1, 3, 4, 1000, 4099, 1 << 20):
+        a = rng.standard_normal(n).astype(np.float32)
+        src, dst = gpu.DeviceArray.from_host(a), gpu.DeviceArray((n,))
+        L.check(gpu.lib().sr_memcpy_d2d(dst.ptr, src.ptr, a.nbytes))
+        assert np.array_equal(dst.to_host(), a)
+    # unaligned pointers take the runtime copy
+    a = rng.standard_normal(1001).astype(np.float32)
+    src, dst = gpu.DeviceArray.from_host(a), gpu.DeviceArray((1001,))
+    L.check(gpu.lib().sr_memcpy_d2d(dst.ptr + 4, src.ptr + 4, 4 * 1000))
+    assert np.array_equal(dst.to_host()[1:], a[1:])
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+def test_two_d_mode(gpu, O, method):
+    """2-D mode: nz = 1 with pz = 1 PSFs (docs/lit/examples/5-2d.jl:36-59)."""
+    nx, nz, nview, px = 64, 1, 60, 13
+    rng = np.random.default_rng(31)
+    mumap = np.zeros((nx, nx, nz), np.float32)
+    mumap[16:48, 20:44, 0] = 0.1
+    psf = np.asarray(O.psf_gauss(nx, px, 1, 1.0, 4.0), np.float32)
+    psfs = np.asfortranarray(np.repeat(psf[:, :, :, None], nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, 1.0, interpmeth=method)
+    assert plan.info()["conv"] == "separable"
+    ang = angles32(nview).astype(np.float64)
+    P32 = O.Problem(mumap, psfs, 1.0, ang, method, np.float32, "fft")
+    image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    g = gpu.project(image, plan)
+    assert g.shape == (nx, 1, nview)
+    assert per_view_maxrel(g, P32.project(image)) <= PROJ_TOL
+    views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    assert maxrel(gpu.backproject(views, plan), P32.backproject(views)) <= PROJ_TOL
+    a = float(np.vdot(views.astype(np.float64), g.astype(np.float64)))
+    b = float(np.vdot(image.astype(np.float64), gpu.backproject(views, plan).astype(np.float64)))
+    assert abs(a - b) <= 1e-5 * abs(a)
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+def test_per_view_psfs_and_angles(gpu, O, method):
+    """Views with their own (separable) PSF stacks and a non-default,
+    non-uniform viewangle list (src/spectplan.jl:45,47,63)."""
+    nx, nz, nview, px = 32, 12, 10, 9
+    rng = np.random.default_rng(37)
+    mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = np.asfortranarray(np.stack(
+        [np.asarray(O.psf_gauss(nx, px, px, 0.5 + 0.2 * v, 2.0 + 0.3 * v), np.float32) for v in range(nview)], 3))
+    ang = np.sort(rng.random(nview) * 2 * np.pi).astype(np.float32)
+    plan = gpu.SPECTplan(mumap, psfs, 0.48, viewangle=ang, interpmeth=method)
+    assert plan.info()["conv"] == "separable"
+    P32 = O.Problem(mumap, psfs, 0.48, ang.astype(np.float64), method, np.float32, "fft")
+    image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    assert per_view_maxrel(gpu.project(image, plan), P32.project(image)) <= PROJ_TOL
+    views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    assert maxrel(gpu.backproject(views, plan), P32.backproject(views)) <= PROJ_TOL
+    idx = [7, 2]
+    rbs = P32.backproject(views[:, :, idx], index=np.array(idx, np.int32))
+    assert maxrel(gpu.backproject(views[:, :, idx], plan, index=idx), rbs) <= PROJ_TOL
