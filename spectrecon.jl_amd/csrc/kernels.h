@@ -97,6 +97,8 @@ constexpr long kMaxPlane = 32768;
 // whether the separable PSF kernels handle this geometry (nx % 4 == 0 and a
 // plane small enough for one strip per thread); otherwise the direct kernel
 bool psf_sep_supported(const Geo& g);
+// the separable PSF passes run on the matrix cores for this geometry / tap count
+bool psf_mfma_supported(const Geo& g, int nt);
 // blocks of the :two rotation adjoint that one CU holds at once (LDS-bound)
 int rot2_adj_blocks_per_cu(const Geo& g);
 

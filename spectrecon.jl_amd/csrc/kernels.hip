@@ -791,6 +791,274 @@ k_psf_fwd_sep(const float* __restrict__ work, Taps t, const int* __restrict__ vi
   }
 }
 
+// --- forward, separable, on the matrix cores ---------------------------------
+// Each pass of the clamped separable convolution is a banded matrix product:
+//   x pass  Q[z][x0+n] = sum_k P[z][x0-hx+k] u[n+2hx-k]      (k < 16+px-1)
+//   z pass  R[z0+m][x] = sum_k w[m+2hz-k] Q[z0-hz+k][x]      (k < 16+pz-1)
+// on zero-padded rows, plus the replicate boundary as one rank-2 step on the
+// edge tiles: clamping contributes in[0]*Tail(x+h+1) + in[n-1]*Head(x+h-n),
+// Tail(k) = sum_{a>=k} t[a], Head(k) = sum_{a<=k} t[a] (so the pads are zeros
+// written once per block).  16x16 output tiles on v_mfma_f32_16x16x4_f32; the
+// tap (Toeplitz) fragments are built per plane in registers, the data
+// fragments are read from LDS: Pz (the plane, rows z, column x+hx, row stride
+// an odd number of float4s: the 16 rows of a fragment hit distinct banks) and
+// Qz (the x-pass result, rows z+hz, stride = 16 mod 64 floats: the 4 rows of
+// a fragment hit distinct bank ranges).  The z-pass accumulates the view sum
+// over the block's planes in the MFMA accumulators; the next plane and taps
+// are prefetched in registers.  Lane l: A (m = l%16, k = l/16), B (k = l/16,
+// n = l%16), D register r (m = 4(l/16) + r, n = l%16).
+// grid (nchunk, nb), block 512 (8 waves); nx, nz multiples of 16.
+typedef float mf4 __attribute__((ext_vector_type(4)));
+// Workgroup barrier for LDS data only: waits for this wave's LDS operations,
+// not for its global loads and stores in flight (__syncthreads' fence drains
+// those too, which would expose every prefetch and store).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+// keeps the compiler's scheduler from moving instructions across this point
+// (the fragment loads of step s+1 stay ahead of the step-s products)
+#define SR_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+constexpr int MF_THREADS = 512;
+constexpr int MF_WAVES = MF_THREADS / 64;
+constexpr int MF_TPW = 8;   // output tiles per wave and pass, at most
+constexpr int MF_PF = 8;    // prefetched float4 per thread (plane <= 16384 floats)
+
+__host__ __device__ constexpr int mf_ksteps(int nt) { return (16 + nt - 1 + 3) / 4; }
+// column of x = 0 in the plane rows: at least hx, a multiple of 4 (16-byte rows stores)
+__host__ __device__ __forceinline__ int mf_xo(int hx) { return (hx + 3) & ~3; }
+__host__ __device__ __forceinline__ int mf_wx(int nx, int hx, int nt) {
+  return quad_stride(max(mf_xo(hx) - hx + nx - 16 + 4 * mf_ksteps(nt), mf_xo(hx) + nx) - 1);
+}
+__host__ __device__ __forceinline__ int mf_wq(int nx) { return ((nx + 47) / 64) * 64 + 16; }
+__host__ __device__ __forceinline__ int mf_rq(int nz, int nt) { return nz - 16 + 4 * mf_ksteps(nt); }
+
+__device__ __forceinline__ mf4 mfma4(float a, float b, mf4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// tail sum Tail(k) (k >= 0) and head sum Head(k) of nt-entry tables
+__device__ __forceinline__ float tail_at(const float* tail, int nt, int k) { return k < nt ? tail[max(k, 0)] : 0.f; }
+__device__ __forceinline__ float head_at(const float* head, int nt, int k) { return k < 0 ? 0.f : head[min(k, nt - 1)]; }
+
+// tap tables of one plane in LDS: u, w and the tail/head sums of u and w,
+// each with OFF zeros before and at least as many after (the Toeplitz
+// fragments index u[n + 2h - k] and u[k - n] without bounds checks)
+struct MfTaps {
+  int nt, off, tp, size;
+  __host__ __device__ explicit MfTaps(int nt_) : nt(nt_) {
+    off = 4 * mf_ksteps(nt_);
+    tp = (off + 4 * mf_ksteps(nt_) + 16 + 3) / 4 * 4;
+    size = 6 * tp;  // u, w, u tail, u head, w tail, w head
+  }
+};
+// LDS of the matrix-core kernels: the two plane buffers and tbp tap groups
+__host__ __device__ inline size_t mf_lds_floats(const Geo& g, int nt, int tbp) {
+  return (size_t)g.nz * mf_wx(g.nx, (g.px - 1) / 2, nt) + (size_t)mf_rq(g.nz, nt) * mf_wq(g.nx) +
+         (size_t)tbp * MfTaps(nt).size;
+}
+// planes whose tap tables are loaded at once: up to 8, within the LDS
+static int mf_tap_planes(const Geo& g, int nt, int Yc) {
+  const size_t base = mf_lds_floats(g, nt, 0), per = MfTaps(nt).size;
+  const long fit = ((long)(kMaxLds / sizeof(float)) - (long)base) / (long)per;
+  return (int)std::max(1L, std::min<long>(std::min(8, Yc), fit));
+}
+
+// The wave's TPW tiles form a run t = wave*TPW + i (the last wave's run is
+// clamped to the last tile: recomputed, not stored).
+template <int TPW>
+__device__ __forceinline__ void mf_tiles(int wave, int ntile, int ntx, int (&z0)[TPW], int (&x0)[TPW],
+                                         bool (&own)[TPW]) {
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    const int t0 = wave * TPW + i, tt = min(t0, ntile - 1), tz = tt / ntx;
+    own[i] = t0 < ntile;
+    z0[i] = 16 * tz;
+    x0[i] = 16 * (tt - tz * ntx);
+  }
+}
+
+template <int TPW, int NW>
+__global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restrict__ work, Taps t,
+                                                            const int* __restrict__ vids, int Yc,
+                                                            float* __restrict__ part, Geo g, int TBP) {
+  extern __shared__ float L[];
+  const int NT = t.ntx;
+  const MfTaps TT(NT);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
+  const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
+  const int ntx = nx >> 4, ntile = ntx * (nz >> 4);
+  const int WX = mf_wx(nx, hx, NT), WQ = mf_wq(nx), RQ = mf_rq(nz, NT), XO = mf_xo(hx);
+  const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
+  const int v = vids[b];
+  const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int m16 = lane & 15, q4 = lane >> 4;
+  float* Pz = L;
+  float* Qz = Pz + nz * WX;
+  float* s_taps = Qz + RQ * WQ;  // TBP planes of TT.size
+  for (int k = tid; k < nz * WX + RQ * WQ + TBP * TT.size; k += (NW * 64)) L[k] = 0.f;
+
+  const long S = (long)nx * ny;
+  const float* wb = work + S * nz * b;
+  const int nq = nx >> 2, nv = nz * nq;
+  // the plane as float4 along x, slot k of this thread = element tid + k*threads
+  // (clamped to the last element: the loads and stores stay unconditional,
+  // so no wait is forced between them; a clamped slot rewrites that element)
+  constexpr int PF = MF_PF * MF_WAVES / NW;
+  int pfo[PF], pfs[PF];
+#pragma unroll
+  for (int k = 0; k < PF; ++k) {
+    const int i = min(tid + k * (NW * 64), nv - 1), z = i / nq, x4 = i - z * nq;
+    pfo[k] = (int)(S * z) + 4 * x4;
+    pfs[k] = z * WX + XO + 4 * x4;
+  }
+  float4 pf[PF];
+  auto prefetch = [&](int y) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) pf[k] = *reinterpret_cast<const float4*>(wb + (long)nx * y + pfo[k]);
+  };
+  // thread tid < 6 NT loads entry tk of table tab (u, w, u tail, u head, w tail, w head)
+  // the tap tables of up to TBP (<= 8) planes at a time: all loads of a
+  // thread first, then the stores (clamped entries repeat the last one)
+  auto load_taps = [&](int yg, int ng) {
+    constexpr int ME = (8 * 6 * 64 + NW * 64 - 1) / (NW * 64);
+    const int n = ng * 6 * NT;
+    float tvv[ME];
+    int tdst[ME];
+#pragma unroll
+    for (int j = 0; j < ME; ++j) {
+      const int e = min(tid + j * NW * 64, n - 1);
+      const int pl = e / (6 * NT), r = e - pl * 6 * NT, tab = r / NT, tk = r - tab * NT;
+      const long plane = (long)v * t.vstride + yg + pl;
+      tdst[j] = pl * TT.size + tab * TT.tp + TT.off + tk;
+      const float* src = tab == 0 ? t.u + plane * NT : tab == 1 ? t.w + plane * NT : t.cum + (plane * 4 + tab - 2) * NT;
+      tvv[j] = src[tk];
+    }
+#pragma unroll
+    for (int j = 0; j < ME; ++j) s_taps[tdst[j]] = tvv[j];
+  };
+  if (y0 < y1) prefetch(y0);
+  int z0s[TPW], x0s[TPW];
+  bool own[TPW];
+  mf_tiles<TPW>(wave, ntile, ntx, z0s, x0s, own);
+  int aoff[TPW], zoff[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    aoff[i] = (z0s[i] + m16) * WX + XO - hx + x0s[i] + q4;
+    zoff[i] = (z0s[i] + q4) * WQ + x0s[i] + m16;
+  }
+  mf4 dv[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) dv[i] = mf4{0.f, 0.f, 0.f, 0.f};
+  __syncthreads();  // pads zeroed before the first plane lands
+
+  for (int y = y0; y < y1; ++y) {
+    if ((y - y0) % TBP == 0) {  // next tap group (the previous group's last reads are done)
+      lds_barrier();
+      load_taps(y, min(TBP, y1 - y));
+    }
+    const float* tb = s_taps + ((y - y0) % TBP) * TT.size;
+#pragma unroll
+    for (int k = 0; k < PF; ++k)
+      *reinterpret_cast<float4*>(Pz + pfs[k]) = make_float4(pf[k].x, pf[k].y, pf[k].z, pf[k].w);
+    lds_barrier();
+    prefetch(min(y + 1, y1 - 1));
+    // Toeplitz fragments: x-pass B[k][n] = u[n+2hx-k] = ub[-k], z-pass A[m][k] = w[m+2hz-k] = wz[-k]
+    const float* ub = tb + TT.off + m16 - q4 + 2 * hx;
+    const float* wz = tb + TT.tp + TT.off + m16 - q4 + 2 * hz;
+    const float* sut = tb + 2 * TT.tp + TT.off;
+    const float* suh = sut + TT.tp;
+    const float* swt = suh + TT.tp;
+    const float* swh = swt + TT.tp;
+    // x pass: K steps in ping-pong fragment registers (step s+1 loads while s
+    // multiplies).  The operands enter swapped, so the tile comes out
+    // transposed: lane (k, n) register r holds Q[z0 + n][x0 + 4k + r], four
+    // consecutive x stored as one float4.  The replicate boundary goes first,
+    // on every tile (its fragments vanish away from the edges).
+    mf4 dx[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int xn = x0s[i] + m16;
+      const float* prow = Pz + (z0s[i] + m16) * WX + XO;
+      const float pe = q4 == 0 ? prow[0] : q4 == 1 ? prow[nx - 1] : 0.f;
+      const float te = q4 == 0 ? tail_at(sut, NT, xn + hx + 1) : q4 == 1 ? head_at(suh, NT, xn + hx - nx) : 0.f;
+      dx[i] = mfma4(te, pe, mf4{0.f, 0.f, 0.f, 0.f});
+    }
+    {
+      float a0[TPW], a1[TPW], b0, b1;
+      auto ld = [&](float (&a)[TPW], float& bb, int s) {
+        bb = ub[-4 * s];
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) a[i] = Pz[aoff[i] + 4 * s];
+      };
+      ld(a0, b0, 0);
+      int s = 0;
+      for (; s + 1 < nsx; s += 2) {
+        ld(a1, b1, s + 1);
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b0, a0[i], dx[i]);
+        SR_SCHED_FENCE();
+        ld(a0, b0, min(s + 2, nsx - 1));
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b1, a1[i], dx[i]);
+      }
+      if (s < nsx) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b0, a0[i], dx[i]);
+      }
+    }
+    // (a clamped duplicate tile stores the same values as its owner)
+#pragma unroll
+    for (int i = 0; i < TPW; ++i)
+      *reinterpret_cast<float4*>(Qz + (hz + z0s[i] + m16) * WQ + x0s[i] + 4 * q4) =
+          make_float4(dx[i][0], dx[i][1], dx[i][2], dx[i][3]);
+    lds_barrier();
+    // z pass: accumulate the view sum, the replicate boundary first
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int zm = z0s[i] + m16;
+      const float ae = q4 == 0 ? tail_at(swt, NT, zm + hz + 1) : q4 == 1 ? head_at(swh, NT, zm + hz - nz) : 0.f;
+      const float be = q4 == 0 ? Qz[hz * WQ + x0s[i] + m16] : q4 == 1 ? Qz[(hz + nz - 1) * WQ + x0s[i] + m16] : 0.f;
+      dv[i] = mfma4(ae, be, dv[i]);
+    }
+    {
+      float bq0[TPW], bq1[TPW], a0, a1;
+      auto ld = [&](float (&bq)[TPW], float& aa, int s) {
+        aa = wz[-4 * s];
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) bq[i] = Qz[zoff[i] + 4 * s * WQ];
+      };
+      ld(bq0, a0, 0);
+      int s = 0;
+      for (; s + 1 < nsz; s += 2) {
+        ld(bq1, a1, s + 1);
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a0, bq0[i], dv[i]);
+        SR_SCHED_FENCE();
+        ld(bq0, a0, min(s + 2, nsz - 1));
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a1, bq1[i], dv[i]);
+      }
+      if (s < nsz) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a0, bq0[i], dv[i]);
+      }
+    }
+    // the next plane's stores touch Pz (last read before the barrier above)
+    // and the other tap buffer; Qz is rewritten only after the next barrier
+  }
+  float* pb = part + ((long)b * nchunk + c) * nx * nz;
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    if (own[i]) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) pb[(long)(z0s[i] + 4 * q4 + r) * nx + x0s[i] + m16] = dv[i][r];
+    }
+  }
+}
+
 // --- forward, direct 2-D ----------------------------------------------------
 // general (non-separable) PSF: strips of 8 x-outputs, pz rows of NT taps each
 template <int NT, int BLOCK>
@@ -1028,6 +1296,188 @@ k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ v
       }
     }
     __syncthreads();
+  }
+}
+
+// --- adjoint, separable, on the matrix cores ---------------------------------
+// Transpose of the matrix-core forward, per plane y of the view:
+//   z adjoint  G[j0+m][x] = sum_k w[k-m] V[j0-hz+k][x]          (correlation)
+//   x adjoint  H[z][j0+n] = sum_k G[z][j0-hx+k] u[k-n]
+// on zero-padded rows; the transposed replicate boundary makes image row
+// (column) 0 sum_i Tail(i+h) in[i] and row n-1 sum_i Head(i+h-n+1) in[i],
+// i.e. the same band with the taps of those rows replaced by their tail /
+// head sums (per-tile fragment tables).  Then H times the attenuation factors is
+// written to the work volume.  LDS: Vz (the view, rows z+hz, stride 16 mod
+// 64), Gx (the z adjoint, rows z, column x+hx), tap buffers.  Each wave takes
+// a run of consecutive tiles, so both halves of a 128-byte output line are
+// stored back to back.  grid (nchunk, nb), block 512.
+template <int TPW, int NW>
+__global__ void __launch_bounds__(NW * 64) k_psf_adj_mfma(const float* __restrict__ views, Taps t,
+                                                            const int* __restrict__ vids, int Yc,
+                                                            float* __restrict__ work, const float* __restrict__ atten,
+                                                            long atten_vstride, Geo g, int TBP) {
+  extern __shared__ float L[];
+  const int NT = t.ntx;
+  const MfTaps TT(NT);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
+  const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
+  const int ntx = nx >> 4, ntile = ntx * (nz >> 4);
+  const int WX = mf_wx(nx, hx, NT), WQ = mf_wq(nx), RQ = mf_rq(nz, NT);
+  const int b = blockIdx.y, c = blockIdx.x;
+  const int v = vids[b];
+  const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int m16 = lane & 15, q4 = lane >> 4;
+  float* Vz = L;
+  float* Gx = Vz + RQ * WQ;
+  float* s_taps = Gx + nz * WX;  // TBP planes of TT.size
+  for (int k = tid; k < RQ * WQ + nz * WX + TBP * TT.size; k += (NW * 64)) L[k] = 0.f;
+  __syncthreads();
+  const int P = nx * nz;
+  const float* vin = views + (long)b * P;
+  for (RowIter it(tid, (NW * 64), nx); it.row < nz; it.next()) Vz[(hz + it.row) * WQ + it.col] = vin[it.row * nx + it.col];
+
+  const long S = (long)nx * ny, V = S * nz;
+  float* wout = work + V * b;
+  const float* av = atten ? atten + atten_vstride * v : nullptr;
+  // the tap tables of up to TBP (<= 8) planes at a time: all loads of a
+  // thread first, then the stores (clamped entries repeat the last one)
+  auto load_taps = [&](int yg, int ng) {
+    constexpr int ME = (8 * 6 * 64 + NW * 64 - 1) / (NW * 64);
+    const int n = ng * 6 * NT;
+    float tvv[ME];
+    int tdst[ME];
+#pragma unroll
+    for (int j = 0; j < ME; ++j) {
+      const int e = min(tid + j * NW * 64, n - 1);
+      const int pl = e / (6 * NT), r = e - pl * 6 * NT, tab = r / NT, tk = r - tab * NT;
+      const long plane = (long)v * t.vstride + yg + pl;
+      tdst[j] = pl * TT.size + tab * TT.tp + TT.off + tk;
+      const float* src = tab == 0 ? t.u + plane * NT : tab == 1 ? t.w + plane * NT : t.cum + (plane * 4 + tab - 2) * NT;
+      tvv[j] = src[tk];
+    }
+#pragma unroll
+    for (int j = 0; j < ME; ++j) s_taps[tdst[j]] = tvv[j];
+  };
+  int z0s[TPW], x0s[TPW];
+  bool own[TPW];
+  mf_tiles<TPW>(wave, ntile, ntx, z0s, x0s, own);
+  int voff[TPW], goff[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    voff[i] = (z0s[i] + q4) * WQ + x0s[i] + m16;  // z adjoint B fragment (rows of Vz)
+    goff[i] = (z0s[i] + m16) * WX + x0s[i] + q4;  // x adjoint A fragment (rows of Gx)
+  }
+
+  for (int y = y0; y < y1; ++y) {
+    if ((y - y0) % TBP == 0) {  // next tap group (the previous group's last reads are done)
+      lds_barrier();
+      load_taps(y, min(TBP, y1 - y));
+    }
+    const float* tb = s_taps + ((y - y0) % TBP) * TT.size;
+    lds_barrier();  // taps (and, first time, Vz) in place; the previous x adjoint has read Gx
+    // Fragment tables per tile and lane: the transposed replicate boundary
+    // replaces the taps of image row/column 0 by their tail sums and of row/
+    // column n-1 by their head sums (same fragment index 4s + k - m).
+    const float* su = tb + TT.off + q4 - m16;
+    const float* sw = su + TT.tp;
+    const float* sut = sw + TT.tp;
+    const float* suh = sut + TT.tp;
+    const float* swt = suh + TT.tp;
+    const float* swh = swt + TT.tp;
+    // z adjoint: Vz -> Gx
+    const float* ap[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i)
+      ap[i] = (z0s[i] == 0 && m16 == 0) ? swt : (z0s[i] + 16 == nz && m16 == 15) ? swh : sw;
+    mf4 dg[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) dg[i] = mf4{0.f, 0.f, 0.f, 0.f};
+    {
+      float bq0[TPW], bq1[TPW], a0[TPW], a1[TPW];
+      auto ld = [&](float (&bq)[TPW], float (&aa)[TPW], int s) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) {
+          aa[i] = ap[i][4 * s];
+          bq[i] = Vz[voff[i] + 4 * s * WQ];
+        }
+      };
+      ld(bq0, a0, 0);
+      int s = 0;
+      for (; s + 1 < nsz; s += 2) {
+        ld(bq1, a1, s + 1);
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dg[i] = mfma4(a0[i], bq0[i], dg[i]);
+        SR_SCHED_FENCE();
+        ld(bq0, a0, min(s + 2, nsz - 1));
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dg[i] = mfma4(a1[i], bq1[i], dg[i]);
+      }
+      if (s < nsz) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dg[i] = mfma4(a0[i], bq0[i], dg[i]);
+      }
+    }
+    // (a clamped duplicate tile stores the same values as its owner)
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      float* go = Gx + (z0s[i] + 4 * q4) * WX + hx + x0s[i] + m16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) go[r * WX] = dg[i][r];
+    }
+    lds_barrier();
+    // x adjoint: Gx -> H, times the attenuation factors, to the work volume
+    float af[TPW][4];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        af[i][r] = av ? av[S * (z0s[i] + 4 * q4 + r) + (long)nx * y + x0s[i] + m16] : 1.f;
+    const float* bp[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i)
+      bp[i] = (x0s[i] == 0 && m16 == 0) ? sut : (x0s[i] + 16 == nx && m16 == 15) ? suh : su;
+    mf4 dh[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) dh[i] = mf4{0.f, 0.f, 0.f, 0.f};
+    {
+      float a0[TPW], a1[TPW], b0[TPW], b1[TPW];
+      auto ld = [&](float (&a)[TPW], float (&bb)[TPW], int s) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) {
+          bb[i] = bp[i][4 * s];
+          a[i] = Gx[goff[i] + 4 * s];
+        }
+      };
+      ld(a0, b0, 0);
+      int s = 0;
+      for (; s + 1 < nsx; s += 2) {
+        ld(a1, b1, s + 1);
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dh[i] = mfma4(a0[i], b0[i], dh[i]);
+        SR_SCHED_FENCE();
+        ld(a0, b0, min(s + 2, nsx - 1));
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dh[i] = mfma4(a1[i], b1[i], dh[i]);
+      }
+      if (s < nsx) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dh[i] = mfma4(a0[i], b0[i], dh[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int z0 = z0s[i], x0 = x0s[i];
+      float* o = wout + (long)nx * y + x0 + m16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) o[S * (z0 + 4 * q4 + r)] = dh[i][r] * af[i][r];
+    }
   }
 }
 
@@ -1274,6 +1724,30 @@ static int pick_block(long strips) {
     default: { constexpr int BLOCK = 1024; __VA_ARGS__; } break; \
   }
 
+// tiles per wave of the matrix-core kernels (the launch's template argument)
+static int mf_tpw(const Geo& g) { return (int)cdiv((long)(g.nx / 16) * (g.nz / 16), MF_WAVES); }
+#define SR_TPW_SWITCH(TV, ...)                                  \
+  switch (TV) {                                                 \
+    case 1: { constexpr int TPW = 1; __VA_ARGS__; } break;      \
+    case 2: { constexpr int TPW = 2; __VA_ARGS__; } break;      \
+    case 3: { constexpr int TPW = 3; __VA_ARGS__; } break;      \
+    case 4: { constexpr int TPW = 4; __VA_ARGS__; } break;      \
+    case 5: { constexpr int TPW = 5; __VA_ARGS__; } break;      \
+    case 6: { constexpr int TPW = 6; __VA_ARGS__; } break;      \
+    case 7: { constexpr int TPW = 7; __VA_ARGS__; } break;      \
+    case 8: { constexpr int TPW = 8; __VA_ARGS__; } break;      \
+    default: return hipErrorInvalidValue;                       \
+  }
+
+// matrix-core separable kernels: 16-multiple planes that the block's
+// registers can prefetch, taps within the fragment tables, LDS within a CU
+bool psf_mfma_supported(const Geo& g, int nt) {
+  if (g.nx % 16 != 0 || g.nz % 16 != 0 || g.nx < 16 || g.nz < 16) return false;
+  if ((long)g.nx * g.nz > 4L * MF_PF * MF_THREADS) return false;
+  if (mf_tpw(g) > MF_TPW) return false;
+  if (6 * nt > MF_THREADS || std::max(g.px, g.pz) > nt) return false;
+  return sizeof(float) * mf_lds_floats(g, nt, 1) <= kMaxLds;
+}
 
 bool psf_sep_supported(const Geo& g) {
   const int SLv = sep_strip_len(g);
@@ -1289,6 +1763,15 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
   nchunk = (g.ny + Yc - 1) / Yc;
   if (nchunk_used) *nchunk_used = nchunk;
   if (nb <= 0) return hipSuccess;
+  if (separable && psf_mfma_supported(g, t.ntx)) {
+    const int tbp = mf_tap_planes(g, t.ntx, Yc);
+    const size_t lds = sizeof(float) * mf_lds_floats(g, t.ntx, tbp);
+    SR_TPW_SWITCH(mf_tpw(g), {
+      allow_lds(k_psf_fwd_mfma<TPW, MF_WAVES>, lds);
+      k_psf_fwd_mfma<TPW, MF_WAVES><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(work, t, vids, Yc, part, g, tbp);
+    });
+    return hipGetLastError();
+  }
   const int SLv = separable ? sep_strip_len(g) : 0;
   if (separable && SLv > 0 && g.nx % 4 == 0) {
     const int sx = (g.nx + SLv - 1) / SLv, sz = (g.nz + SLv - 1) / SLv;
@@ -1333,6 +1816,16 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
   if (nb <= 0) return hipSuccess;
   const int Yc = (g.ny + nchunk - 1) / nchunk;
   nchunk = (g.ny + Yc - 1) / Yc;
+  if (separable && psf_mfma_supported(g, t.ntx)) {
+    const int tbp = mf_tap_planes(g, t.ntx, Yc);
+    const size_t lds = sizeof(float) * mf_lds_floats(g, t.ntx, tbp);
+    SR_TPW_SWITCH(mf_tpw(g), {
+      allow_lds(k_psf_adj_mfma<TPW, MF_WAVES>, lds);
+      k_psf_adj_mfma<TPW, MF_WAVES><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(views, t, vids, Yc, work, atten,
+                                                                            atten_vstride, g, tbp);
+    });
+    return hipGetLastError();
+  }
   if (separable) {
     const int SLv = sep_strip_len(g);
     if (SLv == 0) return hipErrorInvalidValue;

@@ -196,6 +196,7 @@ struct sr_plan_s {
   int vstride = 0;
   int max_batch = 32;
   int ncu = 256;  // compute units of the plan's device
+  bool mfma = false;  // separable PSF passes on the matrix cores
   size_t bytes = 0;
 
   DBuf mumap, atten, taps_u, taps_w, taps_h, taps_c, rotv, tab_b, tab_w;
@@ -228,8 +229,11 @@ struct sr_plan_s {
     return t;
   }
   long atten_vstride() const { return V; }
-  // planes are split into chunks so that a launch has about 512 blocks
+  // planes are split into chunks so that a launch fills the device: one
+  // block per CU for the matrix-core PSF kernels (their LDS takes a CU),
+  // about 512 blocks otherwise
   int nchunk_fwd(int nb) const {
+    if (mfma) return nchunk_adj(nb);
     const int target = std::max(1, (int)std::ceil(512.0 / std::max(1, nb)));
     return std::min(d.ny, target);
   }
@@ -554,13 +558,15 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
     shared = std::memcmp(psfs, psfs + (long)v * d.ny * pxz, sizeof(float) * d.ny * pxz) == 0;
   p->vstride = shared ? 0 : d.ny;
   const int nstore = shared ? d.ny : d.ny * d.nview;
-  bool sep = d.conv != SR_CONV_DIRECT && sr::psf_sep_supported(p->g);
+  p->nt = round_taps(std::max(d.px, d.pz));
+  const bool mfma_ok = sr::psf_mfma_supported(p->g, p->nt);
+  bool sep = d.conv != SR_CONV_DIRECT && (mfma_ok || sr::psf_sep_supported(p->g));
   std::vector<double> u, w;
   if (sep)
     for (int pl = 0; pl < nstore && sep; ++pl) sep = factor_rank1(psfs + (long)pl * pxz, d.px, d.pz, u, w);
   if (d.conv == SR_CONV_SEPARABLE && !sep) return bail(fail(SR_ERR_ARG, "PSF planes are not rank 1"));
   p->separable = sep ? 1 : 0;
-  p->nt = round_taps(std::max(d.px, d.pz));
+  p->mfma = sep && mfma_ok;
   if (sep) {
     std::vector<float> hu((size_t)nstore * p->nt, 0.f), hw((size_t)nstore * p->nt, 0.f);
     for (int pl = 0; pl < nstore; ++pl) {

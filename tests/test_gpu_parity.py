@@ -342,3 +342,24 @@ def test_per_view_psfs_and_angles(gpu, O, method):
     idx = [7, 2]
     rbs = P32.backproject(views[:, :, idx], index=np.array(idx, np.int32))
     assert maxrel(gpu.backproject(views[:, :, idx], plan, index=idx), rbs) <= PROJ_TOL
+
+
+@pytest.mark.parametrize("nx,nz,px,pz", [(32, 16, 7, 7), (64, 32, 15, 15), (48, 16, 37, 37), (32, 32, 9, 5),
+                                          (16, 16, 3, 21)])
+def test_project_matrix_core_sizes(gpu, O, nx, nz, px, pz):
+    """Planes whose sides are multiples of 16 take the matrix-core separable
+    PSF kernels; parity as for every other path, edge tiles included (PSF
+    half-widths up to beyond one tile)."""
+    nview = 6
+    rng = np.random.default_rng(nx + nz + px)
+    mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = np.asfortranarray(np.repeat(np.asarray(O.psf_gauss(nx, px, pz, 1.0, 5.0), np.float32)[:, :, :, None],
+                                       nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    assert plan.info()["conv"] == "separable"
+    ang = angles32(nview).astype(np.float64)
+    P32 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float32, "fft")
+    image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    assert per_view_maxrel(gpu.project(image, plan), P32.project(image)) <= PROJ_TOL
+    views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    assert maxrel(gpu.backproject(views, plan), P32.backproject(views)) <= PROJ_TOL
