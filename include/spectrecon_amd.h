@@ -91,6 +91,16 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
 int sr_plan_destroy(sr_plan plan);
 /* resident device bytes and the PSF method the plan chose (SR_CONV_*) */
 int sr_plan_info(sr_plan plan, size_t* device_bytes, int32_t* conv_method, int32_t* max_batch);
+/* Device layout of the plan's work volumes, attenuation factors and partial
+ * images (no reference counterpart; the caller's arrays keep the reference
+ * layout either way): SR_LAYOUT_X_FASTEST [.][z][y][x], or
+ * SR_LAYOUT_Z_FASTEST [.][y][x][z], which :two plans with matrix-core PSF
+ * passes take (rotations as z-vector gathers, the rotation adjoint through
+ * plan-time per-target source lists).  The environment variable
+ * SR_LAYOUT=zyx at plan creation keeps the x-fastest layout. */
+#define SR_LAYOUT_X_FASTEST 0
+#define SR_LAYOUT_Z_FASTEST 1
+int sr_plan_layout(sr_plan plan, int32_t* layout);
 /* cap on views processed per launch batch (memory/parallelism trade-off) */
 int sr_plan_set_batch(sr_plan plan, int32_t max_batch);
 /* wait for all work queued on the plan's stream */

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 namespace sr {
 
@@ -67,19 +68,38 @@ hipError_t launch_atten_exact(float* mur, int nb, const Geo& g, float dy, hipStr
 // attenuated already); returns the chunk count actually used through
 // *nchunk_used
 hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
-                          int nchunk, int* nchunk_used, const Geo& g, hipStream_t s);
+                          int nchunk, int* nchunk_used, const Geo& g, hipStream_t s, bool zfast = false);
 // views (or the ML-EM ratio y/(view + b) when epi == EPI_RATIO) = sum of partials
 hipError_t launch_view_reduce(const float* part, int nchunk, const int* vids, int nb, float* out, const float* yobs,
                               const float* bkg, int epi, const Geo& g, hipStream_t s);
 // work[b][:,y,:] = atten[v][:,y,:] .* conv_adj(views[b], psf[v,y])
 hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, const int* vids, int nb, float* work,
-                          const float* atten, long atten_vstride, int nchunk, const Geo& g, hipStream_t s);
+                          const float* atten, long atten_vstride, int nchunk, const Geo& g, hipStream_t s, bool zfast = false);
 
 // ---- elementwise --------------------------------------------------------
 hipError_t launch_fill(float* p, float v, long n, hipStream_t s);
 hipError_t launch_zero_to_inf(float* p, long n, hipStream_t s);
 // dst[i] = src[i] for n 16-byte vectors (grid-stride streaming copy)
 hipError_t launch_copy16(void* dst, const void* src, long n, hipStream_t s);
+// z-fastest work layout ([y][x][z] volumes): the rotations as z-vector gathers
+bool zfast_supported(const Geo& g);
+// per-target (source pixel, weight) lists of the :two rotation of every view
+// (rowptr: nview x (nx*nx + 1) absolute entry indices); false when the
+// entries overflow int32 indexing
+bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, std::vector<int>& rowptr, std::vector<int2>& ent);
+// img [z][y][x] -> out [y][x][z]
+hipError_t launch_to_zfast(const float* img, float* out, const Geo& g, hipStream_t s);
+// out[b] = atten[v] .* rot2(img_t) per view, all [y][x][z]; atten may be null
+hipError_t launch_rot2_fwd_z(const float* img_t, float* out, const float* atten, long atten_vstride,
+                             const RotView* rv, const int* vids, int nb, const Geo& g, hipStream_t s);
+// rotated mu volumes [b][y][x][z] -> attenuation factors, in place
+hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_t s);
+// part[grp] [y][x][z] = sum over the group's views of rot2^T(work[b]) through the per-target lists
+hipError_t launch_rot2_adj_z(const float* work, float* part, const int* rowptr, const int2* ent, const int* vids,
+                             int nb, int vpg, const Geo& g, hipStream_t s);
+// out [z][y][x] = combine(sum_g part[g] [y][x][z]) per ImgMode
+hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,
+                               hipStream_t s);
 // *flag = 0, then 1 if some p[k] is not > 0
 hipError_t launch_flag_nonpositive(const float* p, long n, int* flag, hipStream_t s);
 

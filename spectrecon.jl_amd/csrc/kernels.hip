@@ -5,6 +5,7 @@
 // PSF convolutions, whose reference arithmetic is an FFT, use fmaf).
 #include "kernels.h"
 
+#include <cstring>
 #include <mutex>
 #include <unordered_map>
 
@@ -48,7 +49,7 @@ struct BiCoef {
   int j1, j2;          // 0-based image index of the lower taps
   float w11, w12, w21, w22;
 };
-__device__ __forceinline__ BiCoef bilinear_coef(const RotView& r, int pad, int x, int y) {
+__host__ __device__ __forceinline__ BiCoef bilinear_coef(const RotView& r, int pad, int x, int y) {
   const float f1 = (float)(x + pad + 1), f2 = (float)(y + pad + 1);
   const float x1 = (r.xx * f1 + r.xy * f2) + r.x0;
   const float x2 = (r.yx * f1 + r.yy * f2) + r.y0;
@@ -847,10 +848,11 @@ struct MfTaps {
     size = 6 * tp;  // u, w, u tail, u head, w tail, w head
   }
 };
-// LDS of the matrix-core kernels: the two plane buffers and tbp tap groups
+// LDS of the matrix-core kernels: the two plane buffers (the larger of the
+// two layouts of the first) and tbp tap groups
 __host__ __device__ inline size_t mf_lds_floats(const Geo& g, int nt, int tbp) {
-  return (size_t)g.nz * mf_wx(g.nx, (g.px - 1) / 2, nt) + (size_t)mf_rq(g.nz, nt) * mf_wq(g.nx) +
-         (size_t)tbp * MfTaps(nt).size;
+  const size_t p = std::max((size_t)g.nz * mf_wx(g.nx, (g.px - 1) / 2, nt), (size_t)mf_rq(g.nx, nt) * mf_wq(g.nz));
+  return p + (size_t)mf_rq(g.nz, nt) * mf_wq(g.nx) + (size_t)tbp * MfTaps(nt).size;
 }
 // planes whose tap tables are loaded at once: up to 8, within the LDS
 static int mf_tap_planes(const Geo& g, int nt, int Yc) {
@@ -861,19 +863,27 @@ static int mf_tap_planes(const Geo& g, int nt, int Yc) {
 
 // The wave's TPW tiles form a run t = wave*TPW + i (the last wave's run is
 // clamped to the last tile: recomputed, not stored).
+// (ntx > 0: tiles row-major, x fastest; ntx = -ntz: column-major, z fastest)
 template <int TPW>
 __device__ __forceinline__ void mf_tiles(int wave, int ntile, int ntx, int (&z0)[TPW], int (&x0)[TPW],
                                          bool (&own)[TPW]) {
 #pragma unroll
   for (int i = 0; i < TPW; ++i) {
-    const int t0 = wave * TPW + i, tt = min(t0, ntile - 1), tz = tt / ntx;
+    const int t0 = wave * TPW + i, tt = min(t0, ntile - 1);
     own[i] = t0 < ntile;
-    z0[i] = 16 * tz;
-    x0[i] = 16 * (tt - tz * ntx);
+    if (ntx > 0) {
+      const int tz = tt / ntx;
+      z0[i] = 16 * tz;
+      x0[i] = 16 * (tt - tz * ntx);
+    } else {
+      const int tx = tt / -ntx;
+      x0[i] = 16 * tx;
+      z0[i] = 16 * (tt - tx * -ntx);
+    }
   }
 }
 
-template <int TPW, int NW>
+template <int TPW, int NW, bool ZF>
 __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restrict__ work, Taps t,
                                                             const int* __restrict__ vids, int Yc,
                                                             float* __restrict__ part, Geo g, int TBP) {
@@ -885,16 +895,20 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
   const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
   const int ntx = nx >> 4, ntile = ntx * (nz >> 4);
   const int WX = mf_wx(nx, hx, NT), WQ = mf_wq(nx), RQ = mf_rq(nz, NT), XO = mf_xo(hx);
+  // ZF: the plane arrives as [x][z] (z-fastest work volume) and is staged as
+  // rows x + hx of stride WZ (= 16 mod 64) with zero rows around it
+  const int WZ = mf_wq(nz), RX = mf_rq(nx, NT);
   const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
   const int v = vids[b];
   const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int m16 = lane & 15, q4 = lane >> 4;
+  const int PSZ = ZF ? RX * WZ : nz * WX;
   float* Pz = L;
-  float* Qz = Pz + nz * WX;
+  float* Qz = Pz + PSZ;
   float* s_taps = Qz + RQ * WQ;  // TBP planes of TT.size
-  for (int k = tid; k < nz * WX + RQ * WQ + TBP * TT.size; k += (NW * 64)) L[k] = 0.f;
+  for (int k = tid; k < PSZ + RQ * WQ + TBP * TT.size; k += (NW * 64)) L[k] = 0.f;
 
   const long S = (long)nx * ny;
   const float* wb = work + S * nz * b;
@@ -903,17 +917,26 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
   // (clamped to the last element: the loads and stores stay unconditional,
   // so no wait is forced between them; a clamped slot rewrites that element)
   constexpr int PF = MF_PF * MF_WAVES / NW;
+  const int nz4 = nz >> 2;
   int pfo[PF], pfs[PF];
 #pragma unroll
   for (int k = 0; k < PF; ++k) {
-    const int i = min(tid + k * (NW * 64), nv - 1), z = i / nq, x4 = i - z * nq;
-    pfo[k] = (int)(S * z) + 4 * x4;
-    pfs[k] = z * WX + XO + 4 * x4;
+    const int i = min(tid + k * (NW * 64), nv - 1);
+    if (ZF) {  // element i of the contiguous [x][z] plane
+      const int x = i / nz4, z4 = i - x * nz4;
+      pfo[k] = 4 * i;
+      pfs[k] = (hx + x) * WZ + 4 * z4;
+    } else {
+      const int z = i / nq, x4 = i - z * nq;
+      pfo[k] = (int)(S * z) + 4 * x4;
+      pfs[k] = z * WX + XO + 4 * x4;
+    }
   }
+  const long pstride = ZF ? (long)nx * nz : nx;  // plane y starts at wb + pstride * y
   float4 pf[PF];
   auto prefetch = [&](int y) {
 #pragma unroll
-    for (int k = 0; k < PF; ++k) pf[k] = *reinterpret_cast<const float4*>(wb + (long)nx * y + pfo[k]);
+    for (int k = 0; k < PF; ++k) pf[k] = *reinterpret_cast<const float4*>(wb + pstride * y + pfo[k]);
   };
   // thread tid < 6 NT loads entry tk of table tab (u, w, u tail, u head, w tail, w head)
   // the tap tables of up to TBP (<= 8) planes at a time: all loads of a
@@ -942,7 +965,7 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
   int aoff[TPW], zoff[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; ++i) {
-    aoff[i] = (z0s[i] + m16) * WX + XO - hx + x0s[i] + q4;
+    aoff[i] = ZF ? (x0s[i] + q4) * WZ + z0s[i] + m16 : (z0s[i] + m16) * WX + XO - hx + x0s[i] + q4;
     zoff[i] = (z0s[i] + q4) * WQ + x0s[i] + m16;
   }
   mf4 dv[TPW];
@@ -977,8 +1000,10 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
 #pragma unroll
     for (int i = 0; i < TPW; ++i) {
       const int xn = x0s[i] + m16;
-      const float* prow = Pz + (z0s[i] + m16) * WX + XO;
-      const float pe = q4 == 0 ? prow[0] : q4 == 1 ? prow[nx - 1] : 0.f;
+      // P[z][0] and P[z][nx-1] of the tile's row z = z0 + m
+      const float pe = q4 == 0 ? (ZF ? Pz[hx * WZ + z0s[i] + m16] : Pz[(z0s[i] + m16) * WX + XO])
+                       : q4 == 1 ? (ZF ? Pz[(hx + nx - 1) * WZ + z0s[i] + m16] : Pz[(z0s[i] + m16) * WX + XO + nx - 1])
+                                 : 0.f;
       const float te = q4 == 0 ? tail_at(sut, NT, xn + hx + 1) : q4 == 1 ? head_at(suh, NT, xn + hx - nx) : 0.f;
       dx[i] = mfma4(te, pe, mf4{0.f, 0.f, 0.f, 0.f});
     }
@@ -987,7 +1012,7 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
       auto ld = [&](float (&a)[TPW], float& bb, int s) {
         bb = ub[-4 * s];
 #pragma unroll
-        for (int i = 0; i < TPW; ++i) a[i] = Pz[aoff[i] + 4 * s];
+        for (int i = 0; i < TPW; ++i) a[i] = Pz[aoff[i] + 4 * s * (ZF ? WZ : 1)];
       };
       ld(a0, b0, 0);
       int s = 0;
@@ -1311,7 +1336,7 @@ k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ v
 // 64), Gx (the z adjoint, rows z, column x+hx), tap buffers.  Each wave takes
 // a run of consecutive tiles, so both halves of a 128-byte output line are
 // stored back to back.  grid (nchunk, nb), block 512.
-template <int TPW, int NW>
+template <int TPW, int NW, bool ZF>
 __global__ void __launch_bounds__(NW * 64) k_psf_adj_mfma(const float* __restrict__ views, Taps t,
                                                             const int* __restrict__ vids, int Yc,
                                                             float* __restrict__ work, const float* __restrict__ atten,
@@ -1363,7 +1388,8 @@ __global__ void __launch_bounds__(NW * 64) k_psf_adj_mfma(const float* __restric
   };
   int z0s[TPW], x0s[TPW];
   bool own[TPW];
-  mf_tiles<TPW>(wave, ntile, ntx, z0s, x0s, own);
+  // ZF: runs of tiles go down z, so a wave writes a whole z-column of its x range
+  mf_tiles<TPW>(wave, ntile, ZF ? -(nz >> 4) : ntx, z0s, x0s, own);
   int voff[TPW], goff[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; ++i) {
@@ -1433,10 +1459,20 @@ __global__ void __launch_bounds__(NW * 64) k_psf_adj_mfma(const float* __restric
     // x adjoint: Gx -> H, times the attenuation factors, to the work volume
     float af[TPW][4];
 #pragma unroll
-    for (int i = 0; i < TPW; ++i)
+    for (int i = 0; i < TPW; ++i) {
+      if (ZF) {  // [y][x][z]: the lane's 4 z are contiguous
+        const float4 a4 = av ? *reinterpret_cast<const float4*>(av + ((long)y * nx + x0s[i] + m16) * nz + z0s[i] + 4 * q4)
+                             : make_float4(1.f, 1.f, 1.f, 1.f);
+        af[i][0] = a4.x;
+        af[i][1] = a4.y;
+        af[i][2] = a4.z;
+        af[i][3] = a4.w;
+      } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        af[i][r] = av ? av[S * (z0s[i] + 4 * q4 + r) + (long)nx * y + x0s[i] + m16] : 1.f;
+        for (int r = 0; r < 4; ++r)
+          af[i][r] = av ? av[S * (z0s[i] + 4 * q4 + r) + (long)nx * y + x0s[i] + m16] : 1.f;
+      }
+    }
     const float* bp[TPW];
 #pragma unroll
     for (int i = 0; i < TPW; ++i)
@@ -1474,9 +1510,14 @@ __global__ void __launch_bounds__(NW * 64) k_psf_adj_mfma(const float* __restric
 #pragma unroll
     for (int i = 0; i < TPW; ++i) {
       const int z0 = z0s[i], x0 = x0s[i];
-      float* o = wout + (long)nx * y + x0 + m16;
+      if (ZF) {
+        *reinterpret_cast<float4*>(wout + ((long)y * nx + x0 + m16) * nz + z0 + 4 * q4) =
+            make_float4(dh[i][0] * af[i][0], dh[i][1] * af[i][1], dh[i][2] * af[i][2], dh[i][3] * af[i][3]);
+      } else {
+        float* o = wout + (long)nx * y + x0 + m16;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) o[S * (z0 + 4 * q4 + r)] = dh[i][r] * af[i][r];
+        for (int r = 0; r < 4; ++r) o[S * (z0 + 4 * q4 + r)] = dh[i][r] * af[i][r];
+      }
     }
   }
 }
@@ -1547,6 +1588,171 @@ __global__ void __launch_bounds__(BLOCK) k_psf_adj_2d(const float* __restrict__ 
       wout[o] = av ? val * av[o] : val;
     }
     __syncthreads();
+  }
+}
+
+// ===========================================================================
+// z-fastest work layout: volumes [y][x][z] (a pixel's z-vector contiguous, a
+// PSF plane y one contiguous [x][z] block).  The rotations become gathers
+// over z-vectors: lanes run along z (float4 each), the pixel's coefficients
+// are the same for all of its lanes.  nz % 4 == 0, nz / 4 <= 64.
+// ===========================================================================
+// img [z][y][x] (the caller's layout) -> out [y][x][z]; grid (ceil(N/32), ny), block 256
+__global__ void __launch_bounds__(256) k_to_zfast(const float* __restrict__ img, float* __restrict__ out, Geo g) {
+  __shared__ float t[32 * 257];  // [x][z], z < 256, row stride 257
+  const int N = g.nx, nz = g.nz, x0 = blockIdx.x * 32, y = blockIdx.y, tid = threadIdx.x;
+  const int nxb = min(32, N - x0);
+  const long S = (long)N * g.ny;
+  for (int k = tid; k < nz * 32; k += 256) {
+    const int z = k >> 5, x = k & 31;
+    if (x < nxb) t[x * 257 + z] = img[S * z + (long)N * y + x0 + x];
+  }
+  __syncthreads();
+  float* o = out + ((long)y * N + x0) * nz;
+  for (int k = tid; k < nxb * nz; k += 256) {
+    const int x = k / nz, z = k - x * nz;
+    o[k] = t[x * 257 + z];
+  }
+}
+
+// lane -> (pixel of the wave's group, z quad); PW = 64 / (nz/4) pixels per wave
+struct ZLane {
+  int sub, z4, pw;
+  __device__ ZLane(int lane, int nz4) : pw(64 / nz4) {
+    sub = lane / nz4;
+    z4 = lane - sub * nz4;
+  }
+};
+
+__device__ __forceinline__ float4 ld4_or0(const float* p, bool ok) {
+  return ok ? *reinterpret_cast<const float4*>(p) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// out[b][y][x][:] = atten[v][y][x][:] * bilinear(img_t, c + R(2pi - theta)(x,y) - c)[:]
+// (the :two forward of k_rot2_fwd, same Float32 expression, over z-vectors)
+// grid (ceil(N*N / (4*PW)), nb), block 256
+__global__ void __launch_bounds__(256) k_rot2_fwd_z(const float* __restrict__ img_t, float* __restrict__ out,
+                                                    const float* __restrict__ atten, long atten_vstride,
+                                                    const RotView* __restrict__ rv, const int* __restrict__ vids,
+                                                    Geo g) {
+  const int N = g.nx, nz = g.nz, nz4 = nz >> 2;
+  const ZLane zl(threadIdx.x & 63, nz4);
+  const int pix = (blockIdx.x * 4 + (threadIdx.x >> 6)) * zl.pw + zl.sub;
+  if (zl.sub >= zl.pw || pix >= N * N) return;
+  const int b = blockIdx.y, v = vids[b];
+  const RotView r = rv[v];
+  const int x = pix % N, y = pix / N, zo = 4 * zl.z4;
+  const long V = (long)N * N * nz;
+  float4 val;
+  if (r.kind == 0) {
+    val = *reinterpret_cast<const float4*>(img_t + (long)pix * nz + zo);
+  } else {
+    const BiCoef c = bilinear_coef(r, g.pad, x, y);
+    // taps outside the image read 0 (the zero pad); their addresses are clamped in range
+    const bool x0 = (unsigned)c.j1 < (unsigned)N, x1 = (unsigned)(c.j1 + 1) < (unsigned)N;
+    const bool y0 = (unsigned)c.j2 < (unsigned)N, y1 = (unsigned)(c.j2 + 1) < (unsigned)N;
+    const int ja = clampi(c.j1, 0, N - 1), jb = clampi(c.j1 + 1, 0, N - 1);
+    const long ra = (long)clampi(c.j2, 0, N - 1) * N, rb = (long)clampi(c.j2 + 1, 0, N - 1) * N;
+    const float4 s11 = ld4_or0(img_t + (ra + ja) * nz + zo, x0 && y0);
+    const float4 s12 = ld4_or0(img_t + (ra + jb) * nz + zo, x1 && y0);
+    const float4 s21 = ld4_or0(img_t + (rb + ja) * nz + zo, x0 && y1);
+    const float4 s22 = ld4_or0(img_t + (rb + jb) * nz + zo, x1 && y1);
+    // a = s11 w11 + s12 w12, b = s21 w11 + s22 w12, out = a w21 + b w22
+    val.x = (s11.x * c.w11 + s12.x * c.w12) * c.w21 + (s21.x * c.w11 + s22.x * c.w12) * c.w22;
+    val.y = (s11.y * c.w11 + s12.y * c.w12) * c.w21 + (s21.y * c.w11 + s22.y * c.w12) * c.w22;
+    val.z = (s11.z * c.w11 + s12.z * c.w12) * c.w21 + (s21.z * c.w11 + s22.z * c.w12) * c.w22;
+    val.w = (s11.w * c.w11 + s12.w * c.w12) * c.w21 + (s21.w * c.w11 + s22.w * c.w12) * c.w22;
+  }
+  const long o = (long)pix * nz + zo;
+  if (atten) {
+    const float4 a = *reinterpret_cast<const float4*>(atten + atten_vstride * v + o);
+    val.x = val.x * a.x;
+    val.y = val.y * a.y;
+    val.z = val.z * a.z;
+    val.w = val.w * a.w;
+  }
+  *reinterpret_cast<float4*>(out + V * b + o) = val;
+}
+
+// Attenuation factors in place on rotated mu volumes [b][y][x][z], the same
+// summation order as k_atten_exact (src/project.jl:103-113).  Each thread owns
+// a (x, z) column, staged in LDS.  grid (ceil(N*nz / 256), nb), block 256,
+// LDS ny * 256 floats.
+__global__ void __launch_bounds__(256) k_atten_z(float* __restrict__ mur, Geo g, float dy) {
+  extern __shared__ float sc[];
+  const int N = g.nx, ny = g.ny, nz = g.nz, tid = threadIdx.x;
+  const long cols = (long)N * nz, V = cols * ny;
+  const long c = (long)blockIdx.x * 256 + tid;
+  float* p = mur + V * blockIdx.y;
+  const bool ok = c < cols;
+  for (int y = 0; y < ny; ++y) sc[y * 256 + tid] = ok ? p[y * cols + c] : 0.f;
+  const float mdy = -dy;
+  if (ok)
+    for (int y = 0; y < ny; ++y) {
+      float e = -0.5f * sc[y * 256 + tid];
+      for (int j = 0; j <= y; ++j) e = e + sc[j * 256 + tid];
+      e = e * mdy;
+      p[y * cols + c] = expf(e);
+    }
+}
+
+// Adjoint of k_rot2_fwd_z as a gather: target pixel j collects w * work[i][:]
+// over the (source i, weight w) pairs of its plan-time list (the forward's
+// coefficients, w = the product of its two 1-D weights), summed over the
+// group's views in the list's fixed order.  part[grp] is [y][x][z].
+// grid (ceil(N*N / (4*PW)), ngroups), block 256
+__global__ void __launch_bounds__(256) k_rot2_adj_z(const float* __restrict__ work, float* __restrict__ part,
+                                                    const int* __restrict__ rowptr, const int2* __restrict__ ent,
+                                                    const int* __restrict__ vids, int nb, int vpg, Geo g) {
+  const int N = g.nx, NN = N * N, nz = g.nz, nz4 = nz >> 2;
+  const ZLane zl(threadIdx.x & 63, nz4);
+  const int j = (blockIdx.x * 4 + (threadIdx.x >> 6)) * zl.pw + zl.sub;
+  if (zl.sub >= zl.pw || j >= NN) return;
+  const int grp = blockIdx.y, b0 = grp * vpg, b1 = min(nb, b0 + vpg), zo = 4 * zl.z4;
+  const long V = (long)NN * nz;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int b = b0; b < b1; ++b) {
+    const long rb = (long)vids[b] * (NN + 1);
+    const int e0 = rowptr[rb + j], e1 = rowptr[rb + j + 1];
+    const float* src = work + V * b + zo;
+    for (int e = e0; e < e1; ++e) {
+      const int2 en = ent[e];
+      const float w = __int_as_float(en.y);
+      const float4 p = *reinterpret_cast<const float4*>(src + (long)en.x * nz);
+      acc.x = acc.x + w * p.x;
+      acc.y = acc.y + w * p.y;
+      acc.z = acc.z + w * p.z;
+      acc.w = acc.w + w * p.w;
+    }
+  }
+  *reinterpret_cast<float4*>(part + V * grp + (long)j * nz + zo) = acc;
+}
+
+// Image reduction from z-fastest partials: part[g] [y][x][z] -> out [z][y][x]
+// (the caller's layout) with the modes of k_img_reduce, groups summed in the
+// same order.  grid (ceil(N/32), ny), block 256
+__global__ void __launch_bounds__(256) k_img_reduce_z(const float* __restrict__ part, int ng, float* __restrict__ out,
+                                                      const float* __restrict__ asum, int mode, Geo g) {
+  __shared__ float t[32 * 257];
+  const int N = g.nx, nz = g.nz, x0 = blockIdx.x * 32, y = blockIdx.y, tid = threadIdx.x;
+  const int nxb = min(32, N - x0);
+  const long V = (long)N * g.ny * nz, S = (long)N * g.ny;
+  const float* pp = part + ((long)y * N + x0) * nz;
+  for (int k = tid; k < nxb * nz; k += 256) {
+    float sv = pp[k];
+    for (int gg = 1; gg < ng; ++gg) sv += pp[(long)gg * V + k];
+    const int x = k / nz, z = k - x * nz;
+    t[x * 257 + z] = sv;
+  }
+  __syncthreads();
+  for (int k = tid; k < nz * 32; k += 256) {
+    const int z = k >> 5, x = k & 31;
+    if (x >= nxb) continue;
+    const long o = S * z + (long)N * y + x0 + x;
+    const float sv = t[x * 257 + z];
+    if (mode == IMG_WRITE) out[o] = sv;
+    else if (mode == IMG_ACCUM) out[o] = out[o] + sv;
+    else out[o] = out[o] * (sv / asum[o]);  // out .*= back ./ asum, src/ml-os-em.jl:46
   }
 }
 
@@ -1758,7 +1964,7 @@ bool psf_sep_supported(const Geo& g) {
 }
 
 hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
-                          int nchunk, int* nchunk_used, const Geo& g, hipStream_t s) {
+                          int nchunk, int* nchunk_used, const Geo& g, hipStream_t s, bool zfast) {
   const int Yc = (g.ny + nchunk - 1) / nchunk;
   nchunk = (g.ny + Yc - 1) / Yc;
   if (nchunk_used) *nchunk_used = nchunk;
@@ -1767,8 +1973,14 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
     const int tbp = mf_tap_planes(g, t.ntx, Yc);
     const size_t lds = sizeof(float) * mf_lds_floats(g, t.ntx, tbp);
     SR_TPW_SWITCH(mf_tpw(g), {
-      allow_lds(k_psf_fwd_mfma<TPW, MF_WAVES>, lds);
-      k_psf_fwd_mfma<TPW, MF_WAVES><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(work, t, vids, Yc, part, g, tbp);
+      if (zfast) {
+        allow_lds(k_psf_fwd_mfma<TPW, MF_WAVES, true>, lds);
+        k_psf_fwd_mfma<TPW, MF_WAVES, true><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(work, t, vids, Yc, part, g, tbp);
+      } else {
+        allow_lds(k_psf_fwd_mfma<TPW, MF_WAVES, false>, lds);
+        k_psf_fwd_mfma<TPW, MF_WAVES, false><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(work, t, vids, Yc, part, g,
+                                                                                      tbp);
+      }
     });
     return hipGetLastError();
   }
@@ -1789,7 +2001,7 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
     }));
     return hipGetLastError();
   }
-  if (separable) return hipErrorInvalidConfiguration;
+  if (separable || zfast) return hipErrorInvalidConfiguration;
   const int nxs = (g.nx + 7) / 8;
   const int WX = odd_up(8 * nxs + t.ntx - 1);
   const long strips = (long)g.nz * nxs;
@@ -1812,7 +2024,8 @@ hipError_t launch_view_reduce(const float* part, int nchunk, const int* vids, in
 }
 
 hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, const int* vids, int nb, float* work,
-                          const float* atten, long atten_vstride, int nchunk, const Geo& g, hipStream_t s) {
+                          const float* atten, long atten_vstride, int nchunk, const Geo& g, hipStream_t s,
+                          bool zfast) {
   if (nb <= 0) return hipSuccess;
   const int Yc = (g.ny + nchunk - 1) / nchunk;
   nchunk = (g.ny + Yc - 1) / Yc;
@@ -1820,12 +2033,19 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
     const int tbp = mf_tap_planes(g, t.ntx, Yc);
     const size_t lds = sizeof(float) * mf_lds_floats(g, t.ntx, tbp);
     SR_TPW_SWITCH(mf_tpw(g), {
-      allow_lds(k_psf_adj_mfma<TPW, MF_WAVES>, lds);
-      k_psf_adj_mfma<TPW, MF_WAVES><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(views, t, vids, Yc, work, atten,
-                                                                            atten_vstride, g, tbp);
+      if (zfast) {
+        allow_lds(k_psf_adj_mfma<TPW, MF_WAVES, true>, lds);
+        k_psf_adj_mfma<TPW, MF_WAVES, true><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(views, t, vids, Yc, work, atten,
+                                                                                     atten_vstride, g, tbp);
+      } else {
+        allow_lds(k_psf_adj_mfma<TPW, MF_WAVES, false>, lds);
+        k_psf_adj_mfma<TPW, MF_WAVES, false><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(views, t, vids, Yc, work, atten,
+                                                                                      atten_vstride, g, tbp);
+      }
     });
     return hipGetLastError();
   }
+  if (zfast) return hipErrorInvalidConfiguration;
   if (separable) {
     const int SLv = sep_strip_len(g);
     if (SLv == 0) return hipErrorInvalidValue;
@@ -1848,6 +2068,102 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
       k_psf_adj_2d<NT, 256><<<dim3(nchunk, nb), 256, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride, g);
     });
   }
+  return hipGetLastError();
+}
+
+bool zfast_supported(const Geo& g) {
+  return g.nz % 4 == 0 && g.nz / 4 <= 64 && sizeof(float) * g.ny * 256 <= kMaxLds;
+}
+
+// Per-target source lists of the :two forward (the adjoint's gather form):
+// for target pixel q of view v, entries rowptr[v*(NN+1) + q] ..
+// rowptr[v*(NN+1) + q + 1] of ent hold (source pixel p, weight bits) with
+// weight = w1 * w2 of the forward's bilinear coefficients (bilinear_coef,
+// evaluated here on the host with the same Float32 operations; the map also
+// for a kind-0 view, as the scatter adjoint does), ordered by p;
+// zero weights and taps in the zero pad are left out.  Pixel index = y*N + x.
+bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, std::vector<int>& rowptr, std::vector<int2>& ent) {
+  const int N = g.nx;
+  const long NN = (long)N * N;
+  rowptr.assign((size_t)nview * (NN + 1), 0);
+  ent.clear();
+  std::vector<int> cur(NN);
+  struct Tap {
+    int q;
+    float w;
+  };
+  auto taps = [&](const RotView& r, int pix, Tap (&t)[4]) {
+    int n = 0;  // the adjoint always goes through the map (kind 0 only skips it forward)
+    const BiCoef c = bilinear_coef(r, g.pad, pix % N, pix / N);
+    const int jx[2] = {c.j1, c.j1 + 1}, jy[2] = {c.j2, c.j2 + 1};
+    const float wx[2] = {c.w11, c.w12}, wy[2] = {c.w21, c.w22};
+    for (int a = 0; a < 2; ++a)
+      for (int bb = 0; bb < 2; ++bb) {
+        const float w = wx[bb] * wy[a];
+        if (jx[bb] < 0 || jx[bb] >= N || jy[a] < 0 || jy[a] >= N || w == 0.f) continue;
+        t[n++] = {jy[a] * N + jx[bb], w};
+      }
+    return n;
+  };
+  for (int v = 0; v < nview; ++v) {
+    int* rp = rowptr.data() + (size_t)v * (NN + 1);
+    Tap t[4];
+    for (long pix = 0; pix < NN; ++pix) {
+      const int n = taps(rv[v], (int)pix, t);
+      for (int k = 0; k < n; ++k) rp[t[k].q + 1]++;
+    }
+    const long base = (long)ent.size();
+    rp[0] = (int)base;
+    for (long q = 0; q < NN; ++q) rp[q + 1] += rp[q];
+    if ((long)rp[NN] >= (long)INT32_MAX - 4 * NN) return false;
+    ent.resize(rp[NN]);
+    for (long q = 0; q < NN; ++q) cur[q] = rp[q];
+    for (long pix = 0; pix < NN; ++pix) {
+      const int n = taps(rv[v], (int)pix, t);
+      for (int k = 0; k < n; ++k) {
+        int2 e;
+        e.x = (int)pix;
+        std::memcpy(&e.y, &t[k].w, sizeof(float));
+        ent[cur[t[k].q]++] = e;
+      }
+    }
+  }
+  return true;
+}
+
+hipError_t launch_to_zfast(const float* img, float* out, const Geo& g, hipStream_t s) {
+  k_to_zfast<<<dim3(cdiv(g.nx, 32), g.ny), 256, 0, s>>>(img, out, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_rot2_fwd_z(const float* img_t, float* out, const float* atten, long atten_vstride,
+                             const RotView* rv, const int* vids, int nb, const Geo& g, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  const int pw = 64 / (g.nz / 4);
+  k_rot2_fwd_z<<<dim3(cdiv((long)g.nx * g.nx, 4L * pw), nb), 256, 0, s>>>(img_t, out, atten, atten_vstride, rv, vids, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  const size_t lds = sizeof(float) * g.ny * 256;
+  if (lds > kMaxLds) return hipErrorInvalidConfiguration;
+  allow_lds(k_atten_z, lds);
+  k_atten_z<<<dim3(cdiv((long)g.nx * g.nz, 256), nb), 256, lds, s>>>(mur, g, dy);
+  return hipGetLastError();
+}
+
+hipError_t launch_rot2_adj_z(const float* work, float* part, const int* rowptr, const int2* ent, const int* vids,
+                             int nb, int vpg, const Geo& g, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  const int pw = 64 / (g.nz / 4), ng = (nb + vpg - 1) / vpg;
+  k_rot2_adj_z<<<dim3(cdiv((long)g.nx * g.nx, 4L * pw), ng), 256, 0, s>>>(work, part, rowptr, ent, vids, nb, vpg, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,
+                               hipStream_t s) {
+  k_img_reduce_z<<<dim3(cdiv(g.nx, 32), g.ny), 256, 0, s>>>(part, ngroups, out, asum, mode, g);
   return hipGetLastError();
 }
 

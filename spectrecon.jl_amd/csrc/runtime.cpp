@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -197,6 +198,10 @@ struct sr_plan_s {
   int max_batch = 32;
   int ncu = 256;  // compute units of the plan's device
   bool mfma = false;  // separable PSF passes on the matrix cores
+  // z-fastest work layout: work, attenuation and partial images are
+  // [.][y][x][z]; the :two rotations are z-vector gathers and the rotation
+  // adjoint reads plan-time per-target source lists (zrow, zent)
+  bool zfast = false;
   size_t bytes = 0;
 
   DBuf mumap, atten, taps_u, taps_w, taps_h, taps_c, rotv, tab_b, tab_w;
@@ -204,6 +209,7 @@ struct sr_plan_s {
   DBuf st_img, st_views, st_aux1, st_aux2, st_x0;  // staging for host-pointer calls and EM temporaries
   DBuf em_ratio, em_back, em_asum;
   DBuf flag;  // one int: device-side argument checks
+  DBuf zimg, zrow, zent;  // z-fastest image copy, per-target rotation lists
   std::vector<int> h_vids;
 
   // per-kernel event timing (sr_plan_set_timing)
@@ -246,6 +252,26 @@ struct sr_plan_s {
     if (d.interpmeth == SR_INTERP_ONE) {
       *vpg = 1;
       return nb;
+    }
+    if (zfast) {
+      // gather kernel: ceil(nx*nx / (4 * pixels per wave)) blocks of 4 waves
+      // per group, about 8 blocks per CU; a thread walks the group's views
+      // (~2 units each), a group's partial image costs ~3 units to write and
+      // reduce
+      const int pw = 64 / (d.nz / 4);
+      const long blocks = ((long)d.nx * d.nx + 4L * pw - 1) / (4L * pw), slots = (long)ncu * 8;
+      long best = -1;
+      int bv = nb;
+      for (int v = 1; v <= nb; ++v) {
+        const long ng = (nb + v - 1) / v;
+        const long cost = 2 * v * ((blocks * ng + slots - 1) / slots) + 3 * ng;
+        if (best < 0 || cost < best) {
+          best = cost;
+          bv = v;
+        }
+      }
+      *vpg = bv;
+      return (nb + bv - 1) / bv;
     }
     // time ~ (rounds of blocks over the CUs) x (views per block); each group
     // also writes and re-reads one partial image (about 1/16 of a view's time)
@@ -344,11 +370,20 @@ int upload_vids(sr_plan p, const int32_t* index, int32_t nindex) {
 int run_project(sr_plan p, float* views, const float* image, const int* d_vids, int n, const float* yobs,
                 const float* bkg) {
   const sr::Taps t = p->taps();
+  if (p->zfast && n > 0) {
+    KTimer kt(p, sr::K_ROT_FWD);
+    HIP_TRY(sr::launch_to_zfast(image, p->zimg.as<float>(), p->g, p->stream));
+  }
   for (int s0 = 0; s0 < n; s0 += p->max_batch) {
     const int nb = std::min(p->max_batch, n - s0);
     int rc;
     if ((rc = ensure_batch_buffers(p, nb))) return rc;
-    {
+    if (p->zfast) {
+      KTimer kt(p, sr::K_ROT_FWD);
+      HIP_TRY(sr::launch_rot2_fwd_z(p->zimg.as<float>(), p->work.as<float>(), p->atten.as<float>(),
+                                    p->atten_vstride(), p->rotv.as<sr::RotView>(), d_vids + s0, nb, p->g,
+                                    p->stream));
+    } else {
       KTimer kt(p, sr::K_ROT_FWD);
       HIP_TRY(sr::launch_rot_fwd(p->d.interpmeth, image, p->work.as<float>(), p->atten.as<float>(),
                                  p->atten_vstride(), p->rotv.as<sr::RotView>(), d_vids + s0, nb, p->tab_b.as<int>(),
@@ -358,7 +393,7 @@ int run_project(sr_plan p, float* views, const float* image, const int* d_vids, 
     {
       KTimer kt(p, sr::K_PSF_FWD);
       HIP_TRY(sr::launch_psf_fwd(p->separable, p->work.as<float>(), t, d_vids + s0, nb, p->fpart.as<float>(),
-                                 p->nchunk_fwd(nb), &nchunk, p->g, p->stream));
+                                 p->nchunk_fwd(nb), &nchunk, p->g, p->stream, p->zfast));
     }
     {
       KTimer kt(p, sr::K_VIEW_REDUCE);
@@ -392,18 +427,29 @@ int run_backproject(sr_plan p, float* image, const float* views, const int* d_vi
     {
       KTimer kt(p, sr::K_PSF_ADJ);
       HIP_TRY(sr::launch_psf_adj(p->separable, views + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(),
-                                 p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(nb), p->g, p->stream));
+                                 p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(nb), p->g, p->stream,
+                                 p->zfast));
     }
     int vpg;
     const int ng = p->groups_adj(nb, &vpg);
+    int m = (s0 == 0) ? sr::IMG_WRITE : sr::IMG_ACCUM;
+    if (mode == sr::IMG_EM && single) m = sr::IMG_EM;
+    if (p->zfast) {
+      {
+        KTimer kt(p, sr::K_ROT_ADJ);
+        HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zrow.as<int>(),
+                                      p->zent.as<int2>(), d_vids + s0, nb, vpg, p->g, p->stream));
+      }
+      KTimer kt(p, sr::K_IMG_REDUCE);
+      HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), ng, target, asum, m, p->g, p->stream));
+      continue;
+    }
     {
       KTimer kt(p, sr::K_ROT_ADJ);
       HIP_TRY(sr::launch_rot_adj(p->d.interpmeth, p->work.as<float>(), p->ipart.as<float>(),
                                  p->rotv.as<sr::RotView>(), d_vids + s0, nb, vpg, p->tab_b.as<int>(),
                                  p->tab_w.as<float>(), p->g, p->stream));
     }
-    int m = (s0 == 0) ? sr::IMG_WRITE : sr::IMG_ACCUM;
-    if (mode == sr::IMG_EM && single) m = sr::IMG_EM;
     KTimer kt(p, sr::K_IMG_REDUCE);
     HIP_TRY(sr::launch_img_reduce(p->ipart.as<float>(), ng, target, asum, p->V, m, p->stream));
   }
@@ -606,6 +652,27 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   HIP_TRY(hipMemcpy(p->tab_b.p, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->tab_w.p, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice));
 
+  // --- work layout: z-fastest for :two with the matrix-core PSF passes
+  // (SR_LAYOUT=zyx in the environment keeps the x-fastest layout) ---
+  {
+    const char* lay = std::getenv("SR_LAYOUT");
+    const bool force_zyx = lay && std::strcmp(lay, "zyx") == 0;
+    p->zfast = !force_zyx && p->mfma && d.interpmeth == SR_INTERP_TWO && sr::zfast_supported(p->g);
+  }
+  if (p->zfast) {
+    std::vector<int> rowptr;
+    std::vector<int2> ent;
+    p->zfast = sr::build_rot2_lists(rv.data(), d.nview, p->g, rowptr, ent);
+    if (p->zfast) {
+      if ((rc = p->zrow.ensure(sizeof(int) * rowptr.size())) ||
+          (rc = p->zent.ensure(sizeof(int2) * std::max<size_t>(1, ent.size()))) ||
+          (rc = p->zimg.ensure(sizeof(float) * p->V)))
+        return bail(rc);
+      HIP_TRY(hipMemcpy(p->zrow.p, rowptr.data(), sizeof(int) * rowptr.size(), hipMemcpyHostToDevice));
+      if (!ent.empty()) HIP_TRY(hipMemcpy(p->zent.p, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+    }
+  }
+
   // --- mu-map and attenuation factors for every view ---
   if ((rc = p->mumap.ensure(sizeof(float) * p->V))) return bail(rc);
   HIP_TRY(hipMemcpy(p->mumap.p, mumap, sizeof(float) * p->V, hipMemcpyHostToDevice));
@@ -618,6 +685,13 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   for (int v0 = 0; v0 < d.nview; v0 += chunk) {
     const int nb = std::min(chunk, d.nview - v0);
     float* dst = p->atten.as<float>() + p->V * v0;
+    if (p->zfast) {  // [v][y][x][z]
+      if (v0 == 0) HIP_TRY(sr::launch_to_zfast(p->mumap.as<float>(), p->zimg.as<float>(), p->g, p->stream));
+      HIP_TRY(sr::launch_rot2_fwd_z(p->zimg.as<float>(), dst, nullptr, 0, p->rotv.as<sr::RotView>(),
+                                    p->vids.as<int>() + v0, nb, p->g, p->stream));
+      HIP_TRY(sr::launch_atten_z(dst, nb, p->g, d.dy, p->stream));
+      continue;
+    }
     HIP_TRY(sr::launch_rot_fwd(d.interpmeth, p->mumap.as<float>(), dst, nullptr, 0, p->rotv.as<sr::RotView>(),
                                p->vids.as<int>() + v0, nb, p->tab_b.as<int>(), p->tab_w.as<float>(), p->g,
                                p->stream));
@@ -625,7 +699,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   }
   HIP_TRY(hipStreamSynchronize(p->stream));
   p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->taps_c.n + p->rotv.n + p->tab_b.n +
-             p->tab_w.n;
+             p->tab_w.n + p->zimg.n + p->zrow.n + p->zent.n;
   *out = guard.release();
   return SR_OK;
 }
@@ -671,6 +745,12 @@ int sr_plan_info(sr_plan p, size_t* device_bytes, int32_t* conv_method, int32_t*
                     p->st_aux2.n + p->st_x0.n + p->em_ratio.n + p->em_back.n + p->em_asum.n + p->ones.n;
   if (conv_method) *conv_method = p->separable ? SR_CONV_SEPARABLE : SR_CONV_DIRECT;
   if (max_batch) *max_batch = p->max_batch;
+  return SR_OK;
+}
+
+int sr_plan_layout(sr_plan p, int32_t* layout) {
+  if (!p || !layout) return fail(SR_ERR_ARG, "null argument");
+  *layout = p->zfast ? SR_LAYOUT_Z_FASTEST : SR_LAYOUT_X_FASTEST;
   return SR_OK;
 }
 

@@ -159,8 +159,10 @@ class SPECTplan:
     def info(self):
         nbytes, conv, batch = C.c_size_t(), C.c_int32(), C.c_int32()
         check(lib().sr_plan_info(self.handle, C.byref(nbytes), C.byref(conv), C.byref(batch)))
+        layout = C.c_int32()
+        check(lib().sr_plan_layout(self.handle, C.byref(layout)))
         return {"device_bytes": nbytes.value, "conv": {1: "direct", 2: "separable"}[conv.value],
-                "max_batch": batch.value}
+                "max_batch": batch.value, "layout": {0: "x-fastest", 1: "z-fastest"}[layout.value]}
 
     def set_batch(self, n):
         check(lib().sr_plan_set_batch(self.handle, n))

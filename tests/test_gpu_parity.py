@@ -344,12 +344,16 @@ def test_per_view_psfs_and_angles(gpu, O, method):
     assert maxrel(gpu.backproject(views[:, :, idx], plan, index=idx), rbs) <= PROJ_TOL
 
 
+@pytest.mark.parametrize("layout", ["auto", "zyx"])
 @pytest.mark.parametrize("nx,nz,px,pz", [(32, 16, 7, 7), (64, 32, 15, 15), (48, 16, 37, 37), (32, 32, 9, 5),
-                                          (16, 16, 3, 21)])
-def test_project_matrix_core_sizes(gpu, O, nx, nz, px, pz):
+                                          (16, 16, 3, 21), (32, 48, 5, 9)])
+def test_project_matrix_core_sizes(gpu, O, monkeypatch, nx, nz, px, pz, layout):
     """Planes whose sides are multiples of 16 take the matrix-core separable
-    PSF kernels; parity as for every other path, edge tiles included (PSF
-    half-widths up to beyond one tile)."""
+    PSF kernels, and (:two) the z-fastest work layout unless SR_LAYOUT=zyx;
+    parity as for every other path, edge tiles included (PSF half-widths up
+    to beyond one tile), and the adjoint identity."""
+    if layout == "zyx":
+        monkeypatch.setenv("SR_LAYOUT", "zyx")
     nview = 6
     rng = np.random.default_rng(nx + nz + px)
     mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
@@ -357,9 +361,18 @@ def test_project_matrix_core_sizes(gpu, O, nx, nz, px, pz):
                                        nview, 3))
     plan = gpu.SPECTplan(mumap, psfs, 0.48)
     assert plan.info()["conv"] == "separable"
+    assert plan.info()["layout"] == ("z-fastest" if layout == "auto" else "x-fastest")
     ang = angles32(nview).astype(np.float64)
     P32 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float32, "fft")
     image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
-    assert per_view_maxrel(gpu.project(image, plan), P32.project(image)) <= PROJ_TOL
+    g = gpu.project(image, plan)
+    assert per_view_maxrel(g, P32.project(image)) <= PROJ_TOL
     views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
-    assert maxrel(gpu.backproject(views, plan), P32.backproject(views)) <= PROJ_TOL
+    gb = gpu.backproject(views, plan)
+    assert maxrel(gb, P32.backproject(views)) <= PROJ_TOL
+    idx = [4, 1]
+    rbs = P32.backproject(views[:, :, idx], index=np.array(idx, np.int32))
+    assert maxrel(gpu.backproject(views[:, :, idx], plan, index=idx), rbs) <= PROJ_TOL
+    a = float(np.vdot(views.astype(np.float64), g.astype(np.float64)))
+    b = float(np.vdot(image.astype(np.float64), gb.astype(np.float64)))
+    assert abs(a - b) <= 1e-5 * abs(a)
