@@ -107,10 +107,14 @@ int sr_plan_set_batch(sr_plan plan, int32_t max_batch);
 int sr_plan_synchronize(sr_plan plan);
 /* Per-kernel timing with HIP events on the plan's stream (no reference
  * counterpart; for measurement).  While enabled, every kernel launch of the
- * plan is bracketed by events; sr_plan_get_timing returns the summed device
- * time and launch count of one kernel class since timing was (re)enabled:
- * 0 rotate, 1 PSF forward, 2 view reduce, 3 PSF adjoint, 4 rotate adjoint,
- * 5 image reduce / EM update. */
+ * chosen classes is bracketed by events; sr_plan_get_timing returns the
+ * summed device time and launch count of one kernel class since timing was
+ * (re)enabled: 0 rotate, 1 PSF forward, 2 view reduce, 3 PSF adjoint,
+ * 4 rotate adjoint, 5 image reduce / EM update.  `enable`: 0 off, 1 all
+ * classes, otherwise an or of SR_TIME_CLASS(SR_K_*) bits (bracketing only
+ * the classes of interest keeps the event overhead off the others). */
+#define SR_TIME_CLASS(k) (2 << (k))
+#define SR_TIME_ALL 0x7e
 #define SR_K_ROT_FWD 0
 #define SR_K_PSF_FWD 1
 #define SR_K_VIEW_REDUCE 2

@@ -83,10 +83,10 @@ hipError_t launch_zero_to_inf(float* p, long n, hipStream_t s);
 hipError_t launch_copy16(void* dst, const void* src, long n, hipStream_t s);
 // z-fastest work layout ([y][x][z] volumes): the rotations as z-vector gathers
 bool zfast_supported(const Geo& g);
-// per-target (source pixel, weight) lists of the :two rotation of every view
-// (rowptr: nview x (nx*nx + 1) absolute entry indices); false when the
-// entries overflow int32 indexing
-bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, std::vector<int>& rowptr, std::vector<int2>& ent);
+// per-target (source pixel, weight) lists of the :two rotation of every view,
+// padded to *K slots ([view][K][nx*nx]); false when a list exceeds kMaxEll
+constexpr int kMaxEll = 8;
+bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::vector<int2>& ent);
 // img [z][y][x] -> out [y][x][z]
 hipError_t launch_to_zfast(const float* img, float* out, const Geo& g, hipStream_t s);
 // out[b] = atten[v] .* rot2(img_t) per view, all [y][x][z]; atten may be null
@@ -94,9 +94,10 @@ hipError_t launch_rot2_fwd_z(const float* img_t, float* out, const float* atten,
                              const RotView* rv, const int* vids, int nb, const Geo& g, hipStream_t s);
 // rotated mu volumes [b][y][x][z] -> attenuation factors, in place
 hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_t s);
-// part[grp] [y][x][z] = sum over the group's views of rot2^T(work[b]) through the per-target lists
-hipError_t launch_rot2_adj_z(const float* work, float* part, const int* rowptr, const int2* ent, const int* vids,
-                             int nb, int vpg, const Geo& g, hipStream_t s);
+// part [y][x][z] = sum over the batch's views of rot2^T(work[b]) through the
+// K-slot per-target lists
+hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
+                             const Geo& g, hipStream_t s);
 // out [z][y][x] = combine(sum_g part[g] [y][x][z]) per ImgMode
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,
                                hipStream_t s);

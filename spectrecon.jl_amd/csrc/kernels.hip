@@ -1615,6 +1615,14 @@ __global__ void __launch_bounds__(256) k_to_zfast(const float* __restrict__ img,
   }
 }
 
+// Work-groups go round-robin over the 8 XCDs, each with its own L2.  A grid
+// of 8*C blocks is remapped so that XCD k runs logical blocks [k*C, (k+1)*C):
+// neighbouring pixels, which share their gather sources, stay in one L2.
+// Logical blocks >= the real count exit at once.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int C) { return (int)(blockIdx.x % kXcds) * C + (int)(blockIdx.x / kXcds); }
+static inline int xcd_chunk(long blocks) { return (int)((blocks + kXcds - 1) / kXcds); }
+
 // lane -> (pixel of the wave's group, z quad); PW = 64 / (nz/4) pixels per wave
 struct ZLane {
   int sub, z4, pw;
@@ -1630,16 +1638,20 @@ __device__ __forceinline__ float4 ld4_or0(const float* p, bool ok) {
 
 // out[b][y][x][:] = atten[v][y][x][:] * bilinear(img_t, c + R(2pi - theta)(x,y) - c)[:]
 // (the :two forward of k_rot2_fwd, same Float32 expression, over z-vectors)
-// grid (ceil(N*N / (4*PW)), nb), block 256
+// Logical block (b, pixel block): views outermost, so an XCD takes whole
+// views in turn.  grid 8 * xcd_chunk(nb * PB), PB = ceil(N*N / (4*PW)); block 256
 __global__ void __launch_bounds__(256) k_rot2_fwd_z(const float* __restrict__ img_t, float* __restrict__ out,
                                                     const float* __restrict__ atten, long atten_vstride,
                                                     const RotView* __restrict__ rv, const int* __restrict__ vids,
-                                                    Geo g) {
+                                                    int nb, int PB, int C, Geo g) {
+  const int lb = xcd_block(C);
+  if (lb >= nb * PB) return;
+  const int b = lb / PB, blk = lb - b * PB;
   const int N = g.nx, nz = g.nz, nz4 = nz >> 2;
   const ZLane zl(threadIdx.x & 63, nz4);
-  const int pix = (blockIdx.x * 4 + (threadIdx.x >> 6)) * zl.pw + zl.sub;
+  const int pix = (blk * 4 + (threadIdx.x >> 6)) * zl.pw + zl.sub;
   if (zl.sub >= zl.pw || pix >= N * N) return;
-  const int b = blockIdx.y, v = vids[b];
+  const int v = vids[b];
   const RotView r = rv[v];
   const int x = pix % N, y = pix / N, zo = 4 * zl.z4;
   const long V = (long)N * N * nz;
@@ -1696,60 +1708,123 @@ __global__ void __launch_bounds__(256) k_atten_z(float* __restrict__ mur, Geo g,
     }
 }
 
-// Adjoint of k_rot2_fwd_z as a gather: target pixel j collects w * work[i][:]
-// over the (source i, weight w) pairs of its plan-time list (the forward's
-// coefficients, w = the product of its two 1-D weights), summed over the
-// group's views in the list's fixed order.  part[grp] is [y][x][z].
-// grid (ceil(N*N / (4*PW)), ngroups), block 256
-__global__ void __launch_bounds__(256) k_rot2_adj_z(const float* __restrict__ work, float* __restrict__ part,
-                                                    const int* __restrict__ rowptr, const int2* __restrict__ ent,
-                                                    const int* __restrict__ vids, int nb, int vpg, Geo g) {
+// Adjoint of k_rot2_fwd_z as a gather: target pixel j collects w * work[b][i][:]
+// over the K (source i, weight w) slots of its plan-time list (the forward's
+// coefficients, w = the product of its two 1-D weights; unused slots carry
+// w = 0 and i = j), ent laid out [view][K][NN] so that neighbouring targets
+// read neighbouring slots.  Wave s of a block sums views s, s + VS, ... of the
+// batch for the block's PW pixels (two views per step: every slot and source
+// load of both is issued before the first product); the VS wave sums are then
+// added in wave order through LDS and written to part [y][x][z].
+// grid 8 * xcd_chunk(ceil(NN / PW)), block 64 * VS
+template <int K, int VS>
+__global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restrict__ work, float* __restrict__ part,
+                                                          const int2* __restrict__ ent, const int* __restrict__ vids,
+                                                          int nb, int C, Geo g) {
+  __shared__ float4 red[VS][64];
   const int N = g.nx, NN = N * N, nz = g.nz, nz4 = nz >> 2;
-  const ZLane zl(threadIdx.x & 63, nz4);
-  const int j = (blockIdx.x * 4 + (threadIdx.x >> 6)) * zl.pw + zl.sub;
-  if (zl.sub >= zl.pw || j >= NN) return;
-  const int grp = blockIdx.y, b0 = grp * vpg, b1 = min(nb, b0 + vpg), zo = 4 * zl.z4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const ZLane zl(lane, nz4);
+  const int lb = xcd_block(C);
+  if (lb * zl.pw >= NN) return;  // whole block past the end (uniform)
+  const int j = lb * zl.pw + zl.sub;
+  const bool act = zl.sub < zl.pw && j < NN;
+  const int jj = min(j, NN - 1), zo = 4 * (act ? zl.z4 : 0);
   const long V = (long)NN * nz;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int b = b0; b < b1; ++b) {
-    const long rb = (long)vids[b] * (NN + 1);
-    const int e0 = rowptr[rb + j], e1 = rowptr[rb + j + 1];
-    const float* src = work + V * b + zo;
-    for (int e = e0; e < e1; ++e) {
-      const int2 en = ent[e];
-      const float w = __int_as_float(en.y);
-      const float4 p = *reinterpret_cast<const float4*>(src + (long)en.x * nz);
-      acc.x = acc.x + w * p.x;
-      acc.y = acc.y + w * p.y;
-      acc.z = acc.z + w * p.z;
-      acc.w = acc.w + w * p.w;
+  for (int b = wave; b < nb; b += 2 * VS) {
+    const bool two = b + VS < nb;
+    const int b2 = two ? b + VS : b;
+    const int2* ea = ent + (long)vids[b] * K * NN + jj;
+    const int2* eb = ent + (long)vids[b2] * K * NN + jj;
+    int2 sa[K], sb[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      sa[k] = ea[(long)k * NN];
+      sb[k] = eb[(long)k * NN];
+    }
+    const float* wa = work + V * b + zo;
+    const float* wb = work + V * b2 + zo;
+    float4 pa[K], pb[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      pa[k] = *reinterpret_cast<const float4*>(wa + (long)sa[k].x * nz);
+      pb[k] = *reinterpret_cast<const float4*>(wb + (long)sb[k].x * nz);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const float w = __int_as_float(sa[k].y);
+      if (w != 0.f) {
+        acc.x = acc.x + w * pa[k].x;
+        acc.y = acc.y + w * pa[k].y;
+        acc.z = acc.z + w * pa[k].z;
+        acc.w = acc.w + w * pa[k].w;
+      }
+    }
+    if (two) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float w = __int_as_float(sb[k].y);
+        if (w != 0.f) {
+          acc.x = acc.x + w * pb[k].x;
+          acc.y = acc.y + w * pb[k].y;
+          acc.z = acc.z + w * pb[k].z;
+          acc.w = acc.w + w * pb[k].w;
+        }
+      }
     }
   }
-  *reinterpret_cast<float4*>(part + V * grp + (long)j * nz + zo) = acc;
+  if (VS > 1) {
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (wave != 0) return;
+#pragma unroll
+    for (int s = 1; s < VS; ++s) {
+      const float4 o = red[s][lane];
+      acc.x = acc.x + o.x;
+      acc.y = acc.y + o.y;
+      acc.z = acc.z + o.z;
+      acc.w = acc.w + o.w;
+    }
+  }
+  if (act) *reinterpret_cast<float4*>(part + (long)j * nz + zo) = acc;
 }
 
 // Image reduction from z-fastest partials: part[g] [y][x][z] -> out [z][y][x]
 // (the caller's layout) with the modes of k_img_reduce, groups summed in the
-// same order.  grid (ceil(N/32), ny), block 256
+// same order.  A block transposes a 32 (x) x 32 (z) tile of row y through LDS:
+// float4 reads along z, 128-byte row writes along x.
+// grid (ceil(N/32), ny, ceil(nz/32)), block 256
 __global__ void __launch_bounds__(256) k_img_reduce_z(const float* __restrict__ part, int ng, float* __restrict__ out,
                                                       const float* __restrict__ asum, int mode, Geo g) {
-  __shared__ float t[32 * 257];
-  const int N = g.nx, nz = g.nz, x0 = blockIdx.x * 32, y = blockIdx.y, tid = threadIdx.x;
-  const int nxb = min(32, N - x0);
+  __shared__ float t[32 * 33];  // [x][z]
+  const int N = g.nx, nz = g.nz, x0 = blockIdx.x * 32, y = blockIdx.y, z0 = blockIdx.z * 32, tid = threadIdx.x;
   const long V = (long)N * g.ny * nz, S = (long)N * g.ny;
-  const float* pp = part + ((long)y * N + x0) * nz;
-  for (int k = tid; k < nxb * nz; k += 256) {
-    float sv = pp[k];
-    for (int gg = 1; gg < ng; ++gg) sv += pp[(long)gg * V + k];
-    const int x = k / nz, z = k - x * nz;
-    t[x * 257 + z] = sv;
+  {
+    const int x = tid >> 3, z = 4 * (tid & 7);
+    if (x0 + x < N && z0 + z < nz) {
+      const float* pp = part + ((long)y * N + x0 + x) * nz + z0 + z;
+      float4 sv = *reinterpret_cast<const float4*>(pp);
+      for (int gg = 1; gg < ng; ++gg) {
+        const float4 o = *reinterpret_cast<const float4*>(pp + (long)gg * V);
+        sv.x += o.x;
+        sv.y += o.y;
+        sv.z += o.z;
+        sv.w += o.w;
+      }
+      t[x * 33 + z] = sv.x;
+      t[x * 33 + z + 1] = sv.y;
+      t[x * 33 + z + 2] = sv.z;
+      t[x * 33 + z + 3] = sv.w;
+    }
   }
   __syncthreads();
-  for (int k = tid; k < nz * 32; k += 256) {
-    const int z = k >> 5, x = k & 31;
-    if (x >= nxb) continue;
-    const long o = S * z + (long)N * y + x0 + x;
-    const float sv = t[x * 257 + z];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int e = tid + 256 * r, z = e >> 5, x = e & 31;
+    if (x0 + x >= N || z0 + z >= nz) continue;
+    const long o = S * (z0 + z) + (long)N * y + x0 + x;
+    const float sv = t[x * 33 + z];
     if (mode == IMG_WRITE) out[o] = sv;
     else if (mode == IMG_ACCUM) out[o] = out[o] + sv;
     else out[o] = out[o] * (sv / asum[o]);  // out .*= back ./ asum, src/ml-os-em.jl:46
@@ -2075,25 +2150,23 @@ bool zfast_supported(const Geo& g) {
   return g.nz % 4 == 0 && g.nz / 4 <= 64 && sizeof(float) * g.ny * 256 <= kMaxLds;
 }
 
-// Per-target source lists of the :two forward (the adjoint's gather form):
-// for target pixel q of view v, entries rowptr[v*(NN+1) + q] ..
-// rowptr[v*(NN+1) + q + 1] of ent hold (source pixel p, weight bits) with
-// weight = w1 * w2 of the forward's bilinear coefficients (bilinear_coef,
-// evaluated here on the host with the same Float32 operations; the map also
-// for a kind-0 view, as the scatter adjoint does), ordered by p;
-// zero weights and taps in the zero pad are left out.  Pixel index = y*N + x.
-bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, std::vector<int>& rowptr, std::vector<int2>& ent) {
+// Per-target source lists of the :two forward (the adjoint's gather form),
+// padded to K slots: slot k of target pixel q of view v is
+// ent[(v * K + k) * NN + q] = (source pixel p, weight bits), weight = w1 * w2
+// of the forward's bilinear coefficients (bilinear_coef, evaluated here on
+// the host with the same Float32 operations; the map also for a kind-0 view,
+// as the scatter adjoint does), slots ordered by p; zero weights and taps in
+// the zero pad are left out, unused slots are (q, 0).  Pixel index = y*N + x.
+// K = the largest list, at least 4; false when it exceeds kMaxEll.
+bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::vector<int2>& ent) {
   const int N = g.nx;
   const long NN = (long)N * N;
-  rowptr.assign((size_t)nview * (NN + 1), 0);
-  ent.clear();
-  std::vector<int> cur(NN);
   struct Tap {
     int q;
     float w;
   };
   auto taps = [&](const RotView& r, int pix, Tap (&t)[4]) {
-    int n = 0;  // the adjoint always goes through the map (kind 0 only skips it forward)
+    int n = 0;
     const BiCoef c = bilinear_coef(r, g.pad, pix % N, pix / N);
     const int jx[2] = {c.j1, c.j1 + 1}, jy[2] = {c.j2, c.j2 + 1};
     const float wx[2] = {c.w11, c.w12}, wy[2] = {c.w21, c.w22};
@@ -2105,26 +2178,32 @@ bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, std::vector<in
       }
     return n;
   };
+  std::vector<unsigned char> cnt((size_t)nview * NN, 0);
+  int kmax = 4;
   for (int v = 0; v < nview; ++v) {
-    int* rp = rowptr.data() + (size_t)v * (NN + 1);
+    unsigned char* cv = cnt.data() + (size_t)v * NN;
     Tap t[4];
     for (long pix = 0; pix < NN; ++pix) {
       const int n = taps(rv[v], (int)pix, t);
-      for (int k = 0; k < n; ++k) rp[t[k].q + 1]++;
+      for (int k = 0; k < n; ++k) kmax = std::max(kmax, (int)++cv[t[k].q]);
     }
-    const long base = (long)ent.size();
-    rp[0] = (int)base;
-    for (long q = 0; q < NN; ++q) rp[q + 1] += rp[q];
-    if ((long)rp[NN] >= (long)INT32_MAX - 4 * NN) return false;
-    ent.resize(rp[NN]);
-    for (long q = 0; q < NN; ++q) cur[q] = rp[q];
+  }
+  if (kmax > kMaxEll) return false;
+  *K = kmax;
+  ent.assign((size_t)nview * kmax * NN, int2{});
+  for (int v = 0; v < nview; ++v) {
+    int2* ev = ent.data() + (size_t)v * kmax * NN;
+    for (long q = 0; q < NN; ++q)
+      for (int k = 0; k < kmax; ++k) ev[k * NN + q] = int2{(int)q, 0};
+    std::vector<unsigned char> fill(NN, 0);
+    Tap t[4];
     for (long pix = 0; pix < NN; ++pix) {
       const int n = taps(rv[v], (int)pix, t);
       for (int k = 0; k < n; ++k) {
         int2 e;
         e.x = (int)pix;
         std::memcpy(&e.y, &t[k].w, sizeof(float));
-        ent[cur[t[k].q]++] = e;
+        ev[(long)fill[t[k].q]++ * NN + t[k].q] = e;
       }
     }
   }
@@ -2140,7 +2219,8 @@ hipError_t launch_rot2_fwd_z(const float* img_t, float* out, const float* atten,
                              const RotView* rv, const int* vids, int nb, const Geo& g, hipStream_t s) {
   if (nb <= 0) return hipSuccess;
   const int pw = 64 / (g.nz / 4);
-  k_rot2_fwd_z<<<dim3(cdiv((long)g.nx * g.nx, 4L * pw), nb), 256, 0, s>>>(img_t, out, atten, atten_vstride, rv, vids, g);
+  const int PB = (int)cdiv((long)g.nx * g.nx, 4L * pw), C = xcd_chunk((long)nb * PB);
+  k_rot2_fwd_z<<<kXcds * C, 256, 0, s>>>(img_t, out, atten, atten_vstride, rv, vids, nb, PB, C, g);
   return hipGetLastError();
 }
 
@@ -2153,17 +2233,27 @@ hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_rot2_adj_z(const float* work, float* part, const int* rowptr, const int2* ent, const int* vids,
-                             int nb, int vpg, const Geo& g, hipStream_t s) {
+hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
+                             const Geo& g, hipStream_t s) {
   if (nb <= 0) return hipSuccess;
-  const int pw = 64 / (g.nz / 4), ng = (nb + vpg - 1) / vpg;
-  k_rot2_adj_z<<<dim3(cdiv((long)g.nx * g.nx, 4L * pw), ng), 256, 0, s>>>(work, part, rowptr, ent, vids, nb, vpg, g);
-  return hipGetLastError();
+  const int pw = 64 / (g.nz / 4);
+  const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
+  const int vs = nb <= 4 ? 2 : nb <= 8 ? 4 : nb <= 16 ? 8 : 16;
+#define SR_ELL(KK, VV) \
+  if (K == KK && vs == VV) { \
+    k_rot2_adj_ell<KK, VV><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g); \
+    return hipGetLastError(); \
+  }
+#define SR_ELL_K(KK) SR_ELL(KK, 2) SR_ELL(KK, 4) SR_ELL(KK, 8) SR_ELL(KK, 16)
+  SR_ELL_K(4) SR_ELL_K(5) SR_ELL_K(6) SR_ELL_K(7) SR_ELL_K(8)
+#undef SR_ELL_K
+#undef SR_ELL
+  return hipErrorInvalidConfiguration;
 }
 
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,
                                hipStream_t s) {
-  k_img_reduce_z<<<dim3(cdiv(g.nx, 32), g.ny), 256, 0, s>>>(part, ngroups, out, asum, mode, g);
+  k_img_reduce_z<<<dim3(cdiv(g.nx, 32), g.ny, cdiv(g.nz, 32)), 256, 0, s>>>(part, ngroups, out, asum, mode, g);
   return hipGetLastError();
 }
 

@@ -200,7 +200,7 @@ struct sr_plan_s {
   bool mfma = false;  // separable PSF passes on the matrix cores
   // z-fastest work layout: work, attenuation and partial images are
   // [.][y][x][z]; the :two rotations are z-vector gathers and the rotation
-  // adjoint reads plan-time per-target source lists (zrow, zent)
+  // adjoint reads plan-time per-target source lists (zent)
   bool zfast = false;
   size_t bytes = 0;
 
@@ -209,11 +209,12 @@ struct sr_plan_s {
   DBuf st_img, st_views, st_aux1, st_aux2, st_x0;  // staging for host-pointer calls and EM temporaries
   DBuf em_ratio, em_back, em_asum;
   DBuf flag;  // one int: device-side argument checks
-  DBuf zimg, zrow, zent;  // z-fastest image copy, per-target rotation lists
+  DBuf zimg, zent;  // z-fastest image copy, per-target rotation lists
+  int zk = 0;        // slots per target of zent
   std::vector<int> h_vids;
 
   // per-kernel event timing (sr_plan_set_timing)
-  bool timing = false;
+  int timing = 0;  // bit mask of the kernel classes bracketed by events
   struct Pending {
     int cls;
     hipEvent_t a, b;
@@ -253,25 +254,9 @@ struct sr_plan_s {
       *vpg = 1;
       return nb;
     }
-    if (zfast) {
-      // gather kernel: ceil(nx*nx / (4 * pixels per wave)) blocks of 4 waves
-      // per group, about 8 blocks per CU; a thread walks the group's views
-      // (~2 units each), a group's partial image costs ~3 units to write and
-      // reduce
-      const int pw = 64 / (d.nz / 4);
-      const long blocks = ((long)d.nx * d.nx + 4L * pw - 1) / (4L * pw), slots = (long)ncu * 8;
-      long best = -1;
-      int bv = nb;
-      for (int v = 1; v <= nb; ++v) {
-        const long ng = (nb + v - 1) / v;
-        const long cost = 2 * v * ((blocks * ng + slots - 1) / slots) + 3 * ng;
-        if (best < 0 || cost < best) {
-          best = cost;
-          bv = v;
-        }
-      }
-      *vpg = bv;
-      return (nb + bv - 1) / bv;
+    if (zfast) {  // one group: the gather kernel sums all views of the batch in its blocks
+      *vpg = nb;
+      return 1;
     }
     // time ~ (rounds of blocks over the CUs) x (views per block); each group
     // also writes and re-reads one partial image (about 1/16 of a view's time)
@@ -313,7 +298,7 @@ struct KTimer {
   int cls;
   hipEvent_t a = nullptr;
   KTimer(sr_plan p_, int cls_) : p(p_), cls(cls_) {
-    if (p->timing && (a = pool_event(p))) (void)hipEventRecord(a, p->stream);
+    if ((p->timing & SR_TIME_CLASS(cls)) && (a = pool_event(p))) (void)hipEventRecord(a, p->stream);
   }
   ~KTimer() {
     if (!a) return;
@@ -437,8 +422,8 @@ int run_backproject(sr_plan p, float* image, const float* views, const int* d_vi
     if (p->zfast) {
       {
         KTimer kt(p, sr::K_ROT_ADJ);
-        HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zrow.as<int>(),
-                                      p->zent.as<int2>(), d_vids + s0, nb, vpg, p->g, p->stream));
+        HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
+                                      d_vids + s0, nb, p->g, p->stream));
       }
       KTimer kt(p, sr::K_IMG_REDUCE);
       HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), ng, target, asum, m, p->g, p->stream));
@@ -660,16 +645,12 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
     p->zfast = !force_zyx && p->mfma && d.interpmeth == SR_INTERP_TWO && sr::zfast_supported(p->g);
   }
   if (p->zfast) {
-    std::vector<int> rowptr;
     std::vector<int2> ent;
-    p->zfast = sr::build_rot2_lists(rv.data(), d.nview, p->g, rowptr, ent);
+    p->zfast = sr::build_rot2_lists(rv.data(), d.nview, p->g, &p->zk, ent);
     if (p->zfast) {
-      if ((rc = p->zrow.ensure(sizeof(int) * rowptr.size())) ||
-          (rc = p->zent.ensure(sizeof(int2) * std::max<size_t>(1, ent.size()))) ||
-          (rc = p->zimg.ensure(sizeof(float) * p->V)))
+      if ((rc = p->zent.ensure(sizeof(int2) * ent.size())) || (rc = p->zimg.ensure(sizeof(float) * p->V)))
         return bail(rc);
-      HIP_TRY(hipMemcpy(p->zrow.p, rowptr.data(), sizeof(int) * rowptr.size(), hipMemcpyHostToDevice));
-      if (!ent.empty()) HIP_TRY(hipMemcpy(p->zent.p, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(p->zent.p, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
     }
   }
 
@@ -699,7 +680,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   }
   HIP_TRY(hipStreamSynchronize(p->stream));
   p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->taps_c.n + p->rotv.n + p->tab_b.n +
-             p->tab_w.n + p->zimg.n + p->zrow.n + p->zent.n;
+             p->tab_w.n + p->zimg.n + p->zent.n;
   *out = guard.release();
   return SR_OK;
 }
@@ -720,7 +701,7 @@ int sr_plan_set_timing(sr_plan p, int32_t enable) {
   if (!p) return fail(SR_ERR_ARG, "null plan");
   HIP_TRY(hipStreamSynchronize(p->stream));
   resolve_timing(p);
-  p->timing = enable != 0;
+  p->timing = enable == 1 ? SR_TIME_ALL : (enable & SR_TIME_ALL);
   for (int k = 0; k < sr::K_NCLASS; ++k) {
     p->t_ms[k] = 0;
     p->t_n[k] = 0;

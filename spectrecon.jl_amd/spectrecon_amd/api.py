@@ -170,9 +170,13 @@ class SPECTplan:
     def synchronize(self):
         check(lib().sr_plan_synchronize(self.handle))
 
-    def set_timing(self, enable=True):
-        """Bracket every kernel launch of this plan with HIP events."""
-        check(lib().sr_plan_set_timing(self.handle, int(enable)))
+    def set_timing(self, enable=True, classes=None):
+        """Bracket kernel launches of this plan with HIP events: all of them,
+        or only those of the named classes (names of KERNEL_CLASSES)."""
+        mask = 0
+        if enable:
+            mask = 1 if classes is None else sum(2 << L.KERNEL_CLASSES.index(c) for c in classes)
+        check(lib().sr_plan_set_timing(self.handle, mask))
 
     def timing(self):
         """{kernel class: (total device ms, launches)} since set_timing(True)."""
