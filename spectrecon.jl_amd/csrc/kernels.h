@@ -66,9 +66,11 @@ hipError_t launch_atten_exact(float* mur, int nb, const Geo& g, float dy, hipStr
 // ---- PSF ----------------------------------------------------------------
 // per-chunk partial views of the batch from the work volume (rotated and
 // attenuated already); returns the chunk count actually used through
-// *nchunk_used
+// *nchunk_used.  zfast: `work` is the z-fastest image [y][x][z] and the :two
+// rotation (rv) and attenuation ([v][y][x][z]) are applied while staging
 hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
-                          int nchunk, int* nchunk_used, const Geo& g, hipStream_t s, bool zfast = false);
+                          int nchunk, int* nchunk_used, const Geo& g, hipStream_t s, bool zfast = false,
+                          const float* atten = nullptr, long atten_vstride = 0, const RotView* rv = nullptr);
 // views (or the ML-EM ratio y/(view + b) when epi == EPI_RATIO) = sum of partials
 hipError_t launch_view_reduce(const float* part, int nchunk, const int* vids, int nb, float* out, const float* yobs,
                               const float* bkg, int epi, const Geo& g, hipStream_t s);

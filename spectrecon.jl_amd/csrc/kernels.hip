@@ -883,10 +883,70 @@ __device__ __forceinline__ void mf_tiles(int wave, int ntile, int ntx, int (&z0)
   }
 }
 
+// ZF (z-fastest layout, :two rotation fused): plane y of the rotated,
+// attenuated view is not read from a work volume but computed while it is
+// staged: element (x, z-quad) = atten[v][y][x][:] * bilinear(img_t at the
+// rotated position of pixel (x, y)) -- the expression of k_rot2_fwd_z, with
+// taps in the zero pad given weight 0 (their clamped addresses read a real,
+// finite pixel).  The element slots of a thread are split in two parts:
+// MF_RA slots are gathered during the x pass (the pass with more live
+// registers) and stored after it, MF_RB slots during the z pass; img_t is
+// [y][x][z] (nz % 4 == 0), atten [v][y][x][z].
+constexpr int MF_RA = 2, MF_RB = 3;
+struct RotSlot {
+  float4 s11, s12, s21, s22, a;
+};
+// the taps of pixel (x, y) (theta = 0: the forward is a copy of (x, y)) and
+// their weights, 0 for taps in the zero pad
+struct RotTaps {
+  int ja, jb, ra, rb;
+  float wa1, wa2, wb1, wb2, w21, w22;
+};
+__device__ __forceinline__ RotTaps rot_taps(const RotView& r, int N, int pad, int x, int y) {
+  const BiCoef c = bilinear_coef(r, pad, x, y);
+  const bool k0 = r.kind == 0;
+  const int j1 = k0 ? x : c.j1, j2 = k0 ? y : c.j2;
+  const bool x0 = (unsigned)j1 < (unsigned)N, x1 = (unsigned)(j1 + 1) < (unsigned)N;
+  const bool y0 = (unsigned)j2 < (unsigned)N, y1 = (unsigned)(j2 + 1) < (unsigned)N;
+  RotTaps t;
+  t.ja = clampi(j1, 0, N - 1);
+  t.jb = clampi(j1 + 1, 0, N - 1);
+  t.ra = clampi(j2, 0, N - 1) * N;
+  t.rb = clampi(j2 + 1, 0, N - 1) * N;
+  t.wa1 = k0 ? 1.f : (x0 && y0 ? c.w11 : 0.f);
+  t.wa2 = k0 ? 0.f : (x1 && y0 ? c.w12 : 0.f);
+  t.wb1 = k0 ? 0.f : (x0 && y1 ? c.w11 : 0.f);
+  t.wb2 = k0 ? 0.f : (x1 && y1 ? c.w12 : 0.f);
+  t.w21 = k0 ? 1.f : c.w21;
+  t.w22 = k0 ? 0.f : c.w22;
+  return t;
+}
+__device__ __forceinline__ void rot_issue(RotSlot& q, const float* __restrict__ img_t, const float* __restrict__ av,
+                                          const RotView& r, int N, int nz, int pad, int x, int y, int zo) {
+  const RotTaps t = rot_taps(r, N, pad, x, y);
+  q.s11 = *reinterpret_cast<const float4*>(img_t + (long)(t.ra + t.ja) * nz + zo);
+  q.s12 = *reinterpret_cast<const float4*>(img_t + (long)(t.ra + t.jb) * nz + zo);
+  q.s21 = *reinterpret_cast<const float4*>(img_t + (long)(t.rb + t.ja) * nz + zo);
+  q.s22 = *reinterpret_cast<const float4*>(img_t + (long)(t.rb + t.jb) * nz + zo);
+  q.a = *reinterpret_cast<const float4*>(av + ((long)y * N + x) * nz + zo);
+}
+// the weights are recomputed here rather than held with the loads in flight
+__device__ __forceinline__ float4 rot_value(const RotSlot& q, const RotView& r, int N, int pad, int x, int y) {
+  const RotTaps t = rot_taps(r, N, pad, x, y);
+  float4 v;
+  v.x = ((q.s11.x * t.wa1 + q.s12.x * t.wa2) * t.w21 + (q.s21.x * t.wb1 + q.s22.x * t.wb2) * t.w22) * q.a.x;
+  v.y = ((q.s11.y * t.wa1 + q.s12.y * t.wa2) * t.w21 + (q.s21.y * t.wb1 + q.s22.y * t.wb2) * t.w22) * q.a.y;
+  v.z = ((q.s11.z * t.wa1 + q.s12.z * t.wa2) * t.w21 + (q.s21.z * t.wb1 + q.s22.z * t.wb2) * t.w22) * q.a.z;
+  v.w = ((q.s11.w * t.wa1 + q.s12.w * t.wa2) * t.w21 + (q.s21.w * t.wb1 + q.s22.w * t.wb2) * t.w22) * q.a.w;
+  return v;
+}
+
 template <int TPW, int NW, bool ZF>
 __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restrict__ work, Taps t,
                                                             const int* __restrict__ vids, int Yc,
-                                                            float* __restrict__ part, Geo g, int TBP) {
+                                                            float* __restrict__ part, Geo g, int TBP,
+                                                            const float* __restrict__ atten, long atten_vstride,
+                                                            const RotView* __restrict__ rv) {
   extern __shared__ float L[];
   const int NT = t.ntx;
   const MfTaps TT(NT);
@@ -895,8 +955,8 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
   const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
   const int ntx = nx >> 4, ntile = ntx * (nz >> 4);
   const int WX = mf_wx(nx, hx, NT), WQ = mf_wq(nx), RQ = mf_rq(nz, NT), XO = mf_xo(hx);
-  // ZF: the plane arrives as [x][z] (z-fastest work volume) and is staged as
-  // rows x + hx of stride WZ (= 16 mod 64) with zero rows around it
+  // ZF: the plane is staged as rows x + hx of stride WZ (= 16 mod 64) with
+  // zero rows around it
   const int WZ = mf_wq(nz), RX = mf_rq(nx, NT);
   const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
   const int v = vids[b];
@@ -911,32 +971,59 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
   for (int k = tid; k < PSZ + RQ * WQ + TBP * TT.size; k += (NW * 64)) L[k] = 0.f;
 
   const long S = (long)nx * ny;
+  const int nq = nx >> 2, nz4 = nz >> 2;
+  const int nv = nz * nq;  // float4 elements of a plane
+  // x-fastest: the plane as float4 along x, slot k of this thread = element
+  // tid + k*threads (clamped to the last element: the loads and stores stay
+  // unconditional, so no wait is forced between them; a clamped slot
+  // rewrites that element)
+  constexpr int PF = ZF ? 1 : MF_PF * MF_WAVES / NW;
   const float* wb = work + S * nz * b;
-  const int nq = nx >> 2, nv = nz * nq;
-  // the plane as float4 along x, slot k of this thread = element tid + k*threads
-  // (clamped to the last element: the loads and stores stay unconditional,
-  // so no wait is forced between them; a clamped slot rewrites that element)
-  constexpr int PF = MF_PF * MF_WAVES / NW;
-  const int nz4 = nz >> 2;
   int pfo[PF], pfs[PF];
+  float4 pf[PF];
+  // ZF: slot k = element tid + k*threads of the [x][z] plane, (x, z quad)
+  // packed as x << 8 | zq
+  constexpr int RS = ZF ? MF_RA + MF_RB : 1;
+  int rxz[RS];
+  RotSlot ra[ZF ? MF_RA : 1], rb[ZF ? MF_RB : 1];
+  const RotView rot = ZF ? rv[v] : RotView{};
+  const float* av = ZF ? atten + atten_vstride * v : nullptr;
+  if (ZF) {
 #pragma unroll
-  for (int k = 0; k < PF; ++k) {
-    const int i = min(tid + k * (NW * 64), nv - 1);
-    if (ZF) {  // element i of the contiguous [x][z] plane
-      const int x = i / nz4, z4 = i - x * nz4;
-      pfo[k] = 4 * i;
-      pfs[k] = (hx + x) * WZ + 4 * z4;
-    } else {
+    for (int k = 0; k < RS; ++k) {
+      const int i = min(tid + k * (NW * 64), nv - 1), x = i / nz4;
+      rxz[k] = x << 8 | (i - x * nz4);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int i = min(tid + k * (NW * 64), nv - 1);
       const int z = i / nq, x4 = i - z * nq;
       pfo[k] = (int)(S * z) + 4 * x4;
       pfs[k] = z * WX + XO + 4 * x4;
     }
   }
-  const long pstride = ZF ? (long)nx * nz : nx;  // plane y starts at wb + pstride * y
-  float4 pf[PF];
   auto prefetch = [&](int y) {
 #pragma unroll
-    for (int k = 0; k < PF; ++k) pf[k] = *reinterpret_cast<const float4*>(wb + pstride * y + pfo[k]);
+    for (int k = 0; k < PF; ++k) pf[k] = *reinterpret_cast<const float4*>(wb + nx * y + pfo[k]);
+  };
+  // (scheduling fences keep each part's gathers where they are issued, so
+  // that only one part is live at a time)
+  auto rot_half = [&](auto& h, int k0, int y) {
+    SR_SCHED_FENCE();
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(h) / sizeof(h[0])); ++k)
+      rot_issue(h[k], work, av, rot, nx, nz, g.pad, rxz[k0 + k] >> 8, y, 4 * (rxz[k0 + k] & 255));
+    SR_SCHED_FENCE();
+  };
+  auto rot_store = [&](auto& h, int k0, int y) {
+    SR_SCHED_FENCE();
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(h) / sizeof(h[0])); ++k) {
+      const int x = rxz[k0 + k] >> 8, zq = rxz[k0 + k] & 255;
+      const float4 val = rot_value(h[k], rot, nx, g.pad, x, y);
+      *reinterpret_cast<float4*>(Pz + (hx + x) * WZ + 4 * zq) = make_float4(val.x, val.y, val.z, val.w);
+    }
   };
   // thread tid < 6 NT loads entry tk of table tab (u, w, u tail, u head, w tail, w head)
   // the tap tables of up to TBP (<= 8) planes at a time: all loads of a
@@ -958,7 +1045,7 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
 #pragma unroll
     for (int j = 0; j < ME; ++j) s_taps[tdst[j]] = tvv[j];
   };
-  if (y0 < y1) prefetch(y0);
+  if (y0 < y1 && !ZF) prefetch(y0);
   int z0s[TPW], x0s[TPW];
   bool own[TPW];
   mf_tiles<TPW>(wave, ntile, ntx, z0s, x0s, own);
@@ -972,6 +1059,12 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
 #pragma unroll
   for (int i = 0; i < TPW; ++i) dv[i] = mf4{0.f, 0.f, 0.f, 0.f};
   __syncthreads();  // pads zeroed before the first plane lands
+  if (ZF && y0 < y1) {  // the first plane: its latency is exposed once per block
+    rot_half(ra, 0, y0);
+    rot_store(ra, 0, y0);
+    rot_half(rb, MF_RA, y0);
+    rot_store(rb, MF_RA, y0);
+  }
 
   for (int y = y0; y < y1; ++y) {
     if ((y - y0) % TBP == 0) {  // next tap group (the previous group's last reads are done)
@@ -979,11 +1072,15 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
       load_taps(y, min(TBP, y1 - y));
     }
     const float* tb = s_taps + ((y - y0) % TBP) * TT.size;
+    const int yn = min(y + 1, y1 - 1);
+    if (!ZF) {
 #pragma unroll
-    for (int k = 0; k < PF; ++k)
-      *reinterpret_cast<float4*>(Pz + pfs[k]) = make_float4(pf[k].x, pf[k].y, pf[k].z, pf[k].w);
+      for (int k = 0; k < PF; ++k)
+        *reinterpret_cast<float4*>(Pz + pfs[k]) = make_float4(pf[k].x, pf[k].y, pf[k].z, pf[k].w);
+    }
     lds_barrier();
-    prefetch(min(y + 1, y1 - 1));
+    if (ZF) rot_half(ra, 0, yn);
+    else prefetch(yn);
     // Toeplitz fragments: x-pass B[k][n] = u[n+2hx-k] = ub[-k], z-pass A[m][k] = w[m+2hz-k] = wz[-k]
     const float* ub = tb + TT.off + m16 - q4 + 2 * hx;
     const float* wz = tb + TT.tp + TT.off + m16 - q4 + 2 * hz;
@@ -1038,6 +1135,12 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
       *reinterpret_cast<float4*>(Qz + (hz + z0s[i] + m16) * WQ + x0s[i] + 4 * q4) =
           make_float4(dx[i][0], dx[i][1], dx[i][2], dx[i][3]);
     lds_barrier();
+    // ZF: every wave's x pass is done, so Pz takes half A of the next plane
+    // (half B follows after the z pass)
+    if (ZF) {
+      rot_store(ra, 0, yn);
+      rot_half(rb, MF_RA, yn);
+    }
     // z pass: accumulate the view sum, the replicate boundary first
 #pragma unroll
     for (int i = 0; i < TPW; ++i) {
@@ -1071,6 +1174,7 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
         for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a0, bq0[i], dv[i]);
       }
     }
+    if (ZF) rot_store(rb, MF_RA, yn);
     // the next plane's stores touch Pz (last read before the barrier above)
     // and the other tap buffer; Qz is rewritten only after the next barrier
   }
@@ -2039,7 +2143,8 @@ bool psf_sep_supported(const Geo& g) {
 }
 
 hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
-                          int nchunk, int* nchunk_used, const Geo& g, hipStream_t s, bool zfast) {
+                          int nchunk, int* nchunk_used, const Geo& g, hipStream_t s, bool zfast, const float* atten,
+                          long atten_vstride, const RotView* rv) {
   const int Yc = (g.ny + nchunk - 1) / nchunk;
   nchunk = (g.ny + Yc - 1) / Yc;
   if (nchunk_used) *nchunk_used = nchunk;
@@ -2050,11 +2155,12 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
     SR_TPW_SWITCH(mf_tpw(g), {
       if (zfast) {
         allow_lds(k_psf_fwd_mfma<TPW, MF_WAVES, true>, lds);
-        k_psf_fwd_mfma<TPW, MF_WAVES, true><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(work, t, vids, Yc, part, g, tbp);
+        k_psf_fwd_mfma<TPW, MF_WAVES, true><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(
+            work, t, vids, Yc, part, g, tbp, atten, atten_vstride, rv);
       } else {
         allow_lds(k_psf_fwd_mfma<TPW, MF_WAVES, false>, lds);
         k_psf_fwd_mfma<TPW, MF_WAVES, false><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(work, t, vids, Yc, part, g,
-                                                                                      tbp);
+                                                                                      tbp, nullptr, 0, nullptr);
       }
     });
     return hipGetLastError();
@@ -2147,7 +2253,10 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
 }
 
 bool zfast_supported(const Geo& g) {
-  return g.nz % 4 == 0 && g.nz / 4 <= 64 && sizeof(float) * g.ny * 256 <= kMaxLds;
+  // z-vector lanes, the attenuation kernel's LDS columns, the fused forward's
+  // element slots (MF_RA + MF_RB float4 per thread)
+  return g.nz % 4 == 0 && g.nz / 4 <= 64 && sizeof(float) * g.ny * 256 <= kMaxLds &&
+         (long)g.nx * g.nz <= 4L * (MF_RA + MF_RB) * MF_THREADS;
 }
 
 // Per-target source lists of the :two forward (the adjoint's gather form),

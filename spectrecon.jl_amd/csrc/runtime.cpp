@@ -363,12 +363,7 @@ int run_project(sr_plan p, float* views, const float* image, const int* d_vids, 
     const int nb = std::min(p->max_batch, n - s0);
     int rc;
     if ((rc = ensure_batch_buffers(p, nb))) return rc;
-    if (p->zfast) {
-      KTimer kt(p, sr::K_ROT_FWD);
-      HIP_TRY(sr::launch_rot2_fwd_z(p->zimg.as<float>(), p->work.as<float>(), p->atten.as<float>(),
-                                    p->atten_vstride(), p->rotv.as<sr::RotView>(), d_vids + s0, nb, p->g,
-                                    p->stream));
-    } else {
+    if (!p->zfast) {  // (z-fastest: the PSF forward rotates while it stages)
       KTimer kt(p, sr::K_ROT_FWD);
       HIP_TRY(sr::launch_rot_fwd(p->d.interpmeth, image, p->work.as<float>(), p->atten.as<float>(),
                                  p->atten_vstride(), p->rotv.as<sr::RotView>(), d_vids + s0, nb, p->tab_b.as<int>(),
@@ -377,8 +372,9 @@ int run_project(sr_plan p, float* views, const float* image, const int* d_vids, 
     int nchunk = 0;
     {
       KTimer kt(p, sr::K_PSF_FWD);
-      HIP_TRY(sr::launch_psf_fwd(p->separable, p->work.as<float>(), t, d_vids + s0, nb, p->fpart.as<float>(),
-                                 p->nchunk_fwd(nb), &nchunk, p->g, p->stream, p->zfast));
+      HIP_TRY(sr::launch_psf_fwd(p->separable, p->zfast ? p->zimg.as<float>() : p->work.as<float>(), t, d_vids + s0,
+                                 nb, p->fpart.as<float>(), p->nchunk_fwd(nb), &nchunk, p->g, p->stream, p->zfast,
+                                 p->atten.as<float>(), p->atten_vstride(), p->rotv.as<sr::RotView>()));
     }
     {
       KTimer kt(p, sr::K_VIEW_REDUCE);
