@@ -1817,14 +1817,18 @@ __global__ void __launch_bounds__(256) k_atten_z(float* __restrict__ mur, Geo g,
 // coefficients, w = the product of its two 1-D weights; unused slots carry
 // w = 0 and i = j), ent laid out [view][K][NN] so that neighbouring targets
 // read neighbouring slots.  Wave s of a block sums views s, s + VS, ... of the
-// batch for the block's PW pixels (two views per step: every slot and source
-// load of both is issued before the first product); the VS wave sums are then
-// added in wave order through LDS and written to part [y][x][z].
+// batch for the block's PW pixels (one view per step, few registers: many
+// waves in flight hide the slot -> source load chain); the VS wave sums are
+// then added in wave order through LDS and written to part [y][x][z].
+// HA: the source volume of view v is H .* atten[v] (H one volume shared by
+// all views, the product rounded as the PSF adjoint rounds it), for the
+// back-projection of a view-independent PSF adjoint result.
 // grid 8 * xcd_chunk(ceil(NN / PW)), block 64 * VS
-template <int K, int VS>
+template <int K, int VS, bool HA>
 __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restrict__ work, float* __restrict__ part,
                                                           const int2* __restrict__ ent, const int* __restrict__ vids,
-                                                          int nb, int C, Geo g) {
+                                                          int nb, int C, Geo g, const float* __restrict__ atten,
+                                                          long atten_vstride) {
   __shared__ float4 red[VS][64];
   const int N = g.nx, NN = N * N, nz = g.nz, nz4 = nz >> 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1836,24 +1840,22 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
   const int jj = min(j, NN - 1), zo = 4 * (act ? zl.z4 : 0);
   const long V = (long)NN * nz;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int b = wave; b < nb; b += 2 * VS) {
-    const bool two = b + VS < nb;
-    const int b2 = two ? b + VS : b;
+  for (int b = wave; b < nb; b += VS) {
     const int2* ea = ent + (long)vids[b] * K * NN + jj;
-    const int2* eb = ent + (long)vids[b2] * K * NN + jj;
-    int2 sa[K], sb[K];
+    int2 sa[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      sa[k] = ea[(long)k * NN];
-      sb[k] = eb[(long)k * NN];
-    }
-    const float* wa = work + V * b + zo;
-    const float* wb = work + V * b2 + zo;
-    float4 pa[K], pb[K];
+    for (int k = 0; k < K; ++k) sa[k] = ea[(long)k * NN];
+    const float* wa = HA ? work + zo : work + V * b + zo;
+    float4 pa[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      pa[k] = *reinterpret_cast<const float4*>(wa + (long)sa[k].x * nz);
-      pb[k] = *reinterpret_cast<const float4*>(wb + (long)sb[k].x * nz);
+    for (int k = 0; k < K; ++k) pa[k] = *reinterpret_cast<const float4*>(wa + (long)sa[k].x * nz);
+    if (HA) {
+      const float* aa = atten + atten_vstride * vids[b] + zo;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float4 fa = *reinterpret_cast<const float4*>(aa + (long)sa[k].x * nz);
+        pa[k] = make_float4(pa[k].x * fa.x, pa[k].y * fa.y, pa[k].z * fa.z, pa[k].w * fa.w);
+      }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1863,18 +1865,6 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
         acc.y = acc.y + w * pa[k].y;
         acc.z = acc.z + w * pa[k].z;
         acc.w = acc.w + w * pa[k].w;
-      }
-    }
-    if (two) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const float w = __int_as_float(sb[k].y);
-        if (w != 0.f) {
-          acc.x = acc.x + w * pb[k].x;
-          acc.y = acc.y + w * pb[k].y;
-          acc.z = acc.z + w * pb[k].z;
-          acc.w = acc.w + w * pb[k].w;
-        }
       }
     }
   }
@@ -1897,10 +1887,12 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
 // Image reduction from z-fastest partials: part[g] [y][x][z] -> out [z][y][x]
 // (the caller's layout) with the modes of k_img_reduce, groups summed in the
 // same order.  A block transposes a 32 (x) x 32 (z) tile of row y through LDS:
-// float4 reads along z, 128-byte row writes along x.
+// float4 reads along z, 128-byte row writes along x.  zout (optional): the
+// new out values also in [y][x][z] (the next projection's z-fastest image).
 // grid (ceil(N/32), ny, ceil(nz/32)), block 256
 __global__ void __launch_bounds__(256) k_img_reduce_z(const float* __restrict__ part, int ng, float* __restrict__ out,
-                                                      const float* __restrict__ asum, int mode, Geo g) {
+                                                      const float* __restrict__ asum, int mode, Geo g,
+                                                      float* __restrict__ zout) {
   __shared__ float t[32 * 33];  // [x][z]
   const int N = g.nx, nz = g.nz, x0 = blockIdx.x * 32, y = blockIdx.y, z0 = blockIdx.z * 32, tid = threadIdx.x;
   const long V = (long)N * g.ny * nz, S = (long)N * g.ny;
@@ -1929,10 +1921,19 @@ __global__ void __launch_bounds__(256) k_img_reduce_z(const float* __restrict__ 
     if (x0 + x >= N || z0 + z >= nz) continue;
     const long o = S * (z0 + z) + (long)N * y + x0 + x;
     const float sv = t[x * 33 + z];
-    if (mode == IMG_WRITE) out[o] = sv;
-    else if (mode == IMG_ACCUM) out[o] = out[o] + sv;
-    else out[o] = out[o] * (sv / asum[o]);  // out .*= back ./ asum, src/ml-os-em.jl:46
+    float nv;
+    if (mode == IMG_WRITE) nv = sv;
+    else if (mode == IMG_ACCUM) nv = out[o] + sv;
+    else nv = out[o] * (sv / asum[o]);  // out .*= back ./ asum, src/ml-os-em.jl:46
+    out[o] = nv;
+    t[x * 33 + z] = nv;  // (this thread's own element)
   }
+  if (!zout) return;
+  __syncthreads();
+  const int x = tid >> 3, z = 4 * (tid & 7);
+  if (x0 + x < N && z0 + z < nz)
+    *reinterpret_cast<float4*>(zout + ((long)y * N + x0 + x) * nz + z0 + z) =
+        make_float4(t[x * 33 + z], t[x * 33 + z + 1], t[x * 33 + z + 2], t[x * 33 + z + 3]);
 }
 
 // ===========================================================================
@@ -2343,14 +2344,19 @@ hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_
 }
 
 hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
-                             const Geo& g, hipStream_t s) {
+                             const Geo& g, hipStream_t s, const float* atten, long atten_vstride) {
+  const bool ha = atten != nullptr;
   if (nb <= 0) return hipSuccess;
   const int pw = 64 / (g.nz / 4);
   const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
   const int vs = nb <= 4 ? 2 : nb <= 8 ? 4 : nb <= 16 ? 8 : 16;
 #define SR_ELL(KK, VV) \
   if (K == KK && vs == VV) { \
-    k_rot2_adj_ell<KK, VV><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g); \
+    if (ha) \
+      k_rot2_adj_ell<KK, VV, true><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, atten, \
+                                                                 atten_vstride); \
+    else \
+      k_rot2_adj_ell<KK, VV, false><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, nullptr, 0); \
     return hipGetLastError(); \
   }
 #define SR_ELL_K(KK) SR_ELL(KK, 2) SR_ELL(KK, 4) SR_ELL(KK, 8) SR_ELL(KK, 16)
@@ -2361,8 +2367,8 @@ hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, in
 }
 
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,
-                               hipStream_t s) {
-  k_img_reduce_z<<<dim3(cdiv(g.nx, 32), g.ny, cdiv(g.nz, 32)), 256, 0, s>>>(part, ngroups, out, asum, mode, g);
+                               hipStream_t s, float* zout) {
+  k_img_reduce_z<<<dim3(cdiv(g.nx, 32), g.ny, cdiv(g.nz, 32)), 256, 0, s>>>(part, ngroups, out, asum, mode, g, zout);
   return hipGetLastError();
 }
 

@@ -352,10 +352,11 @@ int upload_vids(sr_plan p, const int32_t* index, int32_t nindex) {
 }
 
 // views[ii] = A_{vids[ii]} image (or the ML-EM ratio y/(A x + b) when yobs)
+// (zimg_ready: the plan's z-fastest image copy already holds `image`)
 int run_project(sr_plan p, float* views, const float* image, const int* d_vids, int n, const float* yobs,
-                const float* bkg) {
+                const float* bkg, bool zimg_ready = false) {
   const sr::Taps t = p->taps();
-  if (p->zfast && n > 0) {
+  if (p->zfast && n > 0 && !zimg_ready) {
     KTimer kt(p, sr::K_ROT_FWD);
     HIP_TRY(sr::launch_to_zfast(image, p->zimg.as<float>(), p->g, p->stream));
   }
@@ -386,9 +387,11 @@ int run_project(sr_plan p, float* views, const float* image, const int* d_vids, 
 }
 
 // image = sum_ii A_{vids[ii]}' views[ii]  (mode IMG_WRITE), or with IMG_EM:
-// image .*= (A' views) ./ asum
+// image .*= (A' views) ./ asum.  *zimg_written (optional): set when the
+// z-fastest image copy was written with the new image too
 int run_backproject(sr_plan p, float* image, const float* views, const int* d_vids, int n, int mode,
-                    const float* asum) {
+                    const float* asum, bool* zimg_written = nullptr) {
+  if (zimg_written) *zimg_written = false;
   const sr::Taps t = p->taps();
   const bool single = n <= p->max_batch;
   float* target = image;
@@ -422,7 +425,9 @@ int run_backproject(sr_plan p, float* image, const float* views, const int* d_vi
                                       d_vids + s0, nb, p->g, p->stream));
       }
       KTimer kt(p, sr::K_IMG_REDUCE);
-      HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), ng, target, asum, m, p->g, p->stream));
+      float* zout = (m == sr::IMG_EM && zimg_written) ? p->zimg.as<float>() : nullptr;
+      HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), ng, target, asum, m, p->g, p->stream, zout));
+      if (zout) *zimg_written = true;
       continue;
     }
     {
@@ -840,19 +845,46 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
       (rc = p->em_ratio.ensure(sizeof(float) * p->P * nsub)))
     return rc;
   HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P * nsub, p->stream));
-  for (int nb = 0; nb < nblocks; ++nb) {
+  if (p->zfast && p->vstride == 0 && p->separable) {
+    // one PSF for all views: the PSF adjoint of a ones view is the same
+    // volume H for every view, so A_v' 1 = rot2^T(H .* atten[v]); H is
+    // computed once and the gather applies the attenuation
+    if ((rc = ensure_batch_buffers(p, std::min(nsub, p->max_batch))) || (rc = ensure_batch_buffers(p, 1))) return rc;
+    const sr::Taps t = p->taps();
+    {
+      KTimer kt(p, sr::K_PSF_ADJ);
+      HIP_TRY(sr::launch_psf_adj(p->separable, p->ones.as<float>(), t, d_vids, 1, p->work.as<float>(), nullptr, 0,
+                                 p->nchunk_adj(1), p->g, p->stream, true));
+    }
+    for (int nb = 0; nb < nblocks; ++nb) {
+      float* as = p->em_asum.as<float>() + p->V * nb;
+      {
+        KTimer kt(p, sr::K_ROT_ADJ);
+        HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
+                                      d_vids + nb * nsub, nsub, p->g, p->stream, p->atten.as<float>(),
+                                      p->atten_vstride()));
+      }
+      {
+        KTimer kt(p, sr::K_IMG_REDUCE);
+        HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, as, nullptr, sr::IMG_WRITE, p->g, p->stream));
+      }
+      HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
+    }
+  }
+  for (int nb = 0; nb < nblocks && !(p->zfast && p->vstride == 0 && p->separable); ++nb) {
     float* as = p->em_asum.as<float>() + p->V * nb;
     if ((rc = run_backproject(p, as, p->ones.as<float>(), d_vids + nb * nsub, nsub, sr::IMG_WRITE, nullptr)))
       return rc;
     HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
   }
   HIP_TRY(hipMemcpyAsync(d_out, d_x0, sizeof(float) * p->V, hipMemcpyDeviceToDevice, p->stream));
+  bool zready = false;  // the EM update also refreshed the z-fastest image copy
   for (int it = 0; it < niter; ++it)
     for (int nb = 0; nb < nblocks; ++nb) {
       const int* sv = d_vids + nb * nsub;
-      if ((rc = run_project(p, p->em_ratio.as<float>(), d_out, sv, nsub, d_y, d_b))) return rc;
+      if ((rc = run_project(p, p->em_ratio.as<float>(), d_out, sv, nsub, d_y, d_b, zready))) return rc;
       if ((rc = run_backproject(p, d_out, p->em_ratio.as<float>(), sv, nsub, sr::IMG_EM,
-                                p->em_asum.as<float>() + p->V * nb)))
+                                p->em_asum.as<float>() + p->V * nb, &zready)))
         return rc;
     }
   if (host) {
