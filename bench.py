@@ -43,21 +43,26 @@ def workload():
 
 
 def algorithmic(cls, nb, g):
-    """(bytes, flops) of one launch of a kernel class over nb views."""
+    """(bytes, flops) of one launch of a kernel class over nb views, for the
+    z-fastest :two path: the PSF forward rotates and attenuates while it
+    stages (per view: read the attenuation volume and the image, write the
+    view), the PSF adjoint writes the attenuated volumes the rotation
+    adjoint gathers."""
     nx, ny, nz, p = g
     V, P = nx * ny * nz, nx * nz
-    if cls == "psf_fwd":  # separable clamped conv: 2 passes x p MACs per pixel per plane; read work, write partials
-        return 4 * V * nb, 2 * 2 * p * nx * nz * ny * nb
-    if cls == "psf_adj":  # read view, write attenuated plane stack, read attenuation
-        return 4 * (P + 2 * V) * nb, 2 * 2 * p * nx * nz * ny * nb
-    if cls == "rot_fwd":  # read image, read attenuation, write rotated volume
-        return 4 * 3 * V * nb, 0
-    if cls == "rot_adj":  # read work, write partial images (groups)
-        return 4 * 2 * V * nb, 0
+    sep_flops = 2 * 2 * p * nx * nz * ny * nb  # separable clamped conv: 2 passes x p MACs per voxel
+    if cls == "psf_fwd":  # read attenuation + image, write the view
+        return 4 * (2 * V + P) * nb, sep_flops
+    if cls == "psf_adj":  # read view and attenuation, write the attenuated volume
+        return 4 * (P + 2 * V) * nb, sep_flops
+    if cls == "rot_fwd":  # the image to its z-fastest copy
+        return 4 * 2 * V, 0
+    if cls == "rot_adj":  # read the volumes, write one image
+        return 4 * (V * nb + V), 0
     if cls == "view_reduce":
         return 4 * 2 * P * nb, 0
-    if cls == "img_reduce":
-        return 4 * 3 * V, 0
+    if cls == "img_reduce":  # EM update: read back, x, asum; write x
+        return 4 * 4 * V, 0
     return 0, 0
 
 

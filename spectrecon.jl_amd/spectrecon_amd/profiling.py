@@ -16,7 +16,7 @@ _CLASSES = (
     ("k_rot2_adj", "rot_adj"), ("k_rot1_adj", "rot_adj"),
     ("k_psf_fwd", "psf_fwd"), ("k_psf_adj", "psf_adj"),
     ("k_view_reduce", "view_reduce"), ("k_img_reduce", "img_reduce"),
-    ("k_atten_exact", "atten"), ("k_copy16", "copy"),
+    ("k_atten_exact", "atten"), ("k_atten_z", "atten"), ("k_to_zfast", "rot_fwd"), ("k_copy16", "copy"),
 )
 
 
@@ -41,6 +41,22 @@ def stats_markdown(kernel_stats_csv, command, title):
         out.append(f"| `{short_name(r['Name'])}` | {int(r['Calls'])} | {int(r['TotalDurationNs']) / 1e6:.2f} | "
                    f"{float(r['AverageNs']) / 1e3:.1f} | {int(r['MinNs']) / 1e3:.1f} | {int(r['MaxNs']) / 1e3:.1f} | "
                    f"{float(r['Percentage']):.1f} |")
+    return "\n".join(out) + "\n"
+
+
+def launch_shapes_markdown(kernel_trace_csv, min_calls=8):
+    """Per-(kernel, grid) duration table from a ``--kernel-trace`` CSV: the
+    launch shapes of one kernel (e.g. 16-view subset launches and 64-view
+    launches) averaged separately."""
+    groups = collections.defaultdict(list)
+    for r in csv.DictReader(open(kernel_trace_csv)):
+        grid = (int(r["Grid_Size_X"]), int(r["Grid_Size_Y"]), int(r["Grid_Size_Z"]))
+        groups[(short_name(r["Kernel_Name"]), grid)].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    rows = sorted(((k, v) for k, v in groups.items() if len(v) >= min_calls), key=lambda kv: -sum(kv[1]))
+    out = ["| kernel | grid (threads) | calls | avg us | median us |", "|---|---|---|---|---|"]
+    for (name, grid), v in rows:
+        out.append(f"| `{name}` | {grid[0]}x{grid[1]}x{grid[2]} | {len(v)} | {sum(v) / len(v) / 1e3:.1f} | "
+                   f"{_median(v) / 1e3:.1f} |")
     return "\n".join(out) + "\n"
 
 
