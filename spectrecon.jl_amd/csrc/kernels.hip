@@ -1626,6 +1626,177 @@ __global__ void __launch_bounds__(NW * 64) k_psf_adj_mfma(const float* __restric
   }
 }
 
+// --- adjoint, separable, matrix cores, column-owned (z-fastest) --------------
+// The adjoint in the other pass order (the two 1-D passes act on different
+// axes and commute): the x adjoint first, from the block's view staged once in
+// LDS and only read afterwards, then the z adjoint from a wave-private LDS
+// buffer.  Wave w owns the 16-wide x columns w, w + 8, ... with all nz/16 z
+// tiles of each, so no block barrier separates the passes or the planes (two
+// per tap group only):
+//   x adjoint  H[z][x0+n] = sum_k V[z][x0-hx+k] u'[k-n]
+//   z adjoint  R[z0+m][x] = sum_k w'[k-m] H[z0-hz+k][x]
+// (u', w': the taps, or their tail / head sums for output column / row 0 and
+// n-1, the transposed replicate boundary).  The x adjoint takes its operands
+// swapped, so a lane holds four consecutive x of one z and stores them as a
+// float4 into the wave buffer.  LDS: Vx (rows z, column XO + x, stride WX =
+// 4 odd mod 64: a fragment's 16 rows x 4 columns hit distinct banks), per wave
+// Hw (rows z + hz with zero pads, 16 columns: a fragment's 4 rows x 16 columns
+// hit distinct banks), tap groups.  grid (nchunk, nb), block 512.
+__host__ __device__ inline size_t mf_col_lds_floats(const Geo& g, int nt, int tbp) {
+  return (size_t)g.nz * mf_wx(g.nx, (g.px - 1) / 2, nt) + (size_t)MF_WAVES * mf_rq(g.nz, nt) * 16 +
+         (size_t)tbp * MfTaps(nt).size;
+}
+template <int TPW>
+__global__ void __launch_bounds__(MF_THREADS) k_psf_adj_col(const float* __restrict__ views, Taps t,
+                                                            const int* __restrict__ vids, int Yc,
+                                                            float* __restrict__ work, const float* __restrict__ atten,
+                                                            long atten_vstride, Geo g, int TBP) {
+  extern __shared__ float L[];
+  const int NT = t.ntx;
+  const MfTaps TT(NT);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
+  const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
+  const int ntx = nx >> 4;
+  const int WX = mf_wx(nx, hx, NT), XO = mf_xo(hx), RG = mf_rq(nz, NT);
+  const int b = blockIdx.y, c = blockIdx.x;
+  const int v = vids[b];
+  const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int m16 = lane & 15, q4 = lane >> 4;
+  float* Vx = L;
+  float* Hw = Vx + nz * WX + wave * RG * 16;
+  float* s_taps = Vx + nz * WX + MF_WAVES * RG * 16;  // TBP planes of TT.size
+  for (int k = tid; k < nz * WX + MF_WAVES * RG * 16 + TBP * TT.size; k += MF_THREADS) L[k] = 0.f;
+  __syncthreads();
+  const float* vin = views + (long)b * nx * nz;
+  const int nq = nx >> 2;
+  for (int k = tid; k < nz * nq; k += MF_THREADS) {
+    const int z = k / nq, x4 = k - z * nq;
+    *reinterpret_cast<float4*>(Vx + z * WX + XO + 4 * x4) = *reinterpret_cast<const float4*>(vin + z * nx + 4 * x4);
+  }
+  const long V = (long)nx * ny * nz;
+  float* wout = work + V * b;
+  const float* av = atten ? atten + atten_vstride * v : nullptr;
+  auto load_taps = [&](int yg, int ng) {
+    constexpr int ME = (8 * 6 * 64 + MF_THREADS - 1) / MF_THREADS;
+    const int n = ng * 6 * NT;
+    float tvv[ME];
+    int tdst[ME];
+#pragma unroll
+    for (int j = 0; j < ME; ++j) {
+      const int e = min(tid + j * MF_THREADS, n - 1);
+      const int pl = e / (6 * NT), r = e - pl * 6 * NT, tab = r / NT, tk = r - tab * NT;
+      const long plane = (long)v * t.vstride + yg + pl;
+      tdst[j] = pl * TT.size + tab * TT.tp + TT.off + tk;
+      const float* src = tab == 0 ? t.u + plane * NT : tab == 1 ? t.w + plane * NT : t.cum + (plane * 4 + tab - 2) * NT;
+      tvv[j] = src[tk];
+    }
+#pragma unroll
+    for (int j = 0; j < ME; ++j) s_taps[tdst[j]] = tvv[j];
+  };
+  for (int y = y0; y < y1; ++y) {
+    if ((y - y0) % TBP == 0) {
+      lds_barrier();  // the previous tap group is no longer read (the first time: Vx is in place)
+      load_taps(y, min(TBP, y1 - y));
+      lds_barrier();
+    }
+    const float* tb = s_taps + ((y - y0) % TBP) * TT.size;
+    const float* su = tb + TT.off + q4 - m16;
+    const float* sw = su + TT.tp;
+    const float* sut = sw + TT.tp;
+    const float* suh = sut + TT.tp;
+    const float* swt = suh + TT.tp;
+    const float* swh = swt + TT.tp;
+    const float* ap[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) ap[i] = (i == 0 && m16 == 0) ? swt : (16 * i + 16 == nz && m16 == 15) ? swh : sw;
+    for (int col = wave; col < ntx; col += MF_WAVES) {
+      const int x0 = 16 * col;
+      const long ob = ((long)y * nx + x0 + m16) * nz + 4 * q4;  // + 16 i: lane's 4 z of tile i
+      // attenuation factors of the column's tiles, in flight during the passes
+      float4 af[TPW];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i)
+        af[i] = av ? *reinterpret_cast<const float4*>(av + ob + 16 * i) : make_float4(1.f, 1.f, 1.f, 1.f);
+      // x adjoint (operands swapped: lane (k, n) register r = H[16i + n][x0 + 4k + r])
+      const float* bp = (x0 == 0 && m16 == 0) ? sut : (x0 + 16 == nx && m16 == 15) ? suh : su;
+      const float* va = Vx + m16 * WX + XO - hx + x0 + q4;
+      mf4 dh[TPW];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) dh[i] = mf4{0.f, 0.f, 0.f, 0.f};
+      {
+        float a0[TPW], a1[TPW], b0, b1;
+        auto ld = [&](float (&a)[TPW], float& bb, int s) {
+          bb = bp[4 * s];
+#pragma unroll
+          for (int i = 0; i < TPW; ++i) a[i] = va[16 * i * WX + 4 * s];
+        };
+        ld(a0, b0, 0);
+        int s = 0;
+        for (; s + 1 < nsx; s += 2) {
+          ld(a1, b1, s + 1);
+          SR_SCHED_FENCE();
+#pragma unroll
+          for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b0, a0[i], dh[i]);
+          SR_SCHED_FENCE();
+          ld(a0, b0, min(s + 2, nsx - 1));
+          SR_SCHED_FENCE();
+#pragma unroll
+          for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b1, a1[i], dh[i]);
+        }
+        if (s < nsx) {
+#pragma unroll
+          for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b0, a0[i], dh[i]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < TPW; ++i)
+        *reinterpret_cast<float4*>(Hw + (hz + 16 * i + m16) * 16 + 4 * q4) =
+            make_float4(dh[i][0], dh[i][1], dh[i][2], dh[i][3]);
+      // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      // z adjoint: lane (k, n) register r = R[16i + 4k + r][x0 + n]
+      mf4 dz[TPW];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) dz[i] = mf4{0.f, 0.f, 0.f, 0.f};
+      {
+        float bq0[TPW], bq1[TPW], a0[TPW], a1[TPW];
+        auto ld = [&](float (&bq)[TPW], float (&aa)[TPW], int s) {
+#pragma unroll
+          for (int i = 0; i < TPW; ++i) {
+            aa[i] = ap[i][4 * s];
+            bq[i] = Hw[(16 * i + q4 + 4 * s) * 16 + m16];
+          }
+        };
+        ld(bq0, a0, 0);
+        int s = 0;
+        for (; s + 1 < nsz; s += 2) {
+          ld(bq1, a1, s + 1);
+          SR_SCHED_FENCE();
+#pragma unroll
+          for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a0[i], bq0[i], dz[i]);
+          SR_SCHED_FENCE();
+          ld(bq0, a0, min(s + 2, nsz - 1));
+          SR_SCHED_FENCE();
+#pragma unroll
+          for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a1[i], bq1[i], dz[i]);
+        }
+        if (s < nsz) {
+#pragma unroll
+          for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a0[i], bq0[i], dz[i]);
+        }
+      }
+      asm volatile("" ::: "memory");  // the next column's Hw stores stay after these reads
+#pragma unroll
+      for (int i = 0; i < TPW; ++i)
+        *reinterpret_cast<float4*>(wout + ob + 16 * i) = make_float4(dz[i][0] * af[i].x, dz[i][1] * af[i].y,
+                                                                     dz[i][2] * af[i].z, dz[i][3] * af[i].w);
+    }
+  }
+}
+
 // --- adjoint, direct 2-D ------------------------------------------------------
 // Extended correlation E over [-hx, nx+hx) x [-hz, nz+hz) (the reference's
 // padded work matrix after the inverse FFT), then the replicate-pad fold:
@@ -2211,6 +2382,19 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
   if (nb <= 0) return hipSuccess;
   const int Yc = (g.ny + nchunk - 1) / nchunk;
   nchunk = (g.ny + Yc - 1) / Yc;
+  if (separable && zfast && psf_mfma_supported(g, t.ntx) && g.nz / 16 <= MF_TPW &&
+      sizeof(float) * mf_col_lds_floats(g, t.ntx, 1) <= kMaxLds) {
+    const size_t base = mf_col_lds_floats(g, t.ntx, 0), per = MfTaps(t.ntx).size;
+    const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), ((long)(kMaxLds / sizeof(float)) - (long)base) /
+                                                                         (long)per));
+    const size_t lds = sizeof(float) * mf_col_lds_floats(g, t.ntx, tbp);
+    SR_TPW_SWITCH(g.nz / 16, {
+      allow_lds(k_psf_adj_col<TPW>, lds);
+      k_psf_adj_col<TPW><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride, g,
+                                                                   tbp);
+    });
+    return hipGetLastError();
+  }
   if (separable && psf_mfma_supported(g, t.ntx)) {
     const int tbp = mf_tap_planes(g, t.ntx, Yc);
     const size_t lds = sizeof(float) * mf_lds_floats(g, t.ntx, tbp);
