@@ -1188,6 +1188,210 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
   }
 }
 
+// --- forward, separable, matrix cores, column-owned, :two rotation fused -----
+// z-fastest forward with one block barrier per plane.  Wave w owns the 16-wide
+// x column w (nx <= 128) with all nz/16 z tiles: the x pass reads the block's
+// staged plane, its result goes to a wave-private buffer, and the z pass reads
+// only that buffer.  The plane is double-buffered: plane y+1 is gathered
+// (rotation and attenuation fused, as in k_psf_fwd_mfma) and stored during
+// plane y's passes.  The plane buffers carry no pad rows: the x pass reads row
+// clamp(x0 - hx + k), which is the replicate-padded convolution itself, so
+// only the z pass needs its boundary step (zero pad rows in the wave buffer).
+//   x pass  Q[z][x0+n] = sum_k P[clamp(x0-hx+k)][z] u[n+2hx-k]
+//   z pass  R[z0+m][x] = sum_k w[m+2hz-k] Q[z0-hz+k][x] + in[0]Tail + in[nz-1]Head
+// LDS: two planes (rows x, stride WZ = 16 mod 64 floats), per wave Qw (rows
+// z + hz, 16 columns), TBP tap planes.  grid (nchunk, nb), block 512.
+__host__ __device__ inline size_t mf_fcol_lds_floats(const Geo& g, int nt, int tbp) {
+  return 2 * (size_t)g.nx * mf_wq(g.nz) + (size_t)MF_WAVES * mf_rq(g.nz, nt) * 16 + (size_t)tbp * MfTaps(nt).size;
+}
+template <int TPW>
+__global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restrict__ img_t, Taps t,
+                                                            const int* __restrict__ vids, int Yc,
+                                                            float* __restrict__ part, Geo g, int TBP,
+                                                            const float* __restrict__ atten, long atten_vstride,
+                                                            const RotView* __restrict__ rv) {
+  extern __shared__ float L[];
+  const int NT = t.ntx;
+  const MfTaps TT(NT);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
+  const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
+  const int ntx = nx >> 4;
+  const int WZ = mf_wq(nz), RG = mf_rq(nz, NT), PS = nx * WZ;
+  const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
+  const int v = vids[b];
+  const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int m16 = lane & 15, q4 = lane >> 4;
+  float* Pz = L;  // two plane buffers
+  float* Qw = L + 2 * PS + wave * RG * 16;
+  float* s_taps = L + 2 * PS + MF_WAVES * RG * 16;  // TBP planes of TT.size
+  for (int k = tid; k < 2 * PS + MF_WAVES * RG * 16 + TBP * TT.size; k += MF_THREADS) L[k] = 0.f;
+
+  // element slots: element tid + k*threads of the [x][z] plane, (x, z quad) packed as x << 8 | zq
+  const int nz4 = nz >> 2, nv = nx * nz4;
+  constexpr int RS = MF_RA + MF_RB;
+  int rxz[RS];
+  RotSlot ra[MF_RA], rb[MF_RB];
+  const RotView rot = rv[v];
+  const float* av = atten + atten_vstride * v;
+#pragma unroll
+  for (int k = 0; k < RS; ++k) {
+    const int i = min(tid + k * MF_THREADS, nv - 1), x = i / nz4;
+    rxz[k] = x << 8 | (i - x * nz4);
+  }
+  // (scheduling fences keep each part's gathers where they are issued, so
+  // that only one part is live at a time)
+  auto rot_half = [&](auto& h, int k0, int y) {
+    SR_SCHED_FENCE();
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(h) / sizeof(h[0])); ++k)
+      rot_issue(h[k], img_t, av, rot, nx, nz, g.pad, rxz[k0 + k] >> 8, y, 4 * (rxz[k0 + k] & 255));
+    SR_SCHED_FENCE();
+  };
+  auto rot_store = [&](auto& h, int k0, int y, float* buf) {
+    SR_SCHED_FENCE();
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(h) / sizeof(h[0])); ++k) {
+      const int x = rxz[k0 + k] >> 8, zq = rxz[k0 + k] & 255;
+      const float4 val = rot_value(h[k], rot, nx, g.pad, x, y);
+      *reinterpret_cast<float4*>(buf + x * WZ + 4 * zq) = make_float4(val.x, val.y, val.z, val.w);
+    }
+  };
+  auto load_taps = [&](int yg, int ng) {
+    constexpr int ME = (8 * 6 * 64 + MF_THREADS - 1) / MF_THREADS;
+    const int n = ng * 6 * NT;
+    float tvv[ME];
+    int tdst[ME];
+#pragma unroll
+    for (int j = 0; j < ME; ++j) {
+      const int e = min(tid + j * MF_THREADS, n - 1);
+      const int pl = e / (6 * NT), r = e - pl * 6 * NT, tab = r / NT, tk = r - tab * NT;
+      const long plane = (long)v * t.vstride + yg + pl;
+      tdst[j] = pl * TT.size + tab * TT.tp + TT.off + tk;
+      const float* src = tab == 0 ? t.u + plane * NT : tab == 1 ? t.w + plane * NT : t.cum + (plane * 4 + tab - 2) * NT;
+      tvv[j] = src[tk];
+    }
+#pragma unroll
+    for (int j = 0; j < ME; ++j) s_taps[tdst[j]] = tvv[j];
+  };
+  // waves past the last column recompute it and store nothing
+  const bool own = wave < ntx;
+  const int x0 = 16 * min(wave, ntx - 1);
+  mf4 dv[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) dv[i] = mf4{0.f, 0.f, 0.f, 0.f};
+  __syncthreads();  // zero fill done
+  if (y0 < y1) {  // the first plane: its latency is exposed once per block
+    rot_half(ra, 0, y0);
+    rot_store(ra, 0, y0, Pz);
+    rot_half(rb, MF_RA, y0);
+    rot_store(rb, MF_RA, y0, Pz);
+  }
+
+  for (int y = y0; y < y1; ++y) {
+    const int cur = (y - y0) & 1;
+    lds_barrier();  // plane y staged; every wave is done with plane y-1 (its buffer and taps)
+    if ((y - y0) % TBP == 0) {
+      load_taps(y, min(TBP, y1 - y));
+      lds_barrier();
+    }
+    const float* tb = s_taps + ((y - y0) % TBP) * TT.size;
+    const float* Pc = Pz + cur * PS;
+    float* Pn = Pz + (cur ^ 1) * PS;
+    const int yn = min(y + 1, y1 - 1);
+    rot_half(ra, 0, yn);
+    // x pass (operands swapped: lane (k, n) register r = Q[z0 + n][x0 + 4k + r])
+    const float* ub = tb + TT.off + m16 - q4 + 2 * hx;
+    mf4 dx[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) dx[i] = mf4{0.f, 0.f, 0.f, 0.f};
+    {
+      float a0[TPW], a1[TPW], b0, b1;
+      auto ld = [&](float (&a)[TPW], float& bb, int s) {
+        bb = ub[-4 * s];
+        const float* pr = Pc + clampi(x0 - hx + 4 * s + q4, 0, nx - 1) * WZ + m16;
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) a[i] = pr[16 * i];
+      };
+      ld(a0, b0, 0);
+      int s = 0;
+      for (; s + 1 < nsx; s += 2) {
+        ld(a1, b1, s + 1);
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b0, a0[i], dx[i]);
+        SR_SCHED_FENCE();
+        ld(a0, b0, min(s + 2, nsx - 1));
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b1, a1[i], dx[i]);
+      }
+      if (s < nsx) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b0, a0[i], dx[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < TPW; ++i)
+      *reinterpret_cast<float4*>(Qw + (hz + 16 * i + m16) * 16 + 4 * q4) =
+          make_float4(dx[i][0], dx[i][1], dx[i][2], dx[i][3]);
+    // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    rot_store(ra, 0, yn, Pn);
+    rot_half(rb, MF_RA, yn);
+    // z pass: accumulate the view sum, the replicate boundary first
+    const float* wz = tb + TT.tp + TT.off + m16 - q4 + 2 * hz;
+    const float* swt = tb + 4 * TT.tp + TT.off;
+    const float* swh = swt + TT.tp;
+    {
+      const float be = q4 == 0 ? Qw[hz * 16 + m16] : q4 == 1 ? Qw[(hz + nz - 1) * 16 + m16] : 0.f;
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        const int zm = 16 * i + m16;
+        const float ae = q4 == 0 ? tail_at(swt, NT, zm + hz + 1) : q4 == 1 ? head_at(swh, NT, zm + hz - nz) : 0.f;
+        dv[i] = mfma4(ae, be, dv[i]);
+      }
+    }
+    {
+      float bq0[TPW], bq1[TPW], a0, a1;
+      auto ld = [&](float (&bq)[TPW], float& aa, int s) {
+        aa = wz[-4 * s];
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) bq[i] = Qw[(16 * i + q4 + 4 * s) * 16 + m16];
+      };
+      ld(bq0, a0, 0);
+      int s = 0;
+      for (; s + 1 < nsz; s += 2) {
+        ld(bq1, a1, s + 1);
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a0, bq0[i], dv[i]);
+        SR_SCHED_FENCE();
+        ld(bq0, a0, min(s + 2, nsz - 1));
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a1, bq1[i], dv[i]);
+      }
+      if (s < nsz) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a0, bq0[i], dv[i]);
+      }
+    }
+    asm volatile("" ::: "memory");  // the next plane's Qw stores stay after these reads
+    rot_store(rb, MF_RA, yn, Pn);
+  }
+  if (own) {
+    float* pb = part + ((long)b * nchunk + c) * nx * nz;
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) pb[(long)(16 * i + 4 * q4 + r) * nx + x0 + m16] = dv[i][r];
+    }
+  }
+}
+
 // --- forward, direct 2-D ----------------------------------------------------
 // general (non-separable) PSF: strips of 8 x-outputs, pz rows of NT taps each
 template <int NT, int BLOCK>
@@ -2321,6 +2525,19 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
   nchunk = (g.ny + Yc - 1) / Yc;
   if (nchunk_used) *nchunk_used = nchunk;
   if (nb <= 0) return hipSuccess;
+  if (separable && zfast && psf_mfma_supported(g, t.ntx) && g.nx / 16 <= MF_WAVES && g.nz / 16 <= MF_TPW &&
+      sizeof(float) * mf_fcol_lds_floats(g, t.ntx, 1) <= kMaxLds) {
+    const size_t base = mf_fcol_lds_floats(g, t.ntx, 0), per = MfTaps(t.ntx).size;
+    const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), ((long)(kMaxLds / sizeof(float)) - (long)base) /
+                                                                         (long)per));
+    const size_t lds = sizeof(float) * mf_fcol_lds_floats(g, t.ntx, tbp);
+    SR_TPW_SWITCH(g.nz / 16, {
+      allow_lds(k_psf_fwd_col<TPW>, lds);
+      k_psf_fwd_col<TPW><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(work, t, vids, Yc, part, g, tbp, atten,
+                                                                   atten_vstride, rv);
+    });
+    return hipGetLastError();
+  }
   if (separable && psf_mfma_supported(g, t.ntx)) {
     const int tbp = mf_tap_planes(g, t.ntx, Yc);
     const size_t lds = sizeof(float) * mf_lds_floats(g, t.ntx, tbp);
