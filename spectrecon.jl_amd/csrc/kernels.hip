@@ -1227,7 +1227,8 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
   float* Pz = L;  // two plane buffers
   float* Qw = L + 2 * PS + wave * RG * 16;
   float* s_taps = L + 2 * PS + MF_WAVES * RG * 16;  // TBP planes of TT.size
-  for (int k = tid; k < 2 * PS + MF_WAVES * RG * 16 + TBP * TT.size; k += MF_THREADS) L[k] = 0.f;
+  // zeros where no plane data lands: the wave buffers' pad rows and the tap tables' pads
+  for (int k = tid; k < MF_WAVES * RG * 16 + TBP * TT.size; k += MF_THREADS) L[2 * PS + k] = 0.f;
 
   // element slots: element tid + k*threads of the [x][z] plane, (x, z quad) packed as x << 8 | zq
   const int nz4 = nz >> 2, nv = nx * nz4;
@@ -1282,18 +1283,19 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
   mf4 dv[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; ++i) dv[i] = mf4{0.f, 0.f, 0.f, 0.f};
-  __syncthreads();  // zero fill done
-  if (y0 < y1) {  // the first plane: its latency is exposed once per block
+  if (y0 < y1) {  // the first plane and tap group: all their loads in flight together
     rot_half(ra, 0, y0);
-    rot_store(ra, 0, y0, Pz);
     rot_half(rb, MF_RA, y0);
+    __syncthreads();  // zero fill done
+    load_taps(y0, min(TBP, y1 - y0));
+    rot_store(ra, 0, y0, Pz);
     rot_store(rb, MF_RA, y0, Pz);
   }
 
   for (int y = y0; y < y1; ++y) {
     const int cur = (y - y0) & 1;
     lds_barrier();  // plane y staged; every wave is done with plane y-1 (its buffer and taps)
-    if ((y - y0) % TBP == 0) {
+    if (y > y0 && (y - y0) % TBP == 0) {
       load_taps(y, min(TBP, y1 - y));
       lds_barrier();
     }
@@ -1872,14 +1874,22 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_adj_col(const float* __restr
   float* Vx = L;
   float* Hw = Vx + nz * WX + wave * RG * 16;
   float* s_taps = Vx + nz * WX + MF_WAVES * RG * 16;  // TBP planes of TT.size
-  for (int k = tid; k < nz * WX + MF_WAVES * RG * 16 + TBP * TT.size; k += MF_THREADS) L[k] = 0.f;
-  __syncthreads();
+  // zeros where no data lands: Vx's pad columns, the wave buffers, the tap tables
+  {
+    const int padc = WX - nx;  // columns [0, XO) and [XO + nx, WX) of every row
+    for (int k = tid; k < nz * padc; k += MF_THREADS) {
+      const int z = k / padc, cc = k - z * padc;
+      Vx[z * WX + (cc < XO ? cc : cc + nx)] = 0.f;
+    }
+    for (int k = tid; k < MF_WAVES * RG * 16 + TBP * TT.size; k += MF_THREADS) L[nz * WX + k] = 0.f;
+  }
   const float* vin = views + (long)b * nx * nz;
   const int nq = nx >> 2;
   for (int k = tid; k < nz * nq; k += MF_THREADS) {
     const int z = k / nq, x4 = k - z * nq;
     *reinterpret_cast<float4*>(Vx + z * WX + XO + 4 * x4) = *reinterpret_cast<const float4*>(vin + z * nx + 4 * x4);
   }
+  __syncthreads();  // (the tap tables' zeros land before their entries)
   const long V = (long)nx * ny * nz;
   float* wout = work + V * b;
   const float* av = atten ? atten + atten_vstride * v : nullptr;
