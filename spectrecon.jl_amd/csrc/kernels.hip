@@ -2201,72 +2201,88 @@ __global__ void __launch_bounds__(256) k_atten_z(float* __restrict__ mur, Geo g,
 // over the K (source i, weight w) slots of its plan-time list (the forward's
 // coefficients, w = the product of its two 1-D weights; unused slots carry
 // w = 0 and i = j), ent laid out [view][K][NN] so that neighbouring targets
-// read neighbouring slots.  Wave s of a block sums views s, s + VS, ... of the
-// batch for the block's PW pixels (one view per step, few registers: many
-// waves in flight hide the slot -> source load chain); the VS wave sums are
-// then added in wave order through LDS and written to part [y][x][z].
+// read neighbouring slots.  A lane takes Q adjacent z quads of one pixel
+// (nz/(4Q) lanes per pixel, PW = 64/(nz/(4Q)) pixels per wave).  Wave s of a
+// block sums views s, s + VS, ... of the batch for the block's PW pixels (one
+// view per step, few registers: many waves in flight hide the slot -> source
+// load chain); the VS wave sums are then added in wave order through LDS and
+// written to part [y][x][z].
 // HA: the source volume of view v is H .* atten[v] (H one volume shared by
 // all views, the product rounded as the PSF adjoint rounds it), for the
 // back-projection of a view-independent PSF adjoint result.
 // grid 8 * xcd_chunk(ceil(NN / PW)), block 64 * VS
-template <int K, int VS, bool HA>
+template <int K, int VS, int Q, bool HA>
 __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restrict__ work, float* __restrict__ part,
                                                           const int2* __restrict__ ent, const int* __restrict__ vids,
                                                           int nb, int C, Geo g, const float* __restrict__ atten,
                                                           long atten_vstride) {
-  __shared__ float4 red[VS][64];
-  const int N = g.nx, NN = N * N, nz = g.nz, nz4 = nz >> 2;
+  __shared__ float4 red[VS][64][Q];
+  const int N = g.nx, NN = N * N, nz = g.nz, nl = (nz >> 2) / Q, pw = 64 / nl;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const ZLane zl(lane, nz4);
+  const int sub = lane / nl, zl = lane - sub * nl;
   const int lb = xcd_block(C);
-  if (lb * zl.pw >= NN) return;  // whole block past the end (uniform)
-  const int j = lb * zl.pw + zl.sub;
-  const bool act = zl.sub < zl.pw && j < NN;
-  const int jj = min(j, NN - 1), zo = 4 * (act ? zl.z4 : 0);
+  if (lb * pw >= NN) return;  // whole block past the end (uniform)
+  const int j = lb * pw + sub;
+  const bool act = sub < pw && j < NN;
+  const int jj = min(j, NN - 1), zo = 4 * Q * (act ? zl : 0);
   const long V = (long)NN * nz;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 acc[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int b = wave; b < nb; b += VS) {
     const int2* ea = ent + (long)vids[b] * K * NN + jj;
     int2 sa[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) sa[k] = ea[(long)k * NN];
     const float* wa = HA ? work + zo : work + V * b + zo;
-    float4 pa[K];
+    float4 pa[K][Q];
 #pragma unroll
-    for (int k = 0; k < K; ++k) pa[k] = *reinterpret_cast<const float4*>(wa + (long)sa[k].x * nz);
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) pa[k][q] = *reinterpret_cast<const float4*>(wa + (long)sa[k].x * nz + 4 * q);
     if (HA) {
       const float* aa = atten + atten_vstride * vids[b] + zo;
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const float4 fa = *reinterpret_cast<const float4*>(aa + (long)sa[k].x * nz);
-        pa[k] = make_float4(pa[k].x * fa.x, pa[k].y * fa.y, pa[k].z * fa.z, pa[k].w * fa.w);
-      }
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const float4 fa = *reinterpret_cast<const float4*>(aa + (long)sa[k].x * nz + 4 * q);
+          pa[k][q] = make_float4(pa[k][q].x * fa.x, pa[k][q].y * fa.y, pa[k][q].z * fa.z, pa[k][q].w * fa.w);
+        }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const float w = __int_as_float(sa[k].y);
       if (w != 0.f) {
-        acc.x = acc.x + w * pa[k].x;
-        acc.y = acc.y + w * pa[k].y;
-        acc.z = acc.z + w * pa[k].z;
-        acc.w = acc.w + w * pa[k].w;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          acc[q].x = acc[q].x + w * pa[k][q].x;
+          acc[q].y = acc[q].y + w * pa[k][q].y;
+          acc[q].z = acc[q].z + w * pa[k][q].z;
+          acc[q].w = acc[q].w + w * pa[k][q].w;
+        }
       }
     }
   }
   if (VS > 1) {
-    red[wave][lane] = acc;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) red[wave][lane][q] = acc[q];
     __syncthreads();
     if (wave != 0) return;
 #pragma unroll
-    for (int s = 1; s < VS; ++s) {
-      const float4 o = red[s][lane];
-      acc.x = acc.x + o.x;
-      acc.y = acc.y + o.y;
-      acc.z = acc.z + o.z;
-      acc.w = acc.w + o.w;
-    }
+    for (int s = 1; s < VS; ++s)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const float4 o = red[s][lane][q];
+        acc[q].x = acc[q].x + o.x;
+        acc[q].y = acc[q].y + o.y;
+        acc[q].z = acc[q].z + o.z;
+        acc[q].w = acc[q].w + o.w;
+      }
   }
-  if (act) *reinterpret_cast<float4*>(part + (long)j * nz + zo) = acc;
+  if (act)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) *reinterpret_cast<float4*>(part + (long)j * nz + zo + 4 * q) = acc[q];
 }
 
 // Image reduction from z-fastest partials: part[g] [y][x][z] -> out [z][y][x]
@@ -2758,19 +2774,20 @@ hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, in
                              const Geo& g, hipStream_t s, const float* atten, long atten_vstride) {
   const bool ha = atten != nullptr;
   if (nb <= 0) return hipSuccess;
-  const int pw = 64 / (g.nz / 4);
+  const int q = (g.nz / 4) % 2 == 0 ? 2 : 1;  // z quads per lane
+  const int pw = 64 / (g.nz / 4 / q);
   const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
-  const int vs = nb <= 4 ? 2 : nb <= 8 ? 4 : nb <= 16 ? 8 : 16;
-#define SR_ELL(KK, VV) \
-  if (K == KK && vs == VV) { \
+  const int vs = nb <= 4 ? 2 : nb <= 8 ? 4 : 8;
+#define SR_ELL(KK, VV, QQ) \
+  if (K == KK && vs == VV && q == QQ) { \
     if (ha) \
-      k_rot2_adj_ell<KK, VV, true><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, atten, \
-                                                                 atten_vstride); \
+      k_rot2_adj_ell<KK, VV, QQ, true><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, atten, \
+                                                                     atten_vstride); \
     else \
-      k_rot2_adj_ell<KK, VV, false><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, nullptr, 0); \
+      k_rot2_adj_ell<KK, VV, QQ, false><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, nullptr, 0); \
     return hipGetLastError(); \
   }
-#define SR_ELL_K(KK) SR_ELL(KK, 2) SR_ELL(KK, 4) SR_ELL(KK, 8) SR_ELL(KK, 16)
+#define SR_ELL_K(KK) SR_ELL(KK, 2, 1) SR_ELL(KK, 4, 1) SR_ELL(KK, 8, 1) SR_ELL(KK, 2, 2) SR_ELL(KK, 4, 2) SR_ELL(KK, 8, 2)
   SR_ELL_K(4) SR_ELL_K(5) SR_ELL_K(6) SR_ELL_K(7) SR_ELL_K(8)
 #undef SR_ELL_K
 #undef SR_ELL
