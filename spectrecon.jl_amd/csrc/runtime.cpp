@@ -845,11 +845,16 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
       (rc = p->em_ratio.ensure(sizeof(float) * p->P * nsub)))
     return rc;
   HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P * nsub, p->stream));
+  // z-fastest plans whose subsets fit one launch batch run the whole loop on
+  // the z-fastest image copy: the rotation-adjoint gather applies the EM
+  // update in place, the normalisers are kept z-fastest too, and the caller's
+  // layout is written once at the end
+  const bool fused = p->zfast && nsub <= p->max_batch;
+  if ((rc = ensure_batch_buffers(p, std::min(nsub, p->max_batch)))) return rc;
   if (p->zfast && p->vstride == 0 && p->separable) {
     // one PSF for all views: the PSF adjoint of a ones view is the same
     // volume H for every view, so A_v' 1 = rot2^T(H .* atten[v]); H is
     // computed once and the gather applies the attenuation
-    if ((rc = ensure_batch_buffers(p, std::min(nsub, p->max_batch))) || (rc = ensure_batch_buffers(p, 1))) return rc;
     const sr::Taps t = p->taps();
     {
       KTimer kt(p, sr::K_PSF_ADJ);
@@ -860,33 +865,61 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
       float* as = p->em_asum.as<float>() + p->V * nb;
       {
         KTimer kt(p, sr::K_ROT_ADJ);
-        HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
-                                      d_vids + nb * nsub, nsub, p->g, p->stream, p->atten.as<float>(),
+        HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), fused ? as : p->ipart.as<float>(), p->zent.as<int2>(),
+                                      p->zk, d_vids + nb * nsub, nsub, p->g, p->stream, p->atten.as<float>(),
                                       p->atten_vstride()));
       }
-      {
+      if (!fused) {
         KTimer kt(p, sr::K_IMG_REDUCE);
         HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, as, nullptr, sr::IMG_WRITE, p->g, p->stream));
       }
       HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
     }
-  }
-  for (int nb = 0; nb < nblocks && !(p->zfast && p->vstride == 0 && p->separable); ++nb) {
-    float* as = p->em_asum.as<float>() + p->V * nb;
-    if ((rc = run_backproject(p, as, p->ones.as<float>(), d_vids + nb * nsub, nsub, sr::IMG_WRITE, nullptr)))
-      return rc;
-    HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
-  }
-  HIP_TRY(hipMemcpyAsync(d_out, d_x0, sizeof(float) * p->V, hipMemcpyDeviceToDevice, p->stream));
-  bool zready = false;  // the EM update also refreshed the z-fastest image copy
-  for (int it = 0; it < niter; ++it)
+  } else {
+    if (fused && (rc = p->em_back.ensure(sizeof(float) * p->V))) return rc;
     for (int nb = 0; nb < nblocks; ++nb) {
-      const int* sv = d_vids + nb * nsub;
-      if ((rc = run_project(p, p->em_ratio.as<float>(), d_out, sv, nsub, d_y, d_b, zready))) return rc;
-      if ((rc = run_backproject(p, d_out, p->em_ratio.as<float>(), sv, nsub, sr::IMG_EM,
-                                p->em_asum.as<float>() + p->V * nb, &zready)))
+      float* as = p->em_asum.as<float>() + p->V * nb;
+      if ((rc = run_backproject(p, fused ? p->em_back.as<float>() : as, p->ones.as<float>(), d_vids + nb * nsub, nsub,
+                                sr::IMG_WRITE, nullptr)))
         return rc;
+      if (fused) HIP_TRY(sr::launch_to_zfast(p->em_back.as<float>(), as, p->g, p->stream));
+      HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
     }
+  }
+  if (fused) {
+    {
+      KTimer kt(p, sr::K_ROT_FWD);
+      HIP_TRY(sr::launch_to_zfast(d_x0, p->zimg.as<float>(), p->g, p->stream));
+    }
+    const sr::Taps t = p->taps();
+    for (int it = 0; it < niter; ++it)
+      for (int nb = 0; nb < nblocks; ++nb) {
+        const int* sv = d_vids + nb * nsub;
+        if ((rc = run_project(p, p->em_ratio.as<float>(), nullptr, sv, nsub, d_y, d_b, true))) return rc;
+        {
+          KTimer kt(p, sr::K_PSF_ADJ);
+          HIP_TRY(sr::launch_psf_adj(p->separable, p->em_ratio.as<float>(), t, sv, nsub, p->work.as<float>(),
+                                     p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(nsub), p->g, p->stream,
+                                     true));
+        }
+        KTimer kt(p, sr::K_ROT_ADJ);
+        HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv, nsub,
+                                      p->g, p->stream, nullptr, 0, p->em_asum.as<float>() + p->V * nb));
+      }
+    KTimer kt(p, sr::K_IMG_REDUCE);
+    HIP_TRY(sr::launch_img_reduce_z(p->zimg.as<float>(), 1, d_out, nullptr, sr::IMG_WRITE, p->g, p->stream));
+  } else {
+    HIP_TRY(hipMemcpyAsync(d_out, d_x0, sizeof(float) * p->V, hipMemcpyDeviceToDevice, p->stream));
+    bool zready = false;  // the EM update also refreshed the z-fastest image copy
+    for (int it = 0; it < niter; ++it)
+      for (int nb = 0; nb < nblocks; ++nb) {
+        const int* sv = d_vids + nb * nsub;
+        if ((rc = run_project(p, p->em_ratio.as<float>(), d_out, sv, nsub, d_y, d_b, zready))) return rc;
+        if ((rc = run_backproject(p, d_out, p->em_ratio.as<float>(), sv, nsub, sr::IMG_EM,
+                                  p->em_asum.as<float>() + p->V * nb, &zready)))
+          return rc;
+      }
+  }
   if (host) {
     HIP_TRY(hipMemcpyAsync(out, d_out, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
