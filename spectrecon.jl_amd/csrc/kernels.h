@@ -99,10 +99,11 @@ hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_
 // part [y][x][z] = sum over the batch's views of rot2^T(work[b]) through the
 // K-slot per-target lists; with atten: of rot2^T(work .* atten[v]), work one
 // volume shared by the views; with em_asum: part is the z-fastest image x,
-// updated in place as x .*= sum ./ em_asum (z-fastest)
+// updated in place as x .*= sum ./ em_asum (z-fastest); zero_inf: zero sums
+// are written as Inf (OS-EM normalisers)
 hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
                              const Geo& g, hipStream_t s, const float* atten = nullptr, long atten_vstride = 0,
-                             const float* em_asum = nullptr);
+                             const float* em_asum = nullptr, bool zero_inf = false);
 // out [z][y][x] = combine(sum_g part[g] [y][x][z]) per ImgMode; zout
 // (optional) receives the new out as [y][x][z] too
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,

@@ -2211,13 +2211,15 @@ __global__ void __launch_bounds__(256) k_atten_z(float* __restrict__ mur, Geo g,
 // all views, the product rounded as the PSF adjoint rounds it), for the
 // back-projection of a view-independent PSF adjoint result.
 // em_asum (optional): part holds the z-fastest image x and is updated in place,
-// x .*= back ./ asum (src/ml-os-em.jl:46), asum z-fastest too.
+// x .*= back ./ asum (src/ml-os-em.jl:46), asum z-fastest too.  zero_inf: the
+// sums are OS-EM normalisers, zeros written as Inf (src/ml-os-em.jl:110).
 // grid 8 * xcd_chunk(ceil(NN / PW)), block 64 * VS
 template <int K, int VS, int Q, bool HA>
 __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restrict__ work, float* __restrict__ part,
                                                           const int2* __restrict__ ent, const int* __restrict__ vids,
                                                           int nb, int C, Geo g, const float* __restrict__ atten,
-                                                          long atten_vstride, const float* __restrict__ em_asum) {
+                                                          long atten_vstride, const float* __restrict__ em_asum,
+                                                          int zero_inf) {
   __shared__ float4 red[VS][64][Q];
   const int N = g.nx, NN = N * N, nz = g.nz, nl = (nz >> 2) / Q, pw = 64 / nl;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -2290,6 +2292,10 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
       const float4 a = *reinterpret_cast<const float4*>(em_asum + (long)j * nz + zo + 4 * q);
       const float4 x = *o;
       *o = make_float4(x.x * (acc[q].x / a.x), x.y * (acc[q].y / a.y), x.z * (acc[q].z / a.z), x.w * (acc[q].w / a.w));
+    } else if (zero_inf) {
+      const float inf = __builtin_inff();
+      *o = make_float4(acc[q].x == 0.f ? inf : acc[q].x, acc[q].y == 0.f ? inf : acc[q].y,
+                       acc[q].z == 0.f ? inf : acc[q].z, acc[q].w == 0.f ? inf : acc[q].w);
     } else {
       *o = acc[q];
     }
@@ -2783,7 +2789,7 @@ hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_
 
 hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
                              const Geo& g, hipStream_t s, const float* atten, long atten_vstride,
-                             const float* em_asum) {
+                             const float* em_asum, bool zero_inf) {
   const bool ha = atten != nullptr;
   if (nb <= 0) return hipSuccess;
   const int q = (g.nz / 4) % 2 == 0 ? 2 : 1;  // z quads per lane
@@ -2794,10 +2800,10 @@ hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, in
   if (K == KK && vs == VV && q == QQ) { \
     if (ha) \
       k_rot2_adj_ell<KK, VV, QQ, true><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, atten, \
-                                                                     atten_vstride, em_asum); \
+                                                                     atten_vstride, em_asum, zero_inf); \
     else \
       k_rot2_adj_ell<KK, VV, QQ, false><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, nullptr, 0, \
-                                                                      em_asum); \
+                                                                      em_asum, zero_inf); \
     return hipGetLastError(); \
   }
 #define SR_ELL_K(KK) SR_ELL(KK, 2, 1) SR_ELL(KK, 4, 1) SR_ELL(KK, 8, 1) SR_ELL(KK, 2, 2) SR_ELL(KK, 4, 2) SR_ELL(KK, 8, 2)

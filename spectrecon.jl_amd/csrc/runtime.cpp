@@ -867,13 +867,15 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
         KTimer kt(p, sr::K_ROT_ADJ);
         HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), fused ? as : p->ipart.as<float>(), p->zent.as<int2>(),
                                       p->zk, d_vids + nb * nsub, nsub, p->g, p->stream, p->atten.as<float>(),
-                                      p->atten_vstride()));
+                                      p->atten_vstride(), nullptr, fused));
       }
       if (!fused) {
-        KTimer kt(p, sr::K_IMG_REDUCE);
-        HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, as, nullptr, sr::IMG_WRITE, p->g, p->stream));
+        {
+          KTimer kt(p, sr::K_IMG_REDUCE);
+          HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, as, nullptr, sr::IMG_WRITE, p->g, p->stream));
+        }
+        HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
       }
-      HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
     }
   } else {
     if (fused && (rc = p->em_back.ensure(sizeof(float) * p->V))) return rc;
