@@ -57,12 +57,12 @@ def algorithmic(cls, nb, g):
         return 4 * (P + 2 * V) * nb, sep_flops
     if cls == "rot_fwd":  # the image to its z-fastest copy
         return 4 * 2 * V, 0
-    if cls == "rot_adj":  # read the volumes, write one image
-        return 4 * (V * nb + V), 0
+    if cls == "rot_adj":  # read the volumes, x and the normaliser; write x (the EM update is fused)
+        return 4 * (V * nb + 3 * V), 0
     if cls == "view_reduce":
         return 4 * 2 * P * nb, 0
-    if cls == "img_reduce":  # EM update: read back, x, asum; write x
-        return 4 * 4 * V, 0
+    if cls == "img_reduce":  # the z-fastest image to the caller's layout
+        return 4 * 2 * V, 0
     return 0, 0
 
 
