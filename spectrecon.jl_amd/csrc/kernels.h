@@ -89,6 +89,12 @@ bool zfast_supported(const Geo& g);
 // padded to *K slots ([view][K][nx*nx]); false when a list exceeds kMaxEll
 constexpr int kMaxEll = 8;
 bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::vector<int2>& ent);
+// OS-EM update of the z-fastest image x with the subset normaliser computed
+// in the same pass: asum = sum rot2^T(hvol .* atten[v]) (zeros as Inf, written
+// to asum), x .*= (sum rot2^T(work[b])) ./ asum
+hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* ent, int K, const int* vids, int nb,
+                                     const Geo& g, hipStream_t s, const float* hvol, const float* atten,
+                                     long atten_vstride, float* asum);
 // img [z][y][x] -> out [y][x][z]
 hipError_t launch_to_zfast(const float* img, float* out, const Geo& g, hipStream_t s);
 // out[b] = atten[v] .* rot2(img_t) per view, all [y][x][z]; atten may be null

@@ -2207,20 +2207,25 @@ __global__ void __launch_bounds__(256) k_atten_z(float* __restrict__ mur, Geo g,
 // view per step, few registers: many waves in flight hide the slot -> source
 // load chain); the VS wave sums are then added in wave order through LDS and
 // written to part [y][x][z].
-// HA: the source volume of view v is H .* atten[v] (H one volume shared by
-// all views, the product rounded as the PSF adjoint rounds it), for the
-// back-projection of a view-independent PSF adjoint result.
-// em_asum (optional): part holds the z-fastest image x and is updated in place,
-// x .*= back ./ asum (src/ml-os-em.jl:46), asum z-fastest too.  zero_inf: the
-// sums are OS-EM normalisers, zeros written as Inf (src/ml-os-em.jl:110).
+// MODE 0: sources work[b].  MODE 1 (normaliser): the source volume of view v
+// is H .* atten[v] (H one volume, hvol, shared by all views, the product
+// rounded as the PSF adjoint rounds it), for the back-projection of a
+// view-independent PSF adjoint result.  MODE 2: both sums in one pass (the
+// slots are loaded once): the back-projection of work and the normaliser;
+// the normaliser goes to em_asum (zeros as Inf) and drives the EM update.
+// em_asum (MODE 0, optional): part holds the z-fastest image x and is updated
+// in place, x .*= back ./ asum (src/ml-os-em.jl:46), asum z-fastest too.
+// zero_inf (MODE 0/1): the sums are OS-EM normalisers, zeros written as Inf
+// (src/ml-os-em.jl:110).
 // grid 8 * xcd_chunk(ceil(NN / PW)), block 64 * VS
-template <int K, int VS, int Q, bool HA>
+template <int K, int VS, int Q, int MODE>
 __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restrict__ work, float* __restrict__ part,
                                                           const int2* __restrict__ ent, const int* __restrict__ vids,
-                                                          int nb, int C, Geo g, const float* __restrict__ atten,
-                                                          long atten_vstride, const float* __restrict__ em_asum,
-                                                          int zero_inf) {
-  __shared__ float4 red[VS][64][Q];
+                                                          int nb, int C, Geo g, const float* __restrict__ hvol,
+                                                          const float* __restrict__ atten, long atten_vstride,
+                                                          float* __restrict__ em_asum, int zero_inf) {
+  constexpr int NS = MODE == 2 ? 2 : 1;  // sums carried
+  __shared__ float4 red[VS][64][Q * NS];
   const int N = g.nx, NN = N * N, nz = g.nz, nl = (nz >> 2) / Q, pw = 64 / nl;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int sub = lane / nl, zl = lane - sub * nl;
@@ -2230,53 +2235,65 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
   const bool act = sub < pw && j < NN;
   const int jj = min(j, NN - 1), zo = 4 * Q * (act ? zl : 0);
   const long V = (long)NN * nz;
-  float4 acc[Q];
+  float4 acc[Q * NS];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int q = 0; q < Q * NS; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto fma4 = [](float4& a, float w, const float4& p) {
+    a.x = a.x + w * p.x;
+    a.y = a.y + w * p.y;
+    a.z = a.z + w * p.z;
+    a.w = a.w + w * p.w;
+  };
   for (int b = wave; b < nb; b += VS) {
     const int2* ea = ent + (long)vids[b] * K * NN + jj;
     int2 sa[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) sa[k] = ea[(long)k * NN];
-    const float* wa = HA ? work + zo : work + V * b + zo;
-    float4 pa[K][Q];
+    if (MODE != 1) {  // back-projection of work[b]
+      const float* wa = work + V * b + zo;
+      float4 pa[K][Q];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
+      for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int q = 0; q < Q; ++q) pa[k][q] = *reinterpret_cast<const float4*>(wa + (long)sa[k].x * nz + 4 * q);
-    if (HA) {
+        for (int q = 0; q < Q; ++q) pa[k][q] = *reinterpret_cast<const float4*>(wa + (long)sa[k].x * nz + 4 * q);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float w = __int_as_float(sa[k].y);
+        if (w != 0.f)
+#pragma unroll
+          for (int q = 0; q < Q; ++q) fma4(acc[q], w, pa[k][q]);
+      }
+    }
+    if (MODE != 0) {  // normaliser: H .* atten[v]
+      const float* ha = hvol + zo;
       const float* aa = atten + atten_vstride * vids[b] + zo;
+      float4 pa[K][Q];
 #pragma unroll
       for (int k = 0; k < K; ++k)
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
+          const float4 h = *reinterpret_cast<const float4*>(ha + (long)sa[k].x * nz + 4 * q);
           const float4 fa = *reinterpret_cast<const float4*>(aa + (long)sa[k].x * nz + 4 * q);
-          pa[k][q] = make_float4(pa[k][q].x * fa.x, pa[k][q].y * fa.y, pa[k][q].z * fa.z, pa[k][q].w * fa.w);
+          pa[k][q] = make_float4(h.x * fa.x, h.y * fa.y, h.z * fa.z, h.w * fa.w);
         }
-    }
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const float w = __int_as_float(sa[k].y);
-      if (w != 0.f) {
+      for (int k = 0; k < K; ++k) {
+        const float w = __int_as_float(sa[k].y);
+        if (w != 0.f)
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          acc[q].x = acc[q].x + w * pa[k][q].x;
-          acc[q].y = acc[q].y + w * pa[k][q].y;
-          acc[q].z = acc[q].z + w * pa[k][q].z;
-          acc[q].w = acc[q].w + w * pa[k][q].w;
-        }
+          for (int q = 0; q < Q; ++q) fma4(acc[(NS - 1) * Q + q], w, pa[k][q]);
       }
     }
   }
   if (VS > 1) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) red[wave][lane][q] = acc[q];
+    for (int q = 0; q < Q * NS; ++q) red[wave][lane][q] = acc[q];
     __syncthreads();
     if (wave != 0) return;
 #pragma unroll
     for (int s = 1; s < VS; ++s)
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
+      for (int q = 0; q < Q * NS; ++q) {
         const float4 o = red[s][lane][q];
         acc[q].x = acc[q].x + o.x;
         acc[q].y = acc[q].y + o.y;
@@ -2285,19 +2302,25 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
       }
   }
   if (!act) return;
+  const float inf = __builtin_inff();
+  auto zinf = [&](const float4& a) {
+    return make_float4(a.x == 0.f ? inf : a.x, a.y == 0.f ? inf : a.y, a.z == 0.f ? inf : a.z, a.w == 0.f ? inf : a.w);
+  };
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    float4* o = reinterpret_cast<float4*>(part + (long)j * nz + zo + 4 * q);
-    if (em_asum) {
-      const float4 a = *reinterpret_cast<const float4*>(em_asum + (long)j * nz + zo + 4 * q);
-      const float4 x = *o;
-      *o = make_float4(x.x * (acc[q].x / a.x), x.y * (acc[q].y / a.y), x.z * (acc[q].z / a.z), x.w * (acc[q].w / a.w));
-    } else if (zero_inf) {
-      const float inf = __builtin_inff();
-      *o = make_float4(acc[q].x == 0.f ? inf : acc[q].x, acc[q].y == 0.f ? inf : acc[q].y,
-                       acc[q].z == 0.f ? inf : acc[q].z, acc[q].w == 0.f ? inf : acc[q].w);
+    const long o = (long)j * nz + zo + 4 * q;
+    float4* po = reinterpret_cast<float4*>(part + o);
+    if (MODE == 2) {
+      const float4 a = zinf(acc[Q + q]);
+      *reinterpret_cast<float4*>(em_asum + o) = a;
+      const float4 x = *po;
+      *po = make_float4(x.x * (acc[q].x / a.x), x.y * (acc[q].y / a.y), x.z * (acc[q].z / a.z), x.w * (acc[q].w / a.w));
+    } else if (em_asum) {
+      const float4 a = *reinterpret_cast<const float4*>(em_asum + o);
+      const float4 x = *po;
+      *po = make_float4(x.x * (acc[q].x / a.x), x.y * (acc[q].y / a.y), x.z * (acc[q].z / a.z), x.w * (acc[q].w / a.w));
     } else {
-      *o = acc[q];
+      *po = zero_inf ? zinf(acc[q]) : acc[q];
     }
   }
 }
@@ -2787,30 +2810,48 @@ hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
-                             const Geo& g, hipStream_t s, const float* atten, long atten_vstride,
-                             const float* em_asum, bool zero_inf) {
-  const bool ha = atten != nullptr;
+static hipError_t launch_ell(int mode, const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
+                             const Geo& g, hipStream_t s, const float* hvol, const float* atten, long atten_vstride,
+                             float* em_asum, bool zero_inf) {
   if (nb <= 0) return hipSuccess;
   const int q = (g.nz / 4) % 2 == 0 ? 2 : 1;  // z quads per lane
   const int pw = 64 / (g.nz / 4 / q);
   const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
   const int vs = nb <= 4 ? 2 : nb <= 8 ? 4 : 8;
-#define SR_ELL(KK, VV, QQ) \
-  if (K == KK && vs == VV && q == QQ) { \
-    if (ha) \
-      k_rot2_adj_ell<KK, VV, QQ, true><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, atten, \
-                                                                     atten_vstride, em_asum, zero_inf); \
-    else \
-      k_rot2_adj_ell<KK, VV, QQ, false><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, nullptr, 0, \
-                                                                      em_asum, zero_inf); \
-    return hipGetLastError(); \
+  const int zi = zero_inf ? 1 : 0;
+#define SR_ELL(KK, VV, QQ)                                                                                            \
+  if (K == KK && vs == VV && q == QQ) {                                                                             \
+    if (mode == 0)                                                                                                  \
+      k_rot2_adj_ell<KK, VV, QQ, 0><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, hvol, atten,      \
+                                                                  atten_vstride, em_asum, zi);                      \
+    else if (mode == 1)                                                                                             \
+      k_rot2_adj_ell<KK, VV, QQ, 1><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, hvol, atten,      \
+                                                                  atten_vstride, em_asum, zi);                      \
+    else                                                                                                            \
+      k_rot2_adj_ell<KK, VV, QQ, 2><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, hvol, atten,      \
+                                                                  atten_vstride, em_asum, zi);                      \
+    return hipGetLastError();                                                                                       \
   }
 #define SR_ELL_K(KK) SR_ELL(KK, 2, 1) SR_ELL(KK, 4, 1) SR_ELL(KK, 8, 1) SR_ELL(KK, 2, 2) SR_ELL(KK, 4, 2) SR_ELL(KK, 8, 2)
   SR_ELL_K(4) SR_ELL_K(5) SR_ELL_K(6) SR_ELL_K(7) SR_ELL_K(8)
 #undef SR_ELL_K
 #undef SR_ELL
   return hipErrorInvalidConfiguration;
+}
+
+hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
+                             const Geo& g, hipStream_t s, const float* atten, long atten_vstride,
+                             const float* em_asum, bool zero_inf) {
+  // (with atten: work is H, the normaliser form)
+  return atten ? launch_ell(1, nullptr, part, ent, K, vids, nb, g, s, work, atten, atten_vstride, nullptr, zero_inf)
+               : launch_ell(0, work, part, ent, K, vids, nb, g, s, nullptr, nullptr, 0, const_cast<float*>(em_asum),
+                            zero_inf);
+}
+
+hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* ent, int K, const int* vids, int nb,
+                                     const Geo& g, hipStream_t s, const float* hvol, const float* atten,
+                                     long atten_vstride, float* asum) {
+  return launch_ell(2, work, x, ent, K, vids, nb, g, s, hvol, atten, atten_vstride, asum, true);
 }
 
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,

@@ -210,6 +210,7 @@ struct sr_plan_s {
   DBuf em_ratio, em_back, em_asum;
   DBuf flag;  // one int: device-side argument checks
   DBuf zimg, zent;  // z-fastest image copy, per-target rotation lists
+  DBuf zh;          // OS-EM: the PSF adjoint of a ones view (one PSF stack for all views)
   int zk = 0;        // slots per target of zent
   std::vector<int> h_vids;
 
@@ -681,7 +682,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   }
   HIP_TRY(hipStreamSynchronize(p->stream));
   p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->taps_c.n + p->rotv.n + p->tab_b.n +
-             p->tab_w.n + p->zimg.n + p->zent.n;
+             p->tab_w.n + p->zimg.n + p->zent.n;  // (zh is a per-call buffer)
   *out = guard.release();
   return SR_OK;
 }
@@ -850,34 +851,35 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
   // update in place, the normalisers are kept z-fastest too, and the caller's
   // layout is written once at the end
   const bool fused = p->zfast && nsub <= p->max_batch;
+  // one PSF for all views: the PSF adjoint of a ones view is the same volume
+  // H for every view, so A_v' 1 = rot2^T(H .* atten[v]); H is computed once.
+  // On the fused path each normaliser is then summed by the first
+  // iteration's gather of its subset, in the same pass as the back-projection
+  const bool shared = p->zfast && p->vstride == 0 && p->separable;
   if ((rc = ensure_batch_buffers(p, std::min(nsub, p->max_batch)))) return rc;
-  if (p->zfast && p->vstride == 0 && p->separable) {
-    // one PSF for all views: the PSF adjoint of a ones view is the same
-    // volume H for every view, so A_v' 1 = rot2^T(H .* atten[v]); H is
-    // computed once and the gather applies the attenuation
+  if (shared) {
+    if ((rc = p->zh.ensure(sizeof(float) * p->V))) return rc;
     const sr::Taps t = p->taps();
-    {
-      KTimer kt(p, sr::K_PSF_ADJ);
-      HIP_TRY(sr::launch_psf_adj(p->separable, p->ones.as<float>(), t, d_vids, 1, p->work.as<float>(), nullptr, 0,
-                                 p->nchunk_adj(1), p->g, p->stream, true));
-    }
+    KTimer kt(p, sr::K_PSF_ADJ);
+    HIP_TRY(sr::launch_psf_adj(p->separable, p->ones.as<float>(), t, d_vids, 1, p->zh.as<float>(), nullptr, 0,
+                               p->nchunk_adj(1), p->g, p->stream, true));
+  }
+  if (shared && !fused) {
     for (int nb = 0; nb < nblocks; ++nb) {
       float* as = p->em_asum.as<float>() + p->V * nb;
       {
         KTimer kt(p, sr::K_ROT_ADJ);
-        HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), fused ? as : p->ipart.as<float>(), p->zent.as<int2>(),
-                                      p->zk, d_vids + nb * nsub, nsub, p->g, p->stream, p->atten.as<float>(),
-                                      p->atten_vstride(), nullptr, fused));
+        HIP_TRY(sr::launch_rot2_adj_z(p->zh.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
+                                      d_vids + nb * nsub, nsub, p->g, p->stream, p->atten.as<float>(),
+                                      p->atten_vstride()));
       }
-      if (!fused) {
-        {
-          KTimer kt(p, sr::K_IMG_REDUCE);
-          HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, as, nullptr, sr::IMG_WRITE, p->g, p->stream));
-        }
-        HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
+      {
+        KTimer kt(p, sr::K_IMG_REDUCE);
+        HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, as, nullptr, sr::IMG_WRITE, p->g, p->stream));
       }
+      HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
     }
-  } else {
+  } else if (!shared) {
     if (fused && (rc = p->em_back.ensure(sizeof(float) * p->V))) return rc;
     for (int nb = 0; nb < nblocks; ++nb) {
       float* as = p->em_asum.as<float>() + p->V * nb;
@@ -905,8 +907,14 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
                                      true));
         }
         KTimer kt(p, sr::K_ROT_ADJ);
-        HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv, nsub,
-                                      p->g, p->stream, nullptr, 0, p->em_asum.as<float>() + p->V * nb));
+        float* as = p->em_asum.as<float>() + p->V * nb;
+        if (shared && it == 0)  // this subset's normaliser, summed in the same pass
+          HIP_TRY(sr::launch_rot2_adj_z_em_norm(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk,
+                                                sv, nsub, p->g, p->stream, p->zh.as<float>(), p->atten.as<float>(),
+                                                p->atten_vstride(), as));
+        else
+          HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv,
+                                        nsub, p->g, p->stream, nullptr, 0, as));
       }
     KTimer kt(p, sr::K_IMG_REDUCE);
     HIP_TRY(sr::launch_img_reduce_z(p->zimg.as<float>(), 1, d_out, nullptr, sr::IMG_WRITE, p->g, p->stream));
