@@ -145,6 +145,18 @@ int sr_mlem(sr_plan plan, float* out, const float* x0, const float* y, const flo
 int sr_osem(sr_plan plan, float* out, const float* x0, const float* y, const float* b, int32_t nblocks,
             int32_t niter, int32_t mem);
 
+/* ---- gradients ----------------------------------------------------------------
+ * No reference counterpart (the reference differentiates only through the
+ * linear operators, docs/lit/examples/6-dl.jl:145-156): for
+ * L = sum over the indexed views of ||A(x, mu) - y||^2 with mu the plan's
+ * mu-map, *loss = L, grad_x = dL/dx = A'(2 (A x - y)) and grad_mu = dL/dmu
+ * through the rotated exponential cumulative sum of the attenuation.  y holds
+ * the indexed views in order ([nx,nz,nindex]).  Needs a z-fastest plan
+ * (sr_plan_layout); SR_ERR_UNSUPPORTED otherwise.  `loss` is a host pointer;
+ * the call returns after the stream finished. */
+int sr_grad(sr_plan plan, double* loss, float* grad_x, float* grad_mu, const float* image, const float* y,
+            const int32_t* index, int32_t nindex, int32_t mem);
+
 /* ---- unit operators ----------------------------------------------------------
  * sr_imrotate replaces imrotate!/imrotate_adj! on a stack of nz square n x n
  * slices (src/rotatez.jl:240-416, 3-D forms :472-495). */

@@ -46,6 +46,8 @@ def lib():
             getattr(L, f"or_backproject_{suf}").argtypes = [fp, fp] + common + [_i32p, C.c_int, C.c_int, C.c_int]
             getattr(L, f"or_mlem_{suf}").argtypes = [fp, fp, fp, fp] + common + [C.c_int, C.c_int, C.c_int]
             getattr(L, f"or_osem_{suf}").argtypes = [fp, fp, fp, fp] + common + [C.c_int, C.c_int, C.c_int, C.c_int]
+            getattr(L, f"or_grad_{suf}").argtypes = [fp, fp, fp, fp] + common + [_i32p, C.c_int, C.c_int, C.c_int]
+            getattr(L, f"or_grad_{suf}").restype = C.c_double
             getattr(L, f"or_atten_{suf}").argtypes = [fp, fp, C.c_int, C.c_int, C.c_int, ft, C.c_double, C.c_int]
             getattr(L, f"or_psf_gauss_{suf}").argtypes = [fp, C.c_int, C.c_int, C.c_int, _f64p, _f64p]
         L.or_rot_padsize.argtypes = [C.c_int]
@@ -164,6 +166,17 @@ class Problem:
         getattr(lib(), f"or_osem_{_suf(self.dtype)}")(out, _F(x0, self.dtype), _F(y, self.dtype), _F(b, self.dtype),
                                                       *self._common(), nblocks, niter, self.nthread, self.conv)
         return out
+
+    def grad(self, image, y, index=None):
+        """(L, dL/dx, dL/dmu) for L = sum over the indexed views of ||A x - y||^2,
+        y holding the indexed views in order; mu is the plan's mu-map."""
+        index = np.arange(self.nview, dtype=np.int32) if index is None else np.ascontiguousarray(index, np.int32)
+        gx = np.zeros((self.nx, self.ny, self.nz), self.dtype, order="F")
+        gmu = np.zeros((self.nx, self.ny, self.nz), self.dtype, order="F")
+        loss = getattr(lib(), f"or_grad_{_suf(self.dtype)}")(gx, gmu, _F(image, self.dtype), _F(y, self.dtype),
+                                                             *self._common(), index, len(index), self.nthread,
+                                                             self.conv)
+        return float(loss), gx, gmu
 
     def atten(self, view: int):
         """Attenuation factor exp(-dy(Σ_{j≤y} μr_j - μr_y/2)) for one view."""

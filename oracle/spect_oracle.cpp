@@ -565,6 +565,51 @@ void backproject_view(const Problem<T>& P, ViewWork<T>& W, T* image, const T* vi
   for (long k = 0; k < sxy * nz; ++k) image[k] += W.imgr[k];
 }
 
+// Gradient w.r.t. the mu-map of L = sum_v ||A_v(x, mu) - y_v||^2 through one
+// view, accumulated into gmu (no reference counterpart: the reference's
+// forward, src/project.jl:77-135, differentiated by hand).  resid = dL/dA_v =
+// 2 (A_v x - y_v).  With imgr = R_v x, mur = R_v mu and the attenuation
+// a_y = exp(e_y), e_y = -dy (sum_{j<y} mur_j + mur_y / 2):
+//   G_y = conv_y^T(resid)                    (dL / d(imgr_y .* a_y))
+//   E_y = G_y .* imgr_y .* a_y               (dL / de_y)
+//   dL/dmur_j = -dy (sum_{y>j} E_y + E_j / 2)
+//   dL/dmu += R_v^T(dL/dmur)
+template <class T>
+void grad_mu_view(const Problem<T>& P, ViewWork<T>& W, T* gmu, const T* image, const T* resid, int viewidx) {
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const long sxy = (long)nx * ny;
+  const double th = P.angles[viewidx];
+  for (int z = 0; z < nz; ++z) {
+    imrotate(W.imgr.data() + sxy * z, image + sxy * z, th, W.rot, P.method, false);
+    imrotate(W.mumapr.data() + sxy * z, P.mumap + sxy * z, th, W.rot, P.method, false);
+  }
+  const T* psf_v = P.psfs + (long)P.px * P.pz * ny * viewidx;
+  // E into add_img
+  for (int y = 0; y < ny; ++y) {
+    atten_plane(W.e.data(), W.mumapr.data(), nx, ny, nz, y, P.dy);
+    conv_plane(P, W, W.add_img.data() + (long)nx * y, sxy, resid, nx, psf_v + (long)P.px * P.pz * y, true);
+    for (long z = 0; z < nz; ++z)
+      for (long x = 0; x < nx; ++x) {
+        const long k = x + nx * y + sxy * z;
+        W.add_img[k] = W.add_img[k] * W.imgr[k] * W.e[x + nx * z];
+      }
+  }
+  // reverse scan along y, into imgr
+  for (long z = 0; z < nz; ++z)
+    for (long x = 0; x < nx; ++x) {
+      T acc = T(0);
+      for (int y = ny - 1; y >= 0; --y) {
+        const long k = x + nx * y + sxy * z;
+        const T ev = W.add_img[k];
+        W.imgr[k] = -P.dy * (acc + T(0.5) * ev);
+        acc += ev;
+      }
+    }
+  for (int z = 0; z < nz; ++z)
+    imrotate(W.imgr.data() + sxy * z, W.imgr.data() + sxy * z, th, W.rot, P.method, true);
+  for (long k = 0; k < sxy * nz; ++k) gmu[k] += W.imgr[k];
+}
+
 // spawner (src/spawn.jl:10-17): ≤ nthread contiguous chunks of size
 // ceil(ntask/nthread); chunk c uses buffer c.
 template <class F>
@@ -612,6 +657,28 @@ void backproject(const Problem<T>& P, T* image, const T* views, const int* index
   for (int b = 0; b < nb; ++b)
     for (long k = 0; k < V; ++k) image[k] += acc[b][k];
   for (auto* w : W) delete w;
+}
+
+// L = sum over the indexed views of ||A_v x - y_v||^2 (returned), its
+// gradients w.r.t. x (the back-projection of 2 (A x - y)) and w.r.t. the
+// plan's mu-map (grad_mu_view summed over the views in index order)
+template <class T>
+double grad(const Problem<T>& P, T* gx, T* gmu, const T* image, const T* yobs, const int* index, int nindex,
+            int nthread) {
+  const long S = (long)P.nx * P.nz, V = (long)P.nx * P.ny * P.nz;
+  std::vector<T> views(S * nindex);
+  project(P, views.data(), image, index, nindex, nthread);
+  double loss = 0;
+  for (long k = 0; k < S * nindex; ++k) {
+    const T d = views[k] - yobs[k];
+    loss += (double)d * (double)d;
+    views[k] = T(2) * d;
+  }
+  backproject(P, gx, views.data(), index, nindex, nthread);
+  std::fill(gmu, gmu + V, T(0));
+  ViewWork<T> W(P);
+  for (int ii = 0; ii < nindex; ++ii) grad_mu_view(P, W, gmu, image, views.data() + S * ii, index[ii]);
+  return loss;
 }
 
 // src/ml-os-em.jl:21-49 (mlem!)
@@ -779,6 +846,12 @@ OR_CONV(f64, double)
                      int nblocks, int niter, int nthread, int conv_kind) {                                  \
     auto P = make_problem<T>(nx, ny, nz, px, pz, nview, dy, mumap, psfs, angles, method, conv_kind);        \
     osem(P, out, x0, y, b, nblocks, niter, nthread);                                                        \
+  }                                                                                                         \
+  double or_grad_##SUF(T* gx, T* gmu, const T* image, const T* yobs, const T* mumap, const T* psfs, int nx, \
+                       int ny, int nz, int px, int pz, int nview, T dy, const double* angles, int method,      \
+                       const int* index, int nindex, int nthread, int conv_kind) {                            \
+    auto P = make_problem<T>(nx, ny, nz, px, pz, nview, dy, mumap, psfs, angles, method, conv_kind);        \
+    return grad(P, gx, gmu, image, yobs, index, nindex, nthread);                                           \
   }                                                                                                         \
   void or_atten_##SUF(T* atten, const T* mumap, int nx, int ny, int nz, T dy, double theta, int method) {   \
     RotWork<T> W(nx);                                                                                       \

@@ -114,6 +114,13 @@ hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, in
 // (optional) receives the new out as [y][x][z] too
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,
                                hipStream_t s, float* zout = nullptr);
+// r = 2 (p - y) over n entries; *out = sum (p - y)^2 in double (partial: one
+// double per 256 entries)
+hipError_t launch_resid_loss(const float* p, const float* y, float* r, long n, double* partial, double* out,
+                             hipStream_t s);
+// mu-gradient in the rotated frame, in place over g ([b][y][x][z]):
+// g[j] = -dy (sum_{y>j} g_y w_y + g_j w_j / 2)
+hipError_t launch_mu_grad_scan(float* g, const float* w, int nb, const Geo& geo, float dy, hipStream_t s);
 // *flag = 0, then 1 if some p[k] is not > 0
 hipError_t launch_flag_nonpositive(const float* p, long n, int* flag, hipStream_t s);
 

@@ -2378,6 +2378,65 @@ __global__ void __launch_bounds__(256) k_img_reduce_z(const float* __restrict__ 
 }
 
 // ===========================================================================
+// Gradient of L = ||A(x, mu) - y||^2 (no reference counterpart)
+// ===========================================================================
+// r = 2 (p - y) (r may be p); per-block partial sums of (p - y)^2 in double.
+// grid ceil(n / 256), block 256
+__global__ void __launch_bounds__(256) k_resid_loss(const float* p, const float* __restrict__ y,
+                                                    float* r, long n, double* __restrict__ partial) {
+  __shared__ double sh[256];
+  const long k = (long)blockIdx.x * 256 + threadIdx.x;
+  double d2 = 0;
+  if (k < n) {
+    const float d = p[k] - y[k];
+    r[k] = 2.f * d;
+    d2 = (double)d * (double)d;
+  }
+  sh[threadIdx.x] = d2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// out = sum of the partials in a fixed order (one block of 256)
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partial, int n,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[256];
+  double s = 0;
+  for (int k = threadIdx.x; k < n; k += 256) s += partial[k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// dL/dmu_r of each view in the rotated frame, in place over g: with
+// E_y = g_y * w_y (g = conv^T(r), w = R(x) .* atten, z-fastest [b][y][x][z]),
+// dL/dmu_r[j] = -dy (sum_{y>j} E_y + E_j / 2), a reverse scan along y per
+// (x, z) column.  grid (ceil(nx*nz / 256), nb), block 256
+__global__ void __launch_bounds__(256) k_mu_grad_scan(float* __restrict__ g, const float* __restrict__ w, Geo geo,
+                                                      float dy) {
+  const long cols = (long)geo.nx * geo.nz, c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= cols) return;
+  const long V = cols * geo.ny;
+  float* gb = g + V * blockIdx.y + c;
+  const float* wb = w + V * blockIdx.y + c;
+  const float mdy = -dy;
+  float acc = 0.f;
+  for (int y = geo.ny - 1; y >= 0; --y) {
+    const float e = gb[y * cols] * wb[y * cols];
+    gb[y * cols] = mdy * (acc + 0.5f * e);
+    acc = acc + e;
+  }
+}
+
+// ===========================================================================
 // Image reduction, elementwise
 // ===========================================================================
 __global__ void __launch_bounds__(256) k_img_reduce(const float* __restrict__ part, int ng, long n,
@@ -2857,6 +2916,21 @@ hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* en
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,
                                hipStream_t s, float* zout) {
   k_img_reduce_z<<<dim3(cdiv(g.nx, 32), g.ny, cdiv(g.nz, 32)), 256, 0, s>>>(part, ngroups, out, asum, mode, g, zout);
+  return hipGetLastError();
+}
+
+hipError_t launch_resid_loss(const float* p, const float* y, float* r, long n, double* partial, double* out,
+                             hipStream_t s) {
+  const unsigned nblk = cdiv(n, 256);
+  if (n <= 0) return hipSuccess;
+  k_resid_loss<<<nblk, 256, 0, s>>>(p, y, r, n, partial);
+  k_sum_partials<<<1, 256, 0, s>>>(partial, (int)nblk, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mu_grad_scan(float* g, const float* w, int nb, const Geo& geo, float dy, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  k_mu_grad_scan<<<dim3(cdiv((long)geo.nx * geo.nz, 256), nb), 256, 0, s>>>(g, w, geo, dy);
   return hipGetLastError();
 }
 

@@ -211,6 +211,7 @@ struct sr_plan_s {
   DBuf flag;  // one int: device-side argument checks
   DBuf zimg, zent;  // z-fastest image copy, per-target rotation lists
   DBuf zh;          // OS-EM: the PSF adjoint of a ones view (one PSF stack for all views)
+  DBuf gview, gwork2, gpart, gloss;  // sr_grad: residual views, R(x) .* atten, loss partial sums
   int zk = 0;        // slots per target of zent
   std::vector<int> h_vids;
 
@@ -798,6 +799,77 @@ int sr_backproject(sr_plan p, float* image, const float* views, const int32_t* i
     return SR_OK;
   }
   return finish(p, mem);
+}
+
+int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float* image, const float* y,
+            const int32_t* index, int32_t nindex, int32_t mem) {
+  if (!p || !loss || !grad_x || !grad_mu || !image || !y) return fail(SR_ERR_ARG, "null argument");
+  if (!index && nindex != p->d.nview) return fail(SR_ERR_DIM, "nindex must equal nview when index is NULL");
+  if (!p->zfast)
+    return fail(SR_ERR_UNSUPPORTED,
+                "the mu-gradient needs a z-fastest plan (:two, rank-1 PSFs on planes of multiples of 16)");
+  int rc;
+  if ((rc = upload_vids(p, index, nindex))) return rc;
+  const bool host = (mem & 0xff) == SR_MEM_HOST;
+  const long n = std::max(1, nindex), PN = p->P * n;
+  const float *d_img = image, *d_y = y;
+  float *d_gx = grad_x, *d_gmu = grad_mu;
+  if (host) {
+    if ((rc = stage_in(p, p->st_x0, image, sizeof(float) * p->V)) ||
+        (rc = stage_in(p, p->st_aux1, y, sizeof(float) * PN)) || (rc = p->st_img.ensure(sizeof(float) * p->V)) ||
+        (rc = p->st_aux2.ensure(sizeof(float) * p->V)))
+      return rc;
+    d_img = p->st_x0.as<float>();
+    d_y = p->st_aux1.as<float>();
+    d_gx = p->st_img.as<float>();
+    d_gmu = p->st_aux2.as<float>();
+  }
+  if ((rc = p->gview.ensure(sizeof(float) * PN)) || (rc = p->gpart.ensure(sizeof(double) * ((PN + 255) / 256))) ||
+      (rc = p->gloss.ensure(sizeof(double))) ||
+      (rc = p->gwork2.ensure(sizeof(float) * p->V * std::min<long>(n, p->max_batch))))
+    return rc;
+  const int* d_vids = p->vids.as<int>();
+  float* r = p->gview.as<float>();
+  // r = 2 (A x - y), L = sum (A x - y)^2; the projection leaves x's z-fastest copy in zimg
+  if ((rc = run_project(p, r, d_img, d_vids, nindex, nullptr, nullptr))) return rc;
+  HIP_TRY(sr::launch_resid_loss(r, d_y, r, p->P * nindex, p->gpart.as<double>(), p->gloss.as<double>(), p->stream));
+  // dL/dx = A' r
+  if ((rc = run_backproject(p, d_gx, r, d_vids, nindex, sr::IMG_WRITE, nullptr))) return rc;
+  // dL/dmu = sum_v R_v' (-dy (reverse cumulative sum along y of conv'(r_v) .* R_v x .* atten_v, halved at j))
+  const sr::Taps t = p->taps();
+  if (nindex == 0) HIP_TRY(sr::launch_fill(d_gmu, 0.f, p->V, p->stream));
+  for (int s0 = 0; s0 < nindex; s0 += p->max_batch) {
+    const int nb = std::min(p->max_batch, nindex - s0);
+    if ((rc = ensure_batch_buffers(p, nb))) return rc;
+    {
+      KTimer kt(p, sr::K_PSF_ADJ);
+      HIP_TRY(sr::launch_psf_adj(p->separable, r + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(), nullptr,
+                                 0, p->nchunk_adj(nb), p->g, p->stream, true));
+    }
+    {
+      KTimer kt(p, sr::K_ROT_FWD);
+      HIP_TRY(sr::launch_rot2_fwd_z(p->zimg.as<float>(), p->gwork2.as<float>(), p->atten.as<float>(),
+                                    p->atten_vstride(), p->rotv.as<sr::RotView>(), d_vids + s0, nb, p->g, p->stream));
+    }
+    HIP_TRY(sr::launch_mu_grad_scan(p->work.as<float>(), p->gwork2.as<float>(), nb, p->g, p->d.dy, p->stream));
+    {
+      KTimer kt(p, sr::K_ROT_ADJ);
+      HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
+                                    d_vids + s0, nb, p->g, p->stream));
+    }
+    KTimer kt(p, sr::K_IMG_REDUCE);
+    HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, d_gmu, nullptr, s0 == 0 ? sr::IMG_WRITE : sr::IMG_ACCUM,
+                                    p->g, p->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(loss, p->gloss.p, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  if (host) {
+    HIP_TRY(hipMemcpyAsync(grad_x, d_gx, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipMemcpyAsync(grad_mu, d_gmu, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
+  }
+  // the loss lands in host memory: always wait
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  resolve_timing(p);
+  return SR_OK;
 }
 
 namespace {

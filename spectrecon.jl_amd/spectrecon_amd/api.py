@@ -25,7 +25,7 @@ __all__ = [
     "mlem", "mlem_", "osem", "osem_", "Ablock", "LinearMapAA", "spect_operator",
     "PlanRotate", "plan_rotate", "imrotate", "imrotate_", "imrotate_adj", "imrotate_adj_",
     "PlanPSF", "plan_psf", "fft_conv", "fft_conv_", "fft_conv_adj", "fft_conv_adj_", "fft_conv_adj2_",
-    "psf_gauss", "phantom", "mumap_phantom", "poisson",
+    "psf_gauss", "phantom", "mumap_phantom", "poisson", "gradient",
 ]
 
 _METHODS = {"one": L.SR_INTERP_ONE, "two": L.SR_INTERP_TWO}
@@ -309,6 +309,29 @@ def backproject(views, *args, index=None, interpmeth="two", mode="fast", **kw):
         return backproject(views, plan, index=index)
     finally:
         plan.close()
+
+
+# ---------------------------------------------------------------------------
+# Gradients of L = ||A(x, mu) - y||^2 (no reference counterpart)
+# ---------------------------------------------------------------------------
+def gradient(image, y, plan, index=None):
+    """(L, dL/dx, dL/dmu) for L = sum over the indexed views of ||A(x, mu) - y||^2,
+    mu the plan's mu-map and y the indexed views in order ([nx,nz,n]).  dL/dx is
+    the back-projection of 2 (A x - y); dL/dmu runs through the rotated
+    exponential cumulative sum of the attenuation.  Needs a z-fastest plan."""
+    idx, n = _index(plan, index)
+    nx, ny, nz = plan.imgsize
+    if tuple(np.shape(image)) != plan.imgsize:
+        raise DimensionMismatch("image size")
+    if tuple(np.shape(y)) != (nx, nz, n):
+        raise DimensionMismatch("views size")
+    gx = np.zeros(plan.imgsize, np.float32, order="F")
+    gmu = np.zeros(plan.imgsize, np.float32, order="F")
+    loss = C.c_double()
+    x, yy = _f32(image), _f32(y)
+    check(lib().sr_grad(plan.handle, C.byref(loss), gx.ctypes.data, gmu.ctypes.data, x.ctypes.data, yy.ctypes.data,
+                        idx.ctypes.data if idx is not None else None, n, L.SR_MEM_HOST))
+    return loss.value, gx, gmu
 
 
 # ---------------------------------------------------------------------------

@@ -425,3 +425,78 @@ def test_osem_per_view_psfs_z_fastest(gpu, O):
     og = gpu.osem(x0, y, b, gpu.Ablock(plan, 4), niter=2)
     orr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), 4, 2)
     assert np.linalg.norm(og - orr) <= 1e-4 * np.linalg.norm(orr)
+
+
+def test_config0_mlem(gpu, O):
+    """The small workload: 64x64x32 seeded phantom and mu-map, 60 views,
+    15x15 Gaussian PSF (FWHM 1 -> 4 px over the 64 depths), dy = 0.48,
+    y ~ Poisson(A x + 1), b = 1, 10 ML-EM iterations from ones; relative L2
+    vs the FP64 oracle."""
+    nx, nz, nview = 64, 32, 60
+    image = gpu.phantom(nx, nx, nz, seed=0)
+    mumap = gpu.mumap_phantom(nx, nx, nz)
+    psf = gpu.psf_gauss(ny=nx, px=15, fwhm_start=1.0, fwhm_end=4.0)
+    psfs = np.asfortranarray(np.repeat(psf[:, :, :, None], nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    ytrue = gpu.project(image, plan)
+    b = np.ones_like(ytrue)
+    y = gpu.poisson(ytrue + b, seed=1)
+    x0 = np.ones_like(image)
+    xg = gpu.mlem(x0, y, b, gpu.spect_operator(plan), niter=10)
+    P64 = O.Problem(mumap, psfs, 0.48, gpu.default_viewangle(nview).astype(np.float64), "two", np.float64, "direct",
+                    nthread=8)
+    xr = P64.mlem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), 10)
+    rel = np.linalg.norm(xg - xr) / np.linalg.norm(xr)
+    assert rel <= 1e-4, rel
+
+
+def test_config1_adjoint_identity(gpu):
+    """<y, A x> = <A'y, x> at the full clinical size: 128x128x80, 128 views,
+    37x37 depth-varying PSF (FWHM 1.5 -> 6 px), the benchmark's plan."""
+    nx, nz, nview = 128, 80, 128
+    mumap = gpu.mumap_phantom(nx, nx, nz)
+    psf = gpu.psf_gauss(ny=nx, px=37, fwhm_start=1.5, fwhm_end=6.0)
+    psfs = np.asfortranarray(np.repeat(psf[:, :, :, None], nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    assert plan.info()["layout"] == "z-fastest"
+    rng = np.random.default_rng(57)
+    x = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    yv = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    a = float(np.vdot(yv.astype(np.float64), gpu.project(x, plan).astype(np.float64)))
+    b = float(np.vdot(x.astype(np.float64), gpu.backproject(yv, plan).astype(np.float64)))
+    assert abs(a - b) <= 1e-5 * abs(a), (a, b)
+
+
+@pytest.mark.parametrize("nview", [16, 6])
+def test_gradient_vs_oracle(gpu, O, nview):
+    """L = ||A(x, mu) - y||^2 and its gradients w.r.t. x and the mu-map at
+    32x32x16 (the differentiable-projector check size) against the FP64
+    oracle, whose gradients are pinned by finite differences; all views and a
+    view subset."""
+    nx, nz, dy = 32, 16, 0.48
+    rng = np.random.default_rng(71)
+    mumap = np.asfortranarray((0.2 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = np.asfortranarray(np.repeat(np.asarray(O.psf_gauss(nx, 7, 7, 1.0, 3.0), np.float32)[:, :, :, None],
+                                       nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, dy)
+    assert plan.info()["layout"] == "z-fastest"
+    x = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    y = np.asfortranarray((0.5 * gpu.project(np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32)), plan))
+                          .astype(np.float32))
+    P64 = O.Problem(mumap, psfs, dy, angles32(nview).astype(np.float64), "two", np.float64, "direct")
+    for idx in (None, [nview - 1, 0, 3]):
+        yi = y if idx is None else np.asfortranarray(y[:, :, idx])
+        loss, gx, gmu = gpu.gradient(x, yi, plan, index=idx)
+        lr, gxr, gmr = P64.grad(x.astype(np.float64), yi.astype(np.float64),
+                                None if idx is None else np.array(idx, np.int32))
+        assert abs(loss - lr) <= 1e-5 * lr, (loss, lr)
+        assert np.linalg.norm(gx - gxr) <= 1e-4 * np.linalg.norm(gxr)
+        assert np.linalg.norm(gmu - gmr) <= 1e-4 * np.linalg.norm(gmr), np.linalg.norm(gmu - gmr) / np.linalg.norm(gmr)
+
+
+def test_gradient_needs_z_fastest(gpu, monkeypatch):
+    monkeypatch.setenv("SR_LAYOUT", "zyx")
+    mumap, psfs, image = small_problem(3, 32, 16, 4, 5, 5, True)
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    with pytest.raises(gpu.DeviceError, match="z-fastest"):
+        gpu.gradient(image, np.zeros((32, 16, 4), np.float32), plan)

@@ -385,3 +385,41 @@ def test_oracle_matches_golden(O):
     got = G.compute(O)
     for k in ref.files:
         assert np.allclose(got[k], ref[k], rtol=1e-12, atol=1e-12 * np.abs(ref[k]).max()), k
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+def test_gradient_finite_differences(O, method):
+    """The oracle's gradients of L = ||A(x, mu) - y||^2 against central finite
+    differences of its own FP64 loss: w.r.t. x (L is quadratic in x) and
+    w.r.t. the mu-map (through the rotated exponential cumulative sum)."""
+    nx, nz, nview, dy = 8, 4, 3, 0.7
+    rng = np.random.default_rng(61)
+    mumap = np.asfortranarray(0.3 * rng.random((nx, nx, nz)))
+    psfs = np.asfortranarray(rand_sym_psfs(rng, 3, 3, nx, nview).astype(np.float64))
+    ang = np.array([0.0, 0.7, 2.9])
+    x = np.asfortranarray(rng.random((nx, nx, nz)))
+    y = np.asfortranarray(rng.random((nx, nz, nview)))
+    P = O.Problem(mumap, psfs, dy, ang, method, np.float64, "direct")
+    loss, gx, gmu = P.grad(x, y)
+    r = P.project(x) - y
+    assert abs(loss - float(np.sum(r * r))) <= 1e-12 * loss
+    h = 1e-6
+    for k in [(0, 0, 0), (3, 4, 1), (7, 2, 3), (5, 5, 2)]:
+        e = np.zeros_like(x)
+        e[k] = h
+        fd = (P.grad(x + e, y)[0] - P.grad(x - e, y)[0]) / (2 * h)
+        assert abs(fd - gx[k]) <= 1e-6 * max(1.0, abs(gx[k])), (k, fd, gx[k])
+        mp = np.asfortranarray(mumap.copy())
+        mm = np.asfortranarray(mumap.copy())
+        mp[k] += h
+        mm[k] -= h
+        lp = O.Problem(mp, psfs, dy, ang, method, np.float64, "direct").grad(x, y)[0]
+        lm = O.Problem(mm, psfs, dy, ang, method, np.float64, "direct").grad(x, y)[0]
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - gmu[k]) <= 1e-6 * max(1.0, abs(gmu[k])), (k, fd, gmu[k])
+    # a subset of views: the same as the full gradient of the restricted loss
+    idx = np.array([2, 0], np.int32)
+    ls, gxs, _ = P.grad(x, np.asfortranarray(y[:, :, idx]), idx)
+    r = P.project(x, idx) - y[:, :, idx]
+    assert abs(ls - float(np.sum(r * r))) <= 1e-12 * ls
+    assert np.allclose(gxs, P.backproject(2 * r, idx), rtol=1e-12, atol=1e-14)
