@@ -145,6 +145,18 @@ int sr_mlem(sr_plan plan, float* out, const float* x0, const float* y, const flo
 int sr_osem(sr_plan plan, float* out, const float* x0, const float* y, const float* b, int32_t nblocks,
             int32_t niter, int32_t mem);
 
+/* Batched / served use (no reference counterpart): OS-EM over the strided
+ * subsets nb:nblocks:nview for any 1 <= nblocks <= nview (block sizes differ
+ * by at most one when nblocks does not divide nview; the reference's Ablock,
+ * src/ml-os-em.jl:73, throws then, as sr_osem does).  With nblocks dividing
+ * nview it equals sr_osem. */
+int sr_osem_strided(sr_plan plan, float* out, const float* x0, const float* y, const float* b, int32_t nblocks,
+                    int32_t niter, int32_t mem);
+/* *done = 1 when all work queued on the plan's stream has finished (after an
+ * SR_ASYNC call), else 0; never blocks.  Several plans (each with its own
+ * stream) run their queued work concurrently on the device. */
+int sr_plan_query(sr_plan plan, int32_t* done);
+
 /* ---- gradients ----------------------------------------------------------------
  * No reference counterpart (the reference differentiates only through the
  * linear operators, docs/lit/examples/6-dl.jl:145-156): for

@@ -701,23 +701,26 @@ void mlem(const Problem<T>& P, T* out, const T* x0, const T* y, const T* b, int 
 }
 
 // src/ml-os-em.jl:70-131 (Ablock + osem!): block nb uses views nb:nblocks:nview
+// (with nblocks not dividing nview, as the batched use allows, the blocks'
+// sizes differ by at most one)
 template <class T>
 void osem(const Problem<T>& P, T* out, const T* x0, const T* y, const T* b, int nblocks, int niter, int nthread) {
   const long V = (long)P.nx * P.ny * P.nz, S = (long)P.nx * P.nz;
-  const int nsub = P.nview / nblocks;
   std::vector<std::vector<int>> idx(nblocks);
   for (int nb = 0; nb < nblocks; ++nb)
     for (int v = nb; v < P.nview; v += nblocks) idx[nb].push_back(v);
-  std::vector<T> ones(S * nsub, T(1)), ybar(S * nsub), yratio(S * nsub), back(V);
+  const int nmax = (int)idx[0].size();
+  std::vector<T> ones(S * nmax, T(1)), ybar(S * nmax), yratio(S * nmax), back(V);
   std::vector<std::vector<T>> asum(nblocks, std::vector<T>(V));
   for (int nb = 0; nb < nblocks; ++nb) {
-    backproject(P, asum[nb].data(), ones.data(), idx[nb].data(), nsub, nthread);
+    backproject(P, asum[nb].data(), ones.data(), idx[nb].data(), (int)idx[nb].size(), nthread);
     for (auto& a : asum[nb])
       if (a == T(0)) a = std::numeric_limits<T>::infinity();
   }
   std::copy(x0, x0 + V, out);
   for (int it = 0; it < niter; ++it)
     for (int nb = 0; nb < nblocks; ++nb) {
+      const int nsub = (int)idx[nb].size();
       project(P, ybar.data(), out, idx[nb].data(), nsub, nthread);
       for (int ii = 0; ii < nsub; ++ii) {
         const long v = idx[nb][ii];

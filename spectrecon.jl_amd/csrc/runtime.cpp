@@ -909,7 +909,15 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
   std::vector<int> order;
   for (int nb = 0; nb < nblocks; ++nb)
     for (int v = nb; v < nview; v += nblocks) order.push_back(v);
-  const int nsub = nview / nblocks;
+  // block nb: views nb, nb + nblocks, ... (src/ml-os-em.jl:76); counts
+  // differ by at most one when nblocks does not divide nview
+  std::vector<int> cnt(nblocks), off(nblocks);
+  for (int nb = 0, o = 0; nb < nblocks; ++nb) {
+    cnt[nb] = (nview - nb + nblocks - 1) / nblocks;
+    off[nb] = o;
+    o += cnt[nb];
+  }
+  const int nsub = cnt[0];  // the largest block
   if ((rc = p->vids.ensure(sizeof(int) * nview))) return rc;
   HIP_TRY(hipMemcpyAsync(p->vids.p, order.data(), sizeof(int) * nview, hipMemcpyHostToDevice, p->stream));
   const int* d_vids = p->vids.as<int>();
@@ -942,7 +950,7 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
       {
         KTimer kt(p, sr::K_ROT_ADJ);
         HIP_TRY(sr::launch_rot2_adj_z(p->zh.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
-                                      d_vids + nb * nsub, nsub, p->g, p->stream, p->atten.as<float>(),
+                                      d_vids + off[nb], cnt[nb], p->g, p->stream, p->atten.as<float>(),
                                       p->atten_vstride()));
       }
       {
@@ -955,8 +963,8 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
     if (fused && (rc = p->em_back.ensure(sizeof(float) * p->V))) return rc;
     for (int nb = 0; nb < nblocks; ++nb) {
       float* as = p->em_asum.as<float>() + p->V * nb;
-      if ((rc = run_backproject(p, fused ? p->em_back.as<float>() : as, p->ones.as<float>(), d_vids + nb * nsub, nsub,
-                                sr::IMG_WRITE, nullptr)))
+      if ((rc = run_backproject(p, fused ? p->em_back.as<float>() : as, p->ones.as<float>(), d_vids + off[nb],
+                                cnt[nb], sr::IMG_WRITE, nullptr)))
         return rc;
       if (fused) HIP_TRY(sr::launch_to_zfast(p->em_back.as<float>(), as, p->g, p->stream));
       HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
@@ -970,23 +978,24 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
     const sr::Taps t = p->taps();
     for (int it = 0; it < niter; ++it)
       for (int nb = 0; nb < nblocks; ++nb) {
-        const int* sv = d_vids + nb * nsub;
-        if ((rc = run_project(p, p->em_ratio.as<float>(), nullptr, sv, nsub, d_y, d_b, true))) return rc;
+        const int* sv = d_vids + off[nb];
+        const int ns = cnt[nb];
+        if ((rc = run_project(p, p->em_ratio.as<float>(), nullptr, sv, ns, d_y, d_b, true))) return rc;
         {
           KTimer kt(p, sr::K_PSF_ADJ);
-          HIP_TRY(sr::launch_psf_adj(p->separable, p->em_ratio.as<float>(), t, sv, nsub, p->work.as<float>(),
-                                     p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(nsub), p->g, p->stream,
+          HIP_TRY(sr::launch_psf_adj(p->separable, p->em_ratio.as<float>(), t, sv, ns, p->work.as<float>(),
+                                     p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(ns), p->g, p->stream,
                                      true));
         }
         KTimer kt(p, sr::K_ROT_ADJ);
         float* as = p->em_asum.as<float>() + p->V * nb;
         if (shared && it == 0)  // this subset's normaliser, summed in the same pass
           HIP_TRY(sr::launch_rot2_adj_z_em_norm(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk,
-                                                sv, nsub, p->g, p->stream, p->zh.as<float>(), p->atten.as<float>(),
+                                                sv, ns, p->g, p->stream, p->zh.as<float>(), p->atten.as<float>(),
                                                 p->atten_vstride(), as));
         else
           HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv,
-                                        nsub, p->g, p->stream, nullptr, 0, as));
+                                        ns, p->g, p->stream, nullptr, 0, as));
       }
     KTimer kt(p, sr::K_IMG_REDUCE);
     HIP_TRY(sr::launch_img_reduce_z(p->zimg.as<float>(), 1, d_out, nullptr, sr::IMG_WRITE, p->g, p->stream));
@@ -995,9 +1004,9 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
     bool zready = false;  // the EM update also refreshed the z-fastest image copy
     for (int it = 0; it < niter; ++it)
       for (int nb = 0; nb < nblocks; ++nb) {
-        const int* sv = d_vids + nb * nsub;
-        if ((rc = run_project(p, p->em_ratio.as<float>(), d_out, sv, nsub, d_y, d_b, zready))) return rc;
-        if ((rc = run_backproject(p, d_out, p->em_ratio.as<float>(), sv, nsub, sr::IMG_EM,
+        const int* sv = d_vids + off[nb];
+        if ((rc = run_project(p, p->em_ratio.as<float>(), d_out, sv, cnt[nb], d_y, d_b, zready))) return rc;
+        if ((rc = run_backproject(p, d_out, p->em_ratio.as<float>(), sv, cnt[nb], sr::IMG_EM,
                                   p->em_asum.as<float>() + p->V * nb, &zready)))
           return rc;
       }
@@ -1026,6 +1035,30 @@ int sr_osem(sr_plan p, float* out, const float* x0, const float* y, const float*
   if (p->d.nview % nblocks != 0) return fail(SR_ERR_ARG, "nview must be divisible by nblocks!");
   if (niter < 0) return fail(SR_ERR_ARG, "niter < 0");
   return run_em(p, out, x0, y, b, nblocks, niter, mem);
+}
+
+int sr_osem_strided(sr_plan p, float* out, const float* x0, const float* y, const float* b, int32_t nblocks,
+                    int32_t niter, int32_t mem) {
+  if (!p || !out || !x0 || !y || !b) return fail(SR_ERR_ARG, "null argument");
+  if (nblocks < 1 || nblocks > p->d.nview) return fail(SR_ERR_ARG, "nblocks must be in 1..nview");
+  if (niter < 0) return fail(SR_ERR_ARG, "niter < 0");
+  return run_em(p, out, x0, y, b, nblocks, niter, mem);
+}
+
+int sr_plan_query(sr_plan p, int32_t* done) {
+  if (!p || !done) return fail(SR_ERR_ARG, "null argument");
+  const hipError_t e = hipStreamQuery(p->stream);
+  if (e == hipSuccess) {
+    *done = 1;
+    resolve_timing(p);
+    return SR_OK;
+  }
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    *done = 0;
+    return SR_OK;
+  }
+  return fail(SR_ERR_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
 }
 
 // ---------------------------------------------------------------------------

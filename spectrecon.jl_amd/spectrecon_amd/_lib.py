@@ -59,6 +59,8 @@ SIGNATURES = {
     "sr_plan_info": (C.c_int, [_vp, C.POINTER(C.c_size_t), _ip, _ip]),
     "sr_plan_layout": (C.c_int, [_vp, _ip]),
     "sr_grad": (C.c_int, [_vp, C.POINTER(C.c_double), _vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32]),
+    "sr_osem_strided": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_int32]),
+    "sr_plan_query": (C.c_int, [_vp, _ip]),
     "sr_plan_set_batch": (C.c_int, [_vp, C.c_int32]),
     "sr_plan_synchronize": (C.c_int, [_vp]),
     "sr_plan_set_timing": (C.c_int, [_vp, C.c_int32]),

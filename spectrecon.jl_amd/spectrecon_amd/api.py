@@ -25,7 +25,7 @@ __all__ = [
     "mlem", "mlem_", "osem", "osem_", "Ablock", "LinearMapAA", "spect_operator",
     "PlanRotate", "plan_rotate", "imrotate", "imrotate_", "imrotate_adj", "imrotate_adj_",
     "PlanPSF", "plan_psf", "fft_conv", "fft_conv_", "fft_conv_adj", "fft_conv_adj_", "fft_conv_adj2_",
-    "psf_gauss", "phantom", "mumap_phantom", "poisson", "gradient",
+    "psf_gauss", "phantom", "mumap_phantom", "poisson", "gradient", "osem_strided",
 ]
 
 _METHODS = {"one": L.SR_INTERP_ONE, "two": L.SR_INTERP_TWO}
@@ -309,6 +309,17 @@ def backproject(views, *args, index=None, interpmeth="two", mode="fast", **kw):
         return backproject(views, plan, index=index)
     finally:
         plan.close()
+
+
+def osem_strided(x0, y, b, plan, nblocks, niter=16):
+    """OS-EM over the strided subsets nb:nblocks:nview for any 1 <= nblocks <=
+    nview (subset sizes differ by at most one when nblocks does not divide
+    nview; Ablock / osem keep the reference's throw).  Host arrays in and out;
+    for device arrays call lib().sr_osem_strided with SR_MEM_DEVICE."""
+    out = np.zeros(plan.imgsize, np.float32, order="F")
+    check(lib().sr_osem_strided(plan.handle, out.ctypes.data, _f32(x0).ctypes.data, _f32(y).ctypes.data,
+                                _f32(b).ctypes.data, int(nblocks), int(niter), L.SR_MEM_HOST))
+    return out
 
 
 # ---------------------------------------------------------------------------

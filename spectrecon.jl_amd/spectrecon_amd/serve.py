@@ -1,0 +1,255 @@
+"""Serving and batching of OS-EM reconstructions of different sizes.
+
+No reference counterpart: the reference runs one `osem!` at a time.  Here a
+request loop keeps plans (mu-map rotations, attenuation factors, PSF taps,
+rotation-adjoint lists) resident on the device, pooled per configuration, and
+admits requests by HBM footprint.  Every plan owns one HIP stream, so admitted
+requests run concurrently on the device (one host thread per admitted request
+drives its plan), and a request that arrives while others run is admitted as
+soon as a plan of its configuration and the memory for it are free.  The
+kernels of one plan do not depend on other plans, so a result is the same
+alone and in the mix.
+
+The queue and admission logic is backend-independent (`Server(backend=...)`);
+`DeviceBackend` drives the library.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+
+import numpy as np
+
+
+@dataclasses.dataclass(frozen=True)
+class Config:
+    """What a resident plan is built from (requests with equal configs share plans)."""
+    nx: int
+    nz: int
+    nview: int
+    psf: int
+    dy: float = 0.48
+    fwhm_start: float = 1.5
+    fwhm_end: float = 6.0
+
+    def image_shape(self):
+        return (self.nx, self.nx, self.nz)
+
+    def views_shape(self):
+        return (self.nx, self.nz, self.nview)
+
+    def footprint(self, max_batch=64):
+        """Device bytes of one plan of this config and its per-call buffers
+        (attenuation factors, lists, work volumes, normalisers), an upper
+        estimate used for admission."""
+        V = self.nx * self.nx * self.nz
+        batch = min(self.nview, max_batch)
+        atten = 4 * V * self.nview
+        lists = 8 * 8 * self.nx * self.nx * self.nview
+        work = 4 * V * (batch + 8)
+        images = 4 * V * (2 + 8)
+        views = 4 * self.nx * self.nz * self.nview * 4
+        return atten + lists + work + images + views
+
+
+@dataclasses.dataclass
+class Request:
+    config: Config
+    x0: np.ndarray
+    y: np.ndarray
+    b: np.ndarray
+    nblocks: int
+    niter: int
+    arrival: float = 0.0
+    rid: int = 0
+    result: np.ndarray | None = None
+    start: float | None = None
+    finish: float | None = None
+    rejected: bool = False
+
+    @property
+    def latency(self):
+        return None if self.finish is None else self.finish - self.arrival
+
+
+class DeviceBackend:
+    """Plans and OS-EM on the device through the C-ABI.  An admitted request
+    runs on a host thread of its own that calls `sr_osem_strided` with host
+    arrays: the call stages them through its plan's buffers on the plan's
+    stream and releases the interpreter lock while the device works, so the
+    admitted requests' streams run concurrently (no device-wide allocation,
+    free or null-stream copy is made per request)."""
+
+    def __init__(self):
+        from .api import SPECTplan, mumap_phantom, osem_strided, psf_gauss
+        self._SPECTplan, self._mumap, self._psf, self._osem = SPECTplan, mumap_phantom, psf_gauss, osem_strided
+
+    def make_plan(self, cfg: Config):
+        mumap = self._mumap(cfg.nx, cfg.nx, cfg.nz)
+        psf = self._psf(ny=cfg.nx, px=cfg.psf, fwhm_start=cfg.fwhm_start, fwhm_end=cfg.fwhm_end)
+        psfs = np.asfortranarray(np.repeat(psf[:, :, :, None], cfg.nview, 3))
+        return self._SPECTplan(mumap, psfs, cfg.dy)
+
+    def start(self, plan, req: Request):
+        import threading
+        box = {}
+
+        def work():
+            try:
+                box["out"] = self._osem(req.x0, req.y, req.b, plan, req.nblocks, req.niter)
+            except Exception as e:  # reported by result()
+                box["err"] = e
+        t = threading.Thread(target=work, daemon=True)
+        t.start()
+        return (t, box)
+
+    def done(self, plan, token) -> bool:
+        return not token[0].is_alive()
+
+    def result(self, plan, token):
+        token[0].join()
+        if "err" in token[1]:
+            raise token[1]["err"]
+        return token[1]["out"]
+
+    def release(self, plan, token):
+        pass
+
+    def close_plan(self, plan):
+        plan.close()
+
+
+class Server:
+    """Continuous batching of OS-EM requests over resident, pooled plans.
+
+    - `submit(req)` queues a request (FIFO); a request whose plan alone would
+      exceed the HBM budget is rejected.
+    - `poll()` retires finished requests, then admits queued ones in arrival
+      order while a plan of their config is idle, or a new plan fits in the
+      budget (at most `max_active` in flight).  A queued request that cannot
+      be admitted does not block later ones of other configs.
+    - `run(requests)` replays a timed arrival stream and returns the stats.
+    """
+
+    def __init__(self, hbm_budget: float, max_active: int = 8, backend=None, clock=time.perf_counter):
+        self.budget = float(hbm_budget)
+        self.max_active = max_active
+        self.backend = backend if backend is not None else DeviceBackend()
+        self.clock = clock
+        self.pools = collections.defaultdict(list)  # config -> [plan]
+        self.busy = {}                              # id(plan) -> (plan, request, token)
+        self.resident = 0.0
+        self.queue = collections.deque()
+        self.finished = []
+        self.rejected = []
+        self.max_queued = 0
+
+    def submit(self, req: Request):
+        if req.config.footprint() > self.budget:
+            req.rejected = True
+            self.rejected.append(req)
+            return
+        self.queue.append(req)
+        self.max_queued = max(self.max_queued, len(self.queue))
+
+    def _idle_plan(self, cfg):
+        for plan in self.pools[cfg]:
+            if id(plan) not in self.busy:
+                return plan
+        return None
+
+    def _evict_idle(self, need):
+        """Free idle plans of other configs until `need` bytes fit."""
+        for cfg, plans in list(self.pools.items()):
+            for plan in list(plans):
+                if self.resident + need <= self.budget:
+                    return
+                if id(plan) not in self.busy:
+                    plans.remove(plan)
+                    self.backend.close_plan(plan)
+                    self.resident -= cfg.footprint()
+
+    def poll(self):
+        now = self.clock()
+        for key, (plan, req, token) in list(self.busy.items()):
+            if self.backend.done(plan, token):
+                req.result = self.backend.result(plan, token)
+                self.backend.release(plan, token)
+                req.finish = self.clock()
+                self.finished.append(req)
+                del self.busy[key]
+        pending = collections.deque()
+        while self.queue:
+            req = self.queue.popleft()
+            if len(self.busy) >= self.max_active:
+                pending.append(req)
+                continue
+            plan = self._idle_plan(req.config)
+            if plan is None:
+                need = req.config.footprint()
+                if self.resident + need > self.budget:
+                    self._evict_idle(need)
+                if self.resident + need <= self.budget:
+                    plan = self.backend.make_plan(req.config)
+                    self.pools[req.config].append(plan)
+                    self.resident += need
+            if plan is None:
+                pending.append(req)
+                continue
+            req.start = now
+            self.busy[id(plan)] = (plan, req, self.backend.start(plan, req))
+        self.queue = pending
+        return len(self.busy) + len(self.queue)
+
+    def run(self, requests, poll_interval=2e-4):
+        """Replay requests at their `arrival` offsets (seconds from now)."""
+        t0 = self.clock()
+        order = sorted(requests, key=lambda r: r.arrival)
+        for r in order:
+            r.arrival += t0
+        i = 0
+        while i < len(order) or self.queue or self.busy:
+            now = self.clock()
+            while i < len(order) and order[i].arrival <= now:
+                self.submit(order[i])
+                i += 1
+            if not self.poll() and i < len(order):
+                time.sleep(max(0.0, min(poll_interval, order[i].arrival - self.clock())))
+            elif self.busy:
+                time.sleep(poll_interval)
+        return self.stats(self.clock() - t0)
+
+    def stats(self, wall):
+        lat = sorted(r.latency for r in self.finished)
+        pct = (lambda q: float(np.percentile(lat, q)) * 1e3) if lat else (lambda q: None)
+        return {"requests": len(self.finished) + len(self.rejected), "served": len(self.finished),
+                "rejected": len(self.rejected), "max_queued": self.max_queued, "wall_s": wall,
+                "recons_per_s": len(self.finished) / wall if wall > 0 else None,
+                "latency_ms": {"p50": pct(50), "p99": pct(99)},
+                "plans": {f"{c.nx}x{c.nx}x{c.nz}/{c.nview}": len(p) for c, p in self.pools.items()}}
+
+    def close(self):
+        for plans in self.pools.values():
+            for plan in plans:
+                self.backend.close_plan(plan)
+        self.pools.clear()
+        self.resident = 0.0
+
+
+def synthetic_request(cfg: Config, seed: int, nblocks: int, niter: int, arrival: float = 0.0, rid: int = 0,
+                      scale: float = 5e6):
+    """A request of the served stream: seeded phantom, Poisson counts with mean
+    total `scale` per projection set, b = 0.1 mean, x0 = 1.  The measurement
+    is made by projecting through a temporary plan of the config."""
+    from .api import phantom, poisson, project
+    img = phantom(cfg.nx, cfg.nx, cfg.nz, seed=seed)
+    plan = DeviceBackend().make_plan(cfg)
+    try:
+        ytrue = project(img, plan)
+    finally:
+        plan.close()
+    s = scale / float(ytrue.sum())
+    b = np.full_like(ytrue, 0.1 * float((s * ytrue).mean()))
+    y = poisson(s * ytrue + b, seed=seed)
+    return Request(cfg, np.ones(img.shape, np.float32, order="F"), y, b, nblocks, niter, arrival, rid)

@@ -500,3 +500,52 @@ def test_gradient_needs_z_fastest(gpu, monkeypatch):
     plan = gpu.SPECTplan(mumap, psfs, 0.48)
     with pytest.raises(gpu.DeviceError, match="z-fastest"):
         gpu.gradient(image, np.zeros((32, 16, 4), np.float32), plan)
+
+
+def test_osem_strided_uneven_vs_oracle(gpu, O):
+    """OS-EM over strided subsets whose sizes differ (10 views, 4 blocks:
+    3, 3, 2, 2 views) against the FP64 oracle; with nblocks dividing nview it
+    is osem itself."""
+    nx, nz, nview = 32, 16, 10
+    xtrue, mumap, psfs = em_problem(nx, nz, nview)
+    plan = gpu.SPECTplan(mumap, psfs, 0.8)
+    ytrue = gpu.project(xtrue, plan)
+    b = np.full_like(ytrue, 0.1 * ytrue.mean())
+    y = gpu.poisson(20 * (ytrue + b), seed=5) / 20
+    x0 = np.ones_like(xtrue)
+    P64 = O.Problem(mumap, psfs, 0.8, angles32(nview).astype(np.float64), "two", np.float64, "direct")
+    xg = gpu.osem_strided(x0, y, b, plan, 4, 2)
+    xr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), 4, 2)
+    assert np.linalg.norm(xg - xr) <= 1e-4 * np.linalg.norm(xr)
+    with pytest.raises(gpu.SPECTError, match="nview must be divisible by nblocks!"):
+        gpu.Ablock(plan, 4)
+    np.testing.assert_array_equal(gpu.osem_strided(x0, y, b, plan, 5, 2),
+                                  gpu.osem(x0, y, b, gpu.Ablock(plan, 5), niter=2))
+
+
+def test_serve_mix_equals_alone(gpu):
+    """Requests of three sizes admitted concurrently by the serving loop (each
+    on its own resident plan and stream): every result equals the same
+    request reconstructed alone."""
+    from spectrecon_amd import serve as V
+    cfgs = [V.Config(32, 16, 16, 7, 0.48, 1.0, 3.0), V.Config(64, 32, 12, 15, 0.48, 1.0, 4.0),
+            V.Config(48, 16, 10, 5, 0.48, 1.0, 2.0)]
+    reqs = []
+    for k in range(6):
+        cfg = cfgs[k % 3]
+        reqs.append(V.synthetic_request(cfg, seed=100 + k, nblocks=(4, 5, 3)[k % 3], niter=2, arrival=0.002 * k,
+                                        rid=k, scale=1e5))
+    server = V.Server(hbm_budget=8e9, max_active=4)
+    try:
+        st = server.run(reqs)
+    finally:
+        server.close()
+    assert st["served"] == 6 and st["rejected"] == 0
+    backend = V.DeviceBackend()
+    for r in reqs:
+        plan = backend.make_plan(r.config)
+        try:
+            alone = gpu.osem_strided(r.x0, r.y, r.b, plan, r.nblocks, r.niter)
+        finally:
+            plan.close()
+        np.testing.assert_array_equal(r.result, alone)

@@ -423,3 +423,29 @@ def test_gradient_finite_differences(O, method):
     r = P.project(x, idx) - y[:, :, idx]
     assert abs(ls - float(np.sum(r * r))) <= 1e-12 * ls
     assert np.allclose(gxs, P.backproject(2 * r, idx), rtol=1e-12, atol=1e-14)
+
+
+def test_osem_uneven_blocks_composition(O):
+    """OS-EM over strided blocks nb:nblocks:nview when nblocks does not divide
+    nview (the batched use): equal to the update composed from the pinned
+    project / backproject, block by block."""
+    nx, nz, nview, nblocks, niter = 8, 4, 5, 2, 2
+    rng = np.random.default_rng(67)
+    mumap = np.asfortranarray(0.1 * rng.random((nx, nx, nz)))
+    psfs = np.asfortranarray(rand_sym_psfs(rng, 3, 3, nx, nview).astype(np.float64))
+    P = O.Problem(mumap, psfs, 0.5, np.arange(nview) * 2 * np.pi / nview, "two", np.float64, "direct")
+    y = np.asfortranarray(1 + rng.random((nx, nz, nview)))
+    b = np.asfortranarray(0.1 + np.zeros((nx, nz, nview)))
+    x0 = np.asfortranarray(np.ones((nx, nx, nz)))
+    blocks = [np.arange(k, nview, nblocks, dtype=np.int32) for k in range(nblocks)]
+    asum = []
+    for idx in blocks:
+        a = P.backproject(np.asfortranarray(np.ones((nx, nz, len(idx)))), idx)
+        a[a == 0] = np.inf
+        asum.append(a)
+    x = x0.copy()
+    for _ in range(niter):
+        for idx, a in zip(blocks, asum):
+            ratio = np.asfortranarray(y[:, :, idx] / (P.project(x, idx) + b[:, :, idx]))
+            x = np.asfortranarray(x * (P.backproject(ratio, idx) / a))
+    assert np.allclose(P.osem(x0, y, b, nblocks, niter), x, rtol=1e-12, atol=0)
