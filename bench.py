@@ -98,8 +98,80 @@ def cpu_baseline(image, mumap, psfs, budget_s):
                        f"projections + 40 subset back-projections per reconstruction")}
 
 
+def stream_workload(kind, nreq, rate, seed=7):
+    """The batched and served patterns (one JSON line each; not the headline).
+
+    batched: 32 seeded instances over nx in {64,96,128}, nz in {32,64,80},
+      views in {60,120,128}, square PSF side in {15,25,37}, subsets in {4,8}
+      (strided, uneven where the subsets do not divide the views), OS-EM x 4
+      iterations, all submitted at once: reconstructed one at a time, then
+      concurrently on resident plans; every concurrent result must equal its
+      one-at-a-time result.
+    served: `nreq` requests with Poisson arrivals at `rate` per second, 70%
+      128x128x80 / 128 views / 37x37 PSF and 30% 64x64x32 / 60 views / 15x15
+      PSF, OS-EM 8 x 4 (strided subsets: 60 views over 8 blocks)."""
+    from spectrecon_amd import serve as V
+
+    rng = np.random.default_rng(seed)
+    plans = {}
+    reqs = []
+    if kind == "batched":
+        for k in range(32):
+            nx = int(rng.choice([64, 96, 128]))
+            cfg = V.Config(nx, int(rng.choice([32, 64, 80])), int(rng.choice([60, 120, 128])),
+                           int(rng.choice([15, 25, 37])), DY, 1.0, 4.0)
+            reqs.append(V.synthetic_request(cfg, seed=1 + k, nblocks=int(rng.choice([4, 8])), niter=4, rid=k,
+                                            plans=plans))
+    else:
+        big = V.Config(NX, NZ, NVIEW, PSIZE, DY, 1.5, 6.0)
+        small = V.Config(64, 32, 60, 15, DY, 1.0, 4.0)
+        t = 0.0
+        for k in range(nreq):
+            t += float(rng.exponential(1.0 / rate))
+            cfg = big if rng.random() < 0.7 else small
+            reqs.append(V.synthetic_request(cfg, seed=1000 + k, nblocks=8, niter=4, arrival=t, rid=k, plans=plans))
+    for p in plans.values():
+        p.close()
+    log(f"{kind}: {len(reqs)} requests prepared")
+    out = {"workload": kind, "requests": len(reqs), "data": "synthetic (seeded phantoms, Poisson counts)"}
+    if kind == "batched":
+        import copy
+        # resident plans built first (the batched pattern keeps them), then the
+        # same instances one at a time and concurrently
+        alone = [copy.copy(r) for r in reqs]
+        srv = V.Server(hbm_budget=200e9, max_active=1)
+        try:
+            srv.prepare({r.config for r in reqs}, per_config=2)
+            st1 = srv.run(alone)
+            srv.reset_stats()
+            srv.max_active = 8
+            st8 = srv.run(reqs)
+        finally:
+            srv.close()
+        same = all(np.array_equal(a.result, b.result) for a, b in zip(alone, reqs))
+        out.update({"metric": "batched uneven OS-EM recons/s (32 instances)", "unit": "recons/s",
+                    "value": st8["recons_per_s"], "one_at_a_time": st1["recons_per_s"],
+                    "speedup": st8["recons_per_s"] / st1["recons_per_s"], "results_equal_alone": bool(same),
+                    "concurrent": st8, "sequential": st1})
+    else:
+        srv = V.Server(hbm_budget=200e9, max_active=8)
+        try:
+            srv.prepare({r.config for r in reqs}, per_config=2)
+            st = srv.run(reqs)
+        finally:
+            srv.close()
+        out.update({"metric": f"served OS-EM stream, Poisson arrivals {rate:g}/s", "unit": "recons/s",
+                    "value": st["recons_per_s"], "latency_ms": st["latency_ms"], "rejected": st["rejected"],
+                    "max_queued": st["max_queued"], "stats": st})
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["osem", "batched", "served"], default="osem",
+                    help="osem: the headline OS-EM line (default); batched / served: the concurrent patterns")
+    ap.add_argument("--requests", type=int, default=100, help="served: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="served: Poisson arrival rate per second")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -129,6 +201,8 @@ def main():
     import spectrecon_amd as S
     from spectrecon_amd import _lib as L
 
+    if args.workload != "osem":
+        return stream_workload(args.workload, args.requests, args.rate)
     if S.device_count() == 0:
         raise SystemExit("no HIP device")
     lib = S.lib()

@@ -5,8 +5,10 @@
 // PSF convolutions, whose reference arithmetic is an FFT, use fmaf).
 #include "kernels.h"
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 
 namespace sr {
@@ -2814,20 +2816,31 @@ bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::v
       }
     return n;
   };
+  // views are independent: split them over host threads
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto over_views = [&](auto&& f) {
+    std::vector<std::thread> th;
+    for (int w = 0; w < nth; ++w)
+      th.emplace_back([&, w] {
+        for (int v = w; v < nview; v += nth) f(v);
+      });
+    for (auto& t : th) t.join();
+  };
   std::vector<unsigned char> cnt((size_t)nview * NN, 0);
-  int kmax = 4;
-  for (int v = 0; v < nview; ++v) {
+  std::vector<int> kv(nview, 4);
+  over_views([&](int v) {
     unsigned char* cv = cnt.data() + (size_t)v * NN;
     Tap t[4];
     for (long pix = 0; pix < NN; ++pix) {
       const int n = taps(rv[v], (int)pix, t);
-      for (int k = 0; k < n; ++k) kmax = std::max(kmax, (int)++cv[t[k].q]);
+      for (int k = 0; k < n; ++k) kv[v] = std::max(kv[v], (int)++cv[t[k].q]);
     }
-  }
+  });
+  const int kmax = *std::max_element(kv.begin(), kv.end());
   if (kmax > kMaxEll) return false;
   *K = kmax;
   ent.assign((size_t)nview * kmax * NN, int2{});
-  for (int v = 0; v < nview; ++v) {
+  over_views([&](int v) {
     int2* ev = ent.data() + (size_t)v * kmax * NN;
     for (long q = 0; q < NN; ++q)
       for (int k = 0; k < kmax; ++k) ev[k * NN + q] = int2{(int)q, 0};
@@ -2842,7 +2855,7 @@ bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::v
         ev[(long)fill[t[k].q]++ * NN + t[k].q] = e;
       }
     }
-  }
+  });
   return true;
 }
 

@@ -132,9 +132,13 @@ class Server:
     - `run(requests)` replays a timed arrival stream and returns the stats.
     """
 
-    def __init__(self, hbm_budget: float, max_active: int = 8, backend=None, clock=time.perf_counter):
+    def __init__(self, hbm_budget: float, max_active: int = 8, backend=None, clock=time.perf_counter,
+                 plans_per_config: int = 2):
         self.budget = float(hbm_budget)
         self.max_active = max_active
+        # building a plan takes far longer than a reconstruction: beyond this
+        # many per config, requests wait for a resident plan of their config
+        self.plans_per_config = plans_per_config
         self.backend = backend if backend is not None else DeviceBackend()
         self.clock = clock
         self.pools = collections.defaultdict(list)  # config -> [plan]
@@ -186,7 +190,7 @@ class Server:
                 pending.append(req)
                 continue
             plan = self._idle_plan(req.config)
-            if plan is None:
+            if plan is None and len(self.pools[req.config]) < self.plans_per_config:
                 need = req.config.footprint()
                 if self.resident + need > self.budget:
                     self._evict_idle(need)
@@ -220,6 +224,16 @@ class Server:
                 time.sleep(poll_interval)
         return self.stats(self.clock() - t0)
 
+    def prepare(self, configs, per_config=1):
+        """Build resident plans ahead of the requests (within the budget)."""
+        for cfg in configs:
+            while len(self.pools[cfg]) < per_config and self.resident + cfg.footprint() <= self.budget:
+                self.pools[cfg].append(self.backend.make_plan(cfg))
+                self.resident += cfg.footprint()
+
+    def reset_stats(self):
+        self.finished, self.rejected, self.max_queued = [], [], 0
+
     def stats(self, wall):
         lat = sorted(r.latency for r in self.finished)
         pct = (lambda q: float(np.percentile(lat, q)) * 1e3) if lat else (lambda q: None)
@@ -238,17 +252,24 @@ class Server:
 
 
 def synthetic_request(cfg: Config, seed: int, nblocks: int, niter: int, arrival: float = 0.0, rid: int = 0,
-                      scale: float = 5e6):
+                      scale: float = 5e6, plans=None):
     """A request of the served stream: seeded phantom, Poisson counts with mean
     total `scale` per projection set, b = 0.1 mean, x0 = 1.  The measurement
-    is made by projecting through a temporary plan of the config."""
+    is made by projecting through a plan of the config (kept in `plans`, a
+    dict, when one is given; else a temporary one)."""
     from .api import phantom, poisson, project
     img = phantom(cfg.nx, cfg.nx, cfg.nz, seed=seed)
-    plan = DeviceBackend().make_plan(cfg)
+    plan = plans.get(cfg) if plans is not None else None
+    temp = plan is None
+    if temp:
+        plan = DeviceBackend().make_plan(cfg)
     try:
         ytrue = project(img, plan)
     finally:
-        plan.close()
+        if plans is not None:
+            plans[cfg] = plan
+        elif temp:
+            plan.close()
     s = scale / float(ytrue.sum())
     b = np.full_like(ytrue, 0.1 * float((s * ytrue).mean()))
     y = poisson(s * ytrue + b, seed=seed)

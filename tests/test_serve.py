@@ -83,7 +83,7 @@ def test_reject_over_budget_and_serve_the_rest():
 def test_concurrency_cap_and_pool_growth():
     clock = Clock()
     be = FakeBackend(clock, {A: 1.0})
-    s = Server(hbm_budget=A.footprint() * 10, max_active=3, backend=be, clock=clock)
+    s = Server(hbm_budget=A.footprint() * 10, max_active=3, backend=be, clock=clock, plans_per_config=3)
     for k in range(5):
         s.submit(req(A, k))
     s.poll()
@@ -130,3 +130,17 @@ def test_stats_latency():
     lat = sorted(r.latency for r in s.finished)
     assert lat[0] >= 1.0 and lat[-1] >= 3.0
     assert st["latency_ms"]["p99"] >= st["latency_ms"]["p50"]
+
+
+def test_plans_per_config_cap_and_prepare():
+    clock = Clock()
+    be = FakeBackend(clock, {A: 1.0})
+    s = Server(hbm_budget=A.footprint() * 10, max_active=4, backend=be, clock=clock, plans_per_config=1)
+    s.prepare([A])
+    assert len(be.made) == 1
+    for k in range(3):
+        s.submit(req(A, k))
+    s.poll()
+    assert len(s.busy) == 1 and len(be.made) == 1  # the others wait for the resident plan
+    drive(s, clock, [])
+    assert len(s.finished) == 3 and len(be.made) == 1
