@@ -11,7 +11,7 @@ module SPECTreconAMD
 using LinearAlgebra: mul!
 using LinearMapsAA: LinearMapAA, LinearMapAO
 
-export SPECTplanAMD, project!, backproject!, project, backproject, mlem!, osem!, Ablock
+export SPECTplanAMD, project!, backproject!, project, backproject, mlem!, osem!, Ablock, osem_strided!, gradient
 
 const libspect = joinpath(@__DIR__, "..", "lib", "libspectrecon_amd.so")
 
@@ -120,6 +120,30 @@ function osem!(out::Array{Float32,3}, x0::Array{Float32,3}, ynoisy::Array{Float3
                 (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Int32, Int32, Int32),
                 plan.handle, out, x0, ynoisy, background, nblocks, niter, SR_MEM_HOST))
     return out
+end
+
+# OS-EM over strided subsets that need not divide the views (no reference
+# counterpart: the batched / served use)
+function osem_strided!(out::Array{Float32,3}, x0::Array{Float32,3}, ynoisy::Array{Float32,3},
+        background::Array{Float32,3}, plan::SPECTplanAMD, nblocks::Int; niter::Int = 16)
+    check(ccall((:sr_osem_strided, libspect), Cint,
+                (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Int32, Int32, Int32),
+                plan.handle, out, x0, ynoisy, background, nblocks, niter, SR_MEM_HOST))
+    return out
+end
+
+# (L, dL/dx, dL/dmu) for L = ||A(x, mu) - y||^2 over the indexed views (no
+# reference counterpart).  A ChainRulesCore rrule for a mu-dependent projector
+# calls it with y = A x - dy/2 for the cotangent dy.
+function gradient(image::Array{Float32,3}, y::Array{Float32,3}, plan::SPECTplanAMD;
+                  index::AbstractVector{<:Integer} = 1:plan.nview)
+    ix = Int32.(collect(index) .- 1)
+    gx, gmu, loss = similar(image), similar(image), Ref{Float64}(0)
+    check(ccall((:sr_grad, libspect), Cint,
+                (Ptr{Cvoid}, Ref{Float64}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Int32},
+                 Int32, Int32),
+                plan.handle, loss, gx, gmu, image, y, ix, length(ix), SR_MEM_HOST))
+    return loss[], gx, gmu
 end
 
 end # module
