@@ -166,10 +166,42 @@ def stream_workload(kind, nreq, rate, seed=7):
     print(json.dumps(out), flush=True)
 
 
+def gradient_workload(steps):
+    """L = ||A(x, mu) - y||^2 and its gradients w.r.t. x and mu at 128x128x80,
+    128 views, 37x37 PSF, for a batch of 8 seeded phantoms (device arrays)."""
+    import ctypes as C
+
+    import spectrecon_amd as S
+    from spectrecon_amd import _lib as L
+
+    image, mumap, psfs = workload()
+    plan = S.SPECTplan(mumap, psfs, DY)
+    lib = S.lib()
+    ys = [S.DeviceArray.from_host(0.9 * S.project(S.phantom(NX, NX, NZ, seed=50 + k), plan)) for k in range(8)]
+    xs = [S.DeviceArray.from_host(S.phantom(NX, NX, NZ, seed=k)) for k in range(8)]
+    gx, gmu = S.DeviceArray((NX, NX, NZ)), S.DeviceArray((NX, NX, NZ))
+    loss = C.c_double()
+
+    def batch():
+        for x, y in zip(xs, ys):
+            L.check(lib.sr_grad(plan.handle, C.byref(loss), gx.ptr, gmu.ptr, x.ptr, y.ptr, None, NVIEW,
+                                L.SR_MEM_DEVICE))
+    batch()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch()
+    dt = (time.perf_counter() - t0) / steps
+    print(json.dumps({"workload": "gradient", "metric": "gradients (x and mu) per second, batch of 8 phantoms",
+                      "unit": "gradients/s", "value": 8 / dt, "ms_per_batch": 1e3 * dt, "steps": steps,
+                      "config": {"nx": NX, "nz": NZ, "nview": NVIEW, "psf": PSIZE, "batch": 8},
+                      "data": "synthetic (seeded phantoms; y = 0.9 A x' of other phantoms)"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["osem", "batched", "served"], default="osem",
-                    help="osem: the headline OS-EM line (default); batched / served: the concurrent patterns")
+    ap.add_argument("--workload", choices=["osem", "batched", "served", "gradient"], default="osem",
+                    help="osem: the headline OS-EM line (default); batched / served: the concurrent patterns; "
+                         "gradient: the differentiable projector")
     ap.add_argument("--requests", type=int, default=100, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="served: Poisson arrival rate per second")
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +233,8 @@ def main():
     import spectrecon_amd as S
     from spectrecon_amd import _lib as L
 
+    if args.workload == "gradient":
+        return gradient_workload(max(1, args.steps))
     if args.workload != "osem":
         return stream_workload(args.workload, args.requests, args.rate)
     if S.device_count() == 0:
