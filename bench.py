@@ -98,7 +98,7 @@ def cpu_baseline(image, mumap, psfs, budget_s):
                        f"projections + 40 subset back-projections per reconstruction")}
 
 
-def stream_workload(kind, nreq, rate, seed=7):
+def stream_workload(kind, nreq, rate, seed=7, autotune=True):
     """The batched and served patterns (one JSON line each; not the headline).
 
     batched: 32 seeded instances over nx in {64,96,128}, nz in {32,64,80},
@@ -133,15 +133,18 @@ def stream_workload(kind, nreq, rate, seed=7):
     for p in plans.values():
         p.close()
     log(f"{kind}: {len(reqs)} requests prepared")
+    nsubs = {}
+    for r in reqs:
+        nsubs.setdefault(r.config, set()).add(-(-r.config.nview // r.nblocks))
     out = {"workload": kind, "requests": len(reqs), "data": "synthetic (seeded phantoms, Poisson counts)"}
     if kind == "batched":
         import copy
         # resident plans built first (the batched pattern keeps them), then the
         # same instances one at a time and concurrently
         alone = [copy.copy(r) for r in reqs]
-        srv = V.Server(hbm_budget=200e9, max_active=1)
+        srv = V.Server(hbm_budget=200e9, max_active=1, backend=V.DeviceBackend(autotune=autotune))
         try:
-            srv.prepare({r.config for r in reqs}, per_config=2)
+            srv.prepare({r.config for r in reqs}, per_config=2, nsubs=nsubs)
             st1 = srv.run(alone)
             srv.reset_stats()
             srv.max_active = 8
@@ -154,9 +157,9 @@ def stream_workload(kind, nreq, rate, seed=7):
                     "speedup": st8["recons_per_s"] / st1["recons_per_s"], "results_equal_alone": bool(same),
                     "concurrent": st8, "sequential": st1})
     else:
-        srv = V.Server(hbm_budget=200e9, max_active=8)
+        srv = V.Server(hbm_budget=200e9, max_active=8, backend=V.DeviceBackend(autotune=autotune))
         try:
-            srv.prepare({r.config for r in reqs}, per_config=2)
+            srv.prepare({r.config for r in reqs}, per_config=2, nsubs=nsubs)
             st = srv.run(reqs)
         finally:
             srv.close()
@@ -208,6 +211,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel choices")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the benchmark, copy this many MiB 3 times with the library's copy kernel "
@@ -236,7 +240,7 @@ def main():
     if args.workload == "gradient":
         return gradient_workload(max(1, args.steps))
     if args.workload != "osem":
-        return stream_workload(args.workload, args.requests, args.rate)
+        return stream_workload(args.workload, args.requests, args.rate, autotune=not args.no_autotune)
     if S.device_count() == 0:
         raise SystemExit("no HIP device")
     lib = S.lib()
@@ -246,6 +250,14 @@ def main():
     plan_ms = 1e3 * (time.perf_counter() - t0)
     info = plan.info()
     log(f"plan: {plan_ms:.1f} ms, {info}")
+    # kernel choices for 16-view subsets, timed on this device (part of
+    # setting the plan up, outside the timed region)
+    tune_ms, tune = None, None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = plan.autotune(NVIEW // NBLOCKS)
+        tune_ms = 1e3 * (time.perf_counter() - t0)
+        log(f"autotune: {tune_ms:.1f} ms, {plan.choice()}")
 
     # measurements: y ~ Poisson(s*A x + b), s = 5e6 / sum(A x), b = 0.1 mean(s A x)
     ytrue = S.project(image, plan)
@@ -375,6 +387,9 @@ def main():
         "backproj_ms": back_ms,
         "proj_backproj_ms": proj_ms + back_ms,
         "plan_ms": plan_ms,
+        "autotune_ms": tune_ms,
+        "kernel_choices": plan.choice(),
+        "autotune": tune,
         "osem_host_ms": float(np.median(host_ms)),
         "kernels": kernels,
         "roofline": roof,

@@ -156,6 +156,42 @@ int sr_osem_strided(sr_plan plan, float* out, const float* x0, const float* y, c
  * SR_ASYNC call), else 0; never blocks.  Several plans (each with its own
  * stream) run their queued work concurrently on the device. */
 int sr_plan_query(sr_plan plan, int32_t* done);
+/* HIP-graph replay of the EM kernel sequence (no reference counterpart; for
+ * callers that run many small reconstructions, whose time is in launches).
+ * With enable != 0, sr_mlem / sr_osem / sr_osem_strided run their first call
+ * for a given set of buffers (caller pointers, plan buffers, nblocks, niter)
+ * eagerly, capture the sequence on the second and replay it after that;
+ * argument checks, host staging and result copies stay outside the graph,
+ * and results equal the eager ones bit for bit.  Up to 8 graphs are kept per
+ * plan; enable = 0 frees them.  Calls with timing on run eagerly.
+ * sr_plan_graph_stats reports how many graphs were captured and replayed. */
+int sr_plan_set_graph(sr_plan plan, int32_t enable);
+int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
+
+/* Kernel choices of a plan (no reference counterpart; the reference has one
+ * CPU code path).  choice[SR_CHOICE_*]:
+ *   FWD_FAMILY / ADJ_FAMILY: 0 column-owned matrix-core PSF kernels where they
+ *     fit (z-fastest plans), 1 the tile kernels;
+ *   FWD_CHUNKS / ADJ_CHUNKS: plane chunks per view, in percent of the built-in
+ *     count (one block per compute unit over the batch), 10..1000;
+ *   GATHER_SLICES: view slices per block of the rotation-adjoint gather
+ *     (2, 4, 8; 0 picks by the batch size).
+ * Defaults {0, 100, 0, 100, 0}.  Every choice computes the same operator;
+ * the gather's slice count changes the order of its sums (last-bit
+ * differences).  sr_plan_autotune times the candidates of each choice on
+ * batches of nsub views (the OS-EM subset size) on the plan's stream, keeps
+ * the fastest, and writes a JSON record of every candidate's time to `log`
+ * (NUL-terminated, truncated to log_bytes; may be NULL); nsub <= 0 restores
+ * the defaults.  Changing the choices drops the plan's EM graphs. */
+#define SR_CHOICE_FWD_FAMILY 0
+#define SR_CHOICE_FWD_CHUNKS 1
+#define SR_CHOICE_ADJ_FAMILY 2
+#define SR_CHOICE_ADJ_CHUNKS 3
+#define SR_CHOICE_GATHER_SLICES 4
+#define SR_NCHOICE 5
+int sr_plan_set_choice(sr_plan plan, const int32_t* choice);
+int sr_plan_get_choice(sr_plan plan, int32_t* choice);
+int sr_plan_autotune(sr_plan plan, int32_t nsub, char* log, size_t log_bytes);
 
 /* ---- gradients ----------------------------------------------------------------
  * No reference counterpart (the reference differentiates only through the

@@ -68,15 +68,22 @@ hipError_t launch_atten_exact(float* mur, int nb, const Geo& g, float dy, hipStr
 // attenuated already); returns the chunk count actually used through
 // *nchunk_used.  zfast: `work` is the z-fastest image [y][x][z] and the :two
 // rotation (rv) and attenuation ([v][y][x][z]) are applied while staging
+// family (z-fastest matrix-core path): PSF_AUTO takes the column-owned
+// kernels where they fit, PSF_TILE the tile kernels
+constexpr int PSF_AUTO = 0, PSF_TILE = 1;
+bool psf_fwd_col_ok(const Geo& g, int ntx);
+bool psf_adj_col_ok(const Geo& g, int ntx);
 hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
                           int nchunk, int* nchunk_used, const Geo& g, hipStream_t s, bool zfast = false,
-                          const float* atten = nullptr, long atten_vstride = 0, const RotView* rv = nullptr);
+                          const float* atten = nullptr, long atten_vstride = 0, const RotView* rv = nullptr,
+                          int family = PSF_AUTO);
 // views (or the ML-EM ratio y/(view + b) when epi == EPI_RATIO) = sum of partials
 hipError_t launch_view_reduce(const float* part, int nchunk, const int* vids, int nb, float* out, const float* yobs,
                               const float* bkg, int epi, const Geo& g, hipStream_t s);
 // work[b][:,y,:] = atten[v][:,y,:] .* conv_adj(views[b], psf[v,y])
 hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, const int* vids, int nb, float* work,
-                          const float* atten, long atten_vstride, int nchunk, const Geo& g, hipStream_t s, bool zfast = false);
+                          const float* atten, long atten_vstride, int nchunk, const Geo& g, hipStream_t s, bool zfast = false,
+                          int family = PSF_AUTO);
 
 // ---- elementwise --------------------------------------------------------
 hipError_t launch_fill(float* p, float v, long n, hipStream_t s);
@@ -94,7 +101,7 @@ bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::v
 // to asum), x .*= (sum rot2^T(work[b])) ./ asum
 hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* ent, int K, const int* vids, int nb,
                                      const Geo& g, hipStream_t s, const float* hvol, const float* atten,
-                                     long atten_vstride, float* asum);
+                                     long atten_vstride, float* asum, int vs = 0);
 // img [z][y][x] -> out [y][x][z]
 hipError_t launch_to_zfast(const float* img, float* out, const Geo& g, hipStream_t s);
 // out[b] = atten[v] .* rot2(img_t) per view, all [y][x][z]; atten may be null
@@ -103,13 +110,14 @@ hipError_t launch_rot2_fwd_z(const float* img_t, float* out, const float* atten,
 // rotated mu volumes [b][y][x][z] -> attenuation factors, in place
 hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_t s);
 // part [y][x][z] = sum over the batch's views of rot2^T(work[b]) through the
-// K-slot per-target lists; with atten: of rot2^T(work .* atten[v]), work one
+// K-slot per-target lists (vs: views slices per block, 2, 4 or 8; 0 picks
+// by the batch size); with atten: of rot2^T(work .* atten[v]), work one
 // volume shared by the views; with em_asum: part is the z-fastest image x,
 // updated in place as x .*= sum ./ em_asum (z-fastest); zero_inf: zero sums
 // are written as Inf (OS-EM normalisers)
 hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
                              const Geo& g, hipStream_t s, const float* atten = nullptr, long atten_vstride = 0,
-                             const float* em_asum = nullptr, bool zero_inf = false);
+                             const float* em_asum = nullptr, bool zero_inf = false, int vs = 0);
 // out [z][y][x] = combine(sum_g part[g] [y][x][z]) per ImgMode; zout
 // (optional) receives the new out as [y][x][z] too
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,

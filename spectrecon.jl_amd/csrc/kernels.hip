@@ -2645,15 +2645,23 @@ bool psf_sep_supported(const Geo& g) {
   return need <= 1024;
 }
 
+bool psf_fwd_col_ok(const Geo& g, int ntx) {
+  return psf_mfma_supported(g, ntx) && g.nx / 16 <= MF_WAVES && g.nz / 16 <= MF_TPW &&
+         sizeof(float) * mf_fcol_lds_floats(g, ntx, 1) <= kMaxLds;
+}
+
+bool psf_adj_col_ok(const Geo& g, int ntx) {
+  return psf_mfma_supported(g, ntx) && g.nz / 16 <= MF_TPW && sizeof(float) * mf_col_lds_floats(g, ntx, 1) <= kMaxLds;
+}
+
 hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
                           int nchunk, int* nchunk_used, const Geo& g, hipStream_t s, bool zfast, const float* atten,
-                          long atten_vstride, const RotView* rv) {
+                          long atten_vstride, const RotView* rv, int family) {
   const int Yc = (g.ny + nchunk - 1) / nchunk;
   nchunk = (g.ny + Yc - 1) / Yc;
   if (nchunk_used) *nchunk_used = nchunk;
   if (nb <= 0) return hipSuccess;
-  if (separable && zfast && psf_mfma_supported(g, t.ntx) && g.nx / 16 <= MF_WAVES && g.nz / 16 <= MF_TPW &&
-      sizeof(float) * mf_fcol_lds_floats(g, t.ntx, 1) <= kMaxLds) {
+  if (separable && zfast && family != PSF_TILE && psf_fwd_col_ok(g, t.ntx)) {
     const size_t base = mf_fcol_lds_floats(g, t.ntx, 0), per = MfTaps(t.ntx).size;
     const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), ((long)(kMaxLds / sizeof(float)) - (long)base) /
                                                                          (long)per));
@@ -2722,12 +2730,11 @@ hipError_t launch_view_reduce(const float* part, int nchunk, const int* vids, in
 
 hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, const int* vids, int nb, float* work,
                           const float* atten, long atten_vstride, int nchunk, const Geo& g, hipStream_t s,
-                          bool zfast) {
+                          bool zfast, int family) {
   if (nb <= 0) return hipSuccess;
   const int Yc = (g.ny + nchunk - 1) / nchunk;
   nchunk = (g.ny + Yc - 1) / Yc;
-  if (separable && zfast && psf_mfma_supported(g, t.ntx) && g.nz / 16 <= MF_TPW &&
-      sizeof(float) * mf_col_lds_floats(g, t.ntx, 1) <= kMaxLds) {
+  if (separable && zfast && family != PSF_TILE && psf_adj_col_ok(g, t.ntx)) {
     const size_t base = mf_col_lds_floats(g, t.ntx, 0), per = MfTaps(t.ntx).size;
     const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), ((long)(kMaxLds / sizeof(float)) - (long)base) /
                                                                          (long)per));
@@ -2884,12 +2891,13 @@ hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_
 
 static hipError_t launch_ell(int mode, const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
                              const Geo& g, hipStream_t s, const float* hvol, const float* atten, long atten_vstride,
-                             float* em_asum, bool zero_inf) {
+                             float* em_asum, bool zero_inf, int vs_req) {
   if (nb <= 0) return hipSuccess;
   const int q = (g.nz / 4) % 2 == 0 ? 2 : 1;  // z quads per lane
   const int pw = 64 / (g.nz / 4 / q);
   const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
-  const int vs = nb <= 4 ? 2 : nb <= 8 ? 4 : 8;
+  // view slices (waves) per block: each wave sums every vs-th view
+  const int vs = (vs_req == 2 || vs_req == 4 || vs_req == 8) ? vs_req : nb <= 4 ? 2 : nb <= 8 ? 4 : 8;
   const int zi = zero_inf ? 1 : 0;
 #define SR_ELL(KK, VV, QQ)                                                                                            \
   if (K == KK && vs == VV && q == QQ) {                                                                             \
@@ -2913,17 +2921,18 @@ static hipError_t launch_ell(int mode, const float* work, float* part, const int
 
 hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
                              const Geo& g, hipStream_t s, const float* atten, long atten_vstride,
-                             const float* em_asum, bool zero_inf) {
+                             const float* em_asum, bool zero_inf, int vs) {
   // (with atten: work is H, the normaliser form)
-  return atten ? launch_ell(1, nullptr, part, ent, K, vids, nb, g, s, work, atten, atten_vstride, nullptr, zero_inf)
+  return atten ? launch_ell(1, nullptr, part, ent, K, vids, nb, g, s, work, atten, atten_vstride, nullptr, zero_inf,
+                            vs)
                : launch_ell(0, work, part, ent, K, vids, nb, g, s, nullptr, nullptr, 0, const_cast<float*>(em_asum),
-                            zero_inf);
+                            zero_inf, vs);
 }
 
 hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* ent, int K, const int* vids, int nb,
                                      const Geo& g, hipStream_t s, const float* hvol, const float* atten,
-                                     long atten_vstride, float* asum) {
-  return launch_ell(2, work, x, ent, K, vids, nb, g, s, hvol, atten, atten_vstride, asum, true);
+                                     long atten_vstride, float* asum, int vs) {
+  return launch_ell(2, work, x, ent, K, vids, nb, g, s, hvol, atten, atten_vstride, asum, true, vs);
 }
 
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,

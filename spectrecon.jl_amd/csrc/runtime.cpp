@@ -226,6 +226,17 @@ struct sr_plan_s {
   double t_ms[sr::K_NCLASS] = {};
   long t_n[sr::K_NCLASS] = {};
 
+  // HIP graphs of the EM kernel sequence (sr_plan_set_graph), keyed by every
+  // device address and count the sequence bakes in; most recent last
+  int graph = 0;
+  struct GraphEntry {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  std::vector<std::vector<uintptr_t>> graph_seen;  // keys run once eagerly, not yet captured
+  long graph_captures = 0, graph_replays = 0;
+
   sr::Taps taps() const {
     sr::Taps t;
     t.u = taps_u.as<float>();
@@ -241,14 +252,43 @@ struct sr_plan_s {
   // planes are split into chunks so that a launch fills the device: one
   // block per CU for the matrix-core PSF kernels (their LDS takes a CU),
   // about 512 blocks otherwise
+  // kernel choices (sr_plan_set_choice / sr_plan_autotune), SR_CHOICE_* order:
+  // PSF forward family, its chunk count in percent of the heuristic one, the
+  // same two for the PSF adjoint, view slices of the rotation-adjoint gather
+  int32_t choice[SR_NCHOICE] = {0, 100, 0, 100, 0};
+  // plane chunks per view for the PSF kernels: their blocks take a compute
+  // unit each (LDS), so a launch of c chunks x nb views runs in
+  // ceil(c nb / ncu) rounds of about (planes per chunk + 2) plane times (the
+  // 2: a block's prologue and pipeline fill); the cheapest c, fewest on ties
+  int chunks_base(int nb) const {
+    int best = 1;
+    long best_cost = -1;
+    for (int c = 1; c <= d.ny; ++c) {
+      const int yc = (d.ny + c - 1) / c;
+      if ((d.ny + yc - 1) / yc != c) continue;  // same chunks as a smaller c
+      const long rounds = ((long)c * std::max(1, nb) + ncu - 1) / ncu;
+      const long cost = rounds * (yc + 2);
+      if (best_cost < 0 || cost < best_cost) {
+        best_cost = cost;
+        best = c;
+      }
+    }
+    return best;
+  }
+  int chunks_scaled(int nb, int pct) const {
+    return std::min(d.ny, std::max(1, (int)(((long)chunks_base(nb) * pct + 50) / 100)));
+  }
   int nchunk_fwd(int nb) const {
-    if (mfma) return nchunk_adj(nb);
+    if (mfma) return chunks_scaled(nb, choice[SR_CHOICE_FWD_CHUNKS]);
     const int target = std::max(1, (int)std::ceil(512.0 / std::max(1, nb)));
     return std::min(d.ny, target);
   }
   // PSF adjoint: one block per CU (its LDS holds the padded view and the
   // intermediate plane), the planes of a view split over ceil(ncu/nb) chunks
-  int nchunk_adj(int nb) const { return std::min(d.ny, std::max(1, (ncu + nb - 1) / std::max(1, nb))); }
+  int nchunk_adj(int nb) const { return chunks_scaled(nb, choice[SR_CHOICE_ADJ_CHUNKS]); }
+  int fam_fwd() const { return choice[SR_CHOICE_FWD_FAMILY]; }
+  int fam_adj() const { return choice[SR_CHOICE_ADJ_FAMILY]; }
+  int ell_vs() const { return choice[SR_CHOICE_GATHER_SLICES]; }
   int groups_adj(int nb, int* vpg) const {
     // views per rotation-adjoint group: enough blocks (nz * ngroups) to fill
     // the device; the :one kernel takes one view per block
@@ -330,6 +370,21 @@ void resolve_timing(sr_plan p) {
   p->pending.clear();
 }
 
+__attribute__((format(printf, 2, 3))) void appendf(std::string& s, const char* fmt, ...) {
+  char b[256];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(b, sizeof b, fmt, ap);
+  va_end(ap);
+  s += b;
+}
+
+void drop_graphs(sr_plan p) {
+  for (auto& e : p->graphs) (void)hipGraphExecDestroy(e.exec);
+  p->graphs.clear();
+  p->graph_seen.clear();
+}
+
 int ensure_batch_buffers(sr_plan p, int nb) {
   int rc;
   if ((rc = p->work.ensure(sizeof(float) * p->V * nb))) return rc;
@@ -377,7 +432,7 @@ int run_project(sr_plan p, float* views, const float* image, const int* d_vids, 
       KTimer kt(p, sr::K_PSF_FWD);
       HIP_TRY(sr::launch_psf_fwd(p->separable, p->zfast ? p->zimg.as<float>() : p->work.as<float>(), t, d_vids + s0,
                                  nb, p->fpart.as<float>(), p->nchunk_fwd(nb), &nchunk, p->g, p->stream, p->zfast,
-                                 p->atten.as<float>(), p->atten_vstride(), p->rotv.as<sr::RotView>()));
+                                 p->atten.as<float>(), p->atten_vstride(), p->rotv.as<sr::RotView>(), p->fam_fwd()));
     }
     {
       KTimer kt(p, sr::K_VIEW_REDUCE);
@@ -414,7 +469,7 @@ int run_backproject(sr_plan p, float* image, const float* views, const int* d_vi
       KTimer kt(p, sr::K_PSF_ADJ);
       HIP_TRY(sr::launch_psf_adj(p->separable, views + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(),
                                  p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(nb), p->g, p->stream,
-                                 p->zfast));
+                                 p->zfast, p->fam_adj()));
     }
     int vpg;
     const int ng = p->groups_adj(nb, &vpg);
@@ -424,7 +479,7 @@ int run_backproject(sr_plan p, float* image, const float* views, const int* d_vi
       {
         KTimer kt(p, sr::K_ROT_ADJ);
         HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
-                                      d_vids + s0, nb, p->g, p->stream));
+                                      d_vids + s0, nb, p->g, p->stream, nullptr, 0, nullptr, false, p->ell_vs()));
       }
       KTimer kt(p, sr::K_IMG_REDUCE);
       float* zout = (m == sr::IMG_EM && zimg_written) ? p->zimg.as<float>() : nullptr;
@@ -694,6 +749,7 @@ int sr_plan_destroy(sr_plan p) {
     (void)hipStreamSynchronize(p->stream);
     resolve_timing(p);
     for (hipEvent_t e : p->event_pool) (void)hipEventDestroy(e);
+    for (auto& e : p->graphs) (void)hipGraphExecDestroy(e.exec);
     (void)hipStreamDestroy(p->stream);
   }
   delete p;
@@ -708,6 +764,178 @@ int sr_plan_set_timing(sr_plan p, int32_t enable) {
   for (int k = 0; k < sr::K_NCLASS; ++k) {
     p->t_ms[k] = 0;
     p->t_n[k] = 0;
+  }
+  return SR_OK;
+}
+
+int sr_plan_set_graph(sr_plan p, int32_t enable) {
+  if (!p) return fail(SR_ERR_ARG, "null plan");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->graph = enable ? 1 : 0;
+  if (!p->graph) drop_graphs(p);
+  return SR_OK;
+}
+
+int sr_plan_graph_stats(sr_plan p, int64_t* captures, int64_t* replays) {
+  if (!p) return fail(SR_ERR_ARG, "null plan");
+  if (captures) *captures = p->graph_captures;
+  if (replays) *replays = p->graph_replays;
+  return SR_OK;
+}
+
+int sr_plan_set_choice(sr_plan p, const int32_t* c) {
+  if (!p || !c) return fail(SR_ERR_ARG, "null argument");
+  for (int k : {SR_CHOICE_FWD_FAMILY, SR_CHOICE_ADJ_FAMILY})
+    if (c[k] != 0 && c[k] != 1) return fail(SR_ERR_ARG, "PSF family must be 0 or 1");
+  for (int k : {SR_CHOICE_FWD_CHUNKS, SR_CHOICE_ADJ_CHUNKS})
+    if (c[k] < 10 || c[k] > 1000) return fail(SR_ERR_ARG, "chunk percent must be in 10..1000");
+  const int vs = c[SR_CHOICE_GATHER_SLICES];
+  if (vs != 0 && vs != 2 && vs != 4 && vs != 8) return fail(SR_ERR_ARG, "gather slices must be 0, 2, 4 or 8");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  drop_graphs(p);
+  std::copy(c, c + SR_NCHOICE, p->choice);
+  return SR_OK;
+}
+
+int sr_plan_get_choice(sr_plan p, int32_t* c) {
+  if (!p || !c) return fail(SR_ERR_ARG, "null argument");
+  std::copy(p->choice, p->choice + SR_NCHOICE, c);
+  return SR_OK;
+}
+
+// Times each choice's candidates on the kernel it steers, one choice at a
+// time (the others at their current values), over batches of nsub views
+// spread over the orbit; inputs are the plan's own buffers filled with ones.
+int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
+  if (!p) return fail(SR_ERR_ARG, "null plan");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  drop_graphs(p);
+  const int32_t defaults[SR_NCHOICE] = {0, 100, 0, 100, 0};
+  std::string out;
+  if (nsub <= 0) {
+    std::copy(defaults, defaults + SR_NCHOICE, p->choice);
+    out = "{\"reset\": true}";
+  } else {
+    const int nb = std::min({(int)nsub, p->d.nview, p->max_batch});
+    int rc;
+    if ((rc = ensure_batch_buffers(p, nb)) || (rc = p->em_ratio.ensure(sizeof(float) * p->P * nb)) ||
+        (rc = p->st_img.ensure(sizeof(float) * p->V)) || (rc = p->vids.ensure(sizeof(int) * nb)))
+      return rc;
+    std::vector<int> vid(nb);
+    for (int k = 0; k < nb; ++k) vid[k] = (int)((long)k * p->d.nview / nb);
+    HIP_TRY(hipMemcpyAsync(p->vids.p, vid.data(), sizeof(int) * nb, hipMemcpyHostToDevice, p->stream));
+    const int* dv = p->vids.as<int>();
+    float* img = p->zfast ? p->zimg.as<float>() : p->st_img.as<float>();
+    HIP_TRY(sr::launch_fill(img, 1.f, p->V, p->stream));
+    HIP_TRY(sr::launch_fill(p->em_ratio.as<float>(), 1.f, p->P * nb, p->stream));
+    HIP_TRY(sr::launch_fill(p->work.as<float>(), 1.f, p->V * nb, p->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      return fail(SR_ERR_HIP, "hipEventCreate failed");
+    }
+    const sr::Taps t = p->taps();
+    // microseconds per launch: best of 3 rounds of 4 launches, after one
+    // untimed launch; negative when the candidate fails to launch
+    auto time_us = [&](auto&& launch) -> double {
+      if (launch() != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+      }
+      double best = 1e30;
+      for (int r = 0; r < 3; ++r) {
+        (void)hipEventRecord(e0, p->stream);
+        for (int k = 0; k < 4; ++k) (void)launch();
+        (void)hipEventRecord(e1, p->stream);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          (void)hipGetLastError();
+          return -1;
+        }
+        best = std::min(best, 250.0 * ms);
+      }
+      return best;
+    };
+    const bool psf_choices = p->separable && p->mfma;
+    appendf(out, "{\"nsub\": %d, \"views_per_launch\": %d, \"layout\": \"%s\"", (int)nsub, nb,
+         p->zfast ? "z-fastest" : "x-fastest");
+    // PSF forward (with the view reduction its chunk count sets) and adjoint:
+    // family x chunk percent
+    for (int dir = 0; dir < 2 && psf_choices; ++dir) {
+      const int fk = dir == 0 ? SR_CHOICE_FWD_FAMILY : SR_CHOICE_ADJ_FAMILY;
+      const int ck = dir == 0 ? SR_CHOICE_FWD_CHUNKS : SR_CHOICE_ADJ_CHUNKS;
+      const bool col = p->zfast && (dir == 0 ? sr::psf_fwd_col_ok(p->g, t.ntx) : sr::psf_adj_col_ok(p->g, t.ntx));
+      appendf(out, ", \"%s\": {\"candidates\": [", dir == 0 ? "psf_fwd" : "psf_adj");
+      double best = -1;
+      int bf = p->choice[fk], bc = p->choice[ck], n = 0;
+      for (int fam = col ? 0 : 1; fam <= 1; ++fam) {
+        int last_chunks = -1;
+        for (int pct : {50, 100, 200, 400}) {
+          const int chunks = p->chunks_scaled(nb, pct);
+          if (chunks == last_chunks) continue;
+          last_chunks = chunks;
+          double us;
+          if (dir == 0)
+            us = time_us([&]() {
+              int used = 0;
+              hipError_t e = sr::launch_psf_fwd(p->separable, p->zfast ? img : p->work.as<float>(), t, dv, nb,
+                                                p->fpart.as<float>(), chunks, &used, p->g, p->stream, p->zfast,
+                                                p->atten.as<float>(), p->atten_vstride(), p->rotv.as<sr::RotView>(),
+                                                fam);
+              if (e == hipSuccess)
+                e = sr::launch_view_reduce(p->fpart.as<float>(), used, dv, nb, p->em_ratio.as<float>(), nullptr,
+                                           nullptr, sr::EPI_VIEWS, p->g, p->stream);
+              return e;
+            });
+          else
+            us = time_us([&]() {
+              return sr::launch_psf_adj(p->separable, p->em_ratio.as<float>(), t, dv, nb, p->work.as<float>(),
+                                        p->atten.as<float>(), p->atten_vstride(), chunks, p->g, p->stream, p->zfast,
+                                        fam);
+            });
+          appendf(out, "%s{\"family\": \"%s\", \"chunks\": %d, \"us\": %.2f}", n++ ? ", " : "",
+               fam == 0 ? "column" : "tile", chunks, us);
+          if (us > 0 && (best < 0 || us < best)) {
+            best = us;
+            bf = fam;
+            bc = pct;
+          }
+        }
+      }
+      p->choice[fk] = bf;
+      p->choice[ck] = bc;
+      appendf(out, "], \"choice\": {\"family\": \"%s\", \"chunk_percent\": %d, \"us\": %.2f}}",
+           bf == 0 ? "column" : "tile", bc, best);
+    }
+    // rotation-adjoint gather (z-fastest): view slices per block
+    if (p->zfast) {
+      appendf(out, ", \"gather\": {\"candidates\": [");
+      double best = -1;
+      int bv = p->choice[SR_CHOICE_GATHER_SLICES], n = 0;
+      for (int vs : {2, 4, 8}) {
+        const double us = time_us([&]() {
+          return sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk, dv, nb,
+                                       p->g, p->stream, nullptr, 0, nullptr, false, vs);
+        });
+        appendf(out, "%s{\"slices\": %d, \"us\": %.2f}", n++ ? ", " : "", vs, us);
+        if (us > 0 && (best < 0 || us < best)) {
+          best = us;
+          bv = vs;
+        }
+      }
+      p->choice[SR_CHOICE_GATHER_SLICES] = bv;
+      appendf(out, "], \"choice\": {\"slices\": %d, \"us\": %.2f}}", bv, best);
+    }
+    out += "}";
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(hipStreamSynchronize(p->stream));
+  }
+  if (log && log_bytes) {
+    const size_t n = std::min(out.size(), log_bytes - 1);
+    std::memcpy(log, out.data(), n);
+    log[n] = 0;
   }
   return SR_OK;
 }
@@ -844,7 +1072,7 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
     {
       KTimer kt(p, sr::K_PSF_ADJ);
       HIP_TRY(sr::launch_psf_adj(p->separable, r + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(), nullptr,
-                                 0, p->nchunk_adj(nb), p->g, p->stream, true));
+                                 0, p->nchunk_adj(nb), p->g, p->stream, true, p->fam_adj()));
     }
     {
       KTimer kt(p, sr::K_ROT_FWD);
@@ -855,7 +1083,7 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
     {
       KTimer kt(p, sr::K_ROT_ADJ);
       HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
-                                    d_vids + s0, nb, p->g, p->stream));
+                                    d_vids + s0, nb, p->g, p->stream, nullptr, 0, nullptr, false, p->ell_vs()));
     }
     KTimer kt(p, sr::K_IMG_REDUCE);
     HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, d_gmu, nullptr, s0 == 0 ? sr::IMG_WRITE : sr::IMG_ACCUM,
@@ -873,6 +1101,144 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
 }
 
 namespace {
+
+// One EM call after its arguments are staged and its buffers sized: only
+// kernel launches on the plan's stream, so that the sequence can be captured
+// into a HIP graph and replayed.
+struct EmCall {
+  float* out;
+  const float *x0, *y, *b;
+  const int* vids;                 // subset view lists, block after block
+  std::vector<int> cnt, off;       // views per block, offset of each block in vids
+  int nblocks, niter;
+  bool fused, shared;
+};
+
+int em_body(sr_plan p, const EmCall& c) {
+  const int nsub = c.cnt[0];
+  HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P * nsub, p->stream));
+  if (c.shared) {
+    const sr::Taps t = p->taps();
+    KTimer kt(p, sr::K_PSF_ADJ);
+    HIP_TRY(sr::launch_psf_adj(p->separable, p->ones.as<float>(), t, c.vids, 1, p->zh.as<float>(), nullptr, 0,
+                               p->nchunk_adj(1), p->g, p->stream, true, p->fam_adj()));
+  }
+  int rc;
+  if (c.shared && !c.fused) {
+    for (int nb = 0; nb < c.nblocks; ++nb) {
+      float* as = p->em_asum.as<float>() + p->V * nb;
+      {
+        KTimer kt(p, sr::K_ROT_ADJ);
+        HIP_TRY(sr::launch_rot2_adj_z(p->zh.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
+                                      c.vids + c.off[nb], c.cnt[nb], p->g, p->stream, p->atten.as<float>(),
+                                      p->atten_vstride(), nullptr, false, p->ell_vs()));
+      }
+      {
+        KTimer kt(p, sr::K_IMG_REDUCE);
+        HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, as, nullptr, sr::IMG_WRITE, p->g, p->stream));
+      }
+      HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
+    }
+  } else if (!c.shared) {
+    for (int nb = 0; nb < c.nblocks; ++nb) {
+      float* as = p->em_asum.as<float>() + p->V * nb;
+      if ((rc = run_backproject(p, c.fused ? p->em_back.as<float>() : as, p->ones.as<float>(), c.vids + c.off[nb],
+                                c.cnt[nb], sr::IMG_WRITE, nullptr)))
+        return rc;
+      if (c.fused) HIP_TRY(sr::launch_to_zfast(p->em_back.as<float>(), as, p->g, p->stream));
+      HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
+    }
+  }
+  if (c.fused) {
+    {
+      KTimer kt(p, sr::K_ROT_FWD);
+      HIP_TRY(sr::launch_to_zfast(c.x0, p->zimg.as<float>(), p->g, p->stream));
+    }
+    const sr::Taps t = p->taps();
+    for (int it = 0; it < c.niter; ++it)
+      for (int nb = 0; nb < c.nblocks; ++nb) {
+        const int* sv = c.vids + c.off[nb];
+        const int ns = c.cnt[nb];
+        if ((rc = run_project(p, p->em_ratio.as<float>(), nullptr, sv, ns, c.y, c.b, true))) return rc;
+        {
+          KTimer kt(p, sr::K_PSF_ADJ);
+          HIP_TRY(sr::launch_psf_adj(p->separable, p->em_ratio.as<float>(), t, sv, ns, p->work.as<float>(),
+                                     p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(ns), p->g, p->stream,
+                                     true, p->fam_adj()));
+        }
+        KTimer kt(p, sr::K_ROT_ADJ);
+        float* as = p->em_asum.as<float>() + p->V * nb;
+        if (c.shared && it == 0)  // this subset's normaliser, summed in the same pass
+          HIP_TRY(sr::launch_rot2_adj_z_em_norm(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk,
+                                                sv, ns, p->g, p->stream, p->zh.as<float>(), p->atten.as<float>(),
+                                                p->atten_vstride(), as, p->ell_vs()));
+        else
+          HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv,
+                                        ns, p->g, p->stream, nullptr, 0, as, false, p->ell_vs()));
+      }
+    KTimer kt(p, sr::K_IMG_REDUCE);
+    HIP_TRY(sr::launch_img_reduce_z(p->zimg.as<float>(), 1, c.out, nullptr, sr::IMG_WRITE, p->g, p->stream));
+    return SR_OK;
+  }
+  // out = x0 (a copy kernel: the sequence holds kernels only)
+  HIP_TRY(sr::launch_img_reduce(c.x0, 1, c.out, nullptr, p->V, sr::IMG_WRITE, p->stream));
+  bool zready = false;  // the EM update also refreshed the z-fastest image copy
+  for (int it = 0; it < c.niter; ++it)
+    for (int nb = 0; nb < c.nblocks; ++nb) {
+      const int* sv = c.vids + c.off[nb];
+      if ((rc = run_project(p, p->em_ratio.as<float>(), c.out, sv, c.cnt[nb], c.y, c.b, zready))) return rc;
+      if ((rc = run_backproject(p, c.out, p->em_ratio.as<float>(), sv, c.cnt[nb], sr::IMG_EM,
+                                p->em_asum.as<float>() + p->V * nb, &zready)))
+        return rc;
+    }
+  return SR_OK;
+}
+
+constexpr size_t kMaxGraphs = 8;  // instantiated EM graphs kept per plan
+
+// em_body through a HIP graph: the first call with a given key runs eagerly
+// (kernel attributes are set on first launch), the second captures and
+// replays, later ones replay.  A failed capture runs the sequence eagerly.
+int em_graph(sr_plan p, const EmCall& c) {
+  std::vector<uintptr_t> key = {
+      (uintptr_t)c.out, (uintptr_t)c.x0, (uintptr_t)c.y, (uintptr_t)c.b, (uintptr_t)c.vids,
+      (uintptr_t)c.nblocks, (uintptr_t)c.niter, (uintptr_t)p->max_batch, (uintptr_t)p->work.p,
+      (uintptr_t)p->fpart.p, (uintptr_t)p->ipart.p, (uintptr_t)p->ones.p, (uintptr_t)p->em_asum.p,
+      (uintptr_t)p->em_ratio.p, (uintptr_t)p->em_back.p, (uintptr_t)p->zimg.p, (uintptr_t)p->zh.p};
+  for (auto& e : p->graphs)
+    if (e.key == key) {
+      HIP_TRY(hipGraphLaunch(e.exec, p->stream));
+      ++p->graph_replays;
+      return SR_OK;
+    }
+  auto seen = std::find(p->graph_seen.begin(), p->graph_seen.end(), key);
+  if (seen == p->graph_seen.end()) {
+    p->graph_seen.push_back(key);
+    if (p->graph_seen.size() > kMaxGraphs) p->graph_seen.erase(p->graph_seen.begin());
+    return em_body(p, c);
+  }
+  p->graph_seen.erase(seen);
+  HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = em_body(p, c);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(p->stream, &g);
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = (rc == SR_OK && ec == hipSuccess && g) ? hipGraphInstantiate(&x, g, nullptr, nullptr, 0)
+                                                               : hipErrorUnknown;
+  if (g) (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    (void)hipGetLastError();
+    return em_body(p, c);
+  }
+  if (p->graphs.size() >= kMaxGraphs) {
+    (void)hipGraphExecDestroy(p->graphs.front().exec);
+    p->graphs.erase(p->graphs.begin());
+  }
+  p->graphs.push_back({std::move(key), x});
+  ++p->graph_captures;
+  HIP_TRY(hipGraphLaunch(x, p->stream));
+  return SR_OK;
+}
 
 // shared body of mlem!/osem!: blocks of views nb:nblocks:nview
 // (src/ml-os-em.jl:21-49 with nblocks == 1; :97-131 otherwise)
@@ -894,16 +1260,19 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
     HIP_TRY(hipStreamSynchronize(p->stream));
     if (bad) return fail(SR_ERR_ARG, "need background > 0");
   }
-  const float *d_x0 = x0, *d_y = y, *d_b = b;
-  float* d_out = out;
+  EmCall c;
+  c.x0 = x0;
+  c.y = y;
+  c.b = b;
+  c.out = out;
   if (host) {
     if ((rc = stage_in(p, p->st_x0, x0, sizeof(float) * p->V)) || (rc = stage_in(p, p->st_aux1, y, sizeof(float) * M)) ||
         (rc = stage_in(p, p->st_aux2, b, sizeof(float) * M)) || (rc = p->st_img.ensure(sizeof(float) * p->V)))
       return rc;
-    d_x0 = p->st_x0.as<float>();
-    d_y = p->st_aux1.as<float>();
-    d_b = p->st_aux2.as<float>();
-    d_out = p->st_img.as<float>();
+    c.x0 = p->st_x0.as<float>();
+    c.y = p->st_aux1.as<float>();
+    c.b = p->st_aux2.as<float>();
+    c.out = p->st_img.as<float>();
   }
   // subset view lists, uploaded once
   std::vector<int> order;
@@ -911,108 +1280,41 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
     for (int v = nb; v < nview; v += nblocks) order.push_back(v);
   // block nb: views nb, nb + nblocks, ... (src/ml-os-em.jl:76); counts
   // differ by at most one when nblocks does not divide nview
-  std::vector<int> cnt(nblocks), off(nblocks);
+  c.cnt.resize(nblocks);
+  c.off.resize(nblocks);
   for (int nb = 0, o = 0; nb < nblocks; ++nb) {
-    cnt[nb] = (nview - nb + nblocks - 1) / nblocks;
-    off[nb] = o;
-    o += cnt[nb];
+    c.cnt[nb] = (nview - nb + nblocks - 1) / nblocks;
+    c.off[nb] = o;
+    o += c.cnt[nb];
   }
-  const int nsub = cnt[0];  // the largest block
+  c.nblocks = nblocks;
+  c.niter = niter;
+  const int nsub = c.cnt[0];  // the largest block
   if ((rc = p->vids.ensure(sizeof(int) * nview))) return rc;
   HIP_TRY(hipMemcpyAsync(p->vids.p, order.data(), sizeof(int) * nview, hipMemcpyHostToDevice, p->stream));
-  const int* d_vids = p->vids.as<int>();
-  // asum per block = A_b' 1, zeros -> Inf
-  if ((rc = p->ones.ensure(sizeof(float) * p->P * nsub)) || (rc = p->em_asum.ensure(sizeof(float) * p->V * nblocks)) ||
-      (rc = p->em_ratio.ensure(sizeof(float) * p->P * nsub)))
-    return rc;
-  HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P * nsub, p->stream));
+  c.vids = p->vids.as<int>();
   // z-fastest plans whose subsets fit one launch batch run the whole loop on
   // the z-fastest image copy: the rotation-adjoint gather applies the EM
   // update in place, the normalisers are kept z-fastest too, and the caller's
   // layout is written once at the end
-  const bool fused = p->zfast && nsub <= p->max_batch;
+  c.fused = p->zfast && nsub <= p->max_batch;
   // one PSF for all views: the PSF adjoint of a ones view is the same volume
   // H for every view, so A_v' 1 = rot2^T(H .* atten[v]); H is computed once.
   // On the fused path each normaliser is then summed by the first
   // iteration's gather of its subset, in the same pass as the back-projection
-  const bool shared = p->zfast && p->vstride == 0 && p->separable;
-  if ((rc = ensure_batch_buffers(p, std::min(nsub, p->max_batch)))) return rc;
-  if (shared) {
-    if ((rc = p->zh.ensure(sizeof(float) * p->V))) return rc;
-    const sr::Taps t = p->taps();
-    KTimer kt(p, sr::K_PSF_ADJ);
-    HIP_TRY(sr::launch_psf_adj(p->separable, p->ones.as<float>(), t, d_vids, 1, p->zh.as<float>(), nullptr, 0,
-                               p->nchunk_adj(1), p->g, p->stream, true));
-  }
-  if (shared && !fused) {
-    for (int nb = 0; nb < nblocks; ++nb) {
-      float* as = p->em_asum.as<float>() + p->V * nb;
-      {
-        KTimer kt(p, sr::K_ROT_ADJ);
-        HIP_TRY(sr::launch_rot2_adj_z(p->zh.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
-                                      d_vids + off[nb], cnt[nb], p->g, p->stream, p->atten.as<float>(),
-                                      p->atten_vstride()));
-      }
-      {
-        KTimer kt(p, sr::K_IMG_REDUCE);
-        HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, as, nullptr, sr::IMG_WRITE, p->g, p->stream));
-      }
-      HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
-    }
-  } else if (!shared) {
-    if (fused && (rc = p->em_back.ensure(sizeof(float) * p->V))) return rc;
-    for (int nb = 0; nb < nblocks; ++nb) {
-      float* as = p->em_asum.as<float>() + p->V * nb;
-      if ((rc = run_backproject(p, fused ? p->em_back.as<float>() : as, p->ones.as<float>(), d_vids + off[nb],
-                                cnt[nb], sr::IMG_WRITE, nullptr)))
-        return rc;
-      if (fused) HIP_TRY(sr::launch_to_zfast(p->em_back.as<float>(), as, p->g, p->stream));
-      HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
-    }
-  }
-  if (fused) {
-    {
-      KTimer kt(p, sr::K_ROT_FWD);
-      HIP_TRY(sr::launch_to_zfast(d_x0, p->zimg.as<float>(), p->g, p->stream));
-    }
-    const sr::Taps t = p->taps();
-    for (int it = 0; it < niter; ++it)
-      for (int nb = 0; nb < nblocks; ++nb) {
-        const int* sv = d_vids + off[nb];
-        const int ns = cnt[nb];
-        if ((rc = run_project(p, p->em_ratio.as<float>(), nullptr, sv, ns, d_y, d_b, true))) return rc;
-        {
-          KTimer kt(p, sr::K_PSF_ADJ);
-          HIP_TRY(sr::launch_psf_adj(p->separable, p->em_ratio.as<float>(), t, sv, ns, p->work.as<float>(),
-                                     p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(ns), p->g, p->stream,
-                                     true));
-        }
-        KTimer kt(p, sr::K_ROT_ADJ);
-        float* as = p->em_asum.as<float>() + p->V * nb;
-        if (shared && it == 0)  // this subset's normaliser, summed in the same pass
-          HIP_TRY(sr::launch_rot2_adj_z_em_norm(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk,
-                                                sv, ns, p->g, p->stream, p->zh.as<float>(), p->atten.as<float>(),
-                                                p->atten_vstride(), as));
-        else
-          HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv,
-                                        ns, p->g, p->stream, nullptr, 0, as));
-      }
-    KTimer kt(p, sr::K_IMG_REDUCE);
-    HIP_TRY(sr::launch_img_reduce_z(p->zimg.as<float>(), 1, d_out, nullptr, sr::IMG_WRITE, p->g, p->stream));
-  } else {
-    HIP_TRY(hipMemcpyAsync(d_out, d_x0, sizeof(float) * p->V, hipMemcpyDeviceToDevice, p->stream));
-    bool zready = false;  // the EM update also refreshed the z-fastest image copy
-    for (int it = 0; it < niter; ++it)
-      for (int nb = 0; nb < nblocks; ++nb) {
-        const int* sv = d_vids + off[nb];
-        if ((rc = run_project(p, p->em_ratio.as<float>(), d_out, sv, cnt[nb], d_y, d_b, zready))) return rc;
-        if ((rc = run_backproject(p, d_out, p->em_ratio.as<float>(), sv, cnt[nb], sr::IMG_EM,
-                                  p->em_asum.as<float>() + p->V * nb, &zready)))
-          return rc;
-      }
-  }
+  c.shared = p->zfast && p->vstride == 0 && p->separable;
+  // every buffer the kernel sequence touches is sized here, before it runs
+  // (asum per block = A_b' 1 with zeros -> Inf)
+  if ((rc = p->ones.ensure(sizeof(float) * p->P * nsub)) || (rc = p->em_asum.ensure(sizeof(float) * p->V * nblocks)) ||
+      (rc = p->em_ratio.ensure(sizeof(float) * p->P * nsub)) ||
+      (rc = ensure_batch_buffers(p, std::min(nsub, p->max_batch))) ||
+      (c.shared && (rc = p->zh.ensure(sizeof(float) * p->V))) ||
+      (((c.fused && !c.shared) || (!c.fused && nsub > p->max_batch)) &&
+       (rc = p->em_back.ensure(sizeof(float) * p->V))))
+    return rc;
+  if ((rc = (p->graph && !p->timing) ? em_graph(p, c) : em_body(p, c))) return rc;
   if (host) {
-    HIP_TRY(hipMemcpyAsync(out, d_out, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipMemcpyAsync(out, c.out, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
     resolve_timing(p);
     return SR_OK;

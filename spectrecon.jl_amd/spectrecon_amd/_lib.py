@@ -61,6 +61,11 @@ SIGNATURES = {
     "sr_grad": (C.c_int, [_vp, C.POINTER(C.c_double), _vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32]),
     "sr_osem_strided": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_int32]),
     "sr_plan_query": (C.c_int, [_vp, _ip]),
+    "sr_plan_set_graph": (C.c_int, [_vp, C.c_int32]),
+    "sr_plan_graph_stats": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "sr_plan_set_choice": (C.c_int, [_vp, _ip]),
+    "sr_plan_get_choice": (C.c_int, [_vp, _ip]),
+    "sr_plan_autotune": (C.c_int, [_vp, C.c_int32, C.c_char_p, C.c_size_t]),
     "sr_plan_set_batch": (C.c_int, [_vp, C.c_int32]),
     "sr_plan_synchronize": (C.c_int, [_vp]),
     "sr_plan_set_timing": (C.c_int, [_vp, C.c_int32]),

@@ -178,6 +178,47 @@ class SPECTplan:
             mask = 1 if classes is None else sum(2 << L.KERNEL_CLASSES.index(c) for c in classes)
         check(lib().sr_plan_set_timing(self.handle, mask))
 
+    def set_graph(self, enable=True):
+        """Replay the EM kernel sequence of mlem/osem calls from a HIP graph
+        (captured on the second call with the same buffers); results equal
+        the eager ones bit for bit."""
+        check(lib().sr_plan_set_graph(self.handle, 1 if enable else 0))
+
+    def graph_stats(self):
+        """(graphs captured, graph replays) since the plan was made."""
+        cap, rep = C.c_int64(), C.c_int64()
+        check(lib().sr_plan_graph_stats(self.handle, C.byref(cap), C.byref(rep)))
+        return cap.value, rep.value
+
+    CHOICES = ("psf_fwd_family", "psf_fwd_chunk_percent", "psf_adj_family", "psf_adj_chunk_percent",
+               "gather_slices")
+
+    def choice(self):
+        """The plan's kernel choices: PSF families (0 column-owned, 1 tile),
+        chunk counts in percent of the built-in ones, gather view slices."""
+        c = (C.c_int32 * 5)()
+        check(lib().sr_plan_get_choice(self.handle, c))
+        return dict(zip(self.CHOICES, list(c)))
+
+    def set_choice(self, **kw):
+        """Set some kernel choices by name (the rest keep their values)."""
+        cur = self.choice()
+        for k, v in kw.items():
+            if k not in cur:
+                raise KeyError(k)
+            cur[k] = int(v)
+        c = (C.c_int32 * 5)(*[cur[k] for k in self.CHOICES])
+        check(lib().sr_plan_set_choice(self.handle, c))
+
+    def autotune(self, nsub):
+        """Time the candidates of each kernel choice on batches of nsub views
+        and keep the fastest; returns the record of every candidate's time
+        (nsub <= 0 restores the built-in choices)."""
+        import json
+        buf = C.create_string_buffer(8192)
+        check(lib().sr_plan_autotune(self.handle, int(nsub), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     def timing(self):
         """{kernel class: (total device ms, launches)} since set_timing(True)."""
         out = {}

@@ -81,9 +81,28 @@ class DeviceBackend:
     admitted requests' streams run concurrently (no device-wide allocation,
     free or null-stream copy is made per request)."""
 
-    def __init__(self):
+    def __init__(self, autotune: bool = False):
         from .api import SPECTplan, mumap_phantom, osem_strided, psf_gauss
         self._SPECTplan, self._mumap, self._psf, self._osem = SPECTplan, mumap_phantom, psf_gauss, osem_strided
+        # autotune: kernel choices are timed once per (config, subset size),
+        # on one plan of the config (in Server.prepare, before concurrent
+        # work, or else on first use), and set on whichever plan of the
+        # config serves a request of that size, so that a request's result
+        # does not depend on the plan it lands on
+        self.autotune = autotune
+        self.tuned = {}     # (config, nsub) -> kernel choices
+        self.tune_log = []  # {"config", "nsub", "choice", "ms", "us"} per tuning
+
+    def tune(self, plan, cfg: Config, nsub: int):
+        key = (cfg, nsub)
+        if key in self.tuned:
+            return
+        t0 = time.perf_counter()
+        rec = plan.autotune(nsub)
+        self.tuned[key] = plan.choice()
+        self.tune_log.append({"config": f"{cfg.nx}x{cfg.nx}x{cfg.nz}/{cfg.nview}/psf{cfg.psf}", "nsub": nsub,
+                              "choice": self.tuned[key], "ms": 1e3 * (time.perf_counter() - t0),
+                              "us": {k: rec[k]["choice"]["us"] for k in ("psf_fwd", "psf_adj", "gather") if k in rec}})
 
     def make_plan(self, cfg: Config):
         mumap = self._mumap(cfg.nx, cfg.nx, cfg.nz)
@@ -93,6 +112,13 @@ class DeviceBackend:
 
     def start(self, plan, req: Request):
         import threading
+        if self.autotune:
+            nsub = -(-req.config.nview // req.nblocks)
+            choice = self.tuned.get((req.config, nsub))
+            if choice is None:
+                self.tune(plan, req.config, nsub)
+            elif plan.choice() != choice:
+                plan.set_choice(**choice)
         box = {}
 
         def work():
@@ -224,12 +250,17 @@ class Server:
                 time.sleep(poll_interval)
         return self.stats(self.clock() - t0)
 
-    def prepare(self, configs, per_config=1):
-        """Build resident plans ahead of the requests (within the budget)."""
+    def prepare(self, configs, per_config=1, nsubs=None):
+        """Build resident plans ahead of the requests (within the budget);
+        with nsubs ({config: subset sizes}) and a tuning backend, time each
+        plan's kernel choices for those subset sizes now."""
         for cfg in configs:
             while len(self.pools[cfg]) < per_config and self.resident + cfg.footprint() <= self.budget:
                 self.pools[cfg].append(self.backend.make_plan(cfg))
                 self.resident += cfg.footprint()
+            if nsubs and getattr(self.backend, "autotune", False) and self.pools[cfg]:
+                for n in sorted(nsubs.get(cfg, ())):
+                    self.backend.tune(self.pools[cfg][0], cfg, n)
 
     def reset_stats(self):
         self.finished, self.rejected, self.max_queued = [], [], 0
@@ -241,7 +272,8 @@ class Server:
                 "rejected": len(self.rejected), "max_queued": self.max_queued, "wall_s": wall,
                 "recons_per_s": len(self.finished) / wall if wall > 0 else None,
                 "latency_ms": {"p50": pct(50), "p99": pct(99)},
-                "plans": {f"{c.nx}x{c.nx}x{c.nz}/{c.nview}": len(p) for c, p in self.pools.items()}}
+                "plans": {f"{c.nx}x{c.nx}x{c.nz}/{c.nview}": len(p) for c, p in self.pools.items()},
+                "autotune": list(getattr(self.backend, "tune_log", []))}
 
     def close(self):
         for plans in self.pools.values():

@@ -523,6 +523,126 @@ def test_osem_strided_uneven_vs_oracle(gpu, O):
                                   gpu.osem(x0, y, b, gpu.Ablock(plan, 5), niter=2))
 
 
+CHOICE_CASES = {
+    "tile_kernels": dict(psf_fwd_family=1, psf_adj_family=1),
+    "half_chunks": dict(psf_fwd_chunk_percent=50, psf_adj_chunk_percent=50, gather_slices=2),
+    "quad_chunks": dict(psf_fwd_chunk_percent=400, psf_adj_chunk_percent=400, gather_slices=4),
+    "tile_double": dict(psf_fwd_family=1, psf_adj_family=1, psf_fwd_chunk_percent=200,
+                        psf_adj_chunk_percent=200, gather_slices=8),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CHOICE_CASES))
+@pytest.mark.parametrize("nx,nz,px,pz", [(32, 16, 7, 7), (64, 32, 15, 15), (48, 16, 37, 37), (16, 16, 3, 21),
+                                          (32, 48, 5, 9)])
+def test_kernel_choices_vs_oracle(gpu, O, nx, nz, px, pz, case):
+    """Every kernel choice the autotuner can make (PSF kernel family, plane
+    chunks per view, gather view slices) computes the same operator: parity
+    with the oracle as for the built-in choices, and the adjoint identity."""
+    nview = 6
+    rng = np.random.default_rng(nx + nz + px)
+    mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = np.asfortranarray(np.repeat(np.asarray(O.psf_gauss(nx, px, pz, 1.0, 5.0), np.float32)[:, :, :, None],
+                                       nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    assert plan.info()["layout"] == "z-fastest"
+    plan.set_choice(**CHOICE_CASES[case])
+    assert {k: plan.choice()[k] for k in CHOICE_CASES[case]} == CHOICE_CASES[case]
+    P32 = O.Problem(mumap, psfs, 0.48, angles32(nview).astype(np.float64), "two", np.float32, "fft")
+    image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    g = gpu.project(image, plan)
+    assert per_view_maxrel(g, P32.project(image)) <= PROJ_TOL
+    views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    gb = gpu.backproject(views, plan)
+    assert maxrel(gb, P32.backproject(views)) <= PROJ_TOL
+    a = float(np.vdot(views.astype(np.float64), g.astype(np.float64)))
+    b = float(np.vdot(image.astype(np.float64), gb.astype(np.float64)))
+    assert abs(a - b) <= 1e-5 * abs(a)
+
+
+def test_autotune_records_and_keeps_choices(gpu, O):
+    """The autotuner times every candidate, keeps the fastest of each choice,
+    logs them, and the tuned plan's OS-EM matches the FP64 oracle; invalid
+    choices are rejected and nsub <= 0 restores the built-in ones."""
+    nx, nz, nview = 32, 16, 12
+    xtrue, mumap, psfs = em_problem(nx, nz, nview)
+    plan = gpu.SPECTplan(mumap, psfs, 0.8)
+    rec = plan.autotune(3)
+    assert rec["views_per_launch"] == 3 and rec["layout"] == "z-fastest"
+    ch = plan.choice()
+    for key, fam, pct in (("psf_fwd", "psf_fwd_family", "psf_fwd_chunk_percent"),
+                          ("psf_adj", "psf_adj_family", "psf_adj_chunk_percent")):
+        cands = rec[key]["candidates"]
+        assert len(cands) >= 2 and all(c["us"] > 0 for c in cands)
+        best = min(cands, key=lambda c: c["us"])
+        assert rec[key]["choice"]["us"] == best["us"]
+        assert ch[fam] == {"column": 0, "tile": 1}[rec[key]["choice"]["family"]]
+        assert ch[pct] == rec[key]["choice"]["chunk_percent"]
+    assert [c["slices"] for c in rec["gather"]["candidates"]] == [2, 4, 8]
+    assert ch["gather_slices"] == rec["gather"]["choice"]["slices"]
+    ytrue = gpu.project(xtrue, plan)
+    b = np.full_like(ytrue, 0.1 * ytrue.mean())
+    y = gpu.poisson(20 * (ytrue + b), seed=5) / 20
+    x0 = np.ones_like(xtrue)
+    P64 = O.Problem(mumap, psfs, 0.8, angles32(nview).astype(np.float64), "two", np.float64, "direct")
+    xg = gpu.osem_strided(x0, y, b, plan, 4, 2)
+    xr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), 4, 2)
+    assert np.linalg.norm(xg - xr) <= 1e-4 * np.linalg.norm(xr)
+    for bad in (dict(psf_fwd_family=2), dict(psf_adj_chunk_percent=5), dict(gather_slices=3)):
+        with pytest.raises(gpu.SPECTError):
+            plan.set_choice(**bad)
+    assert plan.autotune(0) == {"reset": True}
+    assert list(plan.choice().values()) == [0, 100, 0, 100, 0]
+
+
+@pytest.mark.parametrize("variant", ["shared", "per_view_psf", "x_fastest", "batch2"])
+def test_graph_replay_equals_eager(gpu, O, monkeypatch, variant):
+    """EM calls replayed from a HIP graph equal the eager calls bit for bit,
+    on every replay, with other work of the plan (a projection that regrows
+    its work buffers, so the graph is captured again) between the replays;
+    device-mode replays read the current contents of their inputs.  Variants
+    cover the four kernel sequences: z-fastest with one PSF (shared
+    normaliser), per-view PSFs, the x-fastest layout, and subsets larger than
+    the launch batch."""
+    from spectrecon_amd import _lib as L
+    nx, nz, nview = 32, 16, 10
+    xtrue, mumap, psfs = em_problem(nx, nz, nview)
+    if variant == "per_view_psf":
+        psfs = np.asfortranarray(np.stack(
+            [np.asarray(O.psf_gauss(nx, 5, 5, 0.8 + 0.1 * v, 3.0 + 0.2 * v), np.float32) for v in range(nview)], 3))
+    if variant == "x_fastest":
+        monkeypatch.setenv("SR_LAYOUT", "zyx")
+    eager, graphed = gpu.SPECTplan(mumap, psfs, 0.8), gpu.SPECTplan(mumap, psfs, 0.8)
+    if variant == "batch2":
+        eager.set_batch(2)
+        graphed.set_batch(2)
+    graphed.set_graph(True)
+    ytrue = gpu.project(xtrue, eager)
+    b = np.full_like(ytrue, 0.1 * ytrue.mean())
+    ys = [gpu.poisson(20 * (ytrue + b), seed=s) / 20 for s in (3, 4)]
+    x0 = np.ones_like(xtrue)
+    ref = [gpu.osem_strided(x0, y, b, eager, 4, 2) for y in ys]
+    for k in range(3):
+        np.testing.assert_array_equal(gpu.osem_strided(x0, ys[0], b, graphed, 4, 2), ref[0])
+    gpu.project(xtrue, graphed)  # all views in one batch: the work buffers grow
+    for k in range(3):
+        np.testing.assert_array_equal(gpu.osem_strided(x0, ys[0], b, graphed, 4, 2), ref[0])
+    cap, rep = graphed.graph_stats()
+    # (captured once more if the projection moved a buffer, else replayed)
+    assert (cap, rep) in ((2, 2), (1, 4)), (cap, rep)
+    # device arrays: replays see new measurement contents
+    dx, dy_, db, dout = (gpu.DeviceArray.from_host(a) for a in (x0, ys[0], b, x0))
+    for k in range(4):
+        y = ys[k % 2]
+        dy_.copy_from(y)
+        L.check(gpu.lib().sr_osem_strided(graphed.handle, dout.ptr, dx.ptr, dy_.ptr, db.ptr, 4, 2, L.SR_MEM_DEVICE))
+        np.testing.assert_array_equal(dout.to_host(), ref[k % 2])
+    assert graphed.graph_stats() == (cap + 1, rep + 2)
+    graphed.set_graph(False)
+    np.testing.assert_array_equal(gpu.osem_strided(x0, ys[1], b, graphed, 4, 2), ref[1])
+    assert graphed.graph_stats() == (cap + 1, rep + 2)
+
+
 def test_serve_mix_equals_alone(gpu):
     """Requests of three sizes admitted concurrently by the serving loop (each
     on its own resident plan and stream): every result equals the same

@@ -144,3 +144,33 @@ def test_plans_per_config_cap_and_prepare():
     assert len(s.busy) == 1 and len(be.made) == 1  # the others wait for the resident plan
     drive(s, clock, [])
     assert len(s.finished) == 3 and len(be.made) == 1
+
+
+def test_prepare_tunes_once_per_config_and_subset_size():
+    """With a tuning backend, prepare() tunes each (config, subset size) once,
+    on one plan of the config, and the stats carry the tuning log."""
+    clock = Clock()
+
+    class TuningBackend(FakeBackend):
+        autotune = True
+
+        def __init__(self, *a):
+            super().__init__(*a)
+            self.tune_log = []
+
+        def tune(self, plan, cfg, nsub):
+            if not any(t["key"] == (cfg, nsub) for t in self.tune_log):
+                self.tune_log.append({"key": (cfg, nsub), "plan": plan})
+
+    be = TuningBackend(clock, {A: 1.0, B: 1.0})
+    srv = Server(hbm_budget=1e12, backend=be, clock=clock, plans_per_config=2)
+    srv.prepare([A, B], per_config=2, nsubs={A: {4, 2}, B: {8}})
+    assert len(be.made) == 4
+    assert sorted((t["key"][0].nx, t["key"][1]) for t in be.tune_log) == [(32, 2), (32, 4), (64, 8)]
+    assert all(t["plan"] == srv.pools[t["key"][0]][0] for t in be.tune_log)
+    srv.prepare([A], per_config=2, nsubs={A: {4}})  # already tuned
+    assert len(be.tune_log) == 3
+    assert srv.stats(1.0)["autotune"] == be.tune_log
+    plain = Server(hbm_budget=1e12, backend=FakeBackend(clock, {A: 1.0}), clock=clock)
+    plain.prepare([A], nsubs={A: {4}})  # a backend without tuning: plans only
+    assert plain.stats(1.0)["autotune"] == []
