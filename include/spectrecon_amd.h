@@ -170,8 +170,12 @@ int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
 
 /* Kernel choices of a plan (no reference counterpart; the reference has one
  * CPU code path).  choice[SR_CHOICE_*]:
- *   FWD_FAMILY / ADJ_FAMILY: 0 column-owned matrix-core PSF kernels where they
- *     fit (z-fastest plans), 1 the tile kernels;
+ *   FWD_FAMILY: 0 the best fitting matrix-core PSF forward (z-fastest plans:
+ *     producer/consumer waves, the gathers overlapping the matrix products),
+ *     1 the tile kernel, 2 the column-owned kernel, 3 producer/consumer with
+ *     8 producer waves;
+ *   ADJ_FAMILY: 0 (or 2) the column-owned adjoint where it fits, 1 the tile
+ *     kernel;
  *   FWD_CHUNKS / ADJ_CHUNKS: plane chunks per view, in percent of the built-in
  *     count (one block per compute unit over the batch), 10..1000;
  *   GATHER_SLICES: view slices per block of the rotation-adjoint gather

@@ -1204,7 +1204,36 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
 // LDS: two planes (rows x, stride WZ = 16 mod 64 floats), per wave Qw (rows
 // z + hz, 16 columns), TBP tap planes.  grid (nchunk, nb), block 512.
 __host__ __device__ inline size_t mf_fcol_lds_floats(const Geo& g, int nt, int tbp) {
-  return 2 * (size_t)g.nx * mf_wq(g.nz) + (size_t)MF_WAVES * mf_rq(g.nz, nt) * 16 + (size_t)tbp * MfTaps(nt).size;
+  return 2 * (size_t)g.nx * mf_wq(g.nz) + (size_t)MF_WAVES * mf_rq(g.nz, nt) * 16 + (size_t)tbp * MfTaps(nt).size +
+         16 * (size_t)g.nx;
+}
+// One pixel x of plane y of the :two rotation, for the fused forward's tap
+// table: e[0] = element offsets ((row * N + col) * nz) of the four taps,
+// clamped into the image; e[1] = the bilinear weights with the x masks in
+// (wx1, wx2) and the y masks in (wy1, wy2).  A tap in the zero pad gets a
+// zero factor, so ((s11 wx1 + s12 wx2) wy1 + (s21 wx1 + s22 wx2) wy2) equals
+// the per-tap masked form of rot_taps / rot_value for finite data.
+// (theta = 0: the copy of (x, y), all four offsets on that pixel)
+__device__ __forceinline__ void rot_entry(const RotView& r, int N, int nz, int pad, int x, int y, float4* e) {
+  int o11, o12, o21, o22;
+  float wx1 = 1.f, wx2 = 0.f, wy1 = 1.f, wy2 = 0.f;
+  if (r.kind == 0) {
+    o11 = o12 = o21 = o22 = (y * N + x) * nz;
+  } else {
+    const BiCoef c = bilinear_coef(r, pad, x, y);
+    const int ja = clampi(c.j1, 0, N - 1), jb = clampi(c.j1 + 1, 0, N - 1);
+    const int ra = clampi(c.j2, 0, N - 1) * N, rb = clampi(c.j2 + 1, 0, N - 1) * N;
+    o11 = (ra + ja) * nz;
+    o12 = (ra + jb) * nz;
+    o21 = (rb + ja) * nz;
+    o22 = (rb + jb) * nz;
+    wx1 = (unsigned)c.j1 < (unsigned)N ? c.w11 : 0.f;
+    wx2 = (unsigned)(c.j1 + 1) < (unsigned)N ? c.w12 : 0.f;
+    wy1 = (unsigned)c.j2 < (unsigned)N ? c.w21 : 0.f;
+    wy2 = (unsigned)(c.j2 + 1) < (unsigned)N ? c.w22 : 0.f;
+  }
+  e[0] = make_float4(__int_as_float(o11), __int_as_float(o12), __int_as_float(o21), __int_as_float(o22));
+  e[1] = make_float4(wx1, wx2, wy1, wy2);
 }
 template <int TPW>
 __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restrict__ img_t, Taps t,
@@ -1229,13 +1258,17 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
   float* Pz = L;  // two plane buffers
   float* Qw = L + 2 * PS + wave * RG * 16;
   float* s_taps = L + 2 * PS + MF_WAVES * RG * 16;  // TBP planes of TT.size
+  // two tap tables of the rotation (planes of even / odd offset from y0),
+  // nx entries of two float4 each (rot_entry)
+  float4* s_rt = reinterpret_cast<float4*>(s_taps + TBP * TT.size);
   // zeros where no plane data lands: the wave buffers' pad rows and the tap tables' pads
   for (int k = tid; k < MF_WAVES * RG * 16 + TBP * TT.size; k += MF_THREADS) L[2 * PS + k] = 0.f;
 
-  // element slots: element tid + k*threads of the [x][z] plane, (x, z quad) packed as x << 8 | zq
+  // element slots: element tid + k*threads of the [x][z] plane, (x, z quad)
+  // packed as x << 8 | zq; xo = the element's offset in a [x][z] plane
   const int nz4 = nz >> 2, nv = nx * nz4;
   constexpr int RS = MF_RA + MF_RB;
-  int rxz[RS];
+  int rxz[RS], xo[RS];
   RotSlot ra[MF_RA], rb[MF_RB];
   const RotView rot = rv[v];
   const float* av = atten + atten_vstride * v;
@@ -1243,23 +1276,43 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
   for (int k = 0; k < RS; ++k) {
     const int i = min(tid + k * MF_THREADS, nv - 1), x = i / nz4;
     rxz[k] = x << 8 | (i - x * nz4);
+    xo[k] = x * nz + 4 * (i - x * nz4);
   }
+  auto rot_table = [&](int y) {  // plane y's entries (one pixel per thread)
+    if (tid < nx) rot_entry(rot, nx, nz, g.pad, tid, y, s_rt + 2 * nx * ((y - y0) & 1) + 2 * tid);
+  };
   // (scheduling fences keep each part's gathers where they are issued, so
   // that only one part is live at a time)
   auto rot_half = [&](auto& h, int k0, int y) {
+    const float4* T = s_rt + 2 * nx * ((y - y0) & 1);
+    const float* ay = av + (long)y * nx * nz;
     SR_SCHED_FENCE();
 #pragma unroll
-    for (int k = 0; k < (int)(sizeof(h) / sizeof(h[0])); ++k)
-      rot_issue(h[k], img_t, av, rot, nx, nz, g.pad, rxz[k0 + k] >> 8, y, 4 * (rxz[k0 + k] & 255));
+    for (int k = 0; k < (int)(sizeof(h) / sizeof(h[0])); ++k) {
+      const float4 o = T[2 * (rxz[k0 + k] >> 8)];
+      const int zo = 4 * (rxz[k0 + k] & 255);
+      h[k].s11 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.x) + zo));
+      h[k].s12 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.y) + zo));
+      h[k].s21 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.z) + zo));
+      h[k].s22 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.w) + zo));
+      h[k].a = *reinterpret_cast<const float4*>(ay + xo[k0 + k]);
+    }
     SR_SCHED_FENCE();
   };
   auto rot_store = [&](auto& h, int k0, int y, float* buf) {
+    const float4* T = s_rt + 2 * nx * ((y - y0) & 1);
     SR_SCHED_FENCE();
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(h) / sizeof(h[0])); ++k) {
       const int x = rxz[k0 + k] >> 8, zq = rxz[k0 + k] & 255;
-      const float4 val = rot_value(h[k], rot, nx, g.pad, x, y);
-      *reinterpret_cast<float4*>(buf + x * WZ + 4 * zq) = make_float4(val.x, val.y, val.z, val.w);
+      const float4 w = T[2 * x + 1];
+      const RotSlot& q = h[k];
+      float4 val;
+      val.x = ((q.s11.x * w.x + q.s12.x * w.y) * w.z + (q.s21.x * w.x + q.s22.x * w.y) * w.w) * q.a.x;
+      val.y = ((q.s11.y * w.x + q.s12.y * w.y) * w.z + (q.s21.y * w.x + q.s22.y * w.y) * w.w) * q.a.y;
+      val.z = ((q.s11.z * w.x + q.s12.z * w.y) * w.z + (q.s21.z * w.x + q.s22.z * w.y) * w.w) * q.a.z;
+      val.w = ((q.s11.w * w.x + q.s12.w * w.y) * w.z + (q.s21.w * w.x + q.s22.w * w.y) * w.w) * q.a.w;
+      *reinterpret_cast<float4*>(buf + x * WZ + 4 * zq) = val;
     }
   };
   auto load_taps = [&](int yg, int ng) {
@@ -1286,9 +1339,11 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
 #pragma unroll
   for (int i = 0; i < TPW; ++i) dv[i] = mf4{0.f, 0.f, 0.f, 0.f};
   if (y0 < y1) {  // the first plane and tap group: all their loads in flight together
+    rot_table(y0);
+    if (y0 + 1 < y1) rot_table(y0 + 1);
+    __syncthreads();  // zero fill and the first two rotation tables done
     rot_half(ra, 0, y0);
     rot_half(rb, MF_RA, y0);
-    __syncthreads();  // zero fill done
     load_taps(y0, min(TBP, y1 - y0));
     rot_store(ra, 0, y0, Pz);
     rot_store(rb, MF_RA, y0, Pz);
@@ -1296,7 +1351,7 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
 
   for (int y = y0; y < y1; ++y) {
     const int cur = (y - y0) & 1;
-    lds_barrier();  // plane y staged; every wave is done with plane y-1 (its buffer and taps)
+    lds_barrier();  // plane y staged; every wave is done with plane y-1 (its buffer, taps and rotation table)
     if (y > y0 && (y - y0) % TBP == 0) {
       load_taps(y, min(TBP, y1 - y));
       lds_barrier();
@@ -1306,6 +1361,9 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
     float* Pn = Pz + (cur ^ 1) * PS;
     const int yn = min(y + 1, y1 - 1);
     rot_half(ra, 0, yn);
+    // plane y + 2's rotation table, into plane y's (read before the barrier
+    // above; its next readers stage plane y + 2, after the next barrier)
+    if (y + 2 < y1) rot_table(y + 2);
     // x pass (operands swapped: lane (k, n) register r = Q[z0 + n][x0 + 4k + r])
     const float* ub = tb + TT.off + m16 - q4 + 2 * hx;
     mf4 dx[TPW];
@@ -1385,6 +1443,239 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
     }
     asm volatile("" ::: "memory");  // the next plane's Qw stores stay after these reads
     rot_store(rb, MF_RA, yn, Pn);
+  }
+  if (own) {
+    float* pb = part + ((long)b * nchunk + c) * nx * nz;
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) pb[(long)(16 * i + 4 * q4 + r) * nx + x0 + m16] = dv[i][r];
+    }
+  }
+}
+
+// --- forward, z-fastest, producer and consumer waves -----------------------
+// The passes of k_psf_fwd_col, with the staging moved to waves of their own:
+// waves 0..7 (consumers) own an x column each and run plane y's x and z
+// passes from LDS, while the NPW producer waves stage plane y + 1 (the
+// gathers of the :two rotation through its tap table, attenuation, bilinear
+// combination), load plane y + 1's PSF taps and build plane y + 2's rotation
+// table.  The gathers and their arithmetic then overlap the matrix products
+// instead of alternating with them; one barrier per plane.
+// LDS: two planes (rows x, stride WZ), per consumer wave Qw, two tap planes
+// (planes of even / odd offset from y0), two rotation tables.
+// grid (nchunk, nb), block 512 + 64 NPW.
+constexpr int WS_RE = 5;  // element slots per producer thread and round
+__host__ __device__ inline size_t mf_fws_lds_floats(const Geo& g, int nt) {
+  return 2 * (size_t)g.nx * mf_wq(g.nz) + (size_t)MF_WAVES * mf_rq(g.nz, nt) * 16 + 2 * (size_t)MfTaps(nt).size +
+         16 * (size_t)g.nx;
+}
+template <int TPW, int NPW>
+__global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const float* __restrict__ img_t, Taps t,
+                                                                    const int* __restrict__ vids, int Yc,
+                                                                    float* __restrict__ part, Geo g,
+                                                                    const float* __restrict__ atten,
+                                                                    long atten_vstride,
+                                                                    const RotView* __restrict__ rv) {
+  constexpr int PT = 64 * NPW;                       // producer threads
+  constexpr int RR = (10240 / 4 + PT * WS_RE - 1) / (PT * WS_RE);  // rounds (nx nz <= 10240)
+  extern __shared__ float L[];
+  const int NT = t.ntx;
+  const MfTaps TT(NT);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
+  const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
+  const int ntx = nx >> 4;
+  const int WZ = mf_wq(nz), RG = mf_rq(nz, NT), PS = nx * WZ;
+  const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
+  const int v = vids[b];
+  const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool producer = wave >= MF_WAVES;
+  const int m16 = lane & 15, q4 = lane >> 4;
+  float* Pz = L;  // two plane buffers
+  float* Qw = L + 2 * PS + min(wave, MF_WAVES - 1) * RG * 16;
+  float* s_taps = L + 2 * PS + MF_WAVES * RG * 16;  // two tap planes of TT.size
+  float4* s_rt = reinterpret_cast<float4*>(s_taps + 2 * TT.size);
+  for (int k = tid; k < MF_WAVES * RG * 16 + 2 * TT.size; k += MF_THREADS + PT) L[2 * PS + k] = 0.f;
+
+  if (producer) {
+    const RotView rot = rv[v];
+    const float* av = atten + atten_vstride * v;
+    const int nz4 = nz >> 2, nv = nx * nz4;
+    const int pt = tid - MF_THREADS;  // producer thread
+    // slots: element pt + (r * WS_RE + k) * PT of the [x][z] plane, packed as
+    // x << 8 | zq (clamped to the last element: a clamped slot rewrites it)
+    int sxz[RR][WS_RE];
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+#pragma unroll
+      for (int k = 0; k < WS_RE; ++k) {
+        const int i = min(pt + (r * WS_RE + k) * PT, nv - 1), x = i / nz4;
+        sxz[r][k] = x << 8 | (i - x * nz4);
+      }
+    auto rot_table = [&](int y) {  // plane y's rotation table, one pixel per producer thread
+      if (pt < nx) rot_entry(rot, nx, nz, g.pad, pt, y, s_rt + 2 * nx * ((y - y0) & 1) + 2 * pt);
+    };
+    auto load_taps = [&](int y) {  // plane y's six tap tables into tap plane (y - y0) & 1
+      const long plane = (long)v * t.vstride + y;
+      float* dst = s_taps + ((y - y0) & 1) * TT.size + TT.off;
+      for (int e = pt; e < 6 * NT; e += PT) {
+        const int tab = e / NT, tk = e - tab * NT;
+        const float* src =
+            tab == 0 ? t.u + plane * NT : tab == 1 ? t.w + plane * NT : t.cum + (plane * 4 + tab - 2) * NT;
+        dst[tab * TT.tp + tk] = src[tk];
+      }
+    };
+    auto stage = [&](int y, float* buf) {  // plane y into buf, RR rounds of WS_RE elements
+      const float4* T = s_rt + 2 * nx * ((y - y0) & 1);
+      const float* ay = av + (long)y * nx * nz;
+#pragma unroll
+      for (int r = 0; r < RR; ++r) {
+        if (r * WS_RE * PT >= nv) break;
+        // (the slot words are opaque here, so their addresses are derived per
+        // round instead of being hoisted out of the plane loop and held live)
+#pragma unroll
+        for (int k = 0; k < WS_RE; ++k) asm volatile("" : "+v"(sxz[r][k]));
+        RotSlot h[WS_RE];
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int k = 0; k < WS_RE; ++k) {
+          const int x = sxz[r][k] >> 8, zq = sxz[r][k] & 255;
+          const float4 o = T[2 * x];
+          h[k].s11 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.x) + 4 * zq));
+          h[k].s12 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.y) + 4 * zq));
+          h[k].s21 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.z) + 4 * zq));
+          h[k].s22 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.w) + 4 * zq));
+          h[k].a = *reinterpret_cast<const float4*>(ay + 4 * __umul24(x, nz4) + 4 * zq);
+        }
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int k = 0; k < WS_RE; ++k) {
+          const int x = sxz[r][k] >> 8, zq = sxz[r][k] & 255;
+          const float4 w = T[2 * x + 1];
+          const RotSlot& q = h[k];
+          float4 val;
+          val.x = ((q.s11.x * w.x + q.s12.x * w.y) * w.z + (q.s21.x * w.x + q.s22.x * w.y) * w.w) * q.a.x;
+          val.y = ((q.s11.y * w.x + q.s12.y * w.y) * w.z + (q.s21.y * w.x + q.s22.y * w.y) * w.w) * q.a.y;
+          val.z = ((q.s11.z * w.x + q.s12.z * w.y) * w.z + (q.s21.z * w.x + q.s22.z * w.y) * w.w) * q.a.z;
+          val.w = ((q.s11.w * w.x + q.s12.w * w.y) * w.z + (q.s21.w * w.x + q.s22.w * w.y) * w.w) * q.a.w;
+          *reinterpret_cast<float4*>(buf + x * WZ + 4 * zq) = val;
+        }
+        SR_SCHED_FENCE();
+      }
+    };
+    if (y0 < y1) {
+      rot_table(y0);
+      if (y0 + 1 < y1) rot_table(y0 + 1);
+    }
+    __syncthreads();  // zero fill and the first two rotation tables done
+    if (y0 < y1) {
+      stage(y0, Pz);
+      load_taps(y0);
+    }
+    // (the same barriers as the consumer loop below, one per plane)
+    for (int y = y0; y < y1; ++y) {
+      lds_barrier();  // plane y and its taps staged; plane y - 1's buffers free
+      if (y + 1 < y1) {
+        stage(y + 1, Pz + (((y - y0) & 1) ^ 1) * PS);
+        load_taps(y + 1);
+      }
+      // plane y + 2's rotation table, into plane y's (read before the barrier above)
+      if (y + 2 < y1) rot_table(y + 2);
+    }
+    return;
+  }
+
+  __syncthreads();  // zero fill and the first two rotation tables done
+  const bool own = wave < ntx;  // (waves past the last column recompute it and store nothing)
+  const int x0 = 16 * min(wave, ntx - 1);
+  mf4 dv[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) dv[i] = mf4{0.f, 0.f, 0.f, 0.f};
+
+  for (int y = y0; y < y1; ++y) {
+    const int cur = (y - y0) & 1;
+    lds_barrier();  // plane y and its taps staged; plane y - 1's buffers free
+    const float* tb = s_taps + cur * TT.size;
+    const float* Pc = Pz + cur * PS;
+    // x pass (operands swapped: lane (k, n) register r = Q[z0 + n][x0 + 4k + r])
+    const float* ub = tb + TT.off + m16 - q4 + 2 * hx;
+    mf4 dx[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) dx[i] = mf4{0.f, 0.f, 0.f, 0.f};
+    {
+      float a0[TPW], a1[TPW], b0, b1;
+      auto ld = [&](float (&a)[TPW], float& bb, int s) {
+        bb = ub[-4 * s];
+        const float* pr = Pc + clampi(x0 - hx + 4 * s + q4, 0, nx - 1) * WZ + m16;
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) a[i] = pr[16 * i];
+      };
+      ld(a0, b0, 0);
+      int s = 0;
+      for (; s + 1 < nsx; s += 2) {
+        ld(a1, b1, s + 1);
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b0, a0[i], dx[i]);
+        SR_SCHED_FENCE();
+        ld(a0, b0, min(s + 2, nsx - 1));
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b1, a1[i], dx[i]);
+      }
+      if (s < nsx) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b0, a0[i], dx[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < TPW; ++i)
+      *reinterpret_cast<float4*>(Qw + (hz + 16 * i + m16) * 16 + 4 * q4) =
+          make_float4(dx[i][0], dx[i][1], dx[i][2], dx[i][3]);
+    // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    // z pass: accumulate the view sum, the replicate boundary first
+    const float* wz = tb + TT.tp + TT.off + m16 - q4 + 2 * hz;
+    const float* swt = tb + 4 * TT.tp + TT.off;
+    const float* swh = swt + TT.tp;
+    {
+      const float be = q4 == 0 ? Qw[hz * 16 + m16] : q4 == 1 ? Qw[(hz + nz - 1) * 16 + m16] : 0.f;
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        const int zm = 16 * i + m16;
+        const float ae = q4 == 0 ? tail_at(swt, NT, zm + hz + 1) : q4 == 1 ? head_at(swh, NT, zm + hz - nz) : 0.f;
+        dv[i] = mfma4(ae, be, dv[i]);
+      }
+    }
+    {
+      float bq0[TPW], bq1[TPW], a0, a1;
+      auto ld = [&](float (&bq)[TPW], float& aa, int s) {
+        aa = wz[-4 * s];
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) bq[i] = Qw[(16 * i + q4 + 4 * s) * 16 + m16];
+      };
+      ld(bq0, a0, 0);
+      int s = 0;
+      for (; s + 1 < nsz; s += 2) {
+        ld(bq1, a1, s + 1);
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a0, bq0[i], dv[i]);
+        SR_SCHED_FENCE();
+        ld(bq0, a0, min(s + 2, nsz - 1));
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a1, bq1[i], dv[i]);
+      }
+      if (s < nsz) {
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a0, bq0[i], dv[i]);
+      }
+    }
+    asm volatile("" ::: "memory");  // the next plane's Qw stores stay after these reads
   }
   if (own) {
     float* pb = part + ((long)b * nchunk + c) * nx * nz;
@@ -2650,6 +2941,11 @@ bool psf_fwd_col_ok(const Geo& g, int ntx) {
          sizeof(float) * mf_fcol_lds_floats(g, ntx, 1) <= kMaxLds;
 }
 
+bool psf_fwd_ws_ok(const Geo& g, int ntx, int npw) {
+  return psf_fwd_col_ok(g, ntx) && g.nx <= 64 * npw && (long)g.nx * g.nz <= 10240 &&
+         sizeof(float) * mf_fws_lds_floats(g, ntx) <= kMaxLds;
+}
+
 bool psf_adj_col_ok(const Geo& g, int ntx) {
   return psf_mfma_supported(g, ntx) && g.nz / 16 <= MF_TPW && sizeof(float) * mf_col_lds_floats(g, ntx, 1) <= kMaxLds;
 }
@@ -2661,6 +2957,22 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
   nchunk = (g.ny + Yc - 1) / Yc;
   if (nchunk_used) *nchunk_used = nchunk;
   if (nb <= 0) return hipSuccess;
+  const int npw = family == PSF_WS8 ? 8 : 4;
+  if (separable && zfast && (family == PSF_AUTO || family == PSF_WS8) && psf_fwd_ws_ok(g, t.ntx, npw)) {
+    const size_t lds = sizeof(float) * mf_fws_lds_floats(g, t.ntx);
+    SR_TPW_SWITCH(g.nz / 16, {
+      if (npw == 8) {
+        allow_lds(k_psf_fwd_ws<TPW, 8>, lds);
+        k_psf_fwd_ws<TPW, 8><<<dim3(nchunk, nb), MF_THREADS + 512, lds, s>>>(work, t, vids, Yc, part, g, atten,
+                                                                             atten_vstride, rv);
+      } else {
+        allow_lds(k_psf_fwd_ws<TPW, 4>, lds);
+        k_psf_fwd_ws<TPW, 4><<<dim3(nchunk, nb), MF_THREADS + 256, lds, s>>>(work, t, vids, Yc, part, g, atten,
+                                                                             atten_vstride, rv);
+      }
+    });
+    return hipGetLastError();
+  }
   if (separable && zfast && family != PSF_TILE && psf_fwd_col_ok(g, t.ntx)) {
     const size_t base = mf_fcol_lds_floats(g, t.ntx, 0), per = MfTaps(t.ntx).size;
     const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), ((long)(kMaxLds / sizeof(float)) - (long)base) /

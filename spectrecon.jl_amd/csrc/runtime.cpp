@@ -656,6 +656,10 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   if (d.conv == SR_CONV_SEPARABLE && !sep) return bail(fail(SR_ERR_ARG, "PSF planes are not rank 1"));
   p->separable = sep ? 1 : 0;
   p->mfma = sep && mfma_ok;
+  // the matrix-core kernels take any table length: the exact one keeps their
+  // LDS tap tables and wave buffers smallest (the VALU kernels are
+  // instantiated per multiple of 8)
+  if (p->mfma) p->nt = std::max(d.px, d.pz);
   if (sep) {
     std::vector<float> hu((size_t)nstore * p->nt, 0.f), hw((size_t)nstore * p->nt, 0.f);
     for (int pl = 0; pl < nstore; ++pl) {
@@ -785,8 +789,10 @@ int sr_plan_graph_stats(sr_plan p, int64_t* captures, int64_t* replays) {
 
 int sr_plan_set_choice(sr_plan p, const int32_t* c) {
   if (!p || !c) return fail(SR_ERR_ARG, "null argument");
-  for (int k : {SR_CHOICE_FWD_FAMILY, SR_CHOICE_ADJ_FAMILY})
-    if (c[k] != 0 && c[k] != 1) return fail(SR_ERR_ARG, "PSF family must be 0 or 1");
+  if (c[SR_CHOICE_FWD_FAMILY] < 0 || c[SR_CHOICE_FWD_FAMILY] > 3)
+    return fail(SR_ERR_ARG, "PSF forward family must be 0..3");
+  if (c[SR_CHOICE_ADJ_FAMILY] < 0 || c[SR_CHOICE_ADJ_FAMILY] > 2)
+    return fail(SR_ERR_ARG, "PSF adjoint family must be 0..2");
   for (int k : {SR_CHOICE_FWD_CHUNKS, SR_CHOICE_ADJ_CHUNKS})
     if (c[k] < 10 || c[k] > 1000) return fail(SR_ERR_ARG, "chunk percent must be in 10..1000");
   const int vs = c[SR_CHOICE_GATHER_SLICES];
@@ -865,11 +871,25 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
     for (int dir = 0; dir < 2 && psf_choices; ++dir) {
       const int fk = dir == 0 ? SR_CHOICE_FWD_FAMILY : SR_CHOICE_ADJ_FAMILY;
       const int ck = dir == 0 ? SR_CHOICE_FWD_CHUNKS : SR_CHOICE_ADJ_CHUNKS;
-      const bool col = p->zfast && (dir == 0 ? sr::psf_fwd_col_ok(p->g, t.ntx) : sr::psf_adj_col_ok(p->g, t.ntx));
+      // families that fit this plan (forward: producer/consumer waves with 4
+      // or 8 producers, column-owned, tile; adjoint: column-owned, tile)
+      std::vector<int> fams;
+      if (dir == 0) {
+        if (p->zfast && sr::psf_fwd_ws_ok(p->g, t.ntx, 4)) fams.push_back(sr::PSF_AUTO);
+        if (p->zfast && sr::psf_fwd_ws_ok(p->g, t.ntx, 8)) fams.push_back(sr::PSF_WS8);
+        if (p->zfast && sr::psf_fwd_col_ok(p->g, t.ntx)) fams.push_back(sr::PSF_COL);
+      } else if (p->zfast && sr::psf_adj_col_ok(p->g, t.ntx)) {
+        fams.push_back(sr::PSF_AUTO);
+      }
+      fams.push_back(sr::PSF_TILE);
+      auto fam_name = [&](int f) {
+        return f == sr::PSF_TILE ? "tile" : f == sr::PSF_COL ? "column" : f == sr::PSF_WS8 ? "overlap8"
+               : dir == 0 ? "overlap" : "column";
+      };
       appendf(out, ", \"%s\": {\"candidates\": [", dir == 0 ? "psf_fwd" : "psf_adj");
       double best = -1;
       int bf = p->choice[fk], bc = p->choice[ck], n = 0;
-      for (int fam = col ? 0 : 1; fam <= 1; ++fam) {
+      for (int fam : fams) {
         int last_chunks = -1;
         for (int pct : {50, 100, 200, 400}) {
           const int chunks = p->chunks_scaled(nb, pct);
@@ -894,8 +914,8 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
                                         p->atten.as<float>(), p->atten_vstride(), chunks, p->g, p->stream, p->zfast,
                                         fam);
             });
-          appendf(out, "%s{\"family\": \"%s\", \"chunks\": %d, \"us\": %.2f}", n++ ? ", " : "",
-               fam == 0 ? "column" : "tile", chunks, us);
+          appendf(out, "%s{\"family\": \"%s\", \"code\": %d, \"chunks\": %d, \"us\": %.2f}", n++ ? ", " : "",
+                  fam_name(fam), fam, chunks, us);
           if (us > 0 && (best < 0 || us < best)) {
             best = us;
             bf = fam;
@@ -905,8 +925,8 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
       }
       p->choice[fk] = bf;
       p->choice[ck] = bc;
-      appendf(out, "], \"choice\": {\"family\": \"%s\", \"chunk_percent\": %d, \"us\": %.2f}}",
-           bf == 0 ? "column" : "tile", bc, best);
+      appendf(out, "], \"choice\": {\"family\": \"%s\", \"code\": %d, \"chunk_percent\": %d, \"us\": %.2f}}",
+              fam_name(bf), bf, bc, best);
     }
     // rotation-adjoint gather (z-fastest): view slices per block
     if (p->zfast) {
