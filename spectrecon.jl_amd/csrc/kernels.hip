@@ -1373,7 +1373,7 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
       float a0[TPW], a1[TPW], b0, b1;
       auto ld = [&](float (&a)[TPW], float& bb, int s) {
         bb = ub[-4 * s];
-        const float* pr = Pc + clampi(x0 - hx + 4 * s + q4, 0, nx - 1) * WZ + m16;
+        const float* pr = Pc + __umul24(clampi(x0 - hx + 4 * s + q4, 0, nx - 1), WZ) + m16;
 #pragma unroll
         for (int i = 0; i < TPW; ++i) a[i] = pr[16 * i];
       };
@@ -1609,7 +1609,7 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
       float a0[TPW], a1[TPW], b0, b1;
       auto ld = [&](float (&a)[TPW], float& bb, int s) {
         bb = ub[-4 * s];
-        const float* pr = Pc + clampi(x0 - hx + 4 * s + q4, 0, nx - 1) * WZ + m16;
+        const float* pr = Pc + __umul24(clampi(x0 - hx + 4 * s + q4, 0, nx - 1), WZ) + m16;
 #pragma unroll
         for (int i = 0; i < TPW; ++i) a[i] = pr[16 * i];
       };
@@ -2216,9 +2216,11 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_adj_col(const float* __restr
     const float* suh = sut + TT.tp;
     const float* swt = suh + TT.tp;
     const float* swh = swt + TT.tp;
-    const float* ap[TPW];
-#pragma unroll
-    for (int i = 0; i < TPW; ++i) ap[i] = (i == 0 && m16 == 0) ? swt : (16 * i + 16 == nz && m16 == 15) ? swh : sw;
+    // z taps: the first tile's row 0 takes the tail sums, the last tile's
+    // row 15 the head sums (nz = 16 TPW), every other row the plain taps, so
+    // a step reads three tap words: plain, first tile, last tile
+    const float* ap0 = m16 == 0 ? swt : (TPW == 1 && m16 == 15) ? swh : sw;
+    const float* apl = m16 == 15 ? swh : sw;
     for (int col = wave; col < ntx; col += MF_WAVES) {
       const int x0 = 16 * col;
       const long ob = ((long)y * nx + x0 + m16) * nz + 4 * q4;  // + 16 i: lane's 4 z of tile i
@@ -2271,9 +2273,10 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_adj_col(const float* __restr
       {
         float bq0[TPW], bq1[TPW], a0[TPW], a1[TPW];
         auto ld = [&](float (&bq)[TPW], float (&aa)[TPW], int s) {
+          const float am = sw[4 * s];
 #pragma unroll
           for (int i = 0; i < TPW; ++i) {
-            aa[i] = ap[i][4 * s];
+            aa[i] = i == 0 ? ap0[4 * s] : i == TPW - 1 ? apl[4 * s] : am;
             bq[i] = Hw[(16 * i + q4 + 4 * s) * 16 + m16];
           }
         };
