@@ -212,6 +212,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel choices")
+    ap.add_argument("--choices", default=None,
+                    help="pin the kernel choices (five comma-separated SR_CHOICE_* values) instead of autotuning")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the benchmark, copy this many MiB 3 times with the library's copy kernel "
@@ -253,7 +255,9 @@ def main():
     # kernel choices for 16-view subsets, timed on this device (part of
     # setting the plan up, outside the timed region)
     tune_ms, tune = None, None
-    if not args.no_autotune:
+    if args.choices:
+        plan.set_choice(**dict(zip(plan.CHOICES, (int(c) for c in args.choices.split(",")))))
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         tune = plan.autotune(NVIEW // NBLOCKS)
         tune_ms = 1e3 * (time.perf_counter() - t0)

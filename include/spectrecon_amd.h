@@ -170,10 +170,10 @@ int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
 
 /* Kernel choices of a plan (no reference counterpart; the reference has one
  * CPU code path).  choice[SR_CHOICE_*]:
- *   FWD_FAMILY: 0 the best fitting matrix-core PSF forward (z-fastest plans:
- *     producer/consumer waves, the gathers overlapping the matrix products),
- *     1 the tile kernel, 2 the column-owned kernel, 3 producer/consumer with
- *     8 producer waves;
+ *   FWD_FAMILY: 0 the built-in matrix-core PSF forward (z-fastest plans:
+ *     producer/consumer waves, the gathers overlapping the matrix products;
+ *     8 producer waves for nz <= 80, else 4), 1 the tile kernel, 2 the
+ *     column-owned kernel, 3 / 4 producer/consumer with 8 / 4 producer waves;
  *   ADJ_FAMILY: 0 (or 2) the column-owned adjoint where it fits, 1 the tile
  *     kernel;
  *   FWD_CHUNKS / ADJ_CHUNKS: plane chunks per view, in percent of the built-in
@@ -186,7 +186,8 @@ int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
  * batches of nsub views (the OS-EM subset size) on the plan's stream, keeps
  * the fastest, and writes a JSON record of every candidate's time to `log`
  * (NUL-terminated, truncated to log_bytes; may be NULL); nsub <= 0 restores
- * the defaults.  Changing the choices drops the plan's EM graphs. */
+ * the defaults.  A candidate replaces a built-in choice only when it is at
+ * least 3% faster.  Changing the choices drops the plan's EM graphs. */
 #define SR_CHOICE_FWD_FAMILY 0
 #define SR_CHOICE_FWD_CHUNKS 1
 #define SR_CHOICE_ADJ_FAMILY 2

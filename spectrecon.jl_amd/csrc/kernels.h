@@ -68,13 +68,15 @@ hipError_t launch_atten_exact(float* mur, int nb, const Geo& g, float dy, hipStr
 // attenuated already); returns the chunk count actually used through
 // *nchunk_used.  zfast: `work` is the z-fastest image [y][x][z] and the :two
 // rotation (rv) and attenuation ([v][y][x][z]) are applied while staging
-// family (z-fastest matrix-core path): PSF_AUTO takes the best kernel that
-// fits (forward: producer/consumer waves with 4 producers, else column-owned;
-// adjoint: column-owned), PSF_TILE the tile kernels, PSF_COL the
-// column-owned ones, PSF_WS8 (forward) producer/consumer with 8 producers
-constexpr int PSF_AUTO = 0, PSF_TILE = 1, PSF_COL = 2, PSF_WS8 = 3;
+// family (z-fastest matrix-core path): PSF_AUTO takes the built-in kernel
+// (forward: producer/consumer waves, psf_fwd_auto_npw producers, else
+// column-owned; adjoint: column-owned), PSF_TILE the tile kernels, PSF_COL
+// the column-owned ones, PSF_WS8 / PSF_WS4 (forward) producer/consumer with
+// 8 / 4 producer waves
+constexpr int PSF_AUTO = 0, PSF_TILE = 1, PSF_COL = 2, PSF_WS8 = 3, PSF_WS4 = 4;
 bool psf_fwd_col_ok(const Geo& g, int ntx);
 bool psf_fwd_ws_ok(const Geo& g, int ntx, int npw);
+int psf_fwd_auto_npw(const Geo& g, int ntx);
 bool psf_adj_col_ok(const Geo& g, int ntx);
 hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const int* vids, int nb, float* part,
                           int nchunk, int* nchunk_used, const Geo& g, hipStream_t s, bool zfast = false,

@@ -2949,6 +2949,13 @@ bool psf_fwd_ws_ok(const Geo& g, int ntx, int npw) {
          sizeof(float) * mf_fws_lds_floats(g, ntx) <= kMaxLds;
 }
 
+// producer waves of the built-in forward: 8 while a column's tiles leave the
+// 1024-thread kernel room (nz <= 80), else 4; 0 when neither fits
+int psf_fwd_auto_npw(const Geo& g, int ntx) {
+  if (g.nz / 16 <= 5 && psf_fwd_ws_ok(g, ntx, 8)) return 8;
+  return psf_fwd_ws_ok(g, ntx, 4) ? 4 : 0;
+}
+
 bool psf_adj_col_ok(const Geo& g, int ntx) {
   return psf_mfma_supported(g, ntx) && g.nz / 16 <= MF_TPW && sizeof(float) * mf_col_lds_floats(g, ntx, 1) <= kMaxLds;
 }
@@ -2960,8 +2967,9 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
   nchunk = (g.ny + Yc - 1) / Yc;
   if (nchunk_used) *nchunk_used = nchunk;
   if (nb <= 0) return hipSuccess;
-  const int npw = family == PSF_WS8 ? 8 : 4;
-  if (separable && zfast && (family == PSF_AUTO || family == PSF_WS8) && psf_fwd_ws_ok(g, t.ntx, npw)) {
+  const int npw = family == PSF_WS8 ? 8 : family == PSF_WS4 ? 4 : psf_fwd_auto_npw(g, t.ntx);
+  if (separable && zfast && (family == PSF_AUTO || family == PSF_WS8 || family == PSF_WS4) && npw > 0 &&
+      psf_fwd_ws_ok(g, t.ntx, npw)) {
     const size_t lds = sizeof(float) * mf_fws_lds_floats(g, t.ntx);
     SR_TPW_SWITCH(g.nz / 16, {
       if (npw == 8) {
@@ -3212,7 +3220,12 @@ static hipError_t launch_ell(int mode, const float* work, float* part, const int
   const int pw = 64 / (g.nz / 4 / q);
   const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
   // view slices (waves) per block: each wave sums every vs-th view
-  const int vs = (vs_req == 2 || vs_req == 4 || vs_req == 8) ? vs_req : nb <= 4 ? 2 : nb <= 8 ? 4 : 8;
+  // (built-in: 2 for up to 4 views, 8 for 5-8 and above 24, 4 for 9-24, as
+  // measured on 8-, 15-, 16- and 30-view batches)
+  const int vs = (vs_req == 2 || vs_req == 4 || vs_req == 8) ? vs_req
+                 : nb <= 4                                    ? 2
+                 : nb <= 8 || nb > 24                         ? 8
+                                                              : 4;
   const int zi = zero_inf ? 1 : 0;
 #define SR_ELL(KK, VV, QQ)                                                                                            \
   if (K == KK && vs == VV && q == QQ) {                                                                             \

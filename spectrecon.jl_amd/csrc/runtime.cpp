@@ -789,8 +789,8 @@ int sr_plan_graph_stats(sr_plan p, int64_t* captures, int64_t* replays) {
 
 int sr_plan_set_choice(sr_plan p, const int32_t* c) {
   if (!p || !c) return fail(SR_ERR_ARG, "null argument");
-  if (c[SR_CHOICE_FWD_FAMILY] < 0 || c[SR_CHOICE_FWD_FAMILY] > 3)
-    return fail(SR_ERR_ARG, "PSF forward family must be 0..3");
+  if (c[SR_CHOICE_FWD_FAMILY] < 0 || c[SR_CHOICE_FWD_FAMILY] > 4)
+    return fail(SR_ERR_ARG, "PSF forward family must be 0..4");
   if (c[SR_CHOICE_ADJ_FAMILY] < 0 || c[SR_CHOICE_ADJ_FAMILY] > 2)
     return fail(SR_ERR_ARG, "PSF adjoint family must be 0..2");
   for (int k : {SR_CHOICE_FWD_CHUNKS, SR_CHOICE_ADJ_CHUNKS})
@@ -812,6 +812,10 @@ int sr_plan_get_choice(sr_plan p, int32_t* c) {
 // Times each choice's candidates on the kernel it steers, one choice at a
 // time (the others at their current values), over batches of nsub views
 // spread over the orbit; inputs are the plan's own buffers filled with ones.
+// A candidate replaces the built-in choice only when it is faster by more
+// than the timing noise (kTuneMargin), so that equal kernels do not flip
+// between runs.
+constexpr double kTuneMargin = 0.97;
 int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
   if (!p) return fail(SR_ERR_ARG, "null plan");
   HIP_TRY(hipStreamSynchronize(p->stream));
@@ -842,7 +846,7 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
       return fail(SR_ERR_HIP, "hipEventCreate failed");
     }
     const sr::Taps t = p->taps();
-    // microseconds per launch: best of 3 rounds of 4 launches, after one
+    // microseconds per launch: best of 5 rounds of 4 launches, after one
     // untimed launch; negative when the candidate fails to launch
     auto time_us = [&](auto&& launch) -> double {
       if (launch() != hipSuccess) {
@@ -850,7 +854,7 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
         return -1;
       }
       double best = 1e30;
-      for (int r = 0; r < 3; ++r) {
+      for (int r = 0; r < 5; ++r) {
         (void)hipEventRecord(e0, p->stream);
         for (int k = 0; k < 4; ++k) (void)launch();
         (void)hipEventRecord(e1, p->stream);
@@ -864,6 +868,15 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
       return best;
     };
     const bool psf_choices = p->separable && p->mfma;
+    // the device ramps up over the first milliseconds of work: about 10 ms of
+    // the built-in PSF passes, untimed, so that the first candidate timed
+    // (the built-in one) is not the one that pays for it
+    for (int k = 0; k < 80; ++k) {
+      int used = 0;
+      HIP_TRY(sr::launch_psf_fwd(p->separable, p->zfast ? img : p->work.as<float>(), t, dv, nb, p->fpart.as<float>(),
+                                 p->nchunk_fwd(nb), &used, p->g, p->stream, p->zfast, p->atten.as<float>(),
+                                 p->atten_vstride(), p->rotv.as<sr::RotView>(), p->fam_fwd()));
+    }
     appendf(out, "{\"nsub\": %d, \"views_per_launch\": %d, \"layout\": \"%s\"", (int)nsub, nb,
          p->zfast ? "z-fastest" : "x-fastest");
     // PSF forward (with the view reduction its chunk count sets) and adjoint:
@@ -873,22 +886,28 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
       const int ck = dir == 0 ? SR_CHOICE_FWD_CHUNKS : SR_CHOICE_ADJ_CHUNKS;
       // families that fit this plan (forward: producer/consumer waves with 4
       // or 8 producers, column-owned, tile; adjoint: column-owned, tile)
+      // (the built-in one first: PSF_AUTO)
       std::vector<int> fams;
+      const int auto_npw = p->zfast ? sr::psf_fwd_auto_npw(p->g, t.ntx) : 0;
       if (dir == 0) {
-        if (p->zfast && sr::psf_fwd_ws_ok(p->g, t.ntx, 4)) fams.push_back(sr::PSF_AUTO);
-        if (p->zfast && sr::psf_fwd_ws_ok(p->g, t.ntx, 8)) fams.push_back(sr::PSF_WS8);
-        if (p->zfast && sr::psf_fwd_col_ok(p->g, t.ntx)) fams.push_back(sr::PSF_COL);
-      } else if (p->zfast && sr::psf_adj_col_ok(p->g, t.ntx)) {
         fams.push_back(sr::PSF_AUTO);
+        if (auto_npw != 8 && p->zfast && sr::psf_fwd_ws_ok(p->g, t.ntx, 8)) fams.push_back(sr::PSF_WS8);
+        if (auto_npw != 4 && p->zfast && sr::psf_fwd_ws_ok(p->g, t.ntx, 4)) fams.push_back(sr::PSF_WS4);
+        if (auto_npw != 0 && p->zfast && sr::psf_fwd_col_ok(p->g, t.ntx)) fams.push_back(sr::PSF_COL);
+        if (p->zfast) fams.push_back(sr::PSF_TILE);
+      } else {
+        fams.push_back(sr::PSF_AUTO);
+        if (p->zfast && sr::psf_adj_col_ok(p->g, t.ntx)) fams.push_back(sr::PSF_TILE);
       }
-      fams.push_back(sr::PSF_TILE);
       auto fam_name = [&](int f) {
-        return f == sr::PSF_TILE ? "tile" : f == sr::PSF_COL ? "column" : f == sr::PSF_WS8 ? "overlap8"
-               : dir == 0 ? "overlap" : "column";
+        if (f == sr::PSF_AUTO)
+          return dir == 1 ? (p->zfast && sr::psf_adj_col_ok(p->g, t.ntx) ? "column" : "tile")
+                 : auto_npw == 8 ? "overlap8" : auto_npw == 4 ? "overlap" : p->zfast && sr::psf_fwd_col_ok(p->g, t.ntx) ? "column" : "tile";
+        return f == sr::PSF_TILE ? "tile" : f == sr::PSF_COL ? "column" : f == sr::PSF_WS8 ? "overlap8" : "overlap";
       };
       appendf(out, ", \"%s\": {\"candidates\": [", dir == 0 ? "psf_fwd" : "psf_adj");
-      double best = -1;
-      int bf = p->choice[fk], bc = p->choice[ck], n = 0;
+      double best = -1, def_us = -1;
+      int bf = defaults[fk], bc = defaults[ck], n = 0;
       for (int fam : fams) {
         int last_chunks = -1;
         for (int pct : {50, 100, 200, 400}) {
@@ -916,12 +935,19 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
             });
           appendf(out, "%s{\"family\": \"%s\", \"code\": %d, \"chunks\": %d, \"us\": %.2f}", n++ ? ", " : "",
                   fam_name(fam), fam, chunks, us);
+          if (fam == defaults[fk] && pct == defaults[ck]) def_us = us;
           if (us > 0 && (best < 0 || us < best)) {
             best = us;
             bf = fam;
             bc = pct;
           }
         }
+      }
+      // the built-in choice stays unless a candidate is clearly faster
+      if (def_us > 0 && best > kTuneMargin * def_us) {
+        bf = defaults[fk];
+        bc = defaults[ck];
+        best = def_us;
       }
       p->choice[fk] = bf;
       p->choice[ck] = bc;
@@ -931,18 +957,26 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
     // rotation-adjoint gather (z-fastest): view slices per block
     if (p->zfast) {
       appendf(out, ", \"gather\": {\"candidates\": [");
-      double best = -1;
-      int bv = p->choice[SR_CHOICE_GATHER_SLICES], n = 0;
-      for (int vs : {2, 4, 8}) {
+      double best = -1, def_us = -1;
+      int bv = 0, n = 0;
+      for (int vs : {0, 2, 4, 8}) {  // (0: the built-in count for this batch)
         const double us = time_us([&]() {
           return sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk, dv, nb,
                                        p->g, p->stream, nullptr, 0, nullptr, false, vs);
         });
         appendf(out, "%s{\"slices\": %d, \"us\": %.2f}", n++ ? ", " : "", vs, us);
+        if (vs == 0) {
+          def_us = us;
+          continue;
+        }
         if (us > 0 && (best < 0 || us < best)) {
           best = us;
           bv = vs;
         }
+      }
+      if (def_us > 0 && best > kTuneMargin * def_us) {
+        bv = 0;
+        best = def_us;
       }
       p->choice[SR_CHOICE_GATHER_SLICES] = bv;
       appendf(out, "], \"choice\": {\"slices\": %d, \"us\": %.2f}}", bv, best);

@@ -83,9 +83,10 @@ def traffic(fetch_csv, write_csv, copy_bytes):
 
     Each counter is scaled by (copy_bytes / what it reads for the copy
     kernel); for every class the median over its most frequent launch shape
-    (grid size) is reported, which in the OS-EM benchmark is the one-subset
-    launch (a few other launches can share the grid size: the median keeps
-    them out)."""
+    (kernel and grid size) is reported, which in the OS-EM benchmark is the
+    one-subset launch of the kernel the timed reconstructions ran (autotuner
+    candidates of the same class are other kernels or other shapes; a few
+    other launches can share the shape: the median keeps them out)."""
     f = _per_dispatch(fetch_csv, "FETCH_SIZE")
     w = _per_dispatch(write_csv, "WRITE_SIZE")
 
@@ -103,18 +104,18 @@ def traffic(fetch_csv, write_csv, copy_bytes):
         for name, grid, val in d.values():
             c = kernel_class(name)
             if c and c != "copy":
-                out[c][grid].append(val)
+                out[c][(short_name(name), grid)].append(val)
         return out
 
     fc, wc = by_class(f), by_class(w)
     res = {}
     for c in fc:
-        grid = max(fc[c], key=lambda gsz: len(fc[c][gsz]))
-        fv, wv = fc[c][grid], wc.get(c, {}).get(grid, [0.0])
+        key = max(fc[c], key=lambda kg: len(fc[c][kg]))
+        fv, wv = fc[c][key], wc.get(c, {}).get(key, [0.0])
         fb = 1024.0 * kf * _median(fv)
         wb = 1024.0 * kw * _median(wv)
-        res[c] = {"grid_size": grid, "dispatches": len(fv), "fetch_bytes": round(fb), "write_bytes": round(wb),
-                  "traffic_bytes": round(fb + wb)}
+        res[c] = {"kernel": key[0], "grid_size": key[1], "dispatches": len(fv), "fetch_bytes": round(fb),
+                  "write_bytes": round(wb), "traffic_bytes": round(fb + wb)}
     return {"calibration": {"copy_bytes": copy_bytes, "fetch_scale": kf, "write_scale": kw}, "kernels": res}
 
 

@@ -527,6 +527,7 @@ CHOICE_CASES = {
     "tile_kernels": dict(psf_fwd_family=1, psf_adj_family=1),
     "column_fwd": dict(psf_fwd_family=2, psf_adj_family=2, gather_slices=8),
     "overlap8_fwd": dict(psf_fwd_family=3, psf_fwd_chunk_percent=200),
+    "overlap4_fwd": dict(psf_fwd_family=4, psf_fwd_chunk_percent=50, gather_slices=2),
     "half_chunks": dict(psf_fwd_chunk_percent=50, psf_adj_chunk_percent=50, gather_slices=2),
     "quad_chunks": dict(psf_fwd_chunk_percent=400, psf_adj_chunk_percent=400, gather_slices=4),
     "tile_double": dict(psf_fwd_family=1, psf_adj_family=1, psf_fwd_chunk_percent=200,
@@ -577,11 +578,13 @@ def test_autotune_records_and_keeps_choices(gpu, O):
         cands = rec[key]["candidates"]
         assert len(cands) >= 2 and all(c["us"] > 0 for c in cands)
         best = min(cands, key=lambda c: c["us"])
-        assert rec[key]["choice"]["us"] == best["us"]
+        # the fastest, unless the built-in choice is within the tuning margin
+        builtin = rec[key]["choice"]["code"] == 0 and rec[key]["choice"]["chunk_percent"] == 100
+        assert rec[key]["choice"]["us"] == best["us"] or (builtin and best["us"] > 0.97 * rec[key]["choice"]["us"])
         assert ch[fam] == rec[key]["choice"]["code"]
         assert rec[key]["choice"]["family"] in ("overlap", "overlap8", "column", "tile")
         assert ch[pct] == rec[key]["choice"]["chunk_percent"]
-    assert [c["slices"] for c in rec["gather"]["candidates"]] == [2, 4, 8]
+    assert [c["slices"] for c in rec["gather"]["candidates"]] == [0, 2, 4, 8]
     assert ch["gather_slices"] == rec["gather"]["choice"]["slices"]
     ytrue = gpu.project(xtrue, plan)
     b = np.full_like(ytrue, 0.1 * ytrue.mean())
@@ -592,7 +595,7 @@ def test_autotune_records_and_keeps_choices(gpu, O):
     xr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), 4, 2)
     assert np.linalg.norm(xg - xr) <= 1e-4 * np.linalg.norm(xr)
     assert {c["family"] for c in rec["psf_fwd"]["candidates"]} == {"overlap", "overlap8", "column", "tile"}
-    for bad in (dict(psf_fwd_family=4), dict(psf_adj_family=3), dict(psf_adj_chunk_percent=5),
+    for bad in (dict(psf_fwd_family=5), dict(psf_adj_family=3), dict(psf_adj_chunk_percent=5),
                 dict(gather_slices=3)):
         with pytest.raises(gpu.SPECTError):
             plan.set_choice(**bad)

@@ -56,3 +56,18 @@ def test_traffic_calibration(tmp_path):
     with pytest.raises(ValueError):
         _write(f, "FETCH_SIZE", [(adj, 100, 1.0)])
         profiling.traffic(str(f), str(w), mib << 20)
+
+
+def test_traffic_keeps_the_kernel_that_ran(tmp_path):
+    """Autotuner candidates of a class (other kernels at the same launch
+    shape, fewer dispatches) stay out of the reported median."""
+    mib = 64
+    f, w = tmp_path / "f.csv", tmp_path / "w.csv"
+    copy = "sr::k_copy16(uint4*, uint4 const*, long)"
+    ran = "void sr::k_psf_fwd_ws<5, 8>(float const*)"
+    cand = "void sr::k_psf_fwd_col<5>(float const*)"
+    _write(f, "FETCH_SIZE", [(copy, 10, mib * 1024)] + [(ran, 100, 1000.0)] * 4 + [(cand, 100, 5000.0)] * 3)
+    _write(w, "WRITE_SIZE", [(copy, 10, mib * 1024)] + [(ran, 100, 10.0)] * 4 + [(cand, 100, 50.0)] * 3)
+    k = profiling.traffic(str(f), str(w), mib << 20)["kernels"]["psf_fwd"]
+    assert k["kernel"] == "sr::k_psf_fwd_ws<5, 8>" and k["dispatches"] == 4
+    assert k["fetch_bytes"] == 1024 * 1000 and k["write_bytes"] == 1024 * 10
