@@ -11,7 +11,8 @@ module SPECTreconAMD
 using LinearAlgebra: mul!
 using LinearMapsAA: LinearMapAA, LinearMapAO
 
-export SPECTplanAMD, project!, backproject!, project, backproject, mlem!, osem!, Ablock, osem_strided!, gradient
+export SPECTplanAMD, project!, backproject!, project, backproject, mlem!, osem!, Ablock, osem_strided!, gradient,
+       autotune!, set_graph!
 
 const libspect = joinpath(@__DIR__, "..", "lib", "libspectrecon_amd.so")
 
@@ -144,6 +145,22 @@ function gradient(image::Array{Float32,3}, y::Array{Float32,3}, plan::SPECTplanA
                  Int32, Int32),
                 plan.handle, loss, gx, gmu, image, y, ix, length(ix), SR_MEM_HOST))
     return loss[], gx, gmu
+end
+
+# Kernel choices timed on this device for batches of nsub views (the OS-EM
+# subset size); returns the JSON record of every candidate (no reference
+# counterpart).  nsub <= 0 restores the built-in choices.
+function autotune!(plan::SPECTplanAMD, nsub::Integer)
+    buf = zeros(UInt8, 16384)
+    check(ccall((:sr_plan_autotune, libspect), Cint, (Ptr{Cvoid}, Int32, Ptr{UInt8}, Csize_t),
+                plan.handle, nsub, buf, length(buf)))
+    return unsafe_string(pointer(buf))
+end
+
+# Replay mlem!/osem! kernel sequences from HIP graphs (no reference counterpart).
+function set_graph!(plan::SPECTplanAMD, enable::Bool = true)
+    check(ccall((:sr_plan_set_graph, libspect), Cint, (Ptr{Cvoid}, Int32), plan.handle, enable ? 1 : 0))
+    return plan
 end
 
 end # module
