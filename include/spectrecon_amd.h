@@ -173,7 +173,9 @@ int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
  *   FWD_FAMILY: 0 the built-in matrix-core PSF forward (z-fastest plans:
  *     producer/consumer waves, the gathers overlapping the matrix products;
  *     8 producer waves for nz <= 80, else 4), 1 the tile kernel, 2 the
- *     column-owned kernel, 3 / 4 producer/consumer with 8 / 4 producer waves;
+ *     column-owned kernel, 3 / 4 producer/consumer with 8 / 4 producer waves,
+ *     5 producer/consumer with consumer waves of two columns whose x and z
+ *     passes share one K loop;
  *   ADJ_FAMILY: 0 (or 2) the column-owned adjoint where it fits, 1 the tile
  *     kernel;
  *   FWD_CHUNKS / ADJ_CHUNKS: plane chunks per view, in percent of the built-in

@@ -72,9 +72,11 @@ hipError_t launch_atten_exact(float* mur, int nb, const Geo& g, float dy, hipStr
 // (forward: producer/consumer waves, psf_fwd_auto_npw producers, else
 // column-owned; adjoint: column-owned), PSF_TILE the tile kernels, PSF_COL
 // the column-owned ones, PSF_WS8 / PSF_WS4 (forward) producer/consumer with
-// 8 / 4 producer waves
-constexpr int PSF_AUTO = 0, PSF_TILE = 1, PSF_COL = 2, PSF_WS8 = 3, PSF_WS4 = 4;
+// 8 / 4 producer waves, PSF_WS2 (forward) 4 producer waves and 4 consumer
+// waves of two columns each
+constexpr int PSF_AUTO = 0, PSF_TILE = 1, PSF_COL = 2, PSF_WS8 = 3, PSF_WS4 = 4, PSF_WS2 = 5;
 bool psf_fwd_col_ok(const Geo& g, int ntx);
+bool psf_fwd_ws2_ok(const Geo& g, int ntx);
 bool psf_fwd_ws_ok(const Geo& g, int ntx, int npw);
 int psf_fwd_auto_npw(const Geo& g, int ntx);
 bool psf_adj_col_ok(const Geo& g, int ntx);

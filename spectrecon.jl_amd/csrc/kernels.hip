@@ -1687,6 +1687,237 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
   }
 }
 
+// --- forward, z-fastest, producer waves and column-pair consumers ----------
+// As k_psf_fwd_ws with 4 producer waves, but 4 consumer waves, each owning
+// two x columns A and B.  A wave's x pass and z pass of one column depend on
+// each other through its LDS buffer; here they are taken from different
+// columns and planes and merged into one K loop, so every step has 2 TPW
+// independent products:
+//   loop 1: x pass of A (plane y)   with the z pass of B (plane y - 1)
+//   loop 2: x pass of B (plane y)   with the z pass of A (plane y)
+// and after the last plane the z pass of B alone.  Each column keeps one
+// wave buffer (a loop's z pass reads it before the next x pass rewrites it).
+// Taps: a ring of three planes (the z pass of B still reads plane y - 1's
+// while the producers load plane y + 1's).  grid (nchunk, nb), block 512.
+constexpr int WS2_TAPS = 3;
+__host__ __device__ inline size_t mf_fws2_lds_floats(const Geo& g, int nt) {
+  return 2 * (size_t)g.nx * mf_wq(g.nz) + (size_t)MF_WAVES * mf_rq(g.nz, nt) * 16 +
+         WS2_TAPS * (size_t)MfTaps(nt).size + 16 * (size_t)g.nx;
+}
+template <int TPW>
+__global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_ws2(const float* __restrict__ img_t, Taps t,
+                                                            const int* __restrict__ vids, int Yc,
+                                                            float* __restrict__ part, Geo g,
+                                                            const float* __restrict__ atten, long atten_vstride,
+                                                            const RotView* __restrict__ rv) {
+  constexpr int NCW = 4;               // consumer waves (two columns each)
+  constexpr int PT = MF_THREADS - 64 * NCW;  // producer threads
+  constexpr int RR = (10240 / 4 + PT * WS_RE - 1) / (PT * WS_RE);
+  extern __shared__ float L[];
+  const int NT = t.ntx;
+  const MfTaps TT(NT);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
+  const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
+  const int ntx = nx >> 4;
+  const int WZ = mf_wq(nz), RG = mf_rq(nz, NT), PS = nx * WZ;
+  const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
+  const int v = vids[b];
+  const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool producer = wave >= NCW;
+  const int m16 = lane & 15, q4 = lane >> 4;
+  float* Pz = L;  // two plane buffers
+  float* s_taps = L + 2 * PS + MF_WAVES * RG * 16;  // WS2_TAPS tap planes of TT.size
+  float4* s_rt = reinterpret_cast<float4*>(s_taps + WS2_TAPS * TT.size);
+  for (int k = tid; k < MF_WAVES * RG * 16 + WS2_TAPS * TT.size; k += MF_THREADS) L[2 * PS + k] = 0.f;
+  auto tap_plane = [&](int y) { return s_taps + ((y - y0) % WS2_TAPS) * TT.size; };
+
+  if (producer) {
+    const RotView rot = rv[v];
+    const float* av = atten + atten_vstride * v;
+    const int nz4 = nz >> 2, nv = nx * nz4;
+    const int pt = tid - 64 * NCW;
+    int sxz[RR][WS_RE];
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+#pragma unroll
+      for (int k = 0; k < WS_RE; ++k) {
+        const int i = min(pt + (r * WS_RE + k) * PT, nv - 1), x = i / nz4;
+        sxz[r][k] = x << 8 | (i - x * nz4);
+      }
+    auto rot_table = [&](int y) {
+      if (pt < nx) rot_entry(rot, nx, nz, g.pad, pt, y, s_rt + 2 * nx * ((y - y0) & 1) + 2 * pt);
+    };
+    auto load_taps = [&](int y) {
+      const long plane = (long)v * t.vstride + y;
+      float* dst = tap_plane(y) + TT.off;
+      for (int e = pt; e < 6 * NT; e += PT) {
+        const int tab = e / NT, tk = e - tab * NT;
+        const float* src =
+            tab == 0 ? t.u + plane * NT : tab == 1 ? t.w + plane * NT : t.cum + (plane * 4 + tab - 2) * NT;
+        dst[tab * TT.tp + tk] = src[tk];
+      }
+    };
+    auto stage = [&](int y, float* buf) {
+      const float4* T = s_rt + 2 * nx * ((y - y0) & 1);
+      const float* ay = av + (long)y * nx * nz;
+#pragma unroll
+      for (int r = 0; r < RR; ++r) {
+        if (r * WS_RE * PT >= nv) break;
+#pragma unroll
+        for (int k = 0; k < WS_RE; ++k) asm volatile("" : "+v"(sxz[r][k]));
+        RotSlot h[WS_RE];
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int k = 0; k < WS_RE; ++k) {
+          const int x = sxz[r][k] >> 8, zq = sxz[r][k] & 255;
+          const float4 o = T[2 * x];
+          h[k].s11 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.x) + 4 * zq));
+          h[k].s12 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.y) + 4 * zq));
+          h[k].s21 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.z) + 4 * zq));
+          h[k].s22 = *reinterpret_cast<const float4*>(img_t + (__float_as_int(o.w) + 4 * zq));
+          h[k].a = *reinterpret_cast<const float4*>(ay + 4 * __umul24(x, nz4) + 4 * zq);
+        }
+        SR_SCHED_FENCE();
+#pragma unroll
+        for (int k = 0; k < WS_RE; ++k) {
+          const int x = sxz[r][k] >> 8, zq = sxz[r][k] & 255;
+          const float4 w = T[2 * x + 1];
+          const RotSlot& q = h[k];
+          float4 val;
+          val.x = ((q.s11.x * w.x + q.s12.x * w.y) * w.z + (q.s21.x * w.x + q.s22.x * w.y) * w.w) * q.a.x;
+          val.y = ((q.s11.y * w.x + q.s12.y * w.y) * w.z + (q.s21.y * w.x + q.s22.y * w.y) * w.w) * q.a.y;
+          val.z = ((q.s11.z * w.x + q.s12.z * w.y) * w.z + (q.s21.z * w.x + q.s22.z * w.y) * w.w) * q.a.z;
+          val.w = ((q.s11.w * w.x + q.s12.w * w.y) * w.z + (q.s21.w * w.x + q.s22.w * w.y) * w.w) * q.a.w;
+          *reinterpret_cast<float4*>(buf + x * WZ + 4 * zq) = val;
+        }
+        SR_SCHED_FENCE();
+      }
+    };
+    if (y0 < y1) {
+      rot_table(y0);
+      if (y0 + 1 < y1) rot_table(y0 + 1);
+    }
+    __syncthreads();  // zero fill and the first two rotation tables done
+    if (y0 < y1) {
+      stage(y0, Pz);
+      load_taps(y0);
+    }
+    for (int y = y0; y < y1; ++y) {
+      lds_barrier();  // plane y and its taps staged; plane y - 1's plane buffer free
+      if (y + 1 < y1) {
+        stage(y + 1, Pz + (((y - y0) & 1) ^ 1) * PS);
+        load_taps(y + 1);  // (into plane y - 2's tap plane)
+      }
+      if (y + 2 < y1) rot_table(y + 2);
+    }
+    return;
+  }
+
+  __syncthreads();  // zero fill and the first two rotation tables done
+  // columns A = 2 wave, B = 2 wave + 1 (past the last column: recomputed, not stored)
+  const bool ownA = 2 * wave < ntx, ownB = 2 * wave + 1 < ntx;
+  const int x0A = 16 * min(2 * wave, ntx - 1), x0B = 16 * min(2 * wave + 1, ntx - 1);
+  float* QA = L + 2 * PS + (2 * wave) * RG * 16;
+  float* QB = QA + RG * 16;
+  mf4 dvA[TPW], dvB[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) dvA[i] = dvB[i] = mf4{0.f, 0.f, 0.f, 0.f};
+  const int ns = max(nsx, nsz);
+
+  // one merged loop: the x pass of the column at x0 from plane buffer Pc
+  // with plane tb's taps into Qx (skipped when !xon), and the z pass of the
+  // column whose x-pass result is in Qz with plane tz's taps into dv (skipped
+  // when !zon)
+  auto merged = [&](bool xon, const float* Pc, const float* tb, int x0, float* Qx, bool zon, const float* tz,
+                    const float* Qz, mf4 (&dv)[TPW]) {
+    mf4 dx[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) dx[i] = mf4{0.f, 0.f, 0.f, 0.f};
+    const float* ub = tb + TT.off + m16 - q4 + 2 * hx;
+    const float* wz = tz + TT.tp + TT.off + m16 - q4 + 2 * hz;
+    if (zon) {  // the z pass's replicate boundary first
+      const float* swt = tz + 4 * TT.tp + TT.off;
+      const float* swh = swt + TT.tp;
+      const float be = q4 == 0 ? Qz[hz * 16 + m16] : q4 == 1 ? Qz[(hz + nz - 1) * 16 + m16] : 0.f;
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        const int zm = 16 * i + m16;
+        const float ae = q4 == 0 ? tail_at(swt, NT, zm + hz + 1) : q4 == 1 ? head_at(swh, NT, zm + hz - nz) : 0.f;
+        dv[i] = mfma4(ae, be, dv[i]);
+      }
+    }
+    float ax0[TPW], ax1[TPW], bx0 = 0.f, bx1 = 0.f, bz0[TPW], bz1[TPW], az0 = 0.f, az1 = 0.f;
+    auto ldx = [&](float (&a)[TPW], float& bb, int s) {
+      bb = ub[-4 * s];
+      const float* pr = Pc + __umul24(clampi(x0 - hx + 4 * s + q4, 0, nx - 1), WZ) + m16;
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) a[i] = pr[16 * i];
+    };
+    auto ldz = [&](float (&bq)[TPW], float& aa, int s) {
+      aa = wz[-4 * s];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) bq[i] = Qz[(16 * i + q4 + 4 * s) * 16 + m16];
+    };
+    auto mmx = [&](const float (&a)[TPW], float bb) {
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) dx[i] = mfma4(bb, a[i], dx[i]);
+    };
+    auto mmz = [&](const float (&bq)[TPW], float aa) {
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) dv[i] = mfma4(aa, bq[i], dv[i]);
+    };
+    const int nx_s = xon ? nsx : 0, nz_s = zon ? nsz : 0;
+    if (nx_s > 0) ldx(ax0, bx0, 0);
+    if (nz_s > 0) ldz(bz0, az0, 0);
+    for (int s = 0; s < ns; s += 2) {
+      if (s + 1 < nx_s) ldx(ax1, bx1, s + 1);
+      if (s + 1 < nz_s) ldz(bz1, az1, s + 1);
+      SR_SCHED_FENCE();
+      if (s < nx_s) mmx(ax0, bx0);
+      if (s < nz_s) mmz(bz0, az0);
+      SR_SCHED_FENCE();
+      if (s + 2 < nx_s) ldx(ax0, bx0, s + 2);
+      if (s + 2 < nz_s) ldz(bz0, az0, s + 2);
+      SR_SCHED_FENCE();
+      if (s + 1 < nx_s) mmx(ax1, bx1);
+      if (s + 1 < nz_s) mmz(bz1, az1);
+    }
+    asm volatile("" ::: "memory");  // the Qx stores below stay after the Qz reads above
+    if (xon) {
+#pragma unroll
+      for (int i = 0; i < TPW; ++i)
+        *reinterpret_cast<float4*>(Qx + (hz + 16 * i + m16) * 16 + 4 * q4) =
+            make_float4(dx[i][0], dx[i][1], dx[i][2], dx[i][3]);
+      // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    }
+  };
+
+  for (int y = y0; y < y1; ++y) {
+    const int cur = (y - y0) & 1;
+    lds_barrier();  // plane y and its taps staged; plane y - 1's plane buffer free
+    const float* Pc = Pz + cur * PS;
+    const float* ty = tap_plane(y);
+    // loop 1: x of A (plane y) with z of B (plane y - 1)
+    merged(true, Pc, ty, x0A, QA, y > y0, y > y0 ? tap_plane(y - 1) : ty, QB, dvB);
+    // loop 2: x of B (plane y) with z of A (plane y)
+    merged(true, Pc, ty, x0B, QB, true, ty, QA, dvA);
+  }
+  if (y0 < y1) merged(false, Pz, tap_plane(y1 - 1), x0B, QB, true, tap_plane(y1 - 1), QB, dvB);
+  float* pb = part + ((long)b * nchunk + c) * nx * nz;
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (ownA) pb[(long)(16 * i + 4 * q4 + r) * nx + x0A + m16] = dvA[i][r];
+      if (ownB) pb[(long)(16 * i + 4 * q4 + r) * nx + x0B + m16] = dvB[i][r];
+    }
+  }
+}
+
 // --- forward, direct 2-D ----------------------------------------------------
 // general (non-separable) PSF: strips of 8 x-outputs, pz rows of NT taps each
 template <int NT, int BLOCK>
@@ -2949,6 +3180,11 @@ bool psf_fwd_ws_ok(const Geo& g, int ntx, int npw) {
          sizeof(float) * mf_fws_lds_floats(g, ntx) <= kMaxLds;
 }
 
+bool psf_fwd_ws2_ok(const Geo& g, int ntx) {
+  return psf_fwd_col_ok(g, ntx) && g.nx <= 256 && (long)g.nx * g.nz <= 10240 &&
+         sizeof(float) * mf_fws2_lds_floats(g, ntx) <= kMaxLds;
+}
+
 // producer waves of the built-in forward: 8 while a column's tiles leave the
 // 1024-thread kernel room (nz <= 80), else 4; 0 when neither fits
 int psf_fwd_auto_npw(const Geo& g, int ntx) {
@@ -2967,6 +3203,15 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
   nchunk = (g.ny + Yc - 1) / Yc;
   if (nchunk_used) *nchunk_used = nchunk;
   if (nb <= 0) return hipSuccess;
+  if (separable && zfast && family == PSF_WS2 && psf_fwd_ws2_ok(g, t.ntx)) {
+    const size_t lds = sizeof(float) * mf_fws2_lds_floats(g, t.ntx);
+    SR_TPW_SWITCH(g.nz / 16, {
+      allow_lds(k_psf_fwd_ws2<TPW>, lds);
+      k_psf_fwd_ws2<TPW><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(work, t, vids, Yc, part, g, atten, atten_vstride,
+                                                                   rv);
+    });
+    return hipGetLastError();
+  }
   const int npw = family == PSF_WS8 ? 8 : family == PSF_WS4 ? 4 : psf_fwd_auto_npw(g, t.ntx);
   if (separable && zfast && (family == PSF_AUTO || family == PSF_WS8 || family == PSF_WS4) && npw > 0 &&
       psf_fwd_ws_ok(g, t.ntx, npw)) {

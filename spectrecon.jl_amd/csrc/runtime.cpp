@@ -789,8 +789,8 @@ int sr_plan_graph_stats(sr_plan p, int64_t* captures, int64_t* replays) {
 
 int sr_plan_set_choice(sr_plan p, const int32_t* c) {
   if (!p || !c) return fail(SR_ERR_ARG, "null argument");
-  if (c[SR_CHOICE_FWD_FAMILY] < 0 || c[SR_CHOICE_FWD_FAMILY] > 4)
-    return fail(SR_ERR_ARG, "PSF forward family must be 0..4");
+  if (c[SR_CHOICE_FWD_FAMILY] < 0 || c[SR_CHOICE_FWD_FAMILY] > 5)
+    return fail(SR_ERR_ARG, "PSF forward family must be 0..5");
   if (c[SR_CHOICE_ADJ_FAMILY] < 0 || c[SR_CHOICE_ADJ_FAMILY] > 2)
     return fail(SR_ERR_ARG, "PSF adjoint family must be 0..2");
   for (int k : {SR_CHOICE_FWD_CHUNKS, SR_CHOICE_ADJ_CHUNKS})
@@ -893,6 +893,7 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
         fams.push_back(sr::PSF_AUTO);
         if (auto_npw != 8 && p->zfast && sr::psf_fwd_ws_ok(p->g, t.ntx, 8)) fams.push_back(sr::PSF_WS8);
         if (auto_npw != 4 && p->zfast && sr::psf_fwd_ws_ok(p->g, t.ntx, 4)) fams.push_back(sr::PSF_WS4);
+        if (p->zfast && sr::psf_fwd_ws2_ok(p->g, t.ntx)) fams.push_back(sr::PSF_WS2);
         if (auto_npw != 0 && p->zfast && sr::psf_fwd_col_ok(p->g, t.ntx)) fams.push_back(sr::PSF_COL);
         if (p->zfast) fams.push_back(sr::PSF_TILE);
       } else {
@@ -903,7 +904,8 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
         if (f == sr::PSF_AUTO)
           return dir == 1 ? (p->zfast && sr::psf_adj_col_ok(p->g, t.ntx) ? "column" : "tile")
                  : auto_npw == 8 ? "overlap8" : auto_npw == 4 ? "overlap" : p->zfast && sr::psf_fwd_col_ok(p->g, t.ntx) ? "column" : "tile";
-        return f == sr::PSF_TILE ? "tile" : f == sr::PSF_COL ? "column" : f == sr::PSF_WS8 ? "overlap8" : "overlap";
+        return f == sr::PSF_TILE ? "tile" : f == sr::PSF_COL ? "column" : f == sr::PSF_WS8 ? "overlap8"
+               : f == sr::PSF_WS2 ? "pairs" : "overlap";
       };
       appendf(out, ", \"%s\": {\"candidates\": [", dir == 0 ? "psf_fwd" : "psf_adj");
       double best = -1, def_us = -1;

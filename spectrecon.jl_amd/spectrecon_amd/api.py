@@ -196,7 +196,8 @@ class SPECTplan:
     def choice(self):
         """The plan's kernel choices: PSF families (forward: 0 built-in
         producer/consumer waves, 1 tile, 2 column-owned, 3 / 4 producer/consumer
-        with 8 / 4 producers; adjoint: 0 column-owned, 1 tile), chunk counts in
+        with 8 / 4 producers, 5 producer/consumer with column-pair consumers;
+        adjoint: 0 column-owned, 1 tile), chunk counts in
         percent of the built-in ones, gather view slices (0 built-in)."""
         c = (C.c_int32 * 5)()
         check(lib().sr_plan_get_choice(self.handle, c))

@@ -528,6 +528,8 @@ CHOICE_CASES = {
     "column_fwd": dict(psf_fwd_family=2, psf_adj_family=2, gather_slices=8),
     "overlap8_fwd": dict(psf_fwd_family=3, psf_fwd_chunk_percent=200),
     "overlap4_fwd": dict(psf_fwd_family=4, psf_fwd_chunk_percent=50, gather_slices=2),
+    "pairs_fwd": dict(psf_fwd_family=5),
+    "pairs_fwd_chunks": dict(psf_fwd_family=5, psf_fwd_chunk_percent=400),
     "half_chunks": dict(psf_fwd_chunk_percent=50, psf_adj_chunk_percent=50, gather_slices=2),
     "quad_chunks": dict(psf_fwd_chunk_percent=400, psf_adj_chunk_percent=400, gather_slices=4),
     "tile_double": dict(psf_fwd_family=1, psf_adj_family=1, psf_fwd_chunk_percent=200,
@@ -594,8 +596,8 @@ def test_autotune_records_and_keeps_choices(gpu, O):
     xg = gpu.osem_strided(x0, y, b, plan, 4, 2)
     xr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), 4, 2)
     assert np.linalg.norm(xg - xr) <= 1e-4 * np.linalg.norm(xr)
-    assert {c["family"] for c in rec["psf_fwd"]["candidates"]} == {"overlap", "overlap8", "column", "tile"}
-    for bad in (dict(psf_fwd_family=5), dict(psf_adj_family=3), dict(psf_adj_chunk_percent=5),
+    assert {c["family"] for c in rec["psf_fwd"]["candidates"]} == {"overlap", "overlap8", "pairs", "column", "tile"}
+    for bad in (dict(psf_fwd_family=6), dict(psf_adj_family=3), dict(psf_adj_chunk_percent=5),
                 dict(gather_slices=3)):
         with pytest.raises(gpu.SPECTError):
             plan.set_choice(**bad)
