@@ -211,6 +211,15 @@ int sr_plan_autotune(sr_plan plan, int32_t nsub, char* log, size_t log_bytes);
  * the call returns after the stream finished. */
 int sr_grad(sr_plan plan, double* loss, float* grad_x, float* grad_mu, const float* image, const float* y,
             const int32_t* index, int32_t nindex, int32_t mem);
+/* The mu vector-Jacobian product of the projector (the suggested
+ * sr_project_vjp_mu of the boundary): grad_mu = d<dy, A(x, mu)>/dmu for view
+ * cotangents dy ([nx,nz,nindex], the indexed views in order), through the
+ * attenuation of src/project.jl:103-116.  It is sr_grad's dL/dmu with the
+ * residual 2 (A x - y) replaced by dy, without the projection and the loss:
+ * what a ChainRulesCore rrule for a mu-dependent project calls (the image
+ * cotangent is backproject(dy)).  Needs a z-fastest plan. */
+int sr_project_vjp_mu(sr_plan plan, float* grad_mu, const float* image, const float* dy, const int32_t* index,
+                      int32_t nindex, int32_t mem);
 
 /* ---- unit operators ----------------------------------------------------------
  * sr_imrotate replaces imrotate!/imrotate_adj! on a stack of nz square n x n

@@ -1085,6 +1085,10 @@ int sr_backproject(sr_plan p, float* image, const float* views, const int32_t* i
   return finish(p, mem);
 }
 
+namespace {
+int run_mu_vjp(sr_plan p, float* d_gmu, const float* r, const int* d_vids, int nindex);
+}  // namespace
+
 int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float* image, const float* y,
             const int32_t* index, int32_t nindex, int32_t mem) {
   if (!p || !loss || !grad_x || !grad_mu || !image || !y) return fail(SR_ERR_ARG, "null argument");
@@ -1119,7 +1123,56 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
   HIP_TRY(sr::launch_resid_loss(r, d_y, r, p->P * nindex, p->gpart.as<double>(), p->gloss.as<double>(), p->stream));
   // dL/dx = A' r
   if ((rc = run_backproject(p, d_gx, r, d_vids, nindex, sr::IMG_WRITE, nullptr))) return rc;
-  // dL/dmu = sum_v R_v' (-dy (reverse cumulative sum along y of conv'(r_v) .* R_v x .* atten_v, halved at j))
+  if ((rc = run_mu_vjp(p, d_gmu, r, d_vids, nindex))) return rc;
+  HIP_TRY(hipMemcpyAsync(loss, p->gloss.p, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  if (host) {
+    HIP_TRY(hipMemcpyAsync(grad_x, d_gx, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipMemcpyAsync(grad_mu, d_gmu, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
+  }
+  // the loss lands in host memory: always wait
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  resolve_timing(p);
+  return SR_OK;
+}
+
+// src/project.jl:103-116 differentiated in mu: grad_mu = sum over the indexed
+// views of the mu-cotangent of <dy, A(x, mu)>
+int sr_project_vjp_mu(sr_plan p, float* grad_mu, const float* image, const float* dy, const int32_t* index,
+                      int32_t nindex, int32_t mem) {
+  if (!p || !grad_mu || !image || !dy) return fail(SR_ERR_ARG, "null argument");
+  if (!index && nindex != p->d.nview) return fail(SR_ERR_DIM, "nindex must equal nview when index is NULL");
+  if (!p->zfast)
+    return fail(SR_ERR_UNSUPPORTED,
+                "the mu-gradient needs a z-fastest plan (:two, rank-1 PSFs on planes of multiples of 16)");
+  int rc;
+  if ((rc = upload_vids(p, index, nindex))) return rc;
+  const bool host = (mem & 0xff) == SR_MEM_HOST;
+  const long PN = p->P * std::max(1, nindex);
+  const float *d_img = image, *d_dy = dy;
+  float* d_gmu = grad_mu;
+  if (host) {
+    if ((rc = stage_in(p, p->st_x0, image, sizeof(float) * p->V)) ||
+        (rc = stage_in(p, p->st_aux1, dy, sizeof(float) * PN)) || (rc = p->st_aux2.ensure(sizeof(float) * p->V)))
+      return rc;
+    d_img = p->st_x0.as<float>();
+    d_dy = p->st_aux1.as<float>();
+    d_gmu = p->st_aux2.as<float>();
+  }
+  if ((rc = p->gwork2.ensure(sizeof(float) * p->V * std::min<long>(std::max(1, nindex), p->max_batch)))) return rc;
+  {  // the image's z-fastest copy, which the rotation of x reads
+    KTimer kt(p, sr::K_ROT_FWD);
+    HIP_TRY(sr::launch_to_zfast(d_img, p->zimg.as<float>(), p->g, p->stream));
+  }
+  if ((rc = run_mu_vjp(p, d_gmu, d_dy, p->vids.as<int>(), nindex))) return rc;
+  if (host) HIP_TRY(hipMemcpyAsync(grad_mu, d_gmu, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
+  return finish(p, mem);
+}
+
+namespace {
+// dL/dmu = sum_v R_v' (-dy (reverse cumulative sum along y of conv'(r_v) .* R_v x .* atten_v, halved at j)) for
+// the view cotangents r (the z-fastest image copy holds x)
+int run_mu_vjp(sr_plan p, float* d_gmu, const float* r, const int* d_vids, int nindex) {
+  int rc;
   const sr::Taps t = p->taps();
   if (nindex == 0) HIP_TRY(sr::launch_fill(d_gmu, 0.f, p->V, p->stream));
   for (int s0 = 0; s0 < nindex; s0 += p->max_batch) {
@@ -1145,16 +1198,9 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
     HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, d_gmu, nullptr, s0 == 0 ? sr::IMG_WRITE : sr::IMG_ACCUM,
                                     p->g, p->stream));
   }
-  HIP_TRY(hipMemcpyAsync(loss, p->gloss.p, sizeof(double), hipMemcpyDeviceToHost, p->stream));
-  if (host) {
-    HIP_TRY(hipMemcpyAsync(grad_x, d_gx, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
-    HIP_TRY(hipMemcpyAsync(grad_mu, d_gmu, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
-  }
-  // the loss lands in host memory: always wait
-  HIP_TRY(hipStreamSynchronize(p->stream));
-  resolve_timing(p);
   return SR_OK;
 }
+}  // namespace
 
 namespace {
 

@@ -12,7 +12,7 @@ using LinearAlgebra: mul!
 using LinearMapsAA: LinearMapAA, LinearMapAO
 
 export SPECTplanAMD, project!, backproject!, project, backproject, mlem!, osem!, Ablock, osem_strided!, gradient,
-       autotune!, set_graph!
+       autotune!, set_graph!, project_vjp_mu
 
 const libspect = joinpath(@__DIR__, "..", "lib", "libspectrecon_amd.so")
 
@@ -145,6 +145,18 @@ function gradient(image::Array{Float32,3}, y::Array{Float32,3}, plan::SPECTplanA
                  Int32, Int32),
                 plan.handle, loss, gx, gmu, image, y, ix, length(ix), SR_MEM_HOST))
     return loss[], gx, gmu
+end
+
+# d<dy, A(x, mu)>/dmu for view cotangents dy (the mu part of the projector's
+# vector-Jacobian product; the image part is backproject(dy)).
+function project_vjp_mu(image::Array{Float32,3}, dy::Array{Float32,3}, plan::SPECTplanAMD;
+                        index::AbstractVector{<:Integer} = 1:plan.nview)
+    ix = Int32.(collect(index) .- 1)
+    gmu = similar(image)
+    check(ccall((:sr_project_vjp_mu, libspect), Cint,
+                (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Int32}, Int32, Int32),
+                plan.handle, gmu, image, dy, ix, length(ix), SR_MEM_HOST))
+    return gmu
 end
 
 # Kernel choices timed on this device for batches of nsub views (the OS-EM

@@ -59,6 +59,7 @@ SIGNATURES = {
     "sr_plan_info": (C.c_int, [_vp, C.POINTER(C.c_size_t), _ip, _ip]),
     "sr_plan_layout": (C.c_int, [_vp, _ip]),
     "sr_grad": (C.c_int, [_vp, C.POINTER(C.c_double), _vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32]),
+    "sr_project_vjp_mu": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32]),
     "sr_osem_strided": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_int32]),
     "sr_plan_query": (C.c_int, [_vp, _ip]),
     "sr_plan_set_graph": (C.c_int, [_vp, C.c_int32]),

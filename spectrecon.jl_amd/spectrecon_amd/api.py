@@ -25,7 +25,7 @@ __all__ = [
     "mlem", "mlem_", "osem", "osem_", "Ablock", "LinearMapAA", "spect_operator",
     "PlanRotate", "plan_rotate", "imrotate", "imrotate_", "imrotate_adj", "imrotate_adj_",
     "PlanPSF", "plan_psf", "fft_conv", "fft_conv_", "fft_conv_adj", "fft_conv_adj_", "fft_conv_adj2_",
-    "psf_gauss", "phantom", "mumap_phantom", "poisson", "gradient", "osem_strided",
+    "psf_gauss", "phantom", "mumap_phantom", "poisson", "gradient", "project_vjp_mu", "osem_strided",
 ]
 
 _METHODS = {"one": L.SR_INTERP_ONE, "two": L.SR_INTERP_TWO}
@@ -369,6 +369,23 @@ def osem_strided(x0, y, b, plan, nblocks, niter=16):
 # ---------------------------------------------------------------------------
 # Gradients of L = ||A(x, mu) - y||^2 (no reference counterpart)
 # ---------------------------------------------------------------------------
+def project_vjp_mu(image, dy, plan, index=None):
+    """d<dy, A(x, mu)>/dmu for view cotangents dy (the indexed views in order,
+    [nx,nz,n]): the mu part of the projector's vector-Jacobian product (the
+    image part is backproject(dy)).  Needs a z-fastest plan."""
+    idx, n = _index(plan, index)
+    nx, ny, nz = plan.imgsize
+    if tuple(np.shape(image)) != plan.imgsize:
+        raise DimensionMismatch("image size")
+    if tuple(np.shape(dy)) != (nx, nz, n):
+        raise DimensionMismatch("views size")
+    gmu = np.zeros(plan.imgsize, np.float32, order="F")
+    x, d = _f32(image), _f32(dy)
+    check(lib().sr_project_vjp_mu(plan.handle, gmu.ctypes.data, x.ctypes.data, d.ctypes.data,
+                                  idx.ctypes.data if idx is not None else None, n, L.SR_MEM_HOST))
+    return gmu
+
+
 def gradient(image, y, plan, index=None):
     """(L, dL/dx, dL/dmu) for L = sum over the indexed views of ||A(x, mu) - y||^2,
     mu the plan's mu-map and y the indexed views in order ([nx,nz,n]).  dL/dx is

@@ -494,6 +494,32 @@ def test_gradient_vs_oracle(gpu, O, nview):
         assert np.linalg.norm(gmu - gmr) <= 1e-4 * np.linalg.norm(gmr), np.linalg.norm(gmu - gmr) / np.linalg.norm(gmr)
 
 
+@pytest.mark.parametrize("nview", [16, 6])
+def test_project_vjp_mu_vs_oracle(gpu, O, nview):
+    """The mu vector-Jacobian product d<dy, A(x, mu)>/dmu against the FP64
+    oracle's mu-gradient at y = A x - dy/2, whose residual 2 (A x - y) is dy
+    (the oracle's gradient is pinned by finite differences); all views and a
+    view subset; the image part of the product is the back-projection."""
+    nx, nz, dy_cm = 32, 16, 0.48
+    rng = np.random.default_rng(73)
+    mumap = np.asfortranarray((0.2 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = np.asfortranarray(np.repeat(np.asarray(O.psf_gauss(nx, 7, 7, 1.0, 3.0), np.float32)[:, :, :, None],
+                                       nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, dy_cm)
+    x = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    P64 = O.Problem(mumap, psfs, dy_cm, angles32(nview).astype(np.float64), "two", np.float64, "direct")
+    for idx in (None, [nview - 1, 0, 3]):
+        n = nview if idx is None else len(idx)
+        cot = np.asfortranarray(rng.standard_normal((nx, nz, n)).astype(np.float32))
+        gmu = gpu.project_vjp_mu(x, cot, plan, index=idx)
+        ia = None if idx is None else np.array(idx, np.int32)
+        y64 = P64.project(x.astype(np.float64), ia) - 0.5 * cot.astype(np.float64)
+        _, _, gmr = P64.grad(x.astype(np.float64), y64, ia)
+        assert np.linalg.norm(gmu - gmr) <= 1e-4 * np.linalg.norm(gmr), np.linalg.norm(gmu - gmr) / np.linalg.norm(gmr)
+    with pytest.raises(gpu.DimensionMismatch):
+        gpu.project_vjp_mu(x, cot[:, :, :1], plan, index=[0, 1])
+
+
 def test_gradient_needs_z_fastest(gpu, monkeypatch):
     monkeypatch.setenv("SR_LAYOUT", "zyx")
     mumap, psfs, image = small_problem(3, 32, 16, 4, 5, 5, True)
