@@ -230,6 +230,13 @@ int sr_imrotate(float* out, const float* img, int32_t n, int32_t nz, float theta
  * replicate-padded convolution of an nx x nz image with a px x pz kernel. */
 int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const float* ker, int32_t px, int32_t pz,
                 int32_t adj, int32_t mem);
+/* The adjoints under their own names: imrotate_adj! (src/rotatez.jl:390-416,
+ * :497-558) and fft_conv_adj! (src/fft_convolve.jl:80-130); the same as
+ * sr_imrotate / sr_fft_conv with adj = 1. */
+int sr_imrotate_adj(float* out, const float* img, int32_t n, int32_t nz, float theta, int32_t interpmeth,
+                    int32_t mem);
+int sr_fft_conv_adj(float* out, const float* img, int32_t nx, int32_t nz, const float* ker, int32_t px, int32_t pz,
+                    int32_t mem);
 
 /* ---- input generators (host only) --------------------------------------------
  * sr_psf_gauss replaces psf_gauss(; ny, px, pz, fwhm_x, fwhm_z)

@@ -1585,4 +1585,14 @@ int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const floa
   return SR_OK;
 }
 
+int sr_imrotate_adj(float* out, const float* img, int32_t n, int32_t nz, float theta, int32_t interpmeth,
+                    int32_t mem) {
+  return sr_imrotate(out, img, n, nz, theta, interpmeth, 1, mem);
+}
+
+int sr_fft_conv_adj(float* out, const float* img, int32_t nx, int32_t nz, const float* ker, int32_t px, int32_t pz,
+                    int32_t mem) {
+  return sr_fft_conv(out, img, nx, nz, ker, px, pz, 1, mem);
+}
+
 }  // extern "C"

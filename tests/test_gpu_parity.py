@@ -520,6 +520,25 @@ def test_project_vjp_mu_vs_oracle(gpu, O, nview):
         gpu.project_vjp_mu(x, cot[:, :, :1], plan, index=[0, 1])
 
 
+def test_named_adjoint_entry_points(gpu):
+    """sr_imrotate_adj and sr_fft_conv_adj are sr_imrotate / sr_fft_conv with adj = 1."""
+    from spectrecon_amd import _lib as L
+    rng = np.random.default_rng(5)
+    img = np.asfortranarray(rng.random((24, 24, 3), dtype=np.float32))
+    a, b = np.zeros_like(img), np.zeros_like(img)
+    for interp in (L.SR_INTERP_ONE, L.SR_INTERP_TWO):
+        L.check(gpu.lib().sr_imrotate(a.ctypes.data, img.ctypes.data, 24, 3, 0.7, interp, 1, L.SR_MEM_HOST))
+        L.check(gpu.lib().sr_imrotate_adj(b.ctypes.data, img.ctypes.data, 24, 3, 0.7, interp, L.SR_MEM_HOST))
+        np.testing.assert_array_equal(a, b)
+    im2 = np.asfortranarray(rng.random((20, 14), dtype=np.float32))
+    ker = np.asfortranarray(np.outer(np.hanning(7), np.hanning(5)).astype(np.float32))
+    c, d = np.zeros_like(im2), np.zeros_like(im2)
+    L.check(gpu.lib().sr_fft_conv(c.ctypes.data, im2.ctypes.data, 20, 14, ker.ctypes.data, 7, 5, 1, L.SR_MEM_HOST))
+    L.check(gpu.lib().sr_fft_conv_adj(d.ctypes.data, im2.ctypes.data, 20, 14, ker.ctypes.data, 7, 5, L.SR_MEM_HOST))
+    np.testing.assert_array_equal(c, d)
+    assert np.abs(c).sum() > 0
+
+
 def test_gradient_needs_z_fastest(gpu, monkeypatch):
     monkeypatch.setenv("SR_LAYOUT", "zyx")
     mumap, psfs, image = small_problem(3, 32, 16, 4, 5, 5, True)
