@@ -650,6 +650,31 @@ def test_autotune_records_and_keeps_choices(gpu, O):
     assert list(plan.choice().values()) == [0, 100, 0, 100, 0]
 
 
+def test_autotune_other_plans(gpu, O, monkeypatch):
+    """x-fastest plans tune only the PSF chunk counts (one family, no gather);
+    plans with a general (non-separable) PSF have nothing to tune; both still
+    project as the oracle does afterwards."""
+    nx, nz, nview = 32, 16, 8
+    xtrue, mumap, psfs = em_problem(nx, nz, nview)
+    monkeypatch.setenv("SR_LAYOUT", "zyx")
+    plan = gpu.SPECTplan(mumap, psfs, 0.8)
+    assert plan.info()["layout"] == "x-fastest"
+    rec = plan.autotune(4)
+    assert rec["layout"] == "x-fastest" and "gather" not in rec
+    assert {c["family"] for c in rec["psf_fwd"]["candidates"]} == {"tile"}
+    P32 = O.Problem(mumap, psfs, 0.8, angles32(nview).astype(np.float64), "two", np.float32, "fft")
+    assert per_view_maxrel(gpu.project(xtrue, plan), P32.project(xtrue)) <= PROJ_TOL
+    monkeypatch.delenv("SR_LAYOUT")
+    rng = np.random.default_rng(9)
+    gpsf = rand_sym_psfs(rng, 5, 5, nx, nview)  # not rank 1: the direct 2-D kernels
+    plan2 = gpu.SPECTplan(mumap, gpsf, 0.8)
+    assert plan2.info()["conv"] == "direct"
+    rec2 = plan2.autotune(4)
+    assert "psf_fwd" not in rec2 and "gather" not in rec2
+    P32b = O.Problem(mumap, gpsf, 0.8, angles32(nview).astype(np.float64), "two", np.float32, "fft")
+    assert per_view_maxrel(gpu.project(xtrue, plan2), P32b.project(xtrue)) <= PROJ_TOL
+
+
 @pytest.mark.parametrize("variant", ["shared", "per_view_psf", "x_fastest", "batch2"])
 def test_graph_replay_equals_eager(gpu, O, monkeypatch, variant):
     """EM calls replayed from a HIP graph equal the eager calls bit for bit,
