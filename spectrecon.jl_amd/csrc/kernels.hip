@@ -2607,6 +2607,103 @@ __global__ void __launch_bounds__(BLOCK) k_psf_adj_2d(const float* __restrict__ 
   }
 }
 
+// The same extended correlation and fold, with the view (zero columns on both
+// sides) staged in LDS once per block, each plane's taps in LDS, and each
+// thread computing an 8-wide x strip of E from a register window, as the
+// forward kernel does.  Out-of-range view columns read zeros and tap columns
+// past px are zero, so every added term is an exact zero: E equals the
+// per-term loop of k_psf_adj_2d for finite data.
+__host__ __device__ inline int adj2d_wv(const Geo& g, int nt) { return odd_up(g.nx + 2 * ((g.px - 1) / 2) + nt + 16); }
+__host__ __device__ inline size_t adj2d_lds_floats(const Geo& g, int nt) {
+  const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
+  return (size_t)odd_up(g.nx + 2 * hx) * (g.nz + 2 * hz) + (size_t)g.nz * adj2d_wv(g, nt) + (size_t)g.pz * nt;
+}
+template <int NT, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_psf_adj_2d_strips(const float* __restrict__ views, Taps t,
+                                                             const int* __restrict__ vids, int Yc,
+                                                             float* __restrict__ work,
+                                                             const float* __restrict__ atten, long atten_vstride,
+                                                             Geo g) {
+  extern __shared__ float L[];
+  const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px, pz = g.pz;
+  const int hx = (px - 1) / 2, hz = (pz - 1) / 2;
+  const int EX = nx + 2 * hx, EZ = nz + 2 * hz;
+  const int WE = odd_up(EX), WV = adj2d_wv(g, NT), OX = 2 * hx + 8;  // view column x at OX + x
+  const int sx = (EX + 7) / 8;                                         // 8-wide strips per E row
+  const int b = blockIdx.y, c = blockIdx.x;
+  const int v = vids[b];
+  const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
+  const long S = (long)nx * ny, V = S * nz;
+  const int P = nx * nz;
+  float* E = L;
+  float* Vs = E + (size_t)WE * EZ;
+  float* ts = Vs + (size_t)nz * WV;
+  const float* vin = views + (long)b * P;
+  float* wout = work + V * b;
+  const float* av = atten ? atten + atten_vstride * v : nullptr;
+  for (int k = threadIdx.x; k < nz * WV; k += BLOCK) {
+    const int z = k / WV, x = k - z * WV - OX;
+    Vs[k] = (x >= 0 && x < nx) ? vin[x + nx * z] : 0.f;
+  }
+  for (int y = y0; y < y1; ++y) {
+    const float* th = t.h + (long)(v * t.vstride + y) * pz * NT;
+    for (int k = threadIdx.x; k < pz * NT; k += BLOCK) ts[k] = th[k];
+    __syncthreads();  // (the view, the first time) and this plane's taps in place
+    // E[ze][xe] = sum_{a,b} h[a,b] V[xe - 2hx + a, ze - 2hz + b]
+    for (int s = threadIdx.x; s < sx * EZ; s += BLOCK) {
+      const int ze = s / sx, xe0 = (s - ze * sx) * 8;
+      float acc[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      for (int bb = 0; bb < pz; ++bb) {
+        const int zz = ze - 2 * hz + bb;
+        if (zz < 0 || zz >= nz) continue;
+        const float* row = Vs + zz * WV + OX - 2 * hx + xe0;
+        const float* hb = ts + bb * NT;
+        float w[NT + 7];
+#pragma unroll
+        for (int j = 0; j < NT + 7; ++j) w[j] = row[j];
+#pragma unroll
+        for (int a = 0; a < NT; ++a) {
+          const float ha = hb[a];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc[k] = fmaf(ha, w[k + a], acc[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (xe0 + k < EX) E[ze * WE + xe0 + k] = acc[k];
+    }
+    __syncthreads();
+    // fold x pads onto x = 0 and x = nx-1, for every extended row
+    for (int ze = threadIdx.x; ze < EZ; ze += BLOCK) {
+      float* row = E + ze * WE;
+      float lo = 0.f, hi = 0.f;
+      for (int i = 0; i < hx; ++i) lo += row[i];
+      for (int i = hx + nx; i < EX; ++i) hi += row[i];
+      row[hx] += lo;
+      row[hx + nx - 1] += hi;
+    }
+    __syncthreads();
+    // fold z pads onto z = 0 and z = nz-1, for every extended column
+    for (int xe = threadIdx.x; xe < EX; xe += BLOCK) {
+      float lo = 0.f, hi = 0.f;
+      for (int j = 0; j < hz; ++j) lo += E[j * WE + xe];
+      for (int j = hz + nz; j < EZ; ++j) hi += E[j * WE + xe];
+      E[hz * WE + xe] += lo;
+      E[(hz + nz - 1) * WE + xe] += hi;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < P; k += BLOCK) {
+      const int z = k / nx, x = k - z * nx;
+      const long o = x + (long)nx * y + S * z;
+      const float val = E[(z + hz) * WE + hx + x];
+      wout[o] = av ? val * av[o] : val;
+    }
+    __syncthreads();
+  }
+}
+
 // ===========================================================================
 // z-fastest work layout: volumes [y][x][z] (a pixel's z-vector contiguous, a
 // PSF plane y one contiguous [x][z] block).  The rotations become gathers
@@ -3344,7 +3441,14 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
       allow_lds(k_psf_adj_sep<NT, SL>, lds);
       k_psf_adj_sep<NT, SL><<<dim3(nchunk, nb), block, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride, g);
     }));
-  } else {
+  } else if (sizeof(float) * adj2d_lds_floats(g, t.ntx) <= kMaxLds) {
+    const size_t lds = sizeof(float) * adj2d_lds_floats(g, t.ntx);
+    SR_NT_SWITCH(t.ntx, {
+      allow_lds(k_psf_adj_2d_strips<NT, 256>, lds);
+      k_psf_adj_2d_strips<NT, 256><<<dim3(nchunk, nb), 256, lds, s>>>(views, t, vids, Yc, work, atten,
+                                                                      atten_vstride, g);
+    });
+  } else {  // (planes too large for the staged view: the per-term loop)
     const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
     const size_t lds = sizeof(float) * (size_t)odd_up(g.nx + 2 * hx) * (g.nz + 2 * hz);
     if (lds > kMaxLds) return hipErrorInvalidConfiguration;
