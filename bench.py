@@ -95,7 +95,51 @@ def cpu_baseline(image, mumap, psfs, budget_s):
     return {"value": 1.0 / t_recon, "unit": "recons/s", "cores": min(cores, nsub), "kind": "port",
             "sample": (f"{len(tps)} x (project + backproject of one {nsub}-view subset) at {NX}x{NX}x{NZ}, "
                        f"{PSIZE}x{PSIZE} PSF, median {tp:.2f} s + {tb:.2f} s, extrapolated to 32 subset "
-                       f"projections + 40 subset back-projections per reconstruction")}
+                       f"projections + 40 subset back-projections per reconstruction"),
+            "calibration": calibration(O, S, cores)}
+
+
+def calibration(O, S, cores):
+    """The reference's published timing problem (time/project.jl:17-43): 64x64x40,
+    60 views, random mu-map, random 19x19 PSFs symmetrised by reversal and
+    transposition (not separable), dy = 4.7952, mode :fast.  project! was
+    published at 289.804 ms (:two) and 378.348 ms (:one) on an unstated CPU
+    (time/project.jl:60-61, :56-57).  Here: the oracle's Float32 restatement
+    on this host's cores (median of 3) and the library on the GPU (median of
+    5, device arrays), with seeded inputs of the same construction."""
+    rng = np.random.default_rng(0)
+    nx, nz, nview, p = 64, 40, 60, 19
+    mumap = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    psfs = rng.random((p, p, nx, nview), dtype=np.float32)
+    psfs = psfs + psfs[::-1, ::-1]
+    psfs = psfs + psfs.transpose(1, 0, 2, 3)
+    psfs = np.asfortranarray(psfs / psfs.sum(axis=(0, 1), keepdims=True))
+    x = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    dy = 4.7952
+    ang = S.default_viewangle(nview).astype(np.float64)
+    out = {"problem": "64x64x40, 60 views, 19x19 non-separable PSF (time/project.jl)",
+           "reference_ms": {"two": 289.804, "one": 378.348}, "cores": cores}
+    for interp in ("two", "one"):
+        P = O.Problem(mumap, psfs, dy, ang, interp, np.float32, "fft", nthread=min(cores, nview))
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            P.project(x)
+            ts.append(time.perf_counter() - t0)
+        out.setdefault("cpu_port_ms", {})[interp] = 1e3 * float(np.median(ts))
+        plan = S.SPECTplan(mumap, psfs, dy, interpmeth=interp)
+        d_x, d_v = S.DeviceArray.from_host(x), S.DeviceArray((nx, nz, nview))
+        lib = S.lib()
+        from spectrecon_amd import _lib as L
+        L.check(lib.sr_project(plan.handle, d_v.ptr, d_x.ptr, None, nview, L.SR_MEM_DEVICE))
+        tg = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            L.check(lib.sr_project(plan.handle, d_v.ptr, d_x.ptr, None, nview, L.SR_MEM_DEVICE))
+            tg.append(time.perf_counter() - t0)
+        out.setdefault("gpu_ms", {})[interp] = 1e3 * float(np.median(tg))
+        plan.close()
+    return out
 
 
 def stream_workload(kind, nreq, rate, seed=7, autotune=True):
