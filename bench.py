@@ -244,11 +244,88 @@ def gradient_workload(steps):
                       "data": "synthetic (seeded phantoms; y = 0.9 A x' of other phantoms)"}), flush=True)
 
 
+def units_workload(reps=20):
+    """Each of the reference's published timings (time/*.jl; unstated CPU)
+    beside the library on the GPU for the same operation and sizes (device
+    arrays, median of `reps` calls; every call returns after its stream
+    finished, so small operations measure launch latency)."""
+    import spectrecon_amd as S
+    from spectrecon_amd import _lib as L
+
+    lib = S.lib()
+    rng = np.random.default_rng(0)
+
+    def med(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return 1e3 * float(np.median(ts))
+
+    rows = []
+    # imrotate! of one 100x100 slice, theta = 3 pi / 16 (time/rotate2.jl:9-40)
+    sl = S.DeviceArray.from_host(np.asfortranarray(rng.random((100, 100, 1), dtype=np.float32)))
+    so = S.DeviceArray((100, 100, 1))
+    th = float(np.float32(3 * np.pi / 16))
+    for interp, name, pub in ((L.SR_INTERP_TWO, "two", (0.162171, 0.168518)), (L.SR_INTERP_ONE, "one", (0.533316, 0.517367))):
+        for adj in (0, 1):
+            rows.append({"op": f"imrotate{'_adj' if adj else ''}! 100x100 slice, :{name}", "published_ms": pub[adj],
+                         "gpu_ms": med(lambda: L.check(lib.sr_imrotate(so.ptr, sl.ptr, 100, 1, th, interp, adj,
+                                                                        L.SR_MEM_DEVICE)))})
+    # 3-D rotation of 128x128x100, theta = pi / 6 (time/rotate3.jl:9-24, :two)
+    v3 = S.DeviceArray.from_host(np.asfortranarray(rng.standard_normal((128, 128, 100)).astype(np.float32)))
+    o3 = S.DeviceArray((128, 128, 100))
+    th3 = float(np.float32(np.pi / 6))
+    rows.append({"op": "imrotate! 128x128x100, :two, threaded", "published_ms": 5.9,
+                 "gpu_ms": med(lambda: L.check(lib.sr_imrotate(o3.ptr, v3.ptr, 128, 100, th3, L.SR_INTERP_TWO, 0,
+                                                                L.SR_MEM_DEVICE)))})
+    # fft_conv! / fft_conv_adj! of a 200x64 plane with a 5x5 box kernel (time/fft_convolve.jl:8-24)
+    img = np.zeros((200, 64), np.float32)
+    img[19:150, 19:40] = rng.random((131, 21))
+    di, do = S.DeviceArray.from_host(np.asfortranarray(img)), S.DeviceArray((200, 64))
+    dk = S.DeviceArray.from_host(np.asfortranarray(np.full((5, 5), 1 / 25, np.float32)))
+    for adj, pub in ((0, 0.565631), (1, 0.561371)):
+        rows.append({"op": f"fft_conv{'_adj' if adj else ''}! 200x64, 5x5", "published_ms": pub,
+                     "gpu_ms": med(lambda: L.check(lib.sr_fft_conv(do.ptr, di.ptr, 200, 64, dk.ptr, 5, 5, adj,
+                                                                    L.SR_MEM_DEVICE)))})
+    # SPECTplan, project!, backproject! on the timing problem (time/SPECTplan.jl, time/project.jl,
+    # time/backproject.jl): 64x64x40, 60 views, random 19x19 PSFs symmetrised, dy = 4.7952
+    nx, nz, nview, p = 64, 40, 60, 19
+    mumap = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    psfs = rng.random((p, p, nx, nview), dtype=np.float32)
+    psfs = psfs + psfs[::-1, ::-1]
+    psfs = psfs + psfs.transpose(1, 0, 2, 3)
+    psfs = np.asfortranarray(psfs / psfs.sum(axis=(0, 1), keepdims=True))
+    dx = S.DeviceArray.from_host(np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32)))
+    dv = S.DeviceArray.from_host(np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32)))
+    t0 = time.perf_counter()
+    S.SPECTplan(mumap, psfs, 4.7952).close()
+    rows.append({"op": "SPECTplan 64x64x40, 60 views, 19x19", "published_ms": 13.585,
+                 "gpu_ms": 1e3 * (time.perf_counter() - t0), "note": "one construction, host inputs"})
+    pubs = {("two", "fast"): (289.804, 208.163), ("one", "fast"): (378.348, 343.419),
+            ("two", "mem"): (618.369, 246.847), ("one", "mem"): (793.918, 379.044)}
+    for (interp, mode), (pp, pb) in pubs.items():
+        plan = S.SPECTplan(mumap, psfs, 4.7952, interpmeth=interp, mode=mode)
+        rows.append({"op": f"project! 64x64x40, 60 views, :{interp}, :{mode}", "published_ms": pp,
+                     "gpu_ms": med(lambda: L.check(lib.sr_project(plan.handle, dv.ptr, dx.ptr, None, nview,
+                                                                   L.SR_MEM_DEVICE)))})
+        rows.append({"op": f"backproject! 64x64x40, 60 views, :{interp}, :{mode}", "published_ms": pb,
+                     "gpu_ms": med(lambda: L.check(lib.sr_backproject(plan.handle, dx.ptr, dv.ptr, None, nview,
+                                                                       L.SR_MEM_DEVICE)))})
+        plan.close()
+    for r in rows:
+        r["speedup"] = r["published_ms"] / r["gpu_ms"]
+    print(json.dumps({"workload": "units", "note": "published: the reference's own timings on an unstated CPU "
+                      "(time/*.jl); gpu: this library on one MI355X", "rows": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["osem", "batched", "served", "gradient"], default="osem",
+    ap.add_argument("--workload", choices=["osem", "batched", "served", "gradient", "units"], default="osem",
                     help="osem: the headline OS-EM line (default); batched / served: the concurrent patterns; "
-                         "gradient: the differentiable projector")
+                         "gradient: the differentiable projector; units: the reference's published timings")
     ap.add_argument("--requests", type=int, default=100, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="served: Poisson arrival rate per second")
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,6 +362,8 @@ def main():
 
     if args.workload == "gradient":
         return gradient_workload(max(1, args.steps))
+    if args.workload == "units":
+        return units_workload()
     if args.workload != "osem":
         return stream_workload(args.workload, args.requests, args.rate, autotune=not args.no_autotune)
     if S.device_count() == 0:

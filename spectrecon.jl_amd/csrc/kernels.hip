@@ -3264,7 +3264,7 @@ bool psf_sep_supported(const Geo& g) {
   if (SLv == 0 || g.nx % 4 != 0) return false;
   const int sx = (g.nx + SLv - 1) / SLv, sz = (g.nz + SLv - 1) / SLv;
   const long need = std::max((long)std::max(g.nz * sx, g.nx * sz), (long)cdiv((long)g.nx * g.nz, 4L * PF_MAX));
-  return need <= 1024;
+  return need <= kSepThreads;  // the separable kernels' launch bound
 }
 
 bool psf_fwd_col_ok(const Geo& g, int ntx) {
@@ -3360,7 +3360,7 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
     // one strip per thread per pass, and enough threads to prefetch the plane
     const long need = std::max((long)std::max(g.nz * sx, g.nx * sz), (long)cdiv((long)g.nx * g.nz, 4L * PF_MAX));
     const int block = std::max(64, (int)cdiv(need, 64) * 64);
-    if (block > 1024) return hipErrorInvalidConfiguration;
+    if (block > kSepThreads) return hipErrorInvalidConfiguration;  // (the kernel's launch bound)
     const size_t area = std::max((size_t)g.nz * quad_stride(SLv * sx + t.ntx - 1),
                                  (size_t)SLv * sx * quad_stride(SLv * sz + t.ntx - 1));
     const size_t lds = sizeof(float) * (area + 2 * t.ntx);
@@ -3434,6 +3434,7 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
     const int sx = (g.nx + SLv - 1) / SLv, sz = (g.nz + SLv - 1) / SLv;
     // one strip per thread and pass; at least one thread per tap-table entry
     const int block = (int)cdiv(std::max(std::max(g.nz * sx, g.nx * sz), 6 * t.ntx), 64) * 64;
+    if (block > kSepThreads) return hipErrorInvalidConfiguration;  // (the kernel's launch bound)
     const size_t lds = sizeof(float) * ((size_t)g.nx * quad_stride(SLv * sz + t.ntx - 1) +
                                         (size_t)g.nz * quad_stride(SLv * sx + t.ntx - 1) + 12 * t.ntx);
     if (lds > kMaxLds) return hipErrorInvalidConfiguration;

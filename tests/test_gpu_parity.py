@@ -73,7 +73,10 @@ def test_imrotate_bounds(gpu, method):
 # PSF convolution
 # ---------------------------------------------------------------------------
 CONV_CASES = [(20, 14, 5, 5), (12, 8, 7, 7), (12, 10, 5, 3), (64, 32, 15, 15), (128, 80, 37, 37), (7, 1, 3, 1),
-              (33, 17, 9, 1), (16, 16, 1, 1)]
+              (33, 17, 9, 1), (16, 16, 1, 1),
+              # the reference's fft_conv timing plane (time/fft_convolve.jl:9-17): its separable strips would need
+              # more threads than the separable kernels' launch bound, so it takes the direct 2-D kernels
+              (200, 64, 5, 5)]
 
 
 @pytest.mark.parametrize("nx,nz,px,pz", CONV_CASES)
