@@ -405,8 +405,9 @@ __device__ __forceinline__ void rotf90_src(int m, bool adj, int Np, int x, int y
 //   J1 = clamp(b+k, 0, Np-1) with weight w1 and J2 = clamp(b+k+1, 0, Np-1)
 // with weight w2, summed as (0 + w1*x[J1]) + w2*x[J2].  kAlongX: interpolate
 // along dim 1 with the table indexed by column y (rotate_x!); else along
-// dim 2 indexed by row x (rotate_y!).  kAdj: the transpose, evaluated as a
-// gather in the reference's scatter order (rows ascending, J1 before J2).
+// dim 2 indexed by row x (rotate_y!).  The adjoint is the transpose,
+// evaluated as a gather in the reference's scatter order (rows ascending, J1
+// before J2).
 template <bool kAlongX>
 __device__ __forceinline__ float shear_fwd_at(const float* L, int Np, const int* tb, const float* tw, int x, int y) {
   const int i = kAlongX ? y : x;  // table row
@@ -419,25 +420,38 @@ __device__ __forceinline__ float shear_fwd_at(const float* L, int Np, const int*
   return w1 * v1 + w2 * v2;
 }
 
+// adjoint at an interior position 0 < j < Np-1 of row i: only J2(j-b-1) and
+// J1(j-b) land there, in that order
 template <bool kAlongX>
-__device__ __forceinline__ float shear_adj_at(const float* L, int Np, const int* tb, const float* tw, int x, int y) {
+__device__ __forceinline__ float shear_adj_interior(const float* L, int Np, const int* tb, const float* tw, int x,
+                                                    int y) {
   const int i = kAlongX ? y : x;
   const int j = kAlongX ? x : y;
   const int b = tb[i];
   const float w1 = tw[2 * i], w2 = tw[2 * i + 1];
-  auto ld = [&](int k) { return kAlongX ? L[k + Np * y] : L[x + Np * k]; };
+  const int k2 = j - b - 1, k1 = j - b;
   float s = 0.f;
-  if (j > 0 && j < Np - 1) {
-    const int k2 = j - b - 1, k1 = j - b;  // J2(k2) == j precedes J1(k1) == j
-    if (k2 >= 0 && k2 < Np) s = s + w2 * ld(k2);
-    if (k1 >= 0 && k1 < Np) s = s + w1 * ld(k1);
-  } else if (j == 0) {
+  if (k2 >= 0 && k2 < Np) s = s + w2 * (kAlongX ? L[k2 + Np * y] : L[x + Np * k2]);
+  if (k1 >= 0 && k1 < Np) s = s + w1 * (kAlongX ? L[k1 + Np * y] : L[x + Np * k1]);
+  return s;
+}
+
+// adjoint at the clamped end j = 0 (hi false) or j = Np-1 (hi true) of row i:
+// every tap whose index clamps there, in scatter order
+template <bool kAlongX>
+__device__ __forceinline__ float shear_adj_end(const float* L, int Np, const int* tb, const float* tw, int i,
+                                               bool hi) {
+  const int b = tb[i];
+  const float w1 = tw[2 * i], w2 = tw[2 * i + 1];
+  auto ld = [&](int k) { return kAlongX ? L[k + Np * i] : L[i + Np * k]; };
+  float s = 0.f;
+  if (!hi) {
     const int kend = min(Np - 1, -b);
     for (int k = 0; k <= kend; ++k) {
       if (b + k <= 0) s = s + w1 * ld(k);
       if (b + k + 1 <= 0) s = s + w2 * ld(k);
     }
-  } else {  // j == Np-1
+  } else {
     for (int k = max(0, Np - 2 - b); k < Np; ++k) {
       if (b + k >= Np - 1) s = s + w1 * ld(k);
       if (b + k + 1 >= Np - 1) s = s + w2 * ld(k);
@@ -446,30 +460,78 @@ __device__ __forceinline__ float shear_adj_at(const float* L, int Np, const int*
   return s;
 }
 
-template <bool kAlongX, bool kAdj>
-__device__ __forceinline__ void shear_pass(float* L, int Np, const int* __restrict__ tb, const float* __restrict__ tw) {
+// The block's elements in registers: thread (tx, ty) holds x = tx + 64*xs,
+// y = ty + 16*ys.
+#define R1_FOR_EACH(...)                                  \
+  _Pragma("unroll") for (int ys = 0; ys < R1_MY; ++ys)    \
+  _Pragma("unroll") for (int xs = 0; xs < R1_MX; ++xs) {  \
+    const int x = tx + 64 * xs, y = ty + 16 * ys;         \
+    __VA_ARGS__                                           \
+  }
+
+// A pass computes rows y0 <= y < y1 of L only; the other rows keep their
+// values (the caller reads none of them, or they are zero and stay zero).
+template <bool kAlongX>
+__device__ __forceinline__ void shear_pass_fwd(float* L, int Np, const int* tb, const float* tw, int y0, int y1) {
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   float r[R1_MY][R1_MX];
-#pragma unroll
-  for (int ys = 0; ys < R1_MY; ++ys)
-#pragma unroll
-    for (int xs = 0; xs < R1_MX; ++xs) {
-      const int x = tx + 64 * xs, y = ty + 16 * ys;
-      if (x < Np && y < Np)
-        r[ys][xs] = kAdj ? shear_adj_at<kAlongX>(L, Np, tb, tw, x, y) : shear_fwd_at<kAlongX>(L, Np, tb, tw, x, y);
-    }
+  R1_FOR_EACH(if (x < Np && y >= y0 && y < y1) r[ys][xs] = shear_fwd_at<kAlongX>(L, Np, tb, tw, x, y);)
   __syncthreads();
-#pragma unroll
-  for (int ys = 0; ys < R1_MY; ++ys)
-#pragma unroll
-    for (int xs = 0; xs < R1_MX; ++xs) {
-      const int x = tx + 64 * xs, y = ty + 16 * ys;
-      if (x < Np && y < Np) L[x + Np * y] = r[ys][xs];
-    }
+  R1_FOR_EACH(if (x < Np && y >= y0 && y < y1) L[x + Np * y] = r[ys][xs];)
   __syncthreads();
 }
 
+// The clamped ends hold loops over up to Np taps; they are spread over all
+// waves (end q = 2*row + hi on lane q/16 of wave q%16) instead of serialising
+// in the waves that own columns 0 and Np-1.
+template <bool kAlongX>
+__device__ __forceinline__ void shear_pass_adj(float* L, int Np, const int* tb, const float* tw, int y0, int y1) {
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int q = tx * 16 + ty, ei = q >> 1;
+  const bool hi = q & 1;
+  const int ej = hi ? Np - 1 : 0, ey = kAlongX ? ei : ej;
+  const bool end = ei < Np && (!hi || Np > 1) && ey >= y0 && ey < y1;
+  float e = 0.f;
+  if (end) e = shear_adj_end<kAlongX>(L, Np, tb, tw, ei, hi);
+  float r[R1_MY][R1_MX];
+  R1_FOR_EACH({
+    const int j = kAlongX ? x : y;
+    if (x < Np && y >= y0 && y < y1 && j > 0 && j < Np - 1)
+      r[ys][xs] = shear_adj_interior<kAlongX>(L, Np, tb, tw, x, y);
+  })
+  __syncthreads();
+  R1_FOR_EACH({
+    const int j = kAlongX ? x : y;
+    if (x < Np && y >= y0 && y < y1 && j > 0 && j < Np - 1) L[x + Np * y] = r[ys][xs];
+  })
+  if (end) L[kAlongX ? ej + Np * ei : ei + Np * ej] = e;
+  __syncthreads();
+}
+
+// Table rows of view v (b: x then y, Np ints each; w: x then y, 2*Np floats
+// each), copied behind the slice in LDS when kLds (the caller's next
+// __syncthreads publishes them), else read in place.
+struct R1Tabs {
+  const int* bx;
+  const int* by;
+  const float* wx;
+  const float* wy;
+};
+template <bool kLds>
+__device__ __forceinline__ R1Tabs r1_tabs(float* L, int Np, int v, const int* __restrict__ tab_b,
+                                          const float* __restrict__ tab_w) {
+  const int* gb = tab_b + (long)v * 2 * Np;
+  const float* gw = tab_w + (long)v * 4 * Np;
+  if (!kLds) return {gb, gb + Np, gw, gw + 2 * Np};
+  int* lb = reinterpret_cast<int*>(L + Np * Np);
+  float* lw = L + Np * Np + 2 * Np;
+  for (int i = threadIdx.x; i < 2 * Np; i += R1_THREADS) lb[i] = gb[i];
+  for (int i = threadIdx.x; i < 4 * Np; i += R1_THREADS) lw[i] = gw[i];
+  return {lb, lb + Np, lw, lw + 2 * Np};
+}
+
 // grid (nz, nb): imrotate! :one of slice z for view vids[b] (src/rotatez.jl:240-278)
+template <bool kLds>
 __global__ void __launch_bounds__(R1_THREADS) k_rot1_fwd(const float* __restrict__ img, float* __restrict__ out,
                                                          const float* __restrict__ atten, long atten_vstride,
                                                          const RotView* __restrict__ rv,
@@ -492,33 +554,35 @@ __global__ void __launch_bounds__(R1_THREADS) k_rot1_fwd(const float* __restrict
       }
     return;
   }
-  for (int y = ty; y < Np; y += 16)
-    for (int x = tx; x < Np; x += 64) {
-      int sx, sy;
-      rotf90_src(r.m, false, Np, x, y, sx, sy);
-      L[x + Np * y] = slice_tap(src, N, sx - pad, sy - pad);
-    }
-  __syncthreads();
-  const int* tbx = tab_b + (long)v * 2 * Np;
-  const int* tby = tbx + Np;
-  const float* twx = tab_w + (long)v * 4 * Np;
-  const float* twy = twx + 2 * Np;
-  if (r.kind == 2) {
-    shear_pass<true, false>(L, Np, tbx, twx);
-    shear_pass<false, false>(L, Np, tby, twy);
+  const R1Tabs t = r1_tabs<kLds>(L, Np, v, tab_b, tab_w);
+  {  // all loads of the slice in flight before the first LDS store; each
+     // thread reads source pixel (x, y) (coalesced for every rotation) and
+     // stores it where rot_f90 puts it, the inverse map of rotf90_src
+    float q[R1_MY][R1_MX];
+    R1_FOR_EACH(if (x < Np && y < Np) q[ys][xs] = slice_tap(src, N, x - pad, y - pad);)
+    R1_FOR_EACH(if (x < Np && y < Np) {
+      int dx, dy;
+      rotf90_src(r.m, true, Np, x, y, dx, dy);
+      L[dx + Np * dy] = q[ys][xs];
+    })
   }
-  for (int y = ty; y < N; y += 16)
-    for (int x = tx; x < N; x += 64) {
-      const float val = (r.kind == 2) ? shear_fwd_at<true>(L, Np, tbx, twx, x + pad, y + pad)
-                                      : L[(x + pad) + Np * (y + pad)];
-      const long k = x + (long)N * y;
-      o[k] = a ? val * a[k] : val;
-    }
+  __syncthreads();
+  if (r.kind == 2) {
+    shear_pass_fwd<true>(L, Np, t.bx, t.wx, 0, Np);
+    shear_pass_fwd<false>(L, Np, t.by, t.wy, pad, pad + N);  // the last pass reads rows pad..pad+N-1
+  }
+  R1_FOR_EACH(if (x < N && y < N) {
+    const float val = (r.kind == 2) ? shear_fwd_at<true>(L, Np, t.bx, t.wx, x + pad, y + pad)
+                                    : L[(x + pad) + Np * (y + pad)];
+    const long k = x + (long)N * y;
+    o[k] = a ? val * a[k] : val;
+  })
 }
 
 // grid (nz, nb): imrotate_adj! :one (src/rotatez.jl:286-322) of work[b]
 // slice z into part[b] (one view per block; the views are summed by
 // k_img_reduce).
+template <bool kLds>
 __global__ void __launch_bounds__(R1_THREADS) k_rot1_adj(const float* __restrict__ work, float* __restrict__ part,
                                                          const RotView* __restrict__ rv,
                                                          const int* __restrict__ vids, const int* __restrict__ tab_b,
@@ -536,25 +600,27 @@ __global__ void __launch_bounds__(R1_THREADS) k_rot1_adj(const float* __restrict
       for (int x = tx; x < N; x += 64) o[x + (long)N * y] = src[x + (long)N * y];
     return;
   }
-  for (int y = ty; y < Np; y += 16)
-    for (int x = tx; x < Np; x += 64) L[x + Np * y] = slice_tap(src, N, x - pad, y - pad);
+  const R1Tabs t = r1_tabs<kLds>(L, Np, v, tab_b, tab_w);
+  {
+    float q[R1_MY][R1_MX];
+    R1_FOR_EACH(if (x < Np && y < Np) q[ys][xs] = slice_tap(src, N, x - pad, y - pad);)
+    R1_FOR_EACH(if (x < Np && y < Np) L[x + Np * y] = q[ys][xs];)
+  }
   __syncthreads();
   if (r.kind == 2) {
-    const int* tbx = tab_b + (long)v * 2 * Np;
-    const int* tby = tbx + Np;
-    const float* twx = tab_w + (long)v * 4 * Np;
-    const float* twy = twx + 2 * Np;
-    shear_pass<true, true>(L, Np, tbx, twx);
-    shear_pass<false, true>(L, Np, tby, twy);
-    shear_pass<true, true>(L, Np, tbx, twx);
+    // rows outside pad..pad+N-1 are zero on entry (and stay zero) and are
+    // not read by the crop, which maps that square onto itself
+    shear_pass_adj<true>(L, Np, t.bx, t.wx, pad, pad + N);
+    shear_pass_adj<false>(L, Np, t.by, t.wy, 0, Np);
+    shear_pass_adj<true>(L, Np, t.bx, t.wx, pad, pad + N);
   }
-  for (int y = ty; y < N; y += 16)
-    for (int x = tx; x < N; x += 64) {
-      int sx, sy;
-      rotf90_src(r.m, true, Np, x + pad, y + pad, sx, sy);
-      o[x + (long)N * y] = L[sx + Np * sy];
-    }
+  R1_FOR_EACH(if (x < N && y < N) {
+    int sx, sy;
+    rotf90_src(r.m, true, Np, x + pad, y + pad, sx, sy);
+    o[x + (long)N * y] = L[sx + Np * sy];
+  })
 }
+#undef R1_FOR_EACH
 
 // ===========================================================================
 // Attenuation factors in the reference's summation order
@@ -3139,10 +3205,17 @@ hipError_t launch_rot_fwd(int interp, const float* img, float* out, const float*
   if (nb <= 0) return hipSuccess;
   const long S = (long)g.nx * g.nx;
   if (interp == 1) {
-    const size_t lds = sizeof(float) * g.Np * g.Np;
+    const size_t lds = sizeof(float) * g.Np * g.Np, lds_t = lds + sizeof(float) * 6 * g.Np;
     if (g.Np > kMaxNpOne || lds > kMaxLds) return hipErrorInvalidConfiguration;
-    allow_lds(k_rot1_fwd, lds);
-    k_rot1_fwd<<<dim3(g.nz, nb), R1_THREADS, lds, s>>>(img, out, atten, atten_vstride, rv, vids, tab_b, tab_w, g);
+    if (lds_t <= kMaxLds) {  // interpolation tables beside the slice
+      allow_lds(k_rot1_fwd<true>, lds_t);
+      k_rot1_fwd<true><<<dim3(g.nz, nb), R1_THREADS, lds_t, s>>>(img, out, atten, atten_vstride, rv, vids, tab_b,
+                                                                 tab_w, g);
+    } else {
+      allow_lds(k_rot1_fwd<false>, lds);
+      k_rot1_fwd<false><<<dim3(g.nz, nb), R1_THREADS, lds, s>>>(img, out, atten, atten_vstride, rv, vids, tab_b,
+                                                                tab_w, g);
+    }
   } else {
     // views per block: enough blocks to fill the device, reuse of the staged slice otherwise
     int vpg = 1;
@@ -3174,10 +3247,15 @@ hipError_t launch_rot_adj(int interp, const float* work, float* part, const RotV
                           int vpg, const int* tab_b, const float* tab_w, const Geo& g, hipStream_t s) {
   if (nb <= 0) return hipSuccess;
   if (interp == 1) {  // one view per block: part holds nb slices-volumes
-    const size_t lds = sizeof(float) * g.Np * g.Np;
+    const size_t lds = sizeof(float) * g.Np * g.Np, lds_t = lds + sizeof(float) * 6 * g.Np;
     if (g.Np > kMaxNpOne || lds > kMaxLds) return hipErrorInvalidConfiguration;
-    allow_lds(k_rot1_adj, lds);
-    k_rot1_adj<<<dim3(g.nz, nb), R1_THREADS, lds, s>>>(work, part, rv, vids, tab_b, tab_w, g);
+    if (lds_t <= kMaxLds) {
+      allow_lds(k_rot1_adj<true>, lds_t);
+      k_rot1_adj<true><<<dim3(g.nz, nb), R1_THREADS, lds_t, s>>>(work, part, rv, vids, tab_b, tab_w, g);
+    } else {
+      allow_lds(k_rot1_adj<false>, lds);
+      k_rot1_adj<false><<<dim3(g.nz, nb), R1_THREADS, lds, s>>>(work, part, rv, vids, tab_b, tab_w, g);
+    }
   } else {
     const int ng = (nb + vpg - 1) / vpg;
     if (g.nx <= 128) {
