@@ -2056,12 +2056,24 @@ __global__ void __launch_bounds__(256) k_view_reduce(const float* __restrict__ p
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= P) return;
   const float* p = part + (long)b * nchunk * P + k;
-  float s = 0.f;
-  for (int c = 0; c < nchunk; ++c) s += p[(long)c * P];
-  if (epi == EPI_RATIO) {
+  float yo = 0.f, bk = 0.f;
+  if (epi == EPI_RATIO) {  // issued ahead of the partials
     const long o = (long)vids[b] * P + k;
-    s = yobs[o] / (s + bkg[o]);  // src/ml-os-em.jl:44, :124-125
+    yo = yobs[o];
+    bk = bkg[o];
   }
+  // chunk partials summed in chunk order, 8 loads in flight at a time
+  float s = 0.f;
+  int c = 0;
+  for (; c + 8 <= nchunk; c += 8) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = p[(long)(c + i) * P];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s += v[i];
+  }
+  for (; c < nchunk; ++c) s += p[(long)c * P];
+  if (epi == EPI_RATIO) s = yo / (s + bk);  // src/ml-os-em.jl:44, :124-125
   out[(long)b * P + k] = s;
 }
 
