@@ -46,6 +46,22 @@ def test_imrotate_vs_oracle(gpu, O, method, n):
         assert maxrel(ga, ra) <= ROT_TOL, ("adj", method, n, th, maxrel(ga, ra))
 
 
+@pytest.mark.parametrize("n", [139, 140])
+def test_imrotate_one_largest(gpu, O, n):
+    """:one at the largest padded sizes: Np = 199 keeps its interpolation
+    tables in LDS beside the slice, Np = 200 reads them from device memory."""
+    rng = np.random.default_rng(n)
+    img = np.asfortranarray(rng.random((n, n, 2), dtype=np.float32))
+    for th in (0.3, 2.0, 4.0, 5.9):
+        th = float(np.float32(th))
+        g = gpu.imrotate(img, th, method="one")
+        r = O.imrotate(img, th, "one")
+        assert maxrel(g, r) <= ROT_TOL, (n, th, maxrel(g, r))
+        ga = gpu.imrotate_adj(img, th, method="one")
+        ra = O.imrotate(img, th, "one", adj=True)
+        assert maxrel(ga, ra) <= ROT_TOL, ("adj", n, th, maxrel(ga, ra))
+
+
 @pytest.mark.parametrize("method", ["one", "two"])
 def test_imrotate_dense_adjoint(gpu, method):
     """Matrix(A') == Matrix(A)' at 16x16, 7 angles (test/adjoint-rotate.jl:43-58):
