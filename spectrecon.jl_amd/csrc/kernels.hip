@@ -2450,16 +2450,20 @@ __global__ void __launch_bounds__(NW * 64) k_psf_adj_mfma(const float* __restric
 // 4 odd mod 64: a fragment's 16 rows x 4 columns hit distinct banks), per wave
 // Hw (rows z + hz with zero pads, 16 columns: a fragment's 4 rows x 16 columns
 // hit distinct banks), tap groups.  grid (nchunk, nb), block 512.
-__host__ __device__ inline size_t mf_col_lds_floats(const Geo& g, int nt, int tbp) {
-  return (size_t)g.nz * mf_wx(g.nx, (g.px - 1) / 2, nt) + (size_t)MF_WAVES * mf_rq(g.nz, nt) * 16 +
+__host__ __device__ inline size_t mf_col_lds_floats(const Geo& g, int nt, int tbp, int nwv = MF_WAVES) {
+  return (size_t)g.nz * mf_wx(g.nx, (g.px - 1) / 2, nt) + (size_t)nwv * mf_rq(g.nz, nt) * 16 +
          (size_t)tbp * MfTaps(nt).size;
 }
-template <int TPW>
-__global__ void __launch_bounds__(MF_THREADS) k_psf_adj_col(const float* __restrict__ views, Taps t,
+// NWV waves per block (8, or 12 where the LDS holds their buffers: three
+// waves per SIMD hide more of each other's LDS and memory waits); the block's
+// (plane, column) items of a tap group are dealt round robin to the waves.
+template <int TPW, int NWV>
+__global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restrict__ views, Taps t,
                                                             const int* __restrict__ vids, int Yc,
                                                             float* __restrict__ work, const float* __restrict__ atten,
                                                             long atten_vstride, Geo g, int TBP) {
   extern __shared__ float L[];
+  constexpr int NTH = 64 * NWV;
   const int NT = t.ntx;
   const MfTaps TT(NT);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -2475,19 +2479,19 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_adj_col(const float* __restr
   const int m16 = lane & 15, q4 = lane >> 4;
   float* Vx = L;
   float* Hw = Vx + nz * WX + wave * RG * 16;
-  float* s_taps = Vx + nz * WX + MF_WAVES * RG * 16;  // TBP planes of TT.size
+  float* s_taps = Vx + nz * WX + NWV * RG * 16;  // TBP planes of TT.size
   // zeros where no data lands: Vx's pad columns, the wave buffers, the tap tables
   {
     const int padc = WX - nx;  // columns [0, XO) and [XO + nx, WX) of every row
-    for (int k = tid; k < nz * padc; k += MF_THREADS) {
+    for (int k = tid; k < nz * padc; k += NTH) {
       const int z = k / padc, cc = k - z * padc;
       Vx[z * WX + (cc < XO ? cc : cc + nx)] = 0.f;
     }
-    for (int k = tid; k < MF_WAVES * RG * 16 + TBP * TT.size; k += MF_THREADS) L[nz * WX + k] = 0.f;
+    for (int k = tid; k < NWV * RG * 16 + TBP * TT.size; k += NTH) L[nz * WX + k] = 0.f;
   }
   const float* vin = views + (long)b * nx * nz;
   const int nq = nx >> 2;
-  for (int k = tid; k < nz * nq; k += MF_THREADS) {
+  for (int k = tid; k < nz * nq; k += NTH) {
     const int z = k / nq, x4 = k - z * nq;
     *reinterpret_cast<float4*>(Vx + z * WX + XO + 4 * x4) = *reinterpret_cast<const float4*>(vin + z * nx + 4 * x4);
   }
@@ -2495,123 +2499,128 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_adj_col(const float* __restr
   const long V = (long)nx * ny * nz;
   float* wout = work + V * b;
   const float* av = atten ? atten + atten_vstride * v : nullptr;
-  auto load_taps = [&](int yg, int ng) {
-    constexpr int ME = (8 * 6 * 64 + MF_THREADS - 1) / MF_THREADS;
+  // the tap tables of up to TBP (<= 8) planes at a time; the next group's
+  // loads are in flight in registers while the current group is convolved
+  constexpr int ME = (8 * 6 * 64 + NTH - 1) / NTH;
+  float tvv[ME];
+  int tdst[ME];
+  auto fetch_taps = [&](int yg, int ng) {
     const int n = ng * 6 * NT;
-    float tvv[ME];
-    int tdst[ME];
 #pragma unroll
     for (int j = 0; j < ME; ++j) {
-      const int e = min(tid + j * MF_THREADS, n - 1);
+      const int e = min(tid + j * NTH, n - 1);
       const int pl = e / (6 * NT), r = e - pl * 6 * NT, tab = r / NT, tk = r - tab * NT;
       const long plane = (long)v * t.vstride + yg + pl;
       tdst[j] = pl * TT.size + tab * TT.tp + TT.off + tk;
       const float* src = tab == 0 ? t.u + plane * NT : tab == 1 ? t.w + plane * NT : t.cum + (plane * 4 + tab - 2) * NT;
       tvv[j] = src[tk];
     }
+  };
+  fetch_taps(y0, min(TBP, y1 - y0));
+  for (int yg = y0; yg < y1; yg += TBP) {
+    const int ng = min(TBP, y1 - yg);
+    lds_barrier();  // the previous tap group is no longer read (the first time: Vx is in place)
 #pragma unroll
     for (int j = 0; j < ME; ++j) s_taps[tdst[j]] = tvv[j];
-  };
-  for (int y = y0; y < y1; ++y) {
-    if ((y - y0) % TBP == 0) {
-      lds_barrier();  // the previous tap group is no longer read (the first time: Vx is in place)
-      load_taps(y, min(TBP, y1 - y));
-      lds_barrier();
-    }
-    const float* tb = s_taps + ((y - y0) % TBP) * TT.size;
-    const float* su = tb + TT.off + q4 - m16;
-    const float* sw = su + TT.tp;
-    const float* sut = sw + TT.tp;
-    const float* suh = sut + TT.tp;
-    const float* swt = suh + TT.tp;
-    const float* swh = swt + TT.tp;
-    // z taps: the first tile's row 0 takes the tail sums, the last tile's
-    // row 15 the head sums (nz = 16 TPW), every other row the plain taps, so
-    // a step reads three tap words: plain, first tile, last tile
-    const float* ap0 = m16 == 0 ? swt : (TPW == 1 && m16 == 15) ? swh : sw;
-    const float* apl = m16 == 15 ? swh : sw;
-    for (int col = wave; col < ntx; col += MF_WAVES) {
-      const int x0 = 16 * col;
-      const long ob = ((long)y * nx + x0 + m16) * nz + 4 * q4;  // + 16 i: lane's 4 z of tile i
-      // attenuation factors of the column's tiles, in flight during the passes
-      float4 af[TPW];
-#pragma unroll
-      for (int i = 0; i < TPW; ++i)
-        af[i] = av ? *reinterpret_cast<const float4*>(av + ob + 16 * i) : make_float4(1.f, 1.f, 1.f, 1.f);
-      // x adjoint (operands swapped: lane (k, n) register r = H[16i + n][x0 + 4k + r])
-      const float* bp = (x0 == 0 && m16 == 0) ? sut : (x0 + 16 == nx && m16 == 15) ? suh : su;
-      const float* va = Vx + m16 * WX + XO - hx + x0 + q4;
-      mf4 dh[TPW];
-#pragma unroll
-      for (int i = 0; i < TPW; ++i) dh[i] = mf4{0.f, 0.f, 0.f, 0.f};
+    lds_barrier();
+    if (yg + TBP < y1) fetch_taps(yg + TBP, min(TBP, y1 - yg - TBP));
+    for (int it = wave; it < ng * ntx; it += NWV) {
+      const int pl = it / ntx, col = it - pl * ntx, y = yg + pl;
+      const float* tb = s_taps + pl * TT.size;
+      const float* su = tb + TT.off + q4 - m16;
+      const float* sw = su + TT.tp;
+      const float* sut = sw + TT.tp;
+      const float* suh = sut + TT.tp;
+      const float* swt = suh + TT.tp;
+      const float* swh = swt + TT.tp;
+      // z taps: the first tile's row 0 takes the tail sums, the last tile's
+      // row 15 the head sums (nz = 16 TPW), every other row the plain taps, so
+      // a step reads three tap words: plain, first tile, last tile
+      const float* ap0 = m16 == 0 ? swt : (TPW == 1 && m16 == 15) ? swh : sw;
+      const float* apl = m16 == 15 ? swh : sw;
       {
-        float a0[TPW], a1[TPW], b0, b1;
-        auto ld = [&](float (&a)[TPW], float& bb, int s) {
-          bb = bp[4 * s];
+        const int x0 = 16 * col;
+        const long ob = ((long)y * nx + x0 + m16) * nz + 4 * q4;  // + 16 i: lane's 4 z of tile i
+        // attenuation factors of the column's tiles, in flight during the passes
+        float4 af[TPW];
 #pragma unroll
-          for (int i = 0; i < TPW; ++i) a[i] = va[16 * i * WX + 4 * s];
-        };
-        ld(a0, b0, 0);
-        int s = 0;
-        for (; s + 1 < nsx; s += 2) {
-          ld(a1, b1, s + 1);
-          SR_SCHED_FENCE();
+        for (int i = 0; i < TPW; ++i)
+          af[i] = av ? *reinterpret_cast<const float4*>(av + ob + 16 * i) : make_float4(1.f, 1.f, 1.f, 1.f);
+        // x adjoint (operands swapped: lane (k, n) register r = H[16i + n][x0 + 4k + r])
+        const float* bp = (x0 == 0 && m16 == 0) ? sut : (x0 + 16 == nx && m16 == 15) ? suh : su;
+        const float* va = Vx + m16 * WX + XO - hx + x0 + q4;
+        mf4 dh[TPW];
 #pragma unroll
-          for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b0, a0[i], dh[i]);
-          SR_SCHED_FENCE();
-          ld(a0, b0, min(s + 2, nsx - 1));
-          SR_SCHED_FENCE();
+        for (int i = 0; i < TPW; ++i) dh[i] = mf4{0.f, 0.f, 0.f, 0.f};
+        {
+          float a0[TPW], a1[TPW], b0, b1;
+          auto ld = [&](float (&a)[TPW], float& bb, int s) {
+            bb = bp[4 * s];
 #pragma unroll
-          for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b1, a1[i], dh[i]);
-        }
-        if (s < nsx) {
+            for (int i = 0; i < TPW; ++i) a[i] = va[16 * i * WX + 4 * s];
+          };
+          ld(a0, b0, 0);
+          int s = 0;
+          for (; s + 1 < nsx; s += 2) {
+            ld(a1, b1, s + 1);
+            SR_SCHED_FENCE();
 #pragma unroll
-          for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b0, a0[i], dh[i]);
-        }
-      }
+            for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b0, a0[i], dh[i]);
+            SR_SCHED_FENCE();
+            ld(a0, b0, min(s + 2, nsx - 1));
+            SR_SCHED_FENCE();
 #pragma unroll
-      for (int i = 0; i < TPW; ++i)
-        *reinterpret_cast<float4*>(Hw + (hz + 16 * i + m16) * 16 + 4 * q4) =
-            make_float4(dh[i][0], dh[i][1], dh[i][2], dh[i][3]);
-      // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
-      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-      // z adjoint: lane (k, n) register r = R[16i + 4k + r][x0 + n]
-      mf4 dz[TPW];
-#pragma unroll
-      for (int i = 0; i < TPW; ++i) dz[i] = mf4{0.f, 0.f, 0.f, 0.f};
-      {
-        float bq0[TPW], bq1[TPW], a0[TPW], a1[TPW];
-        auto ld = [&](float (&bq)[TPW], float (&aa)[TPW], int s) {
-          const float am = sw[4 * s];
-#pragma unroll
-          for (int i = 0; i < TPW; ++i) {
-            aa[i] = i == 0 ? ap0[4 * s] : i == TPW - 1 ? apl[4 * s] : am;
-            bq[i] = Hw[(16 * i + q4 + 4 * s) * 16 + m16];
+            for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b1, a1[i], dh[i]);
           }
-        };
-        ld(bq0, a0, 0);
-        int s = 0;
-        for (; s + 1 < nsz; s += 2) {
-          ld(bq1, a1, s + 1);
-          SR_SCHED_FENCE();
+          if (s < nsx) {
 #pragma unroll
-          for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a0[i], bq0[i], dz[i]);
-          SR_SCHED_FENCE();
-          ld(bq0, a0, min(s + 2, nsz - 1));
-          SR_SCHED_FENCE();
-#pragma unroll
-          for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a1[i], bq1[i], dz[i]);
+            for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b0, a0[i], dh[i]);
+          }
         }
-        if (s < nsz) {
 #pragma unroll
-          for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a0[i], bq0[i], dz[i]);
+        for (int i = 0; i < TPW; ++i)
+          *reinterpret_cast<float4*>(Hw + (hz + 16 * i + m16) * 16 + 4 * q4) =
+              make_float4(dh[i][0], dh[i][1], dh[i][2], dh[i][3]);
+        // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        // z adjoint: lane (k, n) register r = R[16i + 4k + r][x0 + n]
+        mf4 dz[TPW];
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) dz[i] = mf4{0.f, 0.f, 0.f, 0.f};
+        {
+          float bq0[TPW], bq1[TPW], a0[TPW], a1[TPW];
+          auto ld = [&](float (&bq)[TPW], float (&aa)[TPW], int s) {
+            const float am = sw[4 * s];
+#pragma unroll
+            for (int i = 0; i < TPW; ++i) {
+              aa[i] = i == 0 ? ap0[4 * s] : i == TPW - 1 ? apl[4 * s] : am;
+              bq[i] = Hw[(16 * i + q4 + 4 * s) * 16 + m16];
+            }
+          };
+          ld(bq0, a0, 0);
+          int s = 0;
+          for (; s + 1 < nsz; s += 2) {
+            ld(bq1, a1, s + 1);
+            SR_SCHED_FENCE();
+#pragma unroll
+            for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a0[i], bq0[i], dz[i]);
+            SR_SCHED_FENCE();
+            ld(bq0, a0, min(s + 2, nsz - 1));
+            SR_SCHED_FENCE();
+#pragma unroll
+            for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a1[i], bq1[i], dz[i]);
+          }
+          if (s < nsz) {
+#pragma unroll
+            for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a0[i], bq0[i], dz[i]);
+          }
         }
+        asm volatile("" ::: "memory");  // the next column's Hw stores stay after these reads
+#pragma unroll
+        for (int i = 0; i < TPW; ++i)
+          *reinterpret_cast<float4*>(wout + ob + 16 * i) = make_float4(dz[i][0] * af[i].x, dz[i][1] * af[i].y,
+                                                                       dz[i][2] * af[i].z, dz[i][3] * af[i].w);
       }
-      asm volatile("" ::: "memory");  // the next column's Hw stores stay after these reads
-#pragma unroll
-      for (int i = 0; i < TPW; ++i)
-        *reinterpret_cast<float4*>(wout + ob + 16 * i) = make_float4(dz[i][0] * af[i].x, dz[i][1] * af[i].y,
-                                                                     dz[i][2] * af[i].z, dz[i][3] * af[i].w);
     }
   }
 }
@@ -3490,14 +3499,24 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
   const int Yc = (g.ny + nchunk - 1) / nchunk;
   nchunk = (g.ny + Yc - 1) / Yc;
   if (separable && zfast && family != PSF_TILE && psf_adj_col_ok(g, t.ntx)) {
-    const size_t base = mf_col_lds_floats(g, t.ntx, 0), per = MfTaps(t.ntx).size;
-    const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), ((long)(kMaxLds / sizeof(float)) - (long)base) /
-                                                                         (long)per));
-    const size_t lds = sizeof(float) * mf_col_lds_floats(g, t.ntx, tbp);
+    const long per = MfTaps(t.ntx).size;
+    // tap planes that fit beside the view and nwv wave buffers
+    auto room = [&](int nwv) {
+      return ((long)(kMaxLds / sizeof(float)) - (long)mf_col_lds_floats(g, t.ntx, 0, nwv)) / per;
+    };
+    const int nwv = room(12) >= std::min(4, Yc) ? 12 : MF_WAVES;
+    const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), room(nwv)));
+    const size_t lds = sizeof(float) * mf_col_lds_floats(g, t.ntx, tbp, nwv);
     SR_TPW_SWITCH(g.nz / 16, {
-      allow_lds(k_psf_adj_col<TPW>, lds);
-      k_psf_adj_col<TPW><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride, g,
-                                                                   tbp);
+      if (nwv == 12) {
+        allow_lds(k_psf_adj_col<TPW, 12>, lds);
+        k_psf_adj_col<TPW, 12><<<dim3(nchunk, nb), 64 * 12, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride,
+                                                                      g, tbp);
+      } else {
+        allow_lds(k_psf_adj_col<TPW, MF_WAVES>, lds);
+        k_psf_adj_col<TPW, MF_WAVES><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(views, t, vids, Yc, work, atten,
+                                                                               atten_vstride, g, tbp);
+      }
     });
     return hipGetLastError();
   }
