@@ -2911,7 +2911,7 @@ __global__ void __launch_bounds__(256) k_atten_z(float* __restrict__ mur, Geo g,
 // Adjoint of k_rot2_fwd_z as a gather: target pixel j collects w * work[b][i][:]
 // over the K (source i, weight w) slots of its plan-time list (the forward's
 // coefficients, w = the product of its two 1-D weights; unused slots carry
-// w = 0 and i = j), ent laid out [view][K][NN] so that neighbouring targets
+// w = 0 and i = j and load nothing), ent laid out [view][K][NN] so that neighbouring targets
 // read neighbouring slots.  A lane takes Q adjacent z quads of one pixel
 // (nz/(4Q) lanes per pixel, PW = 64/(nz/(4Q)) pixels per wave).  Wave s of a
 // block sums views s, s + VS, ... of the batch for the block's PW pixels (one
@@ -2965,8 +2965,9 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
       float4 pa[K][Q];
 #pragma unroll
       for (int k = 0; k < K; ++k)
+        if (__int_as_float(sa[k].y) != 0.f)  // unused slots load nothing
 #pragma unroll
-        for (int q = 0; q < Q; ++q) pa[k][q] = *reinterpret_cast<const float4*>(wa + (long)sa[k].x * nz + 4 * q);
+          for (int q = 0; q < Q; ++q) pa[k][q] = *reinterpret_cast<const float4*>(wa + (long)sa[k].x * nz + 4 * q);
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const float w = __int_as_float(sa[k].y);
@@ -2981,12 +2982,13 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
       float4 pa[K][Q];
 #pragma unroll
       for (int k = 0; k < K; ++k)
+        if (__int_as_float(sa[k].y) != 0.f)
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          const float4 h = *reinterpret_cast<const float4*>(ha + (long)sa[k].x * nz + 4 * q);
-          const float4 fa = *reinterpret_cast<const float4*>(aa + (long)sa[k].x * nz + 4 * q);
-          pa[k][q] = make_float4(h.x * fa.x, h.y * fa.y, h.z * fa.z, h.w * fa.w);
-        }
+          for (int q = 0; q < Q; ++q) {
+            const float4 h = *reinterpret_cast<const float4*>(ha + (long)sa[k].x * nz + 4 * q);
+            const float4 fa = *reinterpret_cast<const float4*>(aa + (long)sa[k].x * nz + 4 * q);
+            pa[k][q] = make_float4(h.x * fa.x, h.y * fa.y, h.z * fa.z, h.w * fa.w);
+          }
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const float w = __int_as_float(sa[k].y);
