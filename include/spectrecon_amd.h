@@ -181,7 +181,7 @@ int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
  *   FWD_CHUNKS / ADJ_CHUNKS: plane chunks per view, in percent of the built-in
  *     count (one block per compute unit over the batch), 10..1000;
  *   GATHER_SLICES: view slices per block of the rotation-adjoint gather
- *     (2, 4, 8; 0 picks by the batch size).
+ *     (2, 4, 8; 0 picks by the launch's wave count and the batch size).
  * Defaults {0, 100, 0, 100, 0}.  Every choice computes the same operator;
  * the gather's slice count changes the order of its sums (last-bit
  * differences).  sr_plan_autotune times the candidates of each choice on

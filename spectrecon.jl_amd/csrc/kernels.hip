@@ -3680,13 +3680,15 @@ static hipError_t launch_ell(int mode, const float* work, float* part, const int
   const int q = (g.nz / 4) % 2 == 0 ? 2 : 1;  // z quads per lane
   const int pw = 64 / (g.nz / 4 / q);
   const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
-  // view slices (waves) per block: each wave sums every vs-th view
-  // (built-in: 2 for up to 4 views, 8 for 5-8 and above 24, 4 for 9-24, as
-  // measured on 8-, 15-, 16- and 30-view batches)
-  const int vs = (vs_req == 2 || vs_req == 4 || vs_req == 8) ? vs_req
-                 : nb <= 4                                    ? 2
-                 : nb <= 8 || nb > 24                         ? 8
-                                                              : 4;
+  // view slices (waves) per block: each wave sums every vs-th view.
+  // Built-in: the fewest slices (2, 4, 8) that give the launch 4096 waves,
+  // at most the batch's views rounded down to a power of two (measured on
+  // 4- to 64-view batches: 2 wins at 128x128 pixels, 8 at 64x64 from 8 views)
+  int vs = 2;
+  if (vs_req == 2 || vs_req == 4 || vs_req == 8)
+    vs = vs_req;
+  else
+    while (vs < 8 && 2 * vs <= nb && (long)kXcds * C * vs < 4096) vs *= 2;
   const int zi = zero_inf ? 1 : 0;
 #define SR_ELL(KK, VV, QQ)                                                                                            \
   if (K == KK && vs == VV && q == QQ) {                                                                             \
