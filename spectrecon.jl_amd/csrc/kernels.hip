@@ -1584,14 +1584,33 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
     auto rot_table = [&](int y) {  // plane y's rotation table, one pixel per producer thread
       if (pt < nx) rot_entry(rot, nx, nz, g.pad, pt, y, s_rt + 2 * nx * ((y - y0) & 1) + 2 * pt);
     };
-    auto load_taps = [&](int y) {  // plane y's six tap tables into tap plane (y - y0) & 1
+    // plane y's six tap tables: fetched into registers one plane ahead (their
+    // loads overlap the gathers instead of adding a second memory round trip
+    // per plane), then stored into tap plane (y - y0) & 1
+    constexpr int TE = (6 * 64 + PT - 1) / PT;  // (NT <= 64)
+    float tv[TE];
+    auto fetch_taps = [&](int y) {
       const long plane = (long)v * t.vstride + y;
+#pragma unroll
+      for (int j = 0; j < TE; ++j) {
+        const int e = pt + j * PT;
+        if (e < 6 * NT) {
+          const int tab = e / NT, tk = e - tab * NT;
+          const float* src =
+              tab == 0 ? t.u + plane * NT : tab == 1 ? t.w + plane * NT : t.cum + (plane * 4 + tab - 2) * NT;
+          tv[j] = src[tk];
+        }
+      }
+    };
+    auto put_taps = [&](int y) {
       float* dst = s_taps + ((y - y0) & 1) * TT.size + TT.off;
-      for (int e = pt; e < 6 * NT; e += PT) {
-        const int tab = e / NT, tk = e - tab * NT;
-        const float* src =
-            tab == 0 ? t.u + plane * NT : tab == 1 ? t.w + plane * NT : t.cum + (plane * 4 + tab - 2) * NT;
-        dst[tab * TT.tp + tk] = src[tk];
+#pragma unroll
+      for (int j = 0; j < TE; ++j) {
+        const int e = pt + j * PT;
+        if (e < 6 * NT) {
+          const int tab = e / NT, tk = e - tab * NT;
+          dst[tab * TT.tp + tk] = tv[j];
+        }
       }
     };
     auto stage = [&](int y, float* buf) {  // plane y into buf, RR rounds of WS_RE elements
@@ -1638,15 +1657,18 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
     }
     __syncthreads();  // zero fill and the first two rotation tables done
     if (y0 < y1) {
+      fetch_taps(y0);
       stage(y0, Pz);
-      load_taps(y0);
+      put_taps(y0);
+      if (y0 + 1 < y1) fetch_taps(y0 + 1);
     }
     // (the same barriers as the consumer loop below, one per plane)
     for (int y = y0; y < y1; ++y) {
       lds_barrier();  // plane y and its taps staged; plane y - 1's buffers free
       if (y + 1 < y1) {
+        put_taps(y + 1);
+        if (y + 2 < y1) fetch_taps(y + 2);
         stage(y + 1, Pz + (((y - y0) & 1) ^ 1) * PS);
-        load_taps(y + 1);
       }
       // plane y + 2's rotation table, into plane y's (read before the barrier above)
       if (y + 2 < y1) rot_table(y + 2);
