@@ -3577,11 +3577,17 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
     }));
   } else if (sizeof(float) * adj2d_lds_floats(g, t.ntx) <= kMaxLds) {
     const size_t lds = sizeof(float) * adj2d_lds_floats(g, t.ntx);
-    SR_NT_SWITCH(t.ntx, {
-      allow_lds(k_psf_adj_2d_strips<NT, 256>, lds);
-      k_psf_adj_2d_strips<NT, 256><<<dim3(nchunk, nb), 256, lds, s>>>(views, t, vids, Yc, work, atten,
-                                                                      atten_vstride, g);
-    });
+    // threads per block: about 16 waves per CU over the blocks that fit in
+    // its LDS (large planes leave room for one), not more than the strips
+    const long per_cu = std::max<long>(1, (long)(kMaxLds / lds));
+    const long strips = (long)((g.nx + g.px - 1 + 7) / 8) * (g.nz + g.pz - 1);
+    int block = per_cu >= 4 ? 256 : per_cu >= 2 ? 512 : 1024;
+    while (block > 256 && strips <= block / 2) block /= 2;
+    SR_NT_SWITCH(t.ntx, SR_BLOCK_SWITCH(block, {
+      allow_lds(k_psf_adj_2d_strips<NT, BLOCK>, lds);
+      k_psf_adj_2d_strips<NT, BLOCK><<<dim3(nchunk, nb), BLOCK, lds, s>>>(views, t, vids, Yc, work, atten,
+                                                                          atten_vstride, g);
+    }));
   } else {  // (planes too large for the staged view: the per-term loop)
     const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
     const size_t lds = sizeof(float) * (size_t)odd_up(g.nx + 2 * hx) * (g.nz + 2 * hz);
