@@ -178,6 +178,33 @@ def test_project_vs_oracle(gpu, O, method, mode, separable):
     assert maxrel(gbs, rbs) <= PROJ_TOL
 
 
+@pytest.mark.parametrize("nx,nz,p", [(64, 40, 19), (64, 64, 37)])
+def test_general_psf_large_planes(gpu, O, nx, nz, p):
+    """Non-separable PSFs on planes whose adjoint staging takes a large part
+    of LDS: 512-thread blocks (64x40, 19x19: three blocks fit a CU) and
+    1024-thread blocks (64x64, 37x37: one fits); parity with the direct FP32
+    oracle, the FP64 oracle and the adjoint identity."""
+    nview = 3
+    rng = np.random.default_rng(nx + nz + p)
+    mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = rand_sym_psfs(rng, p, p, nx, nview)
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    assert plan.info()["conv"] == "direct"
+    ang = angles32(nview).astype(np.float64)
+    P32 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float32, "direct")
+    P64 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float64, "direct")
+    image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    g = gpu.project(image, plan)
+    assert per_view_maxrel(g, P32.project(image)) <= PROJ_TOL
+    views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    gb = gpu.backproject(views, plan)
+    assert maxrel(gb, P32.backproject(views)) <= PROJ_TOL
+    assert maxrel(gb, P64.backproject(views.astype(np.float64))) <= 1e-5
+    a = float(np.vdot(views.astype(np.float64), g.astype(np.float64)))
+    b = float(np.vdot(image.astype(np.float64), gb.astype(np.float64)))
+    assert abs(a - b) <= 1e-5 * abs(a)
+
+
 @pytest.mark.parametrize("method", ["one", "two"])
 def test_adjoint_dot(gpu, method):
     """<y, Ax> = <A'y, x> at 32x32x20, 30 views, 5x3 PSF (test/adjoint-project.jl:52-107)."""
