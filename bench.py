@@ -63,40 +63,53 @@ def algorithmic(cls, nb, g):
         return 4 * 2 * P * nb, 0
     if cls == "img_reduce":  # the z-fastest image to the caller's layout
         return 4 * 2 * V, 0
+    if cls == "atten":  # running sums along y: read the rotated mu volume, write the factors
+        return 4 * 2 * V * nb, 0
     return 0, 0
 
 
-def cpu_baseline(image, mumap, psfs, budget_s):
-    """The oracle's Float32 restatement (FFT per plane, O(ny^2) attenuation,
-    views chunked over all host cores like spawner) on one OS-EM subset."""
+def cpu_baseline(image, mumap, psfs, x0, y, b, xhat):
+    """The CPU leg, with the oracle as the checker.
+
+    - Baseline: one whole OS-EM reconstruction (8 subsets x 4 iterations) by
+      the oracle's Float32 restatement of the reference (an FFT per plane, the
+      O(ny^2) attenuation, views chunked over the host cores like spawner),
+      timed on this host.
+    - Errors of the GPU's reconstruction `xhat` (from the same x0, y, b):
+      relative L2 against that Float32 reference-path result and against the
+      FP64 oracle's OS-EM."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     import spectrecon_amd as S
 
-    cores = os.cpu_count() or 1
-    nsub = NVIEW // NBLOCKS
+    cores = min(os.cpu_count() or 1, NVIEW // NBLOCKS)
     ang = S.default_viewangle(NVIEW).astype(np.float64)
-    P = O.Problem(mumap, psfs, DY, ang, "two", np.float32, "fft", nthread=min(cores, nsub))
-    # subset projection + back-projection, repeated for about budget_s / 3 (at least twice): median times
-    tps, tbs = [], []
-    t_start = time.perf_counter()
-    while len(tps) < 2 or time.perf_counter() - t_start < budget_s / 3:
-        idx = np.arange(len(tps) % NBLOCKS, NVIEW, NBLOCKS, dtype=np.int32)
-        t0 = time.perf_counter()
-        v = P.project(image, index=idx)
-        tps.append(time.perf_counter() - t0)
-        t0 = time.perf_counter()
-        P.backproject(v, index=idx)
-        tbs.append(time.perf_counter() - t0)
-    tp, tb = float(np.median(tps)), float(np.median(tbs))
-    # one reconstruction = 4 x 8 subset projections + 5 x 8 subset back-projections
-    t_recon = NITER * NBLOCKS * tp + (NITER + 1) * NBLOCKS * tb
-    return {"value": 1.0 / t_recon, "unit": "recons/s", "cores": min(cores, nsub), "kind": "port",
-            "sample": (f"{len(tps)} x (project + backproject of one {nsub}-view subset) at {NX}x{NX}x{NZ}, "
-                       f"{PSIZE}x{PSIZE} PSF, median {tp:.2f} s + {tb:.2f} s, extrapolated to 32 subset "
-                       f"projections + 40 subset back-projections per reconstruction"),
+    P32 = O.Problem(mumap, psfs, DY, ang, "two", np.float32, "fft", nthread=cores)
+    t0 = time.perf_counter()
+    x32 = P32.osem(x0, y, b, NBLOCKS, NITER)
+    t_recon = time.perf_counter() - t0
+    log(f"cpu baseline: one OS-EM reconstruction in {t_recon:.1f} s on {cores} cores")
+    P64 = O.Problem(mumap, psfs, DY, ang, "two", np.float64, "fft", nthread=cores)
+    t0 = time.perf_counter()
+    x64 = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), NBLOCKS, NITER)
+    log(f"fp64 oracle: {time.perf_counter() - t0:.1f} s")
+
+    def rel(a, r):
+        a, r = np.asarray(a, np.float64), np.asarray(r, np.float64)
+        return float(np.linalg.norm(a - r) / np.linalg.norm(r))
+
+    errors = {"gpu_vs_fp32_reference_path_rel_l2": rel(xhat, x32), "gpu_vs_fp64_oracle_rel_l2": rel(xhat, x64),
+              "fp32_reference_path_vs_fp64_oracle_rel_l2": rel(x32, x64),
+              "gpu_vs_fp64_oracle_max_rel": float(np.abs(np.asarray(xhat, np.float64) - x64).max()
+                                                  / np.abs(x64).max()),
+              "note": "the timed reconstruction's output; the FP32 reference path is the oracle's literal "
+                      "restatement (FFT per plane), the FP64 oracle the same algorithm in double"}
+    base = {"value": 1.0 / t_recon, "unit": "recons/s", "cores": cores, "kind": "port",
+            "sample": (f"one OS-EM reconstruction (8 subsets x 4 iterations) at {NX}x{NX}x{NZ}, {NVIEW} views, "
+                       f"{PSIZE}x{PSIZE} PSF by the oracle's Float32 restatement, {t_recon:.1f} s"),
             "calibration": calibration(O, S, cores)}
+    return base, errors
 
 
 def calibration(O, S, cores):
@@ -335,7 +348,6 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel choices")
     ap.add_argument("--choices", default=None,
                     help="pin the kernel choices (five comma-separated SR_CHOICE_* values) instead of autotuning")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the benchmark, copy this many MiB 3 times with the library's copy kernel "
                          "(the known-size reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -477,10 +489,30 @@ def main():
     kernels = {k: {"total_ms": round(v[0], 4), "launches": v[1],
                    "avg_ms": round(v[0] / max(v[1], 1), 5)} for k, v in breakdown.items()}
 
-    cpu = None
+    # HBM (or FP32) fraction of every kernel class of a reconstruction, from
+    # the event-timed one, plus the attenuation scan (plan-time work, timed
+    # over a rebuild of the plan's factors: 4 launches of 32 views)
+    plan.set_timing(True, classes=["atten"])
+    plan.set_mumap(mumap)
+    at_ms, at_n = plan.timing()["atten"]
+    plan.set_timing(False)
+    per_kernel = {}
+    for cls, (ms, n) in list(breakdown.items()) + [("atten", (at_ms, at_n))]:
+        if n == 0:
+            continue
+        nbv = 32 if cls == "atten" else nsub
+        byk, flk = algorithmic(cls, nbv, g)
+        avg = ms / 1e3 / n
+        ent = {"avg_launch_ms": 1e3 * avg, "launches": n, "views_per_launch": nbv if cls != "img_reduce" else None,
+               "algorithmic_bytes": byk, "hbm_GBs": byk / avg / 1e9, "hbm_frac": byk / avg / 1e9 / HBM_PEAK_GBS}
+        if flk:
+            ent.update({"fp32_TFLOPs": flk / avg / 1e12, "fp32_frac": flk / avg / 1e12 / FP32_PEAK_TFLOPS})
+        per_kernel[cls] = ent
+
+    cpu = errors = None
     if not args.no_cpu_baseline:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(image, mumap, psfs, args.cpu_budget)
+        log("cpu baseline and oracle errors ...")
+        cpu, errors = cpu_baseline(image, mumap, psfs, np.ones(image.shape, np.float32), y, b, xhat)
 
     if args.copy_mib > 0:
         n = args.copy_mib << 20
@@ -519,6 +551,8 @@ def main():
         "autotune": tune,
         "osem_host_ms": float(np.median(host_ms)),
         "kernels": kernels,
+        "kernel_roofline": per_kernel,
+        "errors": errors,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

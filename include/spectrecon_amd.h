@@ -89,6 +89,14 @@ int sr_device_synchronize(void);
 int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* psfs, const float* viewangle,
                    sr_plan* plan);
 int sr_plan_destroy(sr_plan plan);
+/* Replaces the plan's mu-map (SPECTplan's mumap, src/spectplan.jl:71) and
+ * rebuilds every view's attenuation factors in place, for a geometry, PSF
+ * and angles that stay (a serving loop whose requests bring their own
+ * mu-maps keeps one plan per geometry).  The factors are
+ * exp(-dy (S_y - mu_r[y]/2)) with S_y the running sum of the rotated mu-map
+ * along y (src/project.jl:103-113 sums each y anew: the same values up to
+ * rounding).  `mumap` [nx,ny,nz] is a host (SR_MEM_HOST) or device pointer. */
+int sr_plan_set_mumap(sr_plan plan, const float* mumap, int32_t mem);
 /* resident device bytes and the PSF method the plan chose (SR_CONV_*) */
 int sr_plan_info(sr_plan plan, size_t* device_bytes, int32_t* conv_method, int32_t* max_batch);
 /* Device layout of the plan's work volumes, attenuation factors and partial
@@ -110,17 +118,19 @@ int sr_plan_synchronize(sr_plan plan);
  * chosen classes is bracketed by events; sr_plan_get_timing returns the
  * summed device time and launch count of one kernel class since timing was
  * (re)enabled: 0 rotate, 1 PSF forward, 2 view reduce, 3 PSF adjoint,
- * 4 rotate adjoint, 5 image reduce / EM update.  `enable`: 0 off, 1 all
+ * 4 rotate adjoint, 5 image reduce / EM update, 6 attenuation factors
+ * (sr_plan_set_mumap).  `enable`: 0 off, 1 all
  * classes, otherwise an or of SR_TIME_CLASS(SR_K_*) bits (bracketing only
  * the classes of interest keeps the event overhead off the others). */
 #define SR_TIME_CLASS(k) (2 << (k))
-#define SR_TIME_ALL 0x7e
+#define SR_TIME_ALL 0xfe
 #define SR_K_ROT_FWD 0
 #define SR_K_PSF_FWD 1
 #define SR_K_VIEW_REDUCE 2
 #define SR_K_PSF_ADJ 3
 #define SR_K_ROT_ADJ 4
 #define SR_K_IMG_REDUCE 5
+#define SR_K_ATTEN 6
 int sr_plan_set_timing(sr_plan plan, int32_t enable);
 int sr_plan_get_timing(sr_plan plan, int32_t kernel, double* total_ms, int64_t* launches);
 
@@ -128,7 +138,7 @@ int sr_plan_get_timing(sr_plan plan, int32_t kernel, double* total_ms, int64_t* 
  * sr_project replaces project!(views, image, plan; index) (src/project.jl:143-163):
  *   views[:,:,ii] = A_{index[ii]} image, ii = 0..nindex-1.
  * sr_backproject replaces backproject!(image, views, plan; index)
- * (src/backproject.jl:340-370): image = sum_ii A_{index[ii]}' views[:,:,ii]
+ * (src/backproject.jl:132-162): image = sum_ii A_{index[ii]}' views[:,:,ii]
  * (image is overwritten, as the reference zeroes it).
  * index == NULL means all views 0..nview-1 (nindex must then be nview). */
 int sr_project(sr_plan plan, float* views, const float* image, const int32_t* index, int32_t nindex, int32_t mem);

@@ -545,7 +545,7 @@ void project_view(const Problem<T>& P, ViewWork<T>& W, T* view, const T* image, 
       for (long x = 0; x < nx; ++x) view[x + nx * z] += W.add_img[x + nx * y + sxy * z];
 }
 
-// src/backproject.jl:277-332 (backproject! for one view, accumulating)
+// src/backproject.jl:69-124 (backproject! for one view, accumulating)
 template <class T>
 void backproject_view(const Problem<T>& P, ViewWork<T>& W, T* image, const T* view, int viewidx) {
   const int nx = P.nx, ny = P.ny, nz = P.nz;
@@ -641,7 +641,7 @@ void project(const Problem<T>& P, T* views, const T* image, const int* index, in
   for (auto* w : W) delete w;
 }
 
-// src/backproject.jl:340-370 (:fast): zero image, per-buffer accumulation,
+// src/backproject.jl:132-162 (:fast): zero image, per-buffer accumulation,
 // then image += add_img[b] in buffer order.
 template <class T>
 void backproject(const Problem<T>& P, T* image, const T* views, const int* index, int nindex, int nthread) {

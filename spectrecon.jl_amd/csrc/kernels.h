@@ -60,8 +60,10 @@ hipError_t launch_img_reduce(const float* part, int ngroups, float* out, const f
                              hipStream_t s);
 
 // attenuation factors, in place over the rotated mu volumes of the batch:
-// a[b][z][y][x] = exp(-dy * (sum_{j<=y} mu_r[j] - mu_r[y]/2)) in reference order
-hipError_t launch_atten_exact(float* mur, int nb, const Geo& g, float dy, hipStream_t s);
+// a = exp(-dy * (sum_{j<=y} mu_r[j] - mu_r[y]/2)) from running sums along y
+// (O(ny) per column, a wave-level segmented scan when the batch has few
+// columns), over [b][y][x][z] volumes (zfast) or [b][z][y][x]
+hipError_t launch_atten_scan(float* mur, int nb, const Geo& g, float dy, bool zfast, hipStream_t s);
 
 // ---- PSF ----------------------------------------------------------------
 // per-chunk partial views of the batch from the work volume (rotated and
@@ -114,8 +116,6 @@ hipError_t launch_to_zfast(const float* img, float* out, const Geo& g, hipStream
 // out[b] = atten[v] .* rot2(img_t) per view, all [y][x][z]; atten may be null
 hipError_t launch_rot2_fwd_z(const float* img_t, float* out, const float* atten, long atten_vstride,
                              const RotView* rv, const int* vids, int nb, const Geo& g, hipStream_t s);
-// rotated mu volumes [b][y][x][z] -> attenuation factors, in place
-hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_t s);
 // part [y][x][z] = sum over the batch's views of rot2^T(work[b]) through the
 // K-slot per-target lists (vs: views slices per block, 2, 4 or 8; 0 picks
 // by the batch size); with atten: of rot2^T(work .* atten[v]), work one
@@ -158,7 +158,12 @@ bool psf_mfma_supported(const Geo& g, int nt);
 // blocks of the :two rotation adjoint that one CU holds at once (LDS-bound)
 int rot2_adj_blocks_per_cu(const Geo& g);
 
+// sets the calling thread's sr_last_error() message and returns `code`
+int record_error(int code, const char* msg);
+
 // kernel classes for per-kernel event timing
-enum KernelClass { K_ROT_FWD = 0, K_PSF_FWD, K_VIEW_REDUCE, K_PSF_ADJ, K_ROT_ADJ, K_IMG_REDUCE, K_OTHER, K_NCLASS };
+enum KernelClass {
+  K_ROT_FWD = 0, K_PSF_FWD, K_VIEW_REDUCE, K_PSF_ADJ, K_ROT_ADJ, K_IMG_REDUCE, K_ATTEN, K_OTHER, K_NCLASS
+};
 
 }  // namespace sr

@@ -623,26 +623,54 @@ __global__ void __launch_bounds__(R1_THREADS) k_rot1_adj(const float* __restrict
 #undef R1_FOR_EACH
 
 // ===========================================================================
-// Attenuation factors in the reference's summation order
+// Attenuation factors: running sums along y
 // ===========================================================================
-// grid (nz, nb), block = N threads (one per x column).  In place: mur[b] holds
-// the rotated mu volume on entry and the factors on exit.
-// src/project.jl:103-113: e = -0.5*mu_r[y]; e += mu_r[j] for j = 1..y; e *= -dy; exp.
-__global__ void __launch_bounds__(1024) k_atten_exact(float* __restrict__ mur, Geo g, float dy) {
-  extern __shared__ float s[];
-  const int N = g.nx, z = blockIdx.x, b = blockIdx.y;
-  const long S = (long)N * N, V = S * g.nz;
-  float* p = mur + V * b + S * z;
-  for (long k = threadIdx.x; k < S; k += blockDim.x) s[k] = p[k];
-  __syncthreads();
+// In place over the rotated mu volumes of a batch, per column (x, z):
+//   a(y) = exp(-dy (S_y - mu_y / 2)),  S_y = sum_{j<=y} mu_j.
+// src/project.jl:103-113 forms each y's sum anew (-mu_y/2 + mu_1 + ... + mu_y,
+// O(ny^2) per column); here S_y is carried from y - 1 (O(ny)), which moves the
+// result by rounding only (the sum is associated differently).
+// SEG lanes of a wave share a column: lane (seg, col) holds y in
+// [seg L, (seg + 1) L), L = ceil(ny / SEG), with W = 64 / SEG columns per
+// wave and the W lanes of one segment on adjacent columns (coalesced).  With
+// SEG > 1 the segment sums are scanned across the column's lanes with wave
+// shuffles and each lane then writes its segment from its prefix; SEG = 1 is
+// one streaming pass.  zfast: volumes [b][y][x][z] (column c = x nz + z, y
+// stride nx nz); else [b][z][y][x] (c = z nx + x, y stride nx).
+// grid (ceil(cols / (4 W)), nb), block 256.
+template <int SEG>
+__global__ void __launch_bounds__(256) k_atten_scan(float* __restrict__ mur, Geo g, float dy, int zfast) {
+  constexpr int W = 64 / SEG;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int seg = lane / W;
+  const long cols = (long)g.nx * g.nz;
+  const long c = ((long)blockIdx.x * 4 + wave) * W + lane % W;
+  const int ny = g.ny, L = (ny + SEG - 1) / SEG;
+  const int y0 = min(ny, seg * L), y1 = min(ny, y0 + L);
+  const bool ok = c < cols;
+  const long base = zfast ? c : (c / g.nx) * (long)g.nx * ny + c % g.nx;
+  const long ys = zfast ? cols : (long)g.nx;
+  float* p = mur + (long)blockIdx.y * cols * ny + (ok ? base : 0);
   const float mdy = -dy;
-  for (int x = threadIdx.x; x < N; x += blockDim.x) {
-    for (int y = 0; y < N; ++y) {
-      float e = -0.5f * s[x + N * y];
-      for (int j = 0; j <= y; ++j) e = e + s[x + N * j];
-      e = e * mdy;
-      p[x + (long)N * y] = expf(e);
+  float s = 0.f;
+  if (SEG > 1) {
+    float t = 0.f;
+    if (ok)
+      for (int y = y0; y < y1; ++y) t = t + p[y * ys];
+    // inclusive scan over the column's segments (lanes col, col + W, ...)
+#pragma unroll
+    for (int d = W; d < 64; d <<= 1) {
+      const float v = __shfl_up(t, d, 64);
+      if (lane >= d) t = t + v;
     }
+    s = __shfl_up(t, W, 64);  // the sum of the earlier segments
+    if (seg == 0) s = 0.f;
+  }
+  if (!ok) return;
+  for (int y = y0; y < y1; ++y) {
+    const float m = p[y * ys];
+    s = s + m;
+    p[y * ys] = expf((s + (-0.5f * m)) * mdy);
   }
 }
 
@@ -2129,7 +2157,7 @@ __device__ __forceinline__ float adj_edge_hi(const float* in, int n, int h, cons
 // Qp = the z-adjoint of the view with zero columns around it (stride WQ, data
 // at column offset hx), the taps and their tail/head sums.  The x-pass strip
 // (z, x0..x0+SL-1) is written straight to the work volume, times the
-// attenuation factor (mul3dj!, src/backproject.jl:316).
+// attenuation factor (mul3dj!, src/backproject.jl:108).
 template <int NT, int SL>
 __global__ void __launch_bounds__(kSepThreads)
 k_psf_adj_sep(const float* __restrict__ views, Taps t, const int* __restrict__ vids, int Yc, float* __restrict__ work,
@@ -2908,28 +2936,6 @@ __global__ void __launch_bounds__(256) k_rot2_fwd_z(const float* __restrict__ im
   *reinterpret_cast<float4*>(out + V * b + o) = val;
 }
 
-// Attenuation factors in place on rotated mu volumes [b][y][x][z], the same
-// summation order as k_atten_exact (src/project.jl:103-113).  Each thread owns
-// a (x, z) column, staged in LDS.  grid (ceil(N*nz / 256), nb), block 256,
-// LDS ny * 256 floats.
-__global__ void __launch_bounds__(256) k_atten_z(float* __restrict__ mur, Geo g, float dy) {
-  extern __shared__ float sc[];
-  const int N = g.nx, ny = g.ny, nz = g.nz, tid = threadIdx.x;
-  const long cols = (long)N * nz, V = cols * ny;
-  const long c = (long)blockIdx.x * 256 + tid;
-  float* p = mur + V * blockIdx.y;
-  const bool ok = c < cols;
-  for (int y = 0; y < ny; ++y) sc[y * 256 + tid] = ok ? p[y * cols + c] : 0.f;
-  const float mdy = -dy;
-  if (ok)
-    for (int y = 0; y < ny; ++y) {
-      float e = -0.5f * sc[y * 256 + tid];
-      for (int j = 0; j <= y; ++j) e = e + sc[j * 256 + tid];
-      e = e * mdy;
-      p[y * cols + c] = expf(e);
-    }
-}
-
 // Adjoint of k_rot2_fwd_z as a gather: target pixel j collects w * work[b][i][:]
 // over the K (source i, weight w) slots of its plan-time list (the forward's
 // coefficients, w = the product of its two 1-D weights; unused slots carry
@@ -3323,15 +3329,28 @@ hipError_t launch_img_reduce(const float* part, int ngroups, float* out, const f
   return hipGetLastError();
 }
 
-hipError_t launch_atten_exact(float* mur, int nb, const Geo& g, float dy, hipStream_t s) {
+hipError_t launch_atten_scan(float* mur, int nb, const Geo& g, float dy, bool zfast, hipStream_t s) {
   if (nb <= 0) return hipSuccess;
-  const int threads = std::min(1024, (int)(cdiv(g.nx, 64) * 64));
-  const size_t lds = sizeof(float) * g.nx * g.nx;
-  if (lds > kMaxLds) return hipErrorInvalidConfiguration;
-  allow_lds(k_atten_exact, lds);
-  k_atten_exact<<<dim3(g.nz, nb), threads, lds, s>>>(mur, g, dy);
+  const long cols = (long)g.nx * g.nz;
+  // lanes per column: one (a streaming pass) while the batch's columns give
+  // a launch about 256K lanes, else the power of two that does, within ny
+  int seg = 1;
+  while (seg < 64 && 2 * seg <= g.ny && cols * nb * seg < (256L << 10)) seg *= 2;
+  const int zf = zfast ? 1 : 0;
+  switch (seg) {
+#define SR_ATTEN_CASE(S) \
+  case S:                \
+    k_atten_scan<S><<<dim3(cdiv(cols, 4L * (64 / S)), nb), 256, 0, s>>>(mur, g, dy, zf); \
+    break;
+    SR_ATTEN_CASE(1) SR_ATTEN_CASE(2) SR_ATTEN_CASE(4) SR_ATTEN_CASE(8) SR_ATTEN_CASE(16) SR_ATTEN_CASE(32)
+    SR_ATTEN_CASE(64)
+#undef SR_ATTEN_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
+
 
 static int pick_block(long strips) {
   if (strips <= (long)PSF_MAXS * 256) return 256;
@@ -3692,14 +3711,6 @@ hipError_t launch_rot2_fwd_z(const float* img_t, float* out, const float* atten,
   return hipGetLastError();
 }
 
-hipError_t launch_atten_z(float* mur, int nb, const Geo& g, float dy, hipStream_t s) {
-  if (nb <= 0) return hipSuccess;
-  const size_t lds = sizeof(float) * g.ny * 256;
-  if (lds > kMaxLds) return hipErrorInvalidConfiguration;
-  allow_lds(k_atten_z, lds);
-  k_atten_z<<<dim3(cdiv((long)g.nx * g.nz, 256), nb), 256, lds, s>>>(mur, g, dy);
-  return hipGetLastError();
-}
 
 static hipError_t launch_ell(int mode, const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
                              const Geo& g, hipStream_t s, const float* hvol, const float* atten, long atten_vstride,

@@ -22,6 +22,7 @@
 
 #include "../../include/spectrecon_amd.h"
 #include "kernels.h"
+#include "taps.h"
 
 namespace {
 
@@ -138,51 +139,28 @@ void rot_params(double theta, int interp, int Np, sr::RotView& rv, int* tb, floa
     }
 }
 
-// Per plane tail/head sums of the separable taps (for the adjoint's
-// replicate-pad fold): [plane][u tail, u head, w tail, w head][nt], in double.
+// the tap tables as vectors (taps.h works on caller-provided arrays)
 std::vector<float> cum_sums(const std::vector<float>& hu, const std::vector<float>& hw, int nplane, int nt) {
   std::vector<float> c((size_t)nplane * 4 * nt);
-  for (int pl = 0; pl < nplane; ++pl)
-    for (int which = 0; which < 2; ++which) {
-      const float* t = (which == 0 ? hu : hw).data() + (size_t)pl * nt;
-      float* tail = c.data() + ((size_t)pl * 4 + 2 * which) * nt;
-      float* head = tail + nt;
-      double acc = 0;
-      for (int k = nt - 1; k >= 0; --k) tail[k] = (float)(acc += t[k]);
-      acc = 0;
-      for (int k = 0; k < nt; ++k) head[k] = (float)(acc += t[k]);
-    }
+  sr::cum_sums(hu.data(), hw.data(), nplane, nt, c.data());
   return c;
 }
 
-int round_taps(int p) { return p <= 8 ? 8 : p <= 16 ? 16 : p <= 24 ? 24 : p <= 32 ? 32 : p <= 40 ? 40 : p <= 48 ? 48 : 64; }
-
-// Rank-1 factorisation h[a,b] ~ u[a] w[b] of a px x pz plane, accepted when
-// every entry agrees to 1e-6 of max|h| (psf_gauss planes are outer products
-// rounded to Float32, src/psf-gauss.jl:40).
 bool factor_rank1(const float* h, int px, int pz, std::vector<double>& u, std::vector<double>& w) {
-  int ia = 0, ib = 0;
-  double hm = 0;
-  for (int b = 0; b < pz; ++b)
-    for (int a = 0; a < px; ++a)
-      if (std::fabs(h[a + px * b]) > hm) {
-        hm = std::fabs(h[a + px * b]);
-        ia = a;
-        ib = b;
-      }
   u.assign(px, 0.0);
   w.assign(pz, 0.0);
-  if (hm == 0) return true;
-  const double piv = h[ia + px * ib];
-  for (int a = 0; a < px; ++a) u[a] = h[a + px * ib];
-  for (int b = 0; b < pz; ++b) w[b] = h[ia + px * b] / piv;
-  for (int b = 0; b < pz; ++b)
-    for (int a = 0; a < px; ++a)
-      if (std::fabs(u[a] * w[b] - (double)h[a + px * b]) > 1e-6 * hm) return false;
-  return true;
+  return sr::factor_rank1(h, px, pz, u.data(), w.data());
 }
 
+using sr::round_taps;
+
 }  // namespace
+
+// the calling thread's error message, for the other translation units
+int sr::record_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
 
 // ===========================================================================
 // Plan
@@ -510,11 +488,47 @@ int finish(sr_plan p, int32_t mem) {
   return SR_OK;
 }
 
-// Host-pointer staging: copies `bytes` from host to a device buffer.
-int stage_in(sr_plan p, DBuf& buf, const void* host, size_t bytes) {
+// Host-pointer staging: copies `bytes` from host to a device buffer of at
+// least max(bytes, min_bytes) bytes (an empty view list still gets a buffer
+// the kernels may address, and nothing is read from the caller's array).
+int stage_in(sr_plan p, DBuf& buf, const void* host, size_t bytes, size_t min_bytes = 0) {
   int rc;
-  if ((rc = buf.ensure(bytes))) return rc;
-  HIP_TRY(hipMemcpyAsync(buf.p, host, bytes, hipMemcpyHostToDevice, p->stream));
+  if ((rc = buf.ensure(std::max(bytes, min_bytes)))) return rc;
+  if (bytes) HIP_TRY(hipMemcpyAsync(buf.p, host, bytes, hipMemcpyHostToDevice, p->stream));
+  return SR_OK;
+}
+
+// The attenuation factors of every view from the plan's mu-map (device):
+// each view's rotated mu volume, then the running sums along y and exp, in
+// batches of 32 views on the plan's stream.
+int build_atten(sr_plan p) {
+  const sr_plan_desc& d = p->d;
+  std::vector<int> all(d.nview);
+  for (int v = 0; v < d.nview; ++v) all[v] = v;
+  int rc;
+  if ((rc = p->vids.ensure(sizeof(int) * d.nview))) return rc;
+  HIP_TRY(hipMemcpyAsync(p->vids.p, all.data(), sizeof(int) * d.nview, hipMemcpyHostToDevice, p->stream));
+  // (the copy reads `all` on the host: it lands before the stream moves on)
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  const int chunk = 32;
+  for (int v0 = 0; v0 < d.nview; v0 += chunk) {
+    const int nb = std::min(chunk, d.nview - v0);
+    float* dst = p->atten.as<float>() + p->V * v0;
+    {
+      KTimer kt(p, sr::K_ROT_FWD);
+      if (p->zfast) {  // [v][y][x][z]
+        if (v0 == 0) HIP_TRY(sr::launch_to_zfast(p->mumap.as<float>(), p->zimg.as<float>(), p->g, p->stream));
+        HIP_TRY(sr::launch_rot2_fwd_z(p->zimg.as<float>(), dst, nullptr, 0, p->rotv.as<sr::RotView>(),
+                                      p->vids.as<int>() + v0, nb, p->g, p->stream));
+      } else {
+        HIP_TRY(sr::launch_rot_fwd(d.interpmeth, p->mumap.as<float>(), dst, nullptr, 0, p->rotv.as<sr::RotView>(),
+                                   p->vids.as<int>() + v0, nb, p->tab_b.as<int>(), p->tab_w.as<float>(), p->g,
+                                   p->stream));
+      }
+    }
+    KTimer kt(p, sr::K_ATTEN);
+    HIP_TRY(sr::launch_atten_scan(dst, nb, p->g, d.dy, p->zfast, p->stream));
+  }
   return SR_OK;
 }
 
@@ -720,26 +734,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   if ((rc = p->mumap.ensure(sizeof(float) * p->V))) return bail(rc);
   HIP_TRY(hipMemcpy(p->mumap.p, mumap, sizeof(float) * p->V, hipMemcpyHostToDevice));
   if ((rc = p->atten.ensure(sizeof(float) * p->V * d.nview))) return bail(rc);
-  std::vector<int> all(d.nview);
-  for (int v = 0; v < d.nview; ++v) all[v] = v;
-  if ((rc = p->vids.ensure(sizeof(int) * d.nview))) return bail(rc);
-  HIP_TRY(hipMemcpy(p->vids.p, all.data(), sizeof(int) * d.nview, hipMemcpyHostToDevice));
-  const int chunk = 32;
-  for (int v0 = 0; v0 < d.nview; v0 += chunk) {
-    const int nb = std::min(chunk, d.nview - v0);
-    float* dst = p->atten.as<float>() + p->V * v0;
-    if (p->zfast) {  // [v][y][x][z]
-      if (v0 == 0) HIP_TRY(sr::launch_to_zfast(p->mumap.as<float>(), p->zimg.as<float>(), p->g, p->stream));
-      HIP_TRY(sr::launch_rot2_fwd_z(p->zimg.as<float>(), dst, nullptr, 0, p->rotv.as<sr::RotView>(),
-                                    p->vids.as<int>() + v0, nb, p->g, p->stream));
-      HIP_TRY(sr::launch_atten_z(dst, nb, p->g, d.dy, p->stream));
-      continue;
-    }
-    HIP_TRY(sr::launch_rot_fwd(d.interpmeth, p->mumap.as<float>(), dst, nullptr, 0, p->rotv.as<sr::RotView>(),
-                               p->vids.as<int>() + v0, nb, p->tab_b.as<int>(), p->tab_w.as<float>(), p->g,
-                               p->stream));
-    HIP_TRY(sr::launch_atten_exact(dst, nb, p->g, d.dy, p->stream));
-  }
+  if ((rc = build_atten(p))) return bail(rc);
   HIP_TRY(hipStreamSynchronize(p->stream));
   p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->taps_c.n + p->rotv.n + p->tab_b.n +
              p->tab_w.n + p->zimg.n + p->zent.n;  // (zh is a per-call buffer)
@@ -758,6 +753,20 @@ int sr_plan_destroy(sr_plan p) {
   }
   delete p;
   return SR_OK;
+}
+
+// A new mu-map for an existing plan: the attenuation factors of every view
+// are rebuilt in place (SPECTplan's mu-dependent state, src/spectplan.jl:71,
+// src/project.jl:77-116); the rotation tables, PSF taps and lists stay.
+int sr_plan_set_mumap(sr_plan p, const float* mumap, int32_t mem) {
+  if (!p || !mumap) return fail(SR_ERR_ARG, "null argument");
+  const bool host = (mem & 0xff) == SR_MEM_HOST;
+  HIP_TRY(hipMemcpyAsync(p->mumap.p, mumap, sizeof(float) * p->V,
+                         host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, p->stream));
+  if (host) HIP_TRY(hipStreamSynchronize(p->stream));  // (the caller may reuse its array at once)
+  int rc;
+  if ((rc = build_atten(p))) return rc;
+  return finish(p, host ? (mem & ~SR_ASYNC) : mem);
 }
 
 int sr_plan_set_timing(sr_plan p, int32_t enable) {
@@ -839,12 +848,17 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
     HIP_TRY(sr::launch_fill(img, 1.f, p->V, p->stream));
     HIP_TRY(sr::launch_fill(p->em_ratio.as<float>(), 1.f, p->P * nb, p->stream));
     HIP_TRY(sr::launch_fill(p->work.as<float>(), 1.f, p->V * nb, p->stream));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIP_TRY(hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) {
-      (void)hipEventDestroy(e0);
-      return fail(SR_ERR_HIP, "hipEventCreate failed");
-    }
+    // the two timing events, destroyed on every return path
+    struct EventPair {
+      hipEvent_t a = nullptr, b = nullptr;
+      ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+      }
+    } ev;
+    HIP_TRY(hipEventCreate(&ev.a));
+    HIP_TRY(hipEventCreate(&ev.b));
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     const sr::Taps t = p->taps();
     // microseconds per launch: best of 5 rounds of 4 launches, after one
     // untimed launch; negative when the candidate fails to launch
@@ -984,8 +998,6 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
       appendf(out, "], \"choice\": {\"slices\": %d, \"us\": %.2f}}", bv, best);
     }
     out += "}";
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     HIP_TRY(hipStreamSynchronize(p->stream));
   }
   if (log && log_bytes) {
@@ -1060,7 +1072,7 @@ int sr_project(sr_plan p, float* views, const float* image, const int32_t* index
   return finish(p, mem);
 }
 
-// src/backproject.jl:340-370
+// src/backproject.jl:132-162
 int sr_backproject(sr_plan p, float* image, const float* views, const int32_t* index, int32_t nindex, int32_t mem) {
   if (!p || !views || !image) return fail(SR_ERR_ARG, "null argument");
   if (!index && nindex != p->d.nview) return fail(SR_ERR_DIM, "nindex must equal nview when index is NULL");
@@ -1070,7 +1082,7 @@ int sr_backproject(sr_plan p, float* image, const float* views, const int32_t* i
   float* d_img = image;
   const float* d_views = views;
   if (host) {
-    if ((rc = stage_in(p, p->st_views, views, sizeof(float) * p->P * std::max(1, nindex)))) return rc;
+    if ((rc = stage_in(p, p->st_views, views, sizeof(float) * p->P * nindex, sizeof(float) * p->P))) return rc;
     if ((rc = p->st_img.ensure(sizeof(float) * p->V))) return rc;
     d_img = p->st_img.as<float>();
     d_views = p->st_views.as<float>();
@@ -1104,7 +1116,8 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
   float *d_gx = grad_x, *d_gmu = grad_mu;
   if (host) {
     if ((rc = stage_in(p, p->st_x0, image, sizeof(float) * p->V)) ||
-        (rc = stage_in(p, p->st_aux1, y, sizeof(float) * PN)) || (rc = p->st_img.ensure(sizeof(float) * p->V)) ||
+        (rc = stage_in(p, p->st_aux1, y, sizeof(float) * p->P * nindex, sizeof(float) * PN)) ||
+        (rc = p->st_img.ensure(sizeof(float) * p->V)) ||
         (rc = p->st_aux2.ensure(sizeof(float) * p->V)))
       return rc;
     d_img = p->st_x0.as<float>();
@@ -1124,7 +1137,9 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
   // dL/dx = A' r
   if ((rc = run_backproject(p, d_gx, r, d_vids, nindex, sr::IMG_WRITE, nullptr))) return rc;
   if ((rc = run_mu_vjp(p, d_gmu, r, d_vids, nindex))) return rc;
-  HIP_TRY(hipMemcpyAsync(loss, p->gloss.p, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  // (no view: the residual kernel writes no loss, and the sum over no views is 0)
+  if (nindex > 0) HIP_TRY(hipMemcpyAsync(loss, p->gloss.p, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  else *loss = 0.0;
   if (host) {
     HIP_TRY(hipMemcpyAsync(grad_x, d_gx, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipMemcpyAsync(grad_mu, d_gmu, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
@@ -1152,7 +1167,8 @@ int sr_project_vjp_mu(sr_plan p, float* grad_mu, const float* image, const float
   float* d_gmu = grad_mu;
   if (host) {
     if ((rc = stage_in(p, p->st_x0, image, sizeof(float) * p->V)) ||
-        (rc = stage_in(p, p->st_aux1, dy, sizeof(float) * PN)) || (rc = p->st_aux2.ensure(sizeof(float) * p->V)))
+        (rc = stage_in(p, p->st_aux1, dy, sizeof(float) * p->P * nindex, sizeof(float) * PN)) ||
+        (rc = p->st_aux2.ensure(sizeof(float) * p->V)))
       return rc;
     d_img = p->st_x0.as<float>();
     d_dy = p->st_aux1.as<float>();

@@ -78,7 +78,7 @@ function project!(views::AbstractArray{Float32,3}, image::AbstractArray{Float32,
     return views
 end
 
-# backproject!(image, views, plan; index)  (src/backproject.jl:340-370)
+# backproject!(image, views, plan; index)  (src/backproject.jl:132-162)
 function backproject!(image::AbstractArray{Float32,3}, views::AbstractArray{Float32,3}, plan::SPECTplanAMD;
         index::AbstractVector{<:Int} = 1:plan.nview)
     ix = idx32(index)

@@ -18,7 +18,7 @@ SR_MEM_HOST, SR_MEM_DEVICE, SR_ASYNC = 0, 1, 0x100
 SR_INTERP_ONE, SR_INTERP_TWO = 1, 2
 SR_MODE_FAST, SR_MODE_MEM = 0, 1
 SR_CONV_AUTO, SR_CONV_DIRECT, SR_CONV_SEPARABLE = 0, 1, 2
-KERNEL_CLASSES = ("rot_fwd", "psf_fwd", "view_reduce", "psf_adj", "rot_adj", "img_reduce")
+KERNEL_CLASSES = ("rot_fwd", "psf_fwd", "view_reduce", "psf_adj", "rot_adj", "img_reduce", "atten")
 
 
 class SPECTError(Exception):
@@ -56,6 +56,7 @@ SIGNATURES = {
     "sr_device_synchronize": (C.c_int, []),
     "sr_plan_create": (C.c_int, [C.POINTER(PlanDesc), _vp, _vp, _vp, C.POINTER(C.c_void_p)]),
     "sr_plan_destroy": (C.c_int, [_vp]),
+    "sr_plan_set_mumap": (C.c_int, [_vp, _vp, C.c_int32]),
     "sr_plan_info": (C.c_int, [_vp, C.POINTER(C.c_size_t), _ip, _ip]),
     "sr_plan_layout": (C.c_int, [_vp, _ip]),
     "sr_grad": (C.c_int, [_vp, C.POINTER(C.c_double), _vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32]),

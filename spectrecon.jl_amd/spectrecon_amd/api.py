@@ -164,6 +164,21 @@ class SPECTplan:
         return {"device_bytes": nbytes.value, "conv": {1: "direct", 2: "separable"}[conv.value],
                 "max_batch": batch.value, "layout": {0: "x-fastest", 1: "z-fastest"}[layout.value]}
 
+    def set_mumap(self, mumap):
+        """A new attenuation map for this plan: every view's attenuation
+        factors are rebuilt on the device (geometry, PSFs and angles stay)."""
+        if isinstance(mumap, DeviceArray):
+            if tuple(mumap.shape) != self.imgsize:
+                raise DimensionMismatch("mumap size")
+            check(lib().sr_plan_set_mumap(self.handle, mumap.ptr, L.SR_MEM_DEVICE))
+            return self
+        mu = _f32(mumap)
+        if mu.shape != self.imgsize:
+            raise DimensionMismatch("mumap size")
+        check(lib().sr_plan_set_mumap(self.handle, mu.ctypes.data, L.SR_MEM_HOST))
+        self.mumap = mu
+        return self
+
     def set_batch(self, n):
         check(lib().sr_plan_set_batch(self.handle, n))
 
@@ -361,6 +376,7 @@ def osem_strided(x0, y, b, plan, nblocks, niter=16):
     nview; Ablock / osem keep the reference's throw).  Host arrays in and out;
     for device arrays call lib().sr_osem_strided with SR_MEM_DEVICE."""
     out = np.zeros(plan.imgsize, np.float32, order="F")
+    _check_plan_shapes(plan, out, x0, y, b)
     check(lib().sr_osem_strided(plan.handle, out.ctypes.data, _f32(x0).ctypes.data, _f32(y).ctypes.data,
                                 _f32(b).ctypes.data, int(nblocks), int(niter), L.SR_MEM_HOST))
     return out
@@ -478,17 +494,50 @@ def _check_em(out, x0, ynoisy, background):
         raise DimensionMismatch("size out and x0 not match")
 
 
+def _check_plan_shapes(plan, out, x0, ynoisy, background):
+    """Before a device-resident EM call: the library reads V floats of x0 and
+    P*nview floats of y and b, and writes V floats of out, so every array must
+    have the plan's sizes (a wrong size would over-read or overwrite memory)."""
+    nx, ny, nz = plan.imgsize
+    for name, a in (("out", out), ("x0", x0)):
+        if tuple(np.shape(a)) != plan.imgsize:
+            raise DimensionMismatch(f"size {name} {tuple(np.shape(a))} and plan image size {plan.imgsize} not match")
+    for name, a in (("ynoisy", ynoisy), ("background", background)):
+        if tuple(np.shape(a)) != (nx, nz, plan.nview):
+            raise DimensionMismatch(f"size {name} {tuple(np.shape(a))} and plan views {(nx, nz, plan.nview)} not match")
+
+
+def _em_fast(fn, out, x0, ynoisy, background, args, niter, chat, show):
+    """Run a device EM call (sr_mlem / sr_osem) on host arrays.  With chat, one
+    iteration per call, printing `show(iter, out)` before each: an EM call
+    keeps no state besides the image, so the iterates equal those of one call
+    with all iterations."""
+    res = _f32(out) if isinstance(out, np.ndarray) else out
+    x, y, b = _f32(x0), _f32(ynoisy), _f32(background)
+    if not chat:
+        check(fn(res.ctypes.data, x.ctypes.data, y.ctypes.data, b.ctypes.data, *args, niter, L.SR_MEM_HOST))
+        _writeback(out, res)
+        return out
+    cur = x.copy(order="F")
+    t0 = time.time()
+    for it in range(niter):
+        show(it + 1, cur, time.time() - t0)
+        check(fn(res.ctypes.data, cur.ctypes.data, y.ctypes.data, b.ctypes.data, *args, 1, L.SR_MEM_HOST))
+        cur[...] = res
+    _writeback(out, cur)
+    return out
+
+
 def mlem_(out, x0, ynoisy, background, A, *, niter=20, chat=False):
     """mlem!(out, x0, ynoisy, background, A; niter): x ← x ⊙ A'(y ⊘ (Ax+b)) ⊘ A'1."""
     _check_em(out, x0, ynoisy, background)
     if tuple(np.shape(ynoisy)) != tuple(np.shape(background)):
         raise DimensionMismatch("size ynoisy and background")
-    if A.plan is not None and A.index is None and not chat:
-        res = _f32(out) if isinstance(out, np.ndarray) else out
-        check(lib().sr_mlem(A.plan.handle, res.ctypes.data, _f32(x0).ctypes.data, _f32(ynoisy).ctypes.data,
-                            _f32(background).ctypes.data, niter, L.SR_MEM_HOST))
-        _writeback(out, res)
-        return out
+    if A.plan is not None and A.index is None:
+        plan = A.plan
+        _check_plan_shapes(plan, out, x0, ynoisy, background)
+        return _em_fast(lambda *a: lib().sr_mlem(plan.handle, *a), out, x0, ynoisy, background, (), niter, chat,
+                        lambda it, x, t: print((it, (float(x.min()), float(x.max())), t)))
     At = A.adjoint()
     asum = At @ np.ones(np.shape(ynoisy), np.float32)
     asum[asum == 0] = np.inf
@@ -517,12 +566,11 @@ def osem_(out, x0, ynoisy, background, Ab, *, niter=16, chat=False):
     plan = Ab[0].plan
     standard = plan is not None and all(
         a.plan is plan and a.index == list(range(nb, nview, nblocks)) for nb, a in enumerate(Ab))
-    if standard and not chat and nview == plan.nview:
-        res = _f32(out) if isinstance(out, np.ndarray) else out
-        check(lib().sr_osem(plan.handle, res.ctypes.data, _f32(x0).ctypes.data, _f32(ynoisy).ctypes.data,
-                            _f32(background).ctypes.data, nblocks, niter, L.SR_MEM_HOST))
-        _writeback(out, res)
-        return out
+    if standard and nview == plan.nview:
+        _check_plan_shapes(plan, out, x0, ynoisy, background)
+        # (chat: one line per iteration, at its first block)
+        return _em_fast(lambda *a: lib().sr_osem(plan.handle, *a), out, x0, ynoisy, background, (nblocks,), niter,
+                        chat, lambda it, x, t: print((it, 1, (float(x.min()), float(x.max())), t)))
     y = np.asarray(ynoisy, np.float32)
     b = np.asarray(background, np.float32)
     asum = []

@@ -170,3 +170,30 @@ def test_no_cpu_fallback(S, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", os.path.join(ROOT, "does-not-exist.so"))
     with pytest.raises(ImportError):
         _lib.lib()
+
+
+def _fake_plan(nx=8, nz=4, nview=4):
+    """A stand-in with a plan's sizes and no device handle: the shape checks
+    must raise before any library call would use it."""
+    from types import SimpleNamespace
+
+    return SimpleNamespace(imgsize=(nx, nx, nz), nview=nview, handle=None)
+
+
+@pytest.mark.parametrize("which", ["x0", "out", "y", "b"])
+def test_em_device_paths_check_shapes(S, which):
+    """mlem_/osem_/osem_strided raise DimensionMismatch for arrays whose
+    sizes differ from the plan's, before the library reads or writes them."""
+    plan = _fake_plan()
+    good = {"x0": np.ones((8, 8, 4), np.float32), "out": np.zeros((8, 8, 4), np.float32),
+            "y": np.ones((8, 4, 4), np.float32), "b": np.ones((8, 4, 4), np.float32)}
+    bad = dict(good)
+    bad[which] = np.ones((8, 8, 3), np.float32) if which in ("x0", "out") else np.ones((8, 4, 3), np.float32)
+    with pytest.raises(S.DimensionMismatch):
+        S.mlem_(bad["out"], bad["x0"], bad["y"], bad["b"] if which != "y" else np.ones((8, 4, 3), np.float32),
+                S.spect_operator(plan), niter=1)
+    with pytest.raises(S.DimensionMismatch):
+        S.osem_(bad["out"], bad["x0"], bad["y"], bad["b"], S.Ablock(plan, 2), niter=1)
+    if which != "out":
+        with pytest.raises(S.DimensionMismatch):
+            S.osem_strided(bad["x0"], bad["y"], bad["b"], plan, 2, 1)

@@ -178,12 +178,15 @@ def test_project_vs_oracle(gpu, O, method, mode, separable):
     assert maxrel(gbs, rbs) <= PROJ_TOL
 
 
-@pytest.mark.parametrize("nx,nz,p", [(64, 40, 19), (64, 64, 37)])
+@pytest.mark.parametrize("nx,nz,p", [(64, 40, 37), (64, 64, 37)])
 def test_general_psf_large_planes(gpu, O, nx, nz, p):
     """Non-separable PSFs on planes whose adjoint staging takes a large part
-    of LDS: 512-thread blocks (64x40, 19x19: three blocks fit a CU) and
-    1024-thread blocks (64x64, 37x37: one fits); parity with the direct FP32
-    oracle, the FP64 oracle and the adjoint identity."""
+    of LDS, so that the strip adjoint runs 512- and 1024-thread blocks.  Its
+    staging holds the padded view (odd_up(nx + 36) x (nz + 36) floats), the
+    strip windows (nz x odd_up(nx + 36 + 40 + 16)) and the taps (37 x 40):
+    64x40 takes 61,744 B, so two blocks fit in 160 KiB and a block has 512
+    threads; 64x64 takes 86,512 B, one block, 1024 threads.  Parity with the
+    direct FP32 oracle, the FP64 oracle and the adjoint identity."""
     nview = 3
     rng = np.random.default_rng(nx + nz + p)
     mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
@@ -543,6 +546,56 @@ def test_config1_adjoint_identity(gpu):
     assert abs(a - b) <= 1e-5 * abs(a), (a, b)
 
 
+def test_config1_benchmark_kernels_vs_oracle(gpu, O):
+    """The benchmarked plan against the oracle: 128x128x80, 128 views, 37x37
+    depth-varying Gaussian PSF (FWHM 1.5 -> 6 px), seeded phantom and mu-map,
+    dy = 0.48, autotuned for 16-view subsets.  Projections and
+    back-projections run one whole OS-EM subset (views 0, 8, ..., 120: view 0
+    is theta = 0, views 32, 64 and 96 are quarter turns, the others lie in
+    between), so the kernels run at the benchmark's launch shape: the fused
+    rotation + PSF forward with all 8 consumer columns busy and 5 z tiles per
+    wave, 16 plane chunks per view, the column adjoint and the 16-view list
+    gather.  Bars: per-view max-relative <= 2e-5 vs the FP32 oracle (the
+    reference's FFT path); <= 1e-5 vs the FP64 oracle; OS-EM 8 x 4 from y ~
+    Poisson(s A x + b) (5e6 counts, b = 0.1 mean) within 1e-4 relative L2 of
+    the FP64 oracle's OS-EM, which also runs the shared-PSF normaliser gather."""
+    nx, nz, nview, nblocks, niter, dy = 128, 80, 128, 8, 4, 0.48
+    image = gpu.phantom(nx, nx, nz, seed=0)
+    mumap = gpu.mumap_phantom(nx, nx, nz)
+    psf = gpu.psf_gauss(ny=nx, px=37, fwhm_start=1.5, fwhm_end=6.0)
+    psfs = np.asfortranarray(np.repeat(psf[:, :, :, None], nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, dy)
+    assert plan.info()["layout"] == "z-fastest"
+    plan.autotune(nview // nblocks)
+    ang = gpu.default_viewangle(nview).astype(np.float64)
+    P32 = O.Problem(mumap, psfs, dy, ang, "two", np.float32, "fft", nthread=16)
+    P64 = O.Problem(mumap, psfs, dy, ang, "two", np.float64, "fft", nthread=16)
+    for first in (0, 3):
+        idx = np.arange(first, nview, nblocks, dtype=np.int32)
+        g = gpu.project(image, plan, index=idx)
+        e32 = per_view_maxrel(g, P32.project(image, idx))
+        e64 = per_view_maxrel(g, P64.project(image.astype(np.float64), idx))
+        print(f"subset {first}: project vs FP32 {e32:.2e}, vs FP64 {e64:.2e}", flush=True)
+        assert e32 <= PROJ_TOL and e64 <= 1e-5, (first, e32, e64)
+        rng = np.random.default_rng(first)
+        views = np.asfortranarray((g * (0.5 + rng.random(g.shape))).astype(np.float32))
+        gb = gpu.backproject(views, plan, index=idx)
+        b32 = maxrel(gb, P32.backproject(views, idx))
+        b64 = maxrel(gb, P64.backproject(views.astype(np.float64), idx))
+        print(f"subset {first}: backproject vs FP32 {b32:.2e}, vs FP64 {b64:.2e}", flush=True)
+        assert b32 <= PROJ_TOL and b64 <= 1e-5, (first, b32, b64)
+    ytrue = gpu.project(image, plan)
+    s = 5e6 / float(ytrue.sum())
+    b = np.full_like(ytrue, 0.1 * float((s * ytrue).mean()))
+    y = gpu.poisson(s * ytrue + b, seed=0)
+    x0 = np.ones_like(image)
+    xg = gpu.osem(x0, y, b, gpu.Ablock(plan, nblocks), niter=niter)
+    xr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), nblocks, niter)
+    rel = float(np.linalg.norm(xg - xr) / np.linalg.norm(xr))
+    print(f"OS-EM 8x4 vs FP64: relative L2 {rel:.2e}", flush=True)
+    assert rel <= 1e-4, rel
+
+
 @pytest.mark.parametrize("nview", [16, 6])
 def test_gradient_vs_oracle(gpu, O, nview):
     """L = ||A(x, mu) - y||^2 and its gradients w.r.t. x and the mu-map at
@@ -613,6 +666,21 @@ def test_named_adjoint_entry_points(gpu):
     L.check(gpu.lib().sr_fft_conv_adj(d.ctypes.data, im2.ctypes.data, 20, 14, ker.ctypes.data, 7, 5, L.SR_MEM_HOST))
     np.testing.assert_array_equal(c, d)
     assert np.abs(c).sum() > 0
+
+
+def test_gradient_empty_index(gpu, O):
+    """sr_grad over no view: the loss is 0 and both gradients are zero."""
+    nx, nz, nview = 32, 16, 4
+    rng = np.random.default_rng(3)
+    mumap = np.asfortranarray((0.1 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = np.asfortranarray(np.repeat(np.asarray(O.psf_gauss(nx, 5, 5, 1.0, 2.0), np.float32)[:, :, :, None],
+                                       nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    x = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    gpu.gradient(x, np.asfortranarray(np.ones((nx, nz, nview), np.float32)), plan)  # leaves a loss behind
+    loss, gx, gmu = gpu.gradient(x, np.zeros((nx, nz, 0), np.float32), plan, index=[])
+    assert loss == 0.0
+    assert not gx.any() and not gmu.any()
 
 
 def test_gradient_needs_z_fastest(gpu, monkeypatch):
@@ -825,3 +893,38 @@ def test_serve_mix_equals_alone(gpu):
         finally:
             plan.close()
         np.testing.assert_array_equal(r.result, alone)
+
+
+@pytest.mark.parametrize("layout", ["z-fastest", "x-fastest"])
+def test_set_mumap_equals_new_plan(gpu, O, monkeypatch, layout):
+    """sr_plan_set_mumap rebuilds every view's attenuation factors in place:
+    projections and an OS-EM after it equal those of a plan built with the
+    new mu-map bit for bit, and match the oracle (whose attenuation sums each
+    y anew, the reference's order) at the projector bar."""
+    if layout == "x-fastest":
+        monkeypatch.setenv("SR_LAYOUT", "zyx")
+    nx, nz, nview = 32, 16, 8
+    rng = np.random.default_rng(21)
+    psfs = np.asfortranarray(np.repeat(np.asarray(O.psf_gauss(nx, 5, 5, 1.0, 2.5), np.float32)[:, :, :, None],
+                                       nview, 3))
+    mu1 = np.asfortranarray((0.1 * rng.random((nx, nx, nz))).astype(np.float32))
+    mu2 = np.asfortranarray((0.3 * rng.random((nx, nx, nz))).astype(np.float32))
+    plan = gpu.SPECTplan(mu1, psfs, 0.8)
+    fresh = gpu.SPECTplan(mu2, psfs, 0.8)
+    assert plan.info()["layout"] == layout
+    image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    before = gpu.project(image, plan)
+    plan.set_mumap(mu2)
+    g = gpu.project(image, plan)
+    np.testing.assert_array_equal(g, gpu.project(image, fresh))
+    assert not np.array_equal(g, before)
+    P32 = O.Problem(mu2, psfs, 0.8, angles32(nview).astype(np.float64), "two", np.float32, "fft")
+    assert per_view_maxrel(g, P32.project(image)) <= PROJ_TOL
+    b = np.full_like(g, 0.1 * g.mean())
+    y = gpu.poisson(20 * (g + b), seed=2) / 20
+    x0 = np.ones_like(image)
+    np.testing.assert_array_equal(gpu.osem(x0, y, b, gpu.Ablock(plan, 4), niter=2),
+                                  gpu.osem(x0, y, b, gpu.Ablock(fresh, 4), niter=2))
+    d = gpu.DeviceArray.from_host(mu1)
+    plan.set_mumap(d)
+    np.testing.assert_array_equal(gpu.project(image, plan), before)
