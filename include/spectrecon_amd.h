@@ -248,6 +248,19 @@ int sr_imrotate_adj(float* out, const float* img, int32_t n, int32_t nz, float t
 int sr_fft_conv_adj(float* out, const float* img, int32_t nx, int32_t nz, const float* ker, int32_t px, int32_t pz,
                     int32_t mem);
 
+/* sr_fft_conv for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and results as
+ * sr_fft_conv, but every buffer beyond the caller's arrays (tap tables, chunk
+ * partials, staging of host arrays) lives in one fixed device array of
+ * 411,392 bytes per GPU, held until the process ends: it calls no
+ * hipMalloc / hipFree and allocates no host memory.  Limits: nx * nz <= 32768
+ * and px, pz <= 64 (SR_ERR_UNSUPPORTED beyond).  Calls on one GPU share that
+ * array, so calls that overlap on one GPU race; each call returns after its
+ * work finished. */
+int sr_fft_conv_noalloc(float* out, const float* img, int32_t nx, int32_t nz, const float* ker, int32_t px,
+                        int32_t pz, int32_t adj, int32_t mem);
+
 /* ---- input generators (host only) --------------------------------------------
  * sr_psf_gauss replaces psf_gauss(; ny, px, pz, fwhm_x, fwhm_z)
  * (src/psf-gauss.jl:24-55); psf is [px, pz, ny]. */

@@ -78,6 +78,8 @@ SIGNATURES = {
     "sr_osem": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_int32]),
     "sr_imrotate": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_int32]),
     "sr_fft_conv": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "sr_fft_conv_noalloc": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int32]),
     "sr_imrotate_adj": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32]),
     "sr_fft_conv_adj": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32]),
     "sr_psf_gauss": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp]),
