@@ -347,7 +347,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel choices")
     ap.add_argument("--choices", default=None,
-                    help="pin the kernel choices (five comma-separated SR_CHOICE_* values) instead of autotuning")
+                    help="pin the kernel choices (six comma-separated SR_CHOICE_* values) instead of autotuning")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the benchmark, copy this many MiB 3 times with the library's copy kernel "
                          "(the known-size reference for the FETCH_SIZE / WRITE_SIZE counters)")

@@ -47,6 +47,7 @@ extern "C" {
 #define SR_CONV_AUTO 0     /* pick per plan (separable when every PSF is rank 1) */
 #define SR_CONV_DIRECT 1   /* direct 2-D clamped convolution                     */
 #define SR_CONV_SEPARABLE 2
+#define SR_CONV_FFT 3      /* plane FFTs as src/fft_convolve.jl (x-fastest layout)   */
 
 typedef struct sr_plan_s* sr_plan;
 
@@ -97,7 +98,10 @@ int sr_plan_destroy(sr_plan plan);
  * along y (src/project.jl:103-113 sums each y anew: the same values up to
  * rounding).  `mumap` [nx,ny,nz] is a host (SR_MEM_HOST) or device pointer. */
 int sr_plan_set_mumap(sr_plan plan, const float* mumap, int32_t mem);
-/* resident device bytes and the PSF method the plan chose (SR_CONV_*) */
+/* resident device bytes and the PSF method the plan runs (SR_CONV_*; with
+ * SR_CONV_AUTO, x-fastest plans whose PSFs are not rank 1 take the plane FFTs
+ * when their padded planes fit the FFT kernels and the PSF has at least
+ * 121 taps, else the direct 2-D kernels; the autotuner may switch) */
 int sr_plan_info(sr_plan plan, size_t* device_bytes, int32_t* conv_method, int32_t* max_batch);
 /* Device layout of the plan's work volumes, attenuation factors and partial
  * images (no reference counterpart; the caller's arrays keep the reference
@@ -191,8 +195,12 @@ int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
  *   FWD_CHUNKS / ADJ_CHUNKS: plane chunks per view, in percent of the built-in
  *     count (one block per compute unit over the batch), 10..1000;
  *   GATHER_SLICES: view slices per block of the rotation-adjoint gather
- *     (2, 4, 8; 0 picks by the launch's wave count and the batch size).
- * Defaults {0, 100, 0, 100, 0}.  Every choice computes the same operator;
+ *     (2, 4, 8; 0 picks by the launch's wave count and the batch size);
+ *   PSF_METHOD: 0 the plan's built-in method, 1 the direct taps (separable
+ *     when every PSF plane is rank 1, else the 2-D clamped convolution), 2
+ *     the plane FFTs (plans with PSF spectra: x-fastest layout and padded
+ *     planes the FFT kernels take; SR_ERR_UNSUPPORTED otherwise).
+ * Defaults {0, 100, 0, 100, 0, 0}.  Every choice computes the same operator;
  * the gather's slice count changes the order of its sums (last-bit
  * differences).  sr_plan_autotune times the candidates of each choice on
  * batches of nsub views (the OS-EM subset size) on the plan's stream, keeps
@@ -205,7 +213,8 @@ int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
 #define SR_CHOICE_ADJ_FAMILY 2
 #define SR_CHOICE_ADJ_CHUNKS 3
 #define SR_CHOICE_GATHER_SLICES 4
-#define SR_NCHOICE 5
+#define SR_CHOICE_PSF_METHOD 5
+#define SR_NCHOICE 6
 int sr_plan_set_choice(sr_plan plan, const int32_t* choice);
 int sr_plan_get_choice(sr_plan plan, int32_t* choice);
 int sr_plan_autotune(sr_plan plan, int32_t nsub, char* log, size_t log_bytes);

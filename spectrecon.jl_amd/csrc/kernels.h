@@ -94,6 +94,32 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
                           const float* atten, long atten_vstride, int nchunk, const Geo& g, hipStream_t s, bool zfast = false,
                           int family = PSF_AUTO);
 
+// ---- PSF by FFT (x-fastest layout; fft_psf.hip) ---------------------------
+// Padded plane sizes and LDS of the FFT kernels for a geometry.
+struct FftPlan {
+  int L1, L2;      // Power2(nx + px - 1), Power2(nz + pz - 1)
+  int up, left;    // pads before x and z (src/helper.jl:9-12)
+  int NPS;         // x-buffer stride per position (row pairs)
+  int boff, poff;  // z-buffer and (adjoint) z-pad sums, float2 offsets after the x-buffer
+  size_t lds_fwd, lds_adj;
+};
+// false when the padded plane or its LDS buffers exceed what the kernels take
+bool psf_fft_plan(const Geo& g, FftPlan* f);
+// W_L1^m then W_L2^m (e^{-2 pi i m / L}), rounded from double
+void psf_fft_twiddles(const FftPlan& f, std::vector<float2>& tw);
+// kernel spectra of nplanes planes (psfs [plane][pz][px]) into spec
+// [plane][L2][L1/2 + 1], scaled by 1/(L1 L2), in the kernels' digit order
+hipError_t launch_psf_spectrum(const float* psfs, int nplanes, const Geo& g, const FftPlan& f, float* spec,
+                               hipStream_t s);
+// as launch_psf_fwd / launch_psf_adj (x-fastest work volumes), through the
+// plane FFTs; spectra of plane v * vstride + y
+hipError_t launch_psf_fwd_fft(const float* work, const float* spec, int vstride, const int* vids, int nb, float* part,
+                              int nchunk, int* nchunk_used, const Geo& g, const FftPlan& f, const float2* tw,
+                              hipStream_t s);
+hipError_t launch_psf_adj_fft(const float* views, const float* spec, int vstride, const int* vids, int nb,
+                              float* work, const float* atten, long atten_vstride, int nchunk, const Geo& g,
+                              const FftPlan& f, const float2* tw, hipStream_t s);
+
 // ---- elementwise --------------------------------------------------------
 hipError_t launch_fill(float* p, float v, long n, hipStream_t s);
 hipError_t launch_zero_to_inf(float* p, long n, hipStream_t s);

@@ -183,6 +183,12 @@ struct sr_plan_s {
   size_t bytes = 0;
 
   DBuf mumap, atten, taps_u, taps_w, taps_h, taps_c, rotv, tab_b, tab_w;
+  // PSF by FFT (x-fastest plans whose padded planes fit the FFT kernels): the
+  // kernels' spectra [plane][L2][L1/2 + 1] and the twiddles
+  DBuf fspec, ftw;
+  sr::FftPlan fft{};
+  bool fft_ok = false;       // the FFT kernels take this plan
+  bool fft_default = false;  // the built-in PSF method is the FFT
   DBuf work, fpart, ipart, vids, ones;
   DBuf st_img, st_views, st_aux1, st_aux2, st_x0;  // staging for host-pointer calls and EM temporaries
   DBuf em_ratio, em_back, em_asum;
@@ -233,7 +239,11 @@ struct sr_plan_s {
   // kernel choices (sr_plan_set_choice / sr_plan_autotune), SR_CHOICE_* order:
   // PSF forward family, its chunk count in percent of the heuristic one, the
   // same two for the PSF adjoint, view slices of the rotation-adjoint gather
-  int32_t choice[SR_NCHOICE] = {0, 100, 0, 100, 0};
+  int32_t choice[SR_NCHOICE] = {0, 100, 0, 100, 0, 0};
+  // the PSF through the plane FFTs (else the direct / separable taps)
+  bool use_fft() const {
+    return fft_ok && (choice[SR_CHOICE_PSF_METHOD] == 2 || (choice[SR_CHOICE_PSF_METHOD] == 0 && fft_default));
+  }
   // plane chunks per view for the PSF kernels: their blocks take a compute
   // unit each (LDS), so a launch of c chunks x nb views runs in
   // ceil(c nb / ncu) rounds of about (planes per chunk + 2) plane times (the
@@ -257,7 +267,7 @@ struct sr_plan_s {
     return std::min(d.ny, std::max(1, (int)(((long)chunks_base(nb) * pct + 50) / 100)));
   }
   int nchunk_fwd(int nb) const {
-    if (mfma) return chunks_scaled(nb, choice[SR_CHOICE_FWD_CHUNKS]);
+    if (mfma || use_fft()) return chunks_scaled(nb, choice[SR_CHOICE_FWD_CHUNKS]);
     const int target = std::max(1, (int)std::ceil(512.0 / std::max(1, nb)));
     return std::min(d.ny, target);
   }
@@ -386,11 +396,29 @@ int upload_vids(sr_plan p, const int32_t* index, int32_t nindex) {
   return SR_OK;
 }
 
+// The PSF forward / adjoint of a batch on the plan's PSF method (x-fastest
+// plans may take the plane FFTs; z-fastest ones fuse the rotation)
+hipError_t psf_fwd(sr_plan p, const float* src, const int* vids, int nb, int nchunk, int* used, int family) {
+  if (!p->zfast && p->use_fft())
+    return sr::launch_psf_fwd_fft(src, p->fspec.as<float>(), p->vstride, vids, nb, p->fpart.as<float>(), nchunk, used,
+                                  p->g, p->fft, p->ftw.as<float2>(), p->stream);
+  return sr::launch_psf_fwd(p->separable, src, p->taps(), vids, nb, p->fpart.as<float>(), nchunk, used, p->g,
+                            p->stream, p->zfast, p->atten.as<float>(), p->atten_vstride(), p->rotv.as<sr::RotView>(),
+                            family);
+}
+hipError_t psf_adj(sr_plan p, const float* views, const int* vids, int nb, float* work, const float* atten, int nchunk,
+                   int family) {
+  if (!p->zfast && p->use_fft())
+    return sr::launch_psf_adj_fft(views, p->fspec.as<float>(), p->vstride, vids, nb, work, atten, p->atten_vstride(),
+                                  nchunk, p->g, p->fft, p->ftw.as<float2>(), p->stream);
+  return sr::launch_psf_adj(p->separable, views, p->taps(), vids, nb, work, atten, atten ? p->atten_vstride() : 0,
+                            nchunk, p->g, p->stream, p->zfast, family);
+}
+
 // views[ii] = A_{vids[ii]} image (or the ML-EM ratio y/(A x + b) when yobs)
 // (zimg_ready: the plan's z-fastest image copy already holds `image`)
 int run_project(sr_plan p, float* views, const float* image, const int* d_vids, int n, const float* yobs,
                 const float* bkg, bool zimg_ready = false) {
-  const sr::Taps t = p->taps();
   if (p->zfast && n > 0 && !zimg_ready) {
     KTimer kt(p, sr::K_ROT_FWD);
     HIP_TRY(sr::launch_to_zfast(image, p->zimg.as<float>(), p->g, p->stream));
@@ -408,9 +436,8 @@ int run_project(sr_plan p, float* views, const float* image, const int* d_vids, 
     int nchunk = 0;
     {
       KTimer kt(p, sr::K_PSF_FWD);
-      HIP_TRY(sr::launch_psf_fwd(p->separable, p->zfast ? p->zimg.as<float>() : p->work.as<float>(), t, d_vids + s0,
-                                 nb, p->fpart.as<float>(), p->nchunk_fwd(nb), &nchunk, p->g, p->stream, p->zfast,
-                                 p->atten.as<float>(), p->atten_vstride(), p->rotv.as<sr::RotView>(), p->fam_fwd()));
+      HIP_TRY(psf_fwd(p, p->zfast ? p->zimg.as<float>() : p->work.as<float>(), d_vids + s0, nb, p->nchunk_fwd(nb),
+                      &nchunk, p->fam_fwd()));
     }
     {
       KTimer kt(p, sr::K_VIEW_REDUCE);
@@ -427,7 +454,6 @@ int run_project(sr_plan p, float* views, const float* image, const int* d_vids, 
 int run_backproject(sr_plan p, float* image, const float* views, const int* d_vids, int n, int mode,
                     const float* asum, bool* zimg_written = nullptr) {
   if (zimg_written) *zimg_written = false;
-  const sr::Taps t = p->taps();
   const bool single = n <= p->max_batch;
   float* target = image;
   if (mode == sr::IMG_EM && !single) {
@@ -445,9 +471,8 @@ int run_backproject(sr_plan p, float* image, const float* views, const int* d_vi
     if ((rc = ensure_batch_buffers(p, nb))) return rc;
     {
       KTimer kt(p, sr::K_PSF_ADJ);
-      HIP_TRY(sr::launch_psf_adj(p->separable, views + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(),
-                                 p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(nb), p->g, p->stream,
-                                 p->zfast, p->fam_adj()));
+      HIP_TRY(psf_adj(p, views + (long)s0 * p->P, d_vids + s0, nb, p->work.as<float>(), p->atten.as<float>(),
+                      p->nchunk_adj(nb), p->fam_adj()));
     }
     int vpg;
     const int ng = p->groups_adj(nb, &vpg);
@@ -668,6 +693,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   if (sep)
     for (int pl = 0; pl < nstore && sep; ++pl) sep = factor_rank1(psfs + (long)pl * pxz, d.px, d.pz, u, w);
   if (d.conv == SR_CONV_SEPARABLE && !sep) return bail(fail(SR_ERR_ARG, "PSF planes are not rank 1"));
+  if (d.conv < SR_CONV_AUTO || d.conv > SR_CONV_FFT) return bail(fail(SR_ERR_ARG, "bad conv"));
   p->separable = sep ? 1 : 0;
   p->mfma = sep && mfma_ok;
   // the matrix-core kernels take any table length: the exact one keeps their
@@ -718,7 +744,8 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   {
     const char* lay = std::getenv("SR_LAYOUT");
     const bool force_zyx = lay && std::strcmp(lay, "zyx") == 0;
-    p->zfast = !force_zyx && p->mfma && d.interpmeth == SR_INTERP_TWO && sr::zfast_supported(p->g);
+    p->zfast = !force_zyx && p->mfma && d.interpmeth == SR_INTERP_TWO && d.conv != SR_CONV_FFT &&
+               sr::zfast_supported(p->g);
   }
   if (p->zfast) {
     std::vector<int2> ent;
@@ -730,6 +757,30 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
     }
   }
 
+  // --- PSF spectra for the plane FFTs (x-fastest plans whose planes fit):
+  // from the PSF planes on the device, in double, once ---
+  if (!p->zfast && sr::psf_fft_plan(p->g, &p->fft)) {
+    const long nf = (long)p->fft.L2 * (p->fft.L1 / 2 + 1);
+    DBuf dpsf;
+    std::vector<float2> tw;
+    sr::psf_fft_twiddles(p->fft, tw);
+    if ((rc = dpsf.ensure(sizeof(float) * pxz * nstore)) || (rc = p->fspec.ensure(sizeof(float) * nf * nstore)) ||
+        (rc = p->ftw.ensure(sizeof(float2) * tw.size())))
+      return bail(rc);
+    HIP_TRY(hipMemcpy(dpsf.p, psfs, sizeof(float) * pxz * nstore, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->ftw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+    HIP_TRY(sr::launch_psf_spectrum(dpsf.as<float>(), nstore, p->g, p->fft, p->fspec.as<float>(), p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    p->fft_ok = true;
+    // built in: the FFT for general PSFs of 11 x 11 taps and more (their direct
+    // 2-D cost grows with px pz, the FFT's with log L); sr_plan_autotune
+    // times both
+    p->fft_default = d.conv == SR_CONV_FFT || (d.conv == SR_CONV_AUTO && !p->separable && pxz >= 121);
+  } else if (d.conv == SR_CONV_FFT) {
+    return bail(fail(SR_ERR_UNSUPPORTED, "the PSF FFT kernels do not take %dx%d planes with %dx%d PSFs", d.nx, d.nz,
+                     d.px, d.pz));
+  }
+
   // --- mu-map and attenuation factors for every view ---
   if ((rc = p->mumap.ensure(sizeof(float) * p->V))) return bail(rc);
   HIP_TRY(hipMemcpy(p->mumap.p, mumap, sizeof(float) * p->V, hipMemcpyHostToDevice));
@@ -737,7 +788,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   if ((rc = build_atten(p))) return bail(rc);
   HIP_TRY(hipStreamSynchronize(p->stream));
   p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->taps_c.n + p->rotv.n + p->tab_b.n +
-             p->tab_w.n + p->zimg.n + p->zent.n;  // (zh is a per-call buffer)
+             p->tab_w.n + p->zimg.n + p->zent.n + p->fspec.n + p->ftw.n;  // (zh is a per-call buffer)
   *out = guard.release();
   return SR_OK;
 }
@@ -806,6 +857,9 @@ int sr_plan_set_choice(sr_plan p, const int32_t* c) {
     if (c[k] < 10 || c[k] > 1000) return fail(SR_ERR_ARG, "chunk percent must be in 10..1000");
   const int vs = c[SR_CHOICE_GATHER_SLICES];
   if (vs != 0 && vs != 2 && vs != 4 && vs != 8) return fail(SR_ERR_ARG, "gather slices must be 0, 2, 4 or 8");
+  if (c[SR_CHOICE_PSF_METHOD] < 0 || c[SR_CHOICE_PSF_METHOD] > 2) return fail(SR_ERR_ARG, "PSF method must be 0..2");
+  if (c[SR_CHOICE_PSF_METHOD] == 2 && !p->fft_ok)
+    return fail(SR_ERR_UNSUPPORTED, "the plan has no PSF spectra (z-fastest, or planes the FFT kernels do not take)");
   HIP_TRY(hipStreamSynchronize(p->stream));
   drop_graphs(p);
   std::copy(c, c + SR_NCHOICE, p->choice);
@@ -829,7 +883,7 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
   if (!p) return fail(SR_ERR_ARG, "null plan");
   HIP_TRY(hipStreamSynchronize(p->stream));
   drop_graphs(p);
-  const int32_t defaults[SR_NCHOICE] = {0, 100, 0, 100, 0};
+  const int32_t defaults[SR_NCHOICE] = {0, 100, 0, 100, 0, 0};
   std::string out;
   if (nsub <= 0) {
     std::copy(defaults, defaults + SR_NCHOICE, p->choice);
@@ -887,15 +941,54 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
     // (the built-in one) is not the one that pays for it
     for (int k = 0; k < 80; ++k) {
       int used = 0;
-      HIP_TRY(sr::launch_psf_fwd(p->separable, p->zfast ? img : p->work.as<float>(), t, dv, nb, p->fpart.as<float>(),
-                                 p->nchunk_fwd(nb), &used, p->g, p->stream, p->zfast, p->atten.as<float>(),
-                                 p->atten_vstride(), p->rotv.as<sr::RotView>(), p->fam_fwd()));
+      HIP_TRY(psf_fwd(p, p->zfast ? img : p->work.as<float>(), dv, nb, p->nchunk_fwd(nb), &used, p->fam_fwd()));
     }
     appendf(out, "{\"nsub\": %d, \"views_per_launch\": %d, \"layout\": \"%s\"", (int)nsub, nb,
          p->zfast ? "z-fastest" : "x-fastest");
+    // PSF method (plans with PSF spectra): the direct taps against the plane
+    // FFTs, forward (with its view reduction) plus adjoint, each at the
+    // built-in chunk counts
+    if (p->fft_ok) {
+      appendf(out, ", \"psf_method\": {\"candidates\": [");
+      double best = -1, def_total = -1;
+      int bm = 0;
+      for (int m : {1, 2}) {
+        p->choice[SR_CHOICE_PSF_METHOD] = m;
+        const double fu = time_us([&]() {
+          int used = 0;
+          hipError_t e = psf_fwd(p, p->work.as<float>(), dv, nb, p->nchunk_fwd(nb), &used, p->fam_fwd());
+          if (e == hipSuccess)
+            e = sr::launch_view_reduce(p->fpart.as<float>(), used, dv, nb, p->em_ratio.as<float>(), nullptr, nullptr,
+                                       sr::EPI_VIEWS, p->g, p->stream);
+          return e;
+        });
+        const double au = time_us([&]() {
+          return psf_adj(p, p->em_ratio.as<float>(), dv, nb, p->work.as<float>(), p->atten.as<float>(),
+                         p->nchunk_adj(nb), p->fam_adj());
+        });
+        const double total = (fu > 0 && au > 0) ? fu + au : -1;
+        const bool is_default = (m == 2) == p->fft_default;
+        appendf(out, "%s{\"method\": \"%s\", \"code\": %d, \"fwd_us\": %.2f, \"adj_us\": %.2f}", m == 1 ? "" : ", ",
+                m == 2 ? "fft" : p->separable ? "separable" : "direct", m, fu, au);
+        if (is_default) def_total = total;
+        if (total > 0 && (best < 0 || total < best)) {
+          best = total;
+          bm = m;
+        }
+      }
+      // the built-in method stays unless the other is clearly faster
+      const int dm = p->fft_default ? 2 : 1;
+      if (def_total > 0 && best > kTuneMargin * def_total) {
+        bm = dm;
+        best = def_total;
+      }
+      p->choice[SR_CHOICE_PSF_METHOD] = bm == dm ? 0 : bm;
+      appendf(out, "], \"choice\": {\"method\": \"%s\", \"code\": %d, \"us\": %.2f}}",
+              bm == 2 ? "fft" : p->separable ? "separable" : "direct", p->choice[SR_CHOICE_PSF_METHOD], best);
+    }
     // PSF forward (with the view reduction its chunk count sets) and adjoint:
     // family x chunk percent
-    for (int dir = 0; dir < 2 && psf_choices; ++dir) {
+    for (int dir = 0; dir < 2 && psf_choices && !p->use_fft(); ++dir) {
       const int fk = dir == 0 ? SR_CHOICE_FWD_FAMILY : SR_CHOICE_ADJ_FAMILY;
       const int ck = dir == 0 ? SR_CHOICE_FWD_CHUNKS : SR_CHOICE_ADJ_CHUNKS;
       // families that fit this plan (forward: producer/consumer waves with 4
@@ -1023,7 +1116,7 @@ int sr_plan_info(sr_plan p, size_t* device_bytes, int32_t* conv_method, int32_t*
   if (device_bytes)
     *device_bytes = p->bytes + p->work.n + p->fpart.n + p->ipart.n + p->st_img.n + p->st_views.n + p->st_aux1.n +
                     p->st_aux2.n + p->st_x0.n + p->em_ratio.n + p->em_back.n + p->em_asum.n + p->ones.n;
-  if (conv_method) *conv_method = p->separable ? SR_CONV_SEPARABLE : SR_CONV_DIRECT;
+  if (conv_method) *conv_method = p->use_fft() ? SR_CONV_FFT : p->separable ? SR_CONV_SEPARABLE : SR_CONV_DIRECT;
   if (max_batch) *max_batch = p->max_batch;
   return SR_OK;
 }

@@ -17,7 +17,7 @@ SR_OK, SR_ERR_ARG, SR_ERR_DIM, SR_ERR_HIP, SR_ERR_ALLOC, SR_ERR_UNSUPPORTED = ra
 SR_MEM_HOST, SR_MEM_DEVICE, SR_ASYNC = 0, 1, 0x100
 SR_INTERP_ONE, SR_INTERP_TWO = 1, 2
 SR_MODE_FAST, SR_MODE_MEM = 0, 1
-SR_CONV_AUTO, SR_CONV_DIRECT, SR_CONV_SEPARABLE = 0, 1, 2
+SR_CONV_AUTO, SR_CONV_DIRECT, SR_CONV_SEPARABLE, SR_CONV_FFT = 0, 1, 2, 3
 KERNEL_CLASSES = ("rot_fwd", "psf_fwd", "view_reduce", "psf_adj", "rot_adj", "img_reduce", "atten")
 
 

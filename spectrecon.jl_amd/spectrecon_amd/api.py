@@ -144,7 +144,8 @@ class SPECTplan:
                 raise SPECTError("bad mode")
         desc = L.PlanDesc(nx, ny, nz, px, pz, nview, self.dy, _METHODS.get(self.interpmeth, 0),
                           _MODES.get(self.mode, -1),
-                          {"auto": L.SR_CONV_AUTO, "direct": L.SR_CONV_DIRECT, "separable": L.SR_CONV_SEPARABLE}[conv])
+                          {"auto": L.SR_CONV_AUTO, "direct": L.SR_CONV_DIRECT, "separable": L.SR_CONV_SEPARABLE,
+                           "fft": L.SR_CONV_FFT}[conv])
         h = C.c_void_p()
         check(lib().sr_plan_create(C.byref(desc), mumap.ctypes.data, psfs.ctypes.data, self.viewangle.ctypes.data,
                                    C.byref(h)))
@@ -161,7 +162,7 @@ class SPECTplan:
         check(lib().sr_plan_info(self.handle, C.byref(nbytes), C.byref(conv), C.byref(batch)))
         layout = C.c_int32()
         check(lib().sr_plan_layout(self.handle, C.byref(layout)))
-        return {"device_bytes": nbytes.value, "conv": {1: "direct", 2: "separable"}[conv.value],
+        return {"device_bytes": nbytes.value, "conv": {1: "direct", 2: "separable", 3: "fft"}[conv.value],
                 "max_batch": batch.value, "layout": {0: "x-fastest", 1: "z-fastest"}[layout.value]}
 
     def set_mumap(self, mumap):
@@ -206,15 +207,16 @@ class SPECTplan:
         return cap.value, rep.value
 
     CHOICES = ("psf_fwd_family", "psf_fwd_chunk_percent", "psf_adj_family", "psf_adj_chunk_percent",
-               "gather_slices")
+               "gather_slices", "psf_method")
 
     def choice(self):
         """The plan's kernel choices: PSF families (forward: 0 built-in
         producer/consumer waves, 1 tile, 2 column-owned, 3 / 4 producer/consumer
         with 8 / 4 producers, 5 producer/consumer with column-pair consumers;
         adjoint: 0 column-owned, 1 tile), chunk counts in
-        percent of the built-in ones, gather view slices (0 built-in)."""
-        c = (C.c_int32 * 5)()
+        percent of the built-in ones, gather view slices (0 built-in), PSF
+        method (0 built-in, 1 direct taps, 2 plane FFTs)."""
+        c = (C.c_int32 * len(self.CHOICES))()
         check(lib().sr_plan_get_choice(self.handle, c))
         return dict(zip(self.CHOICES, list(c)))
 
@@ -225,7 +227,7 @@ class SPECTplan:
             if k not in cur:
                 raise KeyError(k)
             cur[k] = int(v)
-        c = (C.c_int32 * 5)(*[cur[k] for k in self.CHOICES])
+        c = (C.c_int32 * len(self.CHOICES))(*[cur[k] for k in self.CHOICES])
         check(lib().sr_plan_set_choice(self.handle, c))
 
     def autotune(self, nsub):
@@ -272,9 +274,25 @@ def _index(plan, index, n_out=None):
 # ---------------------------------------------------------------------------
 # project! / project (src/project.jl)
 # ---------------------------------------------------------------------------
-def project_(views, image, plan, viewidx=None, *, index=None, asynchronous=False):
+def _view_args(args, name):
+    """The trailing (viewidx) or (buffer_id, viewidx) of the single-view forms
+    (src/project.jl:11-69, :77-135; src/backproject.jl:9-62, :69-124).  The
+    reference's buffer_id picks a thread's work buffers; the device plan owns
+    its buffers, so it is accepted and not used."""
+    if len(args) == 0:
+        return None
+    if len(args) == 1:
+        return int(args[0])
+    if len(args) == 2:
+        return int(args[1])
+    raise TypeError(f"{name}: expected (viewidx) or (buffer_id, viewidx), got {len(args)} arguments")
+
+
+def project_(views, image, plan, *args, index=None, asynchronous=False):
     """project!(views, image, plan; index) -- views[:,:,ii] = A_{index[ii]} image.
-    With a 2-D `views` and an int `viewidx`: project!(view, image, plan, viewidx)."""
+    With a 2-D view: project!(view, image, plan, viewidx) and
+    project!(view, image, plan, buffer_id, viewidx) (0-based viewidx)."""
+    viewidx = _view_args(args, "project!")
     if viewidx is not None:
         if np.ndim(views) != 2:
             raise DimensionMismatch("single-view form needs a 2D view")
@@ -324,13 +342,22 @@ def project(image, *args, index=None, interpmeth="two", mode="fast", **kw):
 # ---------------------------------------------------------------------------
 # backproject! / backproject (src/backproject.jl)
 # ---------------------------------------------------------------------------
-def backproject_(image, views, plan, viewidx=None, *, index=None, asynchronous=False):
+def backproject_(image, views, plan, *args, index=None, asynchronous=False):
     """backproject!(image, views, plan; index) -- image = Σ_ii A_{index[ii]}' views[:,:,ii]
-    (image is overwritten, as the reference zeroes it).  With a 2-D view and an
-    int `viewidx`: the single-view backproject!(image, view, plan, viewidx)."""
+    (image is overwritten, as the reference zeroes it).  With a 2-D view:
+    backproject!(image, view, plan, viewidx) overwrites image with that view's
+    back-projection (src/backproject.jl:9-62), and
+    backproject!(image, view, plan, buffer_id, viewidx) adds it to image, as
+    the reference's per-thread form accumulates (src/backproject.jl:121)."""
+    viewidx = _view_args(args, "backproject!")
     if viewidx is not None:
         if np.ndim(views) != 2:
             raise DimensionMismatch("single-view form needs a 2D view")
+        if len(args) == 2:
+            one = np.zeros(plan.imgsize, np.float32, order="F")
+            backproject_(one, np.asarray(views)[:, :, None], plan, index=[viewidx])
+            image[...] = np.asarray(image) + one
+            return image
         return backproject_(image, np.asarray(views)[:, :, None], plan, index=[viewidx])
     idx, n = _index(plan, index)
     nx, ny, nz = plan.imgsize

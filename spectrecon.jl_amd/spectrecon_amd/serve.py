@@ -102,7 +102,8 @@ class DeviceBackend:
         self.tuned[key] = plan.choice()
         self.tune_log.append({"config": f"{cfg.nx}x{cfg.nx}x{cfg.nz}/{cfg.nview}/psf{cfg.psf}", "nsub": nsub,
                               "choice": self.tuned[key], "ms": 1e3 * (time.perf_counter() - t0),
-                              "us": {k: rec[k]["choice"]["us"] for k in ("psf_fwd", "psf_adj", "gather") if k in rec}})
+                              "us": {k: rec[k]["choice"]["us"] for k in ("psf_method", "psf_fwd", "psf_adj", "gather")
+                                     if k in rec}})
 
     def make_plan(self, cfg: Config):
         mumap = self._mumap(cfg.nx, cfg.nx, cfg.nz)

@@ -791,7 +791,7 @@ def test_autotune_records_and_keeps_choices(gpu, O):
         with pytest.raises(gpu.SPECTError):
             plan.set_choice(**bad)
     assert plan.autotune(0) == {"reset": True}
-    assert list(plan.choice().values()) == [0, 100, 0, 100, 0]
+    assert list(plan.choice().values()) == [0, 100, 0, 100, 0, 0]
 
 
 def test_autotune_other_plans(gpu, O, monkeypatch):
@@ -928,3 +928,95 @@ def test_set_mumap_equals_new_plan(gpu, O, monkeypatch, layout):
     d = gpu.DeviceArray.from_host(mu1)
     plan.set_mumap(d)
     np.testing.assert_array_equal(gpu.project(image, plan), before)
+
+
+# ---------------------------------------------------------------------------
+# PSF through the plane FFTs (x-fastest plans)
+# ---------------------------------------------------------------------------
+FFT_CASES = [
+    # nx, nz, p, per-view PSFs: padded planes L1 x L2
+    (32, 16, 7, False),    # 64 x 32
+    (48, 7, 9, False),     # 64 x 16, odd nz (the last row pair has one row)
+    (32, 15, 5, True),     # 64 x 32, odd nz, a PSF stack per view
+    (64, 40, 19, False),   # 128 x 64: the reference's timing problem (time/project.jl)
+    (16, 24, 9, False),    # 32 x 32
+    (96, 64, 25, False),   # 128 x 128
+    (128, 80, 37, False),  # 256 x 128: the clinical size with a non-separable PSF
+]
+
+
+@pytest.mark.parametrize("nx,nz,p,per_view", FFT_CASES)
+def test_fft_psf_vs_oracle(gpu, O, nx, nz, p, per_view):
+    """Plans that run the PSF through the plane FFTs (conv="fft", x-fastest
+    layout) against the oracle's literal FFT path (FP32, the reference's
+    arithmetic) per view at <= 2e-5, against the FP64 oracle, and the adjoint
+    identity; random point-symmetric PSFs, all views and an index subset."""
+    nview = 4
+    rng = np.random.default_rng(nx * 7 + nz + p)
+    mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
+    if per_view:
+        psfs = rand_sym_psfs(rng, p, p, nx, nview)
+    else:
+        psfs = np.asfortranarray(np.repeat(rand_sym_psfs(rng, p, p, nx, 1), nview, 3))
+    print(f"fft psf: {nx}x{nx}x{nz}, {p}x{p}, per_view={per_view}", flush=True)
+    plan = gpu.SPECTplan(mumap, psfs, 0.48, conv="fft")
+    info = plan.info()
+    assert info["conv"] == "fft" and info["layout"] == "x-fastest"
+    ang = angles32(nview).astype(np.float64)
+    P32 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float32, "fft")
+    P64 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float64, "fft")
+    image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    g = gpu.project(image, plan)
+    assert per_view_maxrel(g, P32.project(image)) <= PROJ_TOL
+    assert per_view_maxrel(g, P64.project(image.astype(np.float64))) <= 1e-5
+    views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    gb = gpu.backproject(views, plan)
+    assert maxrel(gb, P32.backproject(views)) <= PROJ_TOL
+    assert maxrel(gb, P64.backproject(views.astype(np.float64))) <= 1e-5
+    a = float(np.vdot(views.astype(np.float64), g.astype(np.float64)))
+    b = float(np.vdot(image.astype(np.float64), gb.astype(np.float64)))
+    assert abs(a - b) <= 1e-5 * abs(a)
+    idx = [3, 0]
+    gi = gpu.project(image, plan, index=idx)
+    assert per_view_maxrel(gi, P32.project(image, np.array(idx, np.int32))) <= PROJ_TOL
+    gbi = gpu.backproject(np.asfortranarray(views[:, :, idx]), plan, index=idx)
+    assert maxrel(gbi, P32.backproject(np.asfortranarray(views[:, :, idx]), np.array(idx, np.int32))) <= PROJ_TOL
+
+
+@pytest.mark.parametrize("method", ["one", "two"])
+def test_fft_psf_osem_and_methods(gpu, O, method):
+    """OS-EM on a plan whose PSF runs through the plane FFTs, against the FP64
+    oracle; the direct taps and the FFTs (set_choice psf_method) give the
+    same operator; the autotuner times both and logs them."""
+    nx, nz, nview, p = 32, 16, 8, 9
+    rng = np.random.default_rng(5)
+    mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = np.asfortranarray(np.repeat(rand_sym_psfs(rng, 11, 11, nx, 1), nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, 0.8, interpmeth=method)
+    assert plan.info()["conv"] == "fft"  # (11 x 11 general PSF: the FFTs are built in)
+    xtrue = gpu.phantom(nx, nx, nz, seed=3)
+    ytrue = gpu.project(xtrue, plan)
+    b = np.full_like(ytrue, 0.1 * ytrue.mean())
+    y = gpu.poisson(20 * (ytrue + b), seed=4) / 20
+    x0 = np.ones_like(xtrue)
+    xg = gpu.osem(x0, y, b, gpu.Ablock(plan, 4), niter=2)
+    P64 = O.Problem(mumap, psfs, 0.8, angles32(nview).astype(np.float64), method, np.float64, "fft")
+    xr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), 4, 2)
+    assert np.linalg.norm(xg - xr) <= 1e-4 * np.linalg.norm(xr)
+    plan.set_choice(psf_method=1)
+    assert plan.info()["conv"] == "direct"
+    gd = gpu.project(xtrue, plan)
+    assert per_view_maxrel(gd, ytrue) <= 1e-5
+    plan.set_choice(psf_method=0)
+    rec = plan.autotune(2)
+    cands = rec["psf_method"]["candidates"]
+    assert [c["method"] for c in cands] == ["direct", "fft"] and all(c["fwd_us"] > 0 and c["adj_us"] > 0
+                                                                     for c in cands)
+    assert plan.choice()["psf_method"] == rec["psf_method"]["choice"]["code"]
+    P32 = O.Problem(mumap, psfs, 0.8, angles32(nview).astype(np.float64), method, np.float32, "fft")
+    assert per_view_maxrel(gpu.project(xtrue, plan), P32.project(xtrue)) <= PROJ_TOL
+    zf = gpu.SPECTplan(mumap, np.asfortranarray(np.repeat(np.asarray(O.psf_gauss(nx, 5, 5, 1.0, 2.0), np.float32)
+                                                          [:, :, :, None], nview, 3)), 0.8)
+    if zf.info()["layout"] == "z-fastest":
+        with pytest.raises(gpu.DeviceError):
+            zf.set_choice(psf_method=2)
