@@ -257,6 +257,103 @@ def gradient_workload(steps):
                       "data": "synthetic (seeded phantoms; y = 0.9 A x' of other phantoms)"}), flush=True)
 
 
+def psf_workload(reps=5):
+    """The PSF methods on general (non-separable) PSFs, device arrays:
+    projection and back-projection of all views with the direct 2-D taps and
+    with the plane FFTs, and the FFT kernels' FP32 rate counted as
+    5 N log2(N) flops per padded plane (N = L1 L2) and direction (the
+    convention of the reference's FFT path: forward and inverse transform per
+    plane, src/fft_convolve.jl:16-25; the spectral product is not counted).
+
+    - clinical: 128x128x80, 128 views, one random point-symmetric 37x37 PSF per
+      depth (seeded), the phantom mu-map, dy = 0.48;
+    - timing: the reference's timing problem (time/project.jl: 64x64x40, 60
+      views, random 19x19 PSFs symmetrised, dy = 4.7952), on the method the
+      autotuner picks."""
+    import spectrecon_amd as S
+    from spectrecon_amd import _lib as L
+
+    lib = S.lib()
+    out = {"workload": "psf", "unit": "ms", "note": "device-resident project!/backproject! of all views; "
+           "fft_TFLOPs counts 5 N log2 N per padded plane and direction"}
+
+    def timed(plan, dv, dx, nview):
+        def proj():
+            L.check(lib.sr_project(plan.handle, dv.ptr, dx.ptr, None, nview, L.SR_MEM_DEVICE))
+
+        def back():
+            L.check(lib.sr_backproject(plan.handle, dx.ptr, dv.ptr, None, nview, L.SR_MEM_DEVICE))
+        res = {}
+        for name, fn in (("project_ms", proj), ("backproject_ms", back)):
+            fn()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            res[name] = 1e3 * float(np.median(ts))
+        plan.set_timing(True, classes=["psf_fwd", "psf_adj"])
+        proj()
+        back()
+        tm = plan.timing()
+        plan.set_timing(False)
+        for cls in ("psf_fwd", "psf_adj"):
+            res[cls + "_launch_ms"] = tm[cls][0] / max(1, tm[cls][1])
+            res[cls + "_launches"] = tm[cls][1]
+        return res
+
+    def fft_rate(res, nx, nz, p, ny, nview):
+        L1 = 1 << math.ceil(math.log2(nx + p - 1))
+        L2 = 1 << math.ceil(math.log2(nz + p - 1))
+        N = L1 * L2
+        fl = 5 * N * math.log2(N) * ny * nview  # all planes of all views, one direction
+        for cls, key in (("psf_fwd", "fwd"), ("psf_adj", "adj")):
+            t = res[cls + "_launch_ms"] * res[cls + "_launches"] / 1e3
+            res[f"fft_{key}_TFLOPs"] = fl / t / 1e12
+            res[f"fft_{key}_frac"] = fl / t / 1e12 / FP32_PEAK_TFLOPS
+        res["padded_plane"] = [L1, L2]
+
+    rng = np.random.default_rng(0)
+    nx, nz, nview, p = NX, NZ, NVIEW, PSIZE
+    mumap = S.mumap_phantom(nx, nx, nz)
+    k = rng.random((p, p, nx)).astype(np.float32)
+    k = k + k[::-1, ::-1]
+    k = k / k.sum(axis=(0, 1), keepdims=True)
+    k = 0.5 * (k + k[::-1, ::-1])
+    psfs = np.asfortranarray(np.repeat(k[:, :, :, None], nview, 3).astype(np.float32))
+    plan = S.SPECTplan(mumap, psfs, DY)
+    dx = S.DeviceArray.from_host(S.phantom(nx, nx, nz, seed=0))
+    dv = S.DeviceArray((nx, nz, nview))
+    clinical = {"problem": f"{nx}x{nx}x{nz}, {nview} views, {p}x{p} non-separable PSF", "info": plan.info()}
+    for m, name in ((1, "direct"), (2, "fft")):
+        plan.set_choice(psf_method=m)
+        log(f"psf: clinical, {name}")
+        clinical[name] = timed(plan, dv, dx, nview)
+    fft_rate(clinical["fft"], nx, nz, p, nx, nview)
+    plan.close()
+    out["clinical"] = clinical
+
+    nx, nz, nview, p = 64, 40, 60, 19
+    mumap = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    psfs = rng.random((p, p, nx, nview), dtype=np.float32)
+    psfs = psfs + psfs[::-1, ::-1]
+    psfs = psfs + psfs.transpose(1, 0, 2, 3)
+    psfs = np.asfortranarray(psfs / psfs.sum(axis=(0, 1), keepdims=True))
+    plan = S.SPECTplan(mumap, psfs, 4.7952)
+    rec = plan.autotune(nview)
+    dx = S.DeviceArray.from_host(np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32)))
+    dv = S.DeviceArray((nx, nz, nview))
+    log("psf: timing problem")
+    tp = {"problem": "64x64x40, 60 views, 19x19 random PSFs (time/project.jl)", "autotune": rec.get("psf_method"),
+          "info": plan.info(), "published_ms": {"project": 289.804, "backproject": 208.163}}
+    tp.update(timed(plan, dv, dx, nview))
+    if plan.info()["conv"] == "fft":
+        fft_rate(tp, nx, nz, p, nx, nview)
+    plan.close()
+    out["timing_problem"] = tp
+    print(json.dumps(out), flush=True)
+
+
 def units_workload(reps=20):
     """Each of the reference's published timings (time/*.jl; unstated CPU)
     beside the library on the GPU for the same operation and sizes (device
@@ -336,9 +433,10 @@ def units_workload(reps=20):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["osem", "batched", "served", "gradient", "units"], default="osem",
+    ap.add_argument("--workload", choices=["osem", "batched", "served", "gradient", "units", "psf"], default="osem",
                     help="osem: the headline OS-EM line (default); batched / served: the concurrent patterns; "
-                         "gradient: the differentiable projector; units: the reference's published timings")
+                         "gradient: the differentiable projector; units: the reference's published timings; "
+                         "psf: direct vs FFT PSF methods on general PSFs")
     ap.add_argument("--requests", type=int, default=100, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="served: Poisson arrival rate per second")
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,6 +474,8 @@ def main():
         return gradient_workload(max(1, args.steps))
     if args.workload == "units":
         return units_workload()
+    if args.workload == "psf":
+        return psf_workload()
     if args.workload != "osem":
         return stream_workload(args.workload, args.requests, args.rate, autotune=not args.no_autotune)
     if S.device_count() == 0:

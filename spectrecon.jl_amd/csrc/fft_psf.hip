@@ -10,11 +10,13 @@
 // the fly as A = (Z[f] + conj Z[-f]) / 2, B = -i (Z[f] - conj Z[-f]) / 2.  The z
 // transform then runs over the half spectrum f = 0..L1/2 only.  Each 1-D
 // transform is two radix-R passes in LDS (L = R1 R2, R <= 16, in-register
-// DFTs): the forward passes are decimation in frequency and leave the
-// spectrum in digit-reversed order (position k1 + R1 k2 holds frequency
-// k2 + R2 k1), the inverse passes are decimation in time and take that order
-// back to natural order, so no reordering pass exists.  The kernel's
-// spectrum is stored in the same order.
+// DFTs): the forward passes are decimation in frequency, the inverse passes
+// decimation in time.  The x passes leave the x spectrum in natural order
+// (the z passes read neighbouring frequencies on distinct banks); the z
+// passes leave the z spectrum in digit-reversed order (position k1 + R1 k2
+// holds frequency k2 + R2 k1) and the inverse z passes take that order back
+// to natural order, so no reordering pass exists.  The kernel's spectrum is
+// stored in the z passes' order.
 //
 // The PSFs are point-symmetric (src/spectplan.jl:83), so the spectrum of a
 // plane's kernel, centred at index 0 (what pad2sizezero! followed by
@@ -129,13 +131,6 @@ __device__ __forceinline__ void dft(float2 (&a)[R]) {
   }
 }
 
-// storage position of x-frequency f after the forward x passes
-template <int L1>
-__device__ __forceinline__ int xpos(int f) {
-  constexpr int R1 = Split<L1>::R1, R2 = Split<L1>::R2;
-  return f / R2 + R1 * (f % R2);
-}
-
 // LDS regions (float2): the twiddles W_L1^m, W_L2^m; region A, the x-buffer
 // [pos][pair] (stride NPS) and in turn the rows' half spectra I [r][f]
 // (stride NF); region B, the z-buffer of one half of the columns [pos][fl]
@@ -154,7 +149,7 @@ __device__ __forceinline__ void z_pass1_fwd(const float2* A, float2* B, const fl
   for (int it = threadIdx.x; it < NFH * R1z; it += NT) {
     const int fl = it % NFH, n1 = it / NFH, f = h * NFH + fl;
     if (f >= NF) continue;
-    const int pf = xpos<L1>(f), pm = xpos<L1>((L1 - f) & (L1 - 1));
+    const int pf = f, pm = (L1 - f) & (L1 - 1);
     float2 a[R2z];
 #pragma unroll
     for (int n2 = 0; n2 < R2z; ++n2) {
@@ -196,8 +191,8 @@ struct SpecItems {
 // x passes of the forward transform of a plane's row pairs into A: pass 1
 // gathers row pair j at padded positions n1 + R1x n2 straight from global
 // memory (LOAD(p, j) gives the pair's two values at padded position p), pass 2
-// completes the transform in place.  Items of pass 1 are (n1, j) with n1
-// fastest, so a wave reads consecutive x.
+// completes the transform.  Items of pass 1 are (n1, j) with n1 fastest, so a
+// wave reads consecutive x.
 template <int NT, int L1, class Load>
 __device__ __forceinline__ void x_passes_fwd(float2* A, const float2* tw1, int NPS, int NP, Load load) {
   constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2;
@@ -211,14 +206,30 @@ __device__ __forceinline__ void x_passes_fwd(float2* A, const float2* tw1, int N
     for (int k2 = 0; k2 < R2x; ++k2) A[(n1 + R1x * k2) * NPS + j] = k2 ? cmul(a[k2], tw1[n1 * k2]) : a[k2];
   }
   __syncthreads();
-  for (int it = threadIdx.x; it < NP * R2x; it += NT) {
-    const int j = it % NP, k2 = it / NP;
-    float2 a[R1x];
+  // pass 2 leaves frequency f at position f (natural order, read all items
+  // first): the z passes then read neighbouring frequencies from
+  // neighbouring rows of A, NPS (odd) float2 apart, on distinct LDS banks
+  constexpr int RA = kReadAll * kNTF / NT;
+  float2 xr[RA][R1x];
 #pragma unroll
-    for (int n1 = 0; n1 < R1x; ++n1) a[n1] = A[(n1 + R1x * k2) * NPS + j];
-    dft<R1x, false>(a);
+  for (int s = 0; s < RA; ++s) {
+    const int it = threadIdx.x + s * NT;
+    if (it < NP * R2x) {
+      const int j = it % NP, k2 = it / NP;
 #pragma unroll
-    for (int k1 = 0; k1 < R1x; ++k1) A[(k1 + R1x * k2) * NPS + j] = a[k1];
+      for (int n1 = 0; n1 < R1x; ++n1) xr[s][n1] = A[(n1 + R1x * k2) * NPS + j];
+      dft<R1x, false>(xr[s]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < RA; ++s) {
+    const int it = threadIdx.x + s * NT;
+    if (it < NP * R2x) {
+      const int j = it % NP, k2 = it / NP;
+#pragma unroll
+      for (int k1 = 0; k1 < R1x; ++k1) A[(k2 + R2x * k1) * NPS + j] = xr[s][k1];
+    }
   }
   __syncthreads();
 }
@@ -639,7 +650,7 @@ bool psf_fft_plan(const Geo& g, FftPlan* fp) {
   f.up = (f.L1 - g.nx + 1) / 2;  // ceil((L - n) / 2), src/helper.jl:9-12
   f.left = (f.L2 - g.nz + 1) / 2;
   const int NP = (g.nz + 1) / 2;
-  f.NPS = NP;
+  f.NPS = NP | 1;  // (odd: rows of A start on different LDS banks)
   const int NF = f.L1 / 2 + 1, NFH = (NF + 1) / 2;
   const int R1z = split_r1(f.L2), R2x = f.L1 / split_r1(f.L1);
   const long a = std::max((long)f.L1 * f.NPS, (long)g.nz * NF);  // x-buffer / rows' half spectra
