@@ -161,9 +161,10 @@ def stream_workload(kind, nreq, rate, seed=7, autotune=True):
     batched: 32 seeded instances over nx in {64,96,128}, nz in {32,64,80},
       views in {60,120,128}, square PSF side in {15,25,37}, subsets in {4,8}
       (strided, uneven where the subsets do not divide the views), OS-EM x 4
-      iterations, all submitted at once: reconstructed one at a time, then
-      concurrently on resident plans; every concurrent result must equal its
-      one-at-a-time result.
+      iterations, device-resident inputs on resident plans: all from one
+      sr_osem_batch call (concurrent streams, HIP-graph replays) against one
+      call per instance; every batched result must equal its one-at-a-time
+      result.
     served: `nreq` requests with Poisson arrivals at `rate` per second, 70%
       128x128x80 / 128 views / 37x37 PSF and 30% 64x64x32 / 60 views / 15x15
       PSF, OS-EM 8 x 4 (strided subsets: 60 views over 8 blocks)."""
@@ -172,13 +173,11 @@ def stream_workload(kind, nreq, rate, seed=7, autotune=True):
     rng = np.random.default_rng(seed)
     plans = {}
     reqs = []
+    inst = []
     if kind == "batched":
-        for k in range(32):
-            nx = int(rng.choice([64, 96, 128]))
-            cfg = V.Config(nx, int(rng.choice([32, 64, 80])), int(rng.choice([60, 120, 128])),
-                           int(rng.choice([15, 25, 37])), DY, 1.0, 4.0)
-            reqs.append(V.synthetic_request(cfg, seed=1 + k, nblocks=int(rng.choice([4, 8])), niter=4, rid=k,
-                                            plans=plans))
+        for k, (cfg, nblocks) in enumerate(V.batched_instances(seed)):
+            inst.append((cfg, nblocks))
+            reqs.append(V.synthetic_request(cfg, seed=1 + k, nblocks=nblocks, niter=4, rid=k, plans=plans))
     else:
         big = V.Config(NX, NZ, NVIEW, PSIZE, DY, 1.5, 6.0)
         small = V.Config(64, 32, 60, 15, DY, 1.0, 4.0)
@@ -187,6 +186,8 @@ def stream_workload(kind, nreq, rate, seed=7, autotune=True):
             t += float(rng.exponential(1.0 / rate))
             cfg = big if rng.random() < 0.7 else small
             reqs.append(V.synthetic_request(cfg, seed=1000 + k, nblocks=8, niter=4, arrival=t, rid=k, plans=plans))
+            if (k + 1) % 500 == 0:
+                log(f"served: {k + 1} requests prepared")
     for p in plans.values():
         p.close()
     log(f"{kind}: {len(reqs)} requests prepared")
@@ -195,29 +196,60 @@ def stream_workload(kind, nreq, rate, seed=7, autotune=True):
         nsubs.setdefault(r.config, set()).add(-(-r.config.nview // r.nblocks))
     out = {"workload": kind, "requests": len(reqs), "data": "synthetic (seeded phantoms, Poisson counts)"}
     if kind == "batched":
-        import copy
-        # resident plans built first (the batched pattern keeps them), then the
-        # same instances one at a time and concurrently
-        alone = [copy.copy(r) for r in reqs]
-        srv = V.Server(hbm_budget=200e9, max_active=1, backend=V.DeviceBackend(autotune=autotune))
-        try:
-            srv.prepare({r.config for r in reqs}, per_config=2, nsubs=nsubs)
-            st1 = srv.run(alone)
-            srv.reset_stats()
-            srv.max_active = 8
-            st8 = srv.run(reqs)
-        finally:
-            srv.close()
-        same = all(np.array_equal(a.result, b.result) for a, b in zip(alone, reqs))
-        out.update({"metric": "batched uneven OS-EM recons/s (32 instances)", "unit": "recons/s",
-                    "value": st8["recons_per_s"], "one_at_a_time": st1["recons_per_s"],
-                    "speedup": st8["recons_per_s"] / st1["recons_per_s"], "results_equal_alone": bool(same),
-                    "concurrent": st8, "sequential": st1})
+        # one resident plan per instance (built and, per configuration and
+        # subset size, autotuned first: the batched pattern keeps them), the
+        # inputs uploaded; then all instances from one sr_osem_batch call
+        # (their plans' streams run concurrently; each plan replays its EM
+        # sequence from a HIP graph after two calls), against one call per
+        # instance, each waited for
+        backend = V.DeviceBackend(autotune=autotune)
+        t0 = time.perf_counter()
+        bplans = [backend.make_plan(cfg) for cfg, _ in inst]
+        for plan, r in zip(bplans, reqs):
+            if autotune:
+                nsub = -(-r.config.nview // r.nblocks)
+                if (r.config, nsub) not in backend.tuned:
+                    backend.tune(plan, r.config, nsub)
+                plan.set_choice(**backend.tuned[(r.config, nsub)])
+        prep_s = time.perf_counter() - t0
+        batch = V.Batch(bplans, reqs)
+        for _ in range(2):  # (eager first call, graph capture on the second)
+            batch.run()
+        t_one = float(np.median([batch.run(sequential=True) for _ in range(3)]))
+        alone = batch.results()
+        # the plain path: the drop-in call per request (host arrays, staged by
+        # the library), one request after the other
+        import spectrecon_amd as S
+        t0 = time.perf_counter()
+        plain = [S.osem_strided(r.x0, r.y, r.b, p, r.nblocks, r.niter) for p, r in zip(bplans, reqs)]
+        t_plain = time.perf_counter() - t0
+        t_all = float(np.median([batch.run() for _ in range(5)]))
+        res = batch.results()
+        same = all(np.array_equal(a, b) for a, b in zip(alone, res)) and all(
+            np.array_equal(a, b) for a, b in zip(plain, res))
+        caps = [p.graph_stats() for p in bplans]
+        batch.free()
+        for p in bplans:
+            backend.close_plan(p)
+        out.update({"metric": f"batched uneven OS-EM recons/s ({len(reqs)} instances, one sr_osem_batch call)",
+                    "unit": "recons/s", "value": len(reqs) / t_all,
+                    "plain_one_at_a_time": len(reqs) / t_plain, "speedup_vs_plain": t_plain / t_all,
+                    "resident_one_at_a_time": len(reqs) / t_one, "speedup_vs_resident": t_one / t_all,
+                    "results_equal_alone": bool(same), "batch_ms": 1e3 * t_all, "plain_ms": 1e3 * t_plain,
+                    "resident_sequential_ms": 1e3 * t_one, "prepare_s": prep_s,
+                    "baselines": "plain: one drop-in call per instance with host arrays (sr_osem_strided, "
+                                 "SR_MEM_HOST), each returning before the next; resident: one sr_osem_batch call "
+                                 "per instance on the same device-resident inputs and graph replays, each waited "
+                                 "for",
+                    "graphs": {"captured": sum(c[0] for c in caps), "replays": sum(c[1] for c in caps)},
+                    "instances": [{"nx": c.nx, "nz": c.nz, "nview": c.nview, "psf": c.psf, "nblocks": nb}
+                                  for c, nb in inst],
+                    "autotune": backend.tune_log})
     else:
         srv = V.Server(hbm_budget=200e9, max_active=8, backend=V.DeviceBackend(autotune=autotune))
         try:
             srv.prepare({r.config for r in reqs}, per_config=2, nsubs=nsubs)
-            st = srv.run(reqs)
+            st = srv.run(reqs, progress=lambda i: log(f"served: {i} requests submitted"))
         finally:
             srv.close()
         out.update({"metric": f"served OS-EM stream, Poisson arrivals {rate:g}/s", "unit": "recons/s",

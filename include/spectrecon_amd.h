@@ -37,6 +37,10 @@ extern "C" {
 #define SR_MEM_HOST 0
 #define SR_MEM_DEVICE 1
 #define SR_ASYNC 0x100
+/* mlem / osem with device arrays: the caller has checked background > 0, so
+ * the call skips its device scan of b (which waits for the stream) and an
+ * SR_ASYNC call returns without any host wait */
+#define SR_NO_CHECK 0x200
 
 #define SR_INTERP_ONE 1 /* :one, 3-pass 1-D linear interpolation          */
 #define SR_INTERP_TWO 2 /* :two, single-pass bilinear (reference default) */
@@ -170,6 +174,29 @@ int sr_osem_strided(sr_plan plan, float* out, const float* x0, const float* y, c
  * SR_ASYNC call), else 0; never blocks.  Several plans (each with its own
  * stream) run their queued work concurrently on the device. */
 int sr_plan_query(sr_plan plan, int32_t* done);
+/* An asynchronous copy on the plan's stream, ordered with the plan's work
+ * (no reference counterpart): for serving loops that stage request arrays
+ * through pinned host memory (sr_host_alloc) while other plans compute. */
+#define SR_COPY_H2D 1
+#define SR_COPY_D2H 2
+#define SR_COPY_D2D 3
+int sr_plan_memcpy_async(sr_plan plan, void* dst, const void* src, size_t bytes, int32_t kind);
+/* page-locked host memory (hipHostMalloc), for asynchronous copies */
+int sr_host_alloc(void** ptr, size_t bytes);
+int sr_host_free(void* ptr);
+/* Batched OS-EM (no reference counterpart): n reconstructions, instance i on
+ * plans[i] (distinct plans) with device arrays outs[i], x0s[i], ys[i], bs[i]
+ * ([nx,nz,nview] of that plan) over the strided subsets of sr_osem_strided
+ * with nblocks[i] blocks and niters[i] iterations.  All instances are checked
+ * first (the reference's messages; nothing runs if one fails; b > 0 unless
+ * SR_NO_CHECK), then every instance's kernel sequence is queued on its plan's
+ * stream (replayed from the plan's HIP graph when sr_plan_set_graph is on),
+ * so the device runs the instances concurrently.  Returns after all finished,
+ * or at once with SR_ASYNC (then sr_plan_query / sr_plan_synchronize per
+ * plan).  `mem` must be SR_MEM_DEVICE (| SR_ASYNC | SR_NO_CHECK). */
+int sr_osem_batch(int32_t n, const sr_plan* plans, float* const* outs, const float* const* x0s,
+                  const float* const* ys, const float* const* bs, const int32_t* nblocks, const int32_t* niters,
+                  int32_t mem);
 /* HIP-graph replay of the EM kernel sequence (no reference counterpart; for
  * callers that run many small reconstructions, whose time is in launches).
  * With enable != 0, sr_mlem / sr_osem / sr_osem_strided run their first call

@@ -197,7 +197,7 @@ struct sr_plan_s {
   DBuf zh;          // OS-EM: the PSF adjoint of a ones view (one PSF stack for all views)
   DBuf gview, gwork2, gpart, gloss;  // sr_grad: residual views, R(x) .* atten, loss partial sums
   int zk = 0;        // slots per target of zent
-  std::vector<int> h_vids;
+  std::vector<int> h_vids, h_order;
 
   // per-kernel event timing (sr_plan_set_timing)
   int timing = 0;  // bit mask of the kernel classes bracketed by events
@@ -1461,9 +1461,11 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
   int rc;
   // background > 0 (src/ml-os-em.jl:30, :106), checked before any update:
   // on the host for host arrays, else by a device scan that returns one flag
+  // (which waits for the stream: SR_NO_CHECK, for callers that checked b,
+  // leaves an SR_ASYNC call free of host waits)
   if (host) {
     if ((rc = check_background(b, (size_t)M))) return rc;
-  } else {
+  } else if (!(mem & SR_NO_CHECK)) {
     if ((rc = p->flag.ensure(sizeof(int)))) return rc;
     HIP_TRY(sr::launch_flag_nonpositive(b, M, p->flag.as<int>(), p->stream));
     int bad = 0;
@@ -1485,8 +1487,10 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
     c.b = p->st_aux2.as<float>();
     c.out = p->st_img.as<float>();
   }
-  // subset view lists, uploaded once
-  std::vector<int> order;
+  // subset view lists, uploaded once (from a plan member: the copy may still
+  // read it after an SR_ASYNC call returns)
+  std::vector<int>& order = p->h_order;
+  order.clear();
   for (int nb = 0; nb < nblocks; ++nb)
     for (int v = nb; v < nview; v += nblocks) order.push_back(v);
   // block nb: views nb, nb + nblocks, ... (src/ml-os-em.jl:76); counts
@@ -1556,6 +1560,81 @@ int sr_osem_strided(sr_plan p, float* out, const float* x0, const float* y, cons
   if (nblocks < 1 || nblocks > p->d.nview) return fail(SR_ERR_ARG, "nblocks must be in 1..nview");
   if (niter < 0) return fail(SR_ERR_ARG, "niter < 0");
   return run_em(p, out, x0, y, b, nblocks, niter, mem);
+}
+
+int sr_plan_memcpy_async(sr_plan p, void* dst, const void* src, size_t bytes, int32_t kind) {
+  if (!p || (!dst && bytes) || (!src && bytes)) return fail(SR_ERR_ARG, "null argument");
+  if (kind != SR_COPY_H2D && kind != SR_COPY_D2H && kind != SR_COPY_D2D) return fail(SR_ERR_ARG, "bad copy kind");
+  if (bytes == 0) return SR_OK;
+  const hipMemcpyKind k = kind == SR_COPY_H2D   ? hipMemcpyHostToDevice
+                          : kind == SR_COPY_D2H ? hipMemcpyDeviceToHost
+                                                : hipMemcpyDeviceToDevice;
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, p->stream));
+  return SR_OK;
+}
+
+int sr_host_alloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(SR_ERR_ARG, "null pointer");
+  *ptr = nullptr;
+  const hipError_t e = hipHostMalloc(ptr, std::max<size_t>(bytes, 1), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return fail(SR_ERR_ALLOC, "hipHostMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  return SR_OK;
+}
+
+int sr_host_free(void* ptr) {
+  if (ptr) HIP_TRY(hipHostFree(ptr));
+  return SR_OK;
+}
+
+// Many reconstructions, each on its own plan (and stream), from one call:
+// all are checked first (the reference's messages; nothing runs when one
+// fails), then each one's EM sequence is queued on its plan's stream (a HIP
+// graph replay where the plan has graphs on), so the device runs them
+// concurrently; returns when all finished, or at once with SR_ASYNC.
+int sr_osem_batch(int32_t n, const sr_plan* plans, float* const* outs, const float* const* x0s,
+                  const float* const* ys, const float* const* bs, const int32_t* nblocks, const int32_t* niters,
+                  int32_t mem) {
+  if (n < 0 || (n > 0 && (!plans || !outs || !x0s || !ys || !bs || !nblocks || !niters)))
+    return fail(SR_ERR_ARG, "null argument");
+  if ((mem & 0xff) != SR_MEM_DEVICE) return fail(SR_ERR_ARG, "sr_osem_batch takes device arrays (SR_MEM_DEVICE)");
+  for (int i = 0; i < n; ++i) {
+    sr_plan p = plans[i];
+    if (!p || !outs[i] || !x0s[i] || !ys[i] || !bs[i]) return fail(SR_ERR_ARG, "null argument (instance %d)", i);
+    if (nblocks[i] < 1 || nblocks[i] > p->d.nview) return fail(SR_ERR_ARG, "nblocks must be in 1..nview (instance %d)", i);
+    if (niters[i] < 0) return fail(SR_ERR_ARG, "niter < 0 (instance %d)", i);
+    for (int k = 0; k < i; ++k)
+      if (plans[k] == p) return fail(SR_ERR_ARG, "instances %d and %d share a plan (a plan runs one call at a time)", k, i);
+  }
+  if (!(mem & SR_NO_CHECK)) {
+    // every background scanned on its plan's stream, the flags read once
+    for (int i = 0; i < n; ++i) {
+      sr_plan p = plans[i];
+      int rc;
+      if ((rc = p->flag.ensure(sizeof(int)))) return rc;
+      HIP_TRY(sr::launch_flag_nonpositive(bs[i], p->P * p->d.nview, p->flag.as<int>(), p->stream));
+    }
+    for (int i = 0; i < n; ++i) {
+      int bad = 0;
+      HIP_TRY(hipMemcpyAsync(&bad, plans[i]->flag.p, sizeof(int), hipMemcpyDeviceToHost, plans[i]->stream));
+      HIP_TRY(hipStreamSynchronize(plans[i]->stream));
+      if (bad) return fail(SR_ERR_ARG, "need background > 0");
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    const int rc = run_em(plans[i], outs[i], x0s[i], ys[i], bs[i], nblocks[i], niters[i],
+                          SR_MEM_DEVICE | SR_ASYNC | SR_NO_CHECK);
+    if (rc) return rc;
+  }
+  if (mem & SR_ASYNC) return SR_OK;
+  for (int i = 0; i < n; ++i) {
+    HIP_TRY(hipStreamSynchronize(plans[i]->stream));
+    resolve_timing(plans[i]);
+  }
+  return SR_OK;
 }
 
 int sr_plan_query(sr_plan p, int32_t* done) {

@@ -14,7 +14,8 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_PKG), "lib", "libspectrecon_amd.so")
 
 SR_OK, SR_ERR_ARG, SR_ERR_DIM, SR_ERR_HIP, SR_ERR_ALLOC, SR_ERR_UNSUPPORTED = range(6)
-SR_MEM_HOST, SR_MEM_DEVICE, SR_ASYNC = 0, 1, 0x100
+SR_MEM_HOST, SR_MEM_DEVICE, SR_ASYNC, SR_NO_CHECK = 0, 1, 0x100, 0x200
+SR_COPY_H2D, SR_COPY_D2H, SR_COPY_D2D = 1, 2, 3
 SR_INTERP_ONE, SR_INTERP_TWO = 1, 2
 SR_MODE_FAST, SR_MODE_MEM = 0, 1
 SR_CONV_AUTO, SR_CONV_DIRECT, SR_CONV_SEPARABLE, SR_CONV_FFT = 0, 1, 2, 3
@@ -63,6 +64,10 @@ SIGNATURES = {
     "sr_project_vjp_mu": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32]),
     "sr_osem_strided": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_int32]),
     "sr_plan_query": (C.c_int, [_vp, _ip]),
+    "sr_plan_memcpy_async": (C.c_int, [_vp, _vp, _vp, C.c_size_t, C.c_int32]),
+    "sr_host_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "sr_host_free": (C.c_int, [_vp]),
+    "sr_osem_batch": (C.c_int, [C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int32]),
     "sr_plan_set_graph": (C.c_int, [_vp, C.c_int32]),
     "sr_plan_graph_stats": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "sr_plan_set_choice": (C.c_int, [_vp, _ip]),

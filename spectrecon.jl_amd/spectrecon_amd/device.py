@@ -59,3 +59,31 @@ class DeviceArray:
 
     def __repr__(self):
         return f"DeviceArray(shape={self.shape}, dtype={self.dtype})"
+
+
+class PinnedArray:
+    """Page-locked host FP32 memory (sr_host_alloc) with a numpy view
+    (`.array`, column-major): the source or target of asynchronous copies on a
+    plan's stream (sr_plan_memcpy_async)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().sr_host_alloc(C.byref(p), max(self.nbytes, 4)))
+        self.ptr = p.value
+        buf = (C.c_char * max(self.nbytes, 4)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, self.dtype, count=int(np.prod(self.shape))).reshape(self.shape, order="F")
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().sr_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

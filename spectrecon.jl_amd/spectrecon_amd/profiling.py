@@ -16,7 +16,7 @@ _CLASSES = (
     ("k_rot2_adj", "rot_adj"), ("k_rot1_adj", "rot_adj"),
     ("k_psf_fwd", "psf_fwd"), ("k_psf_adj", "psf_adj"),
     ("k_view_reduce", "view_reduce"), ("k_img_reduce", "img_reduce"),
-    ("k_atten_exact", "atten"), ("k_atten_z", "atten"), ("k_to_zfast", "rot_fwd"), ("k_copy16", "copy"),
+    ("k_atten_scan", "atten"), ("k_to_zfast", "rot_fwd"), ("k_copy16", "copy"),
 )
 
 

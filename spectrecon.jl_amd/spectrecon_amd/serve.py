@@ -4,11 +4,12 @@ No reference counterpart: the reference runs one `osem!` at a time.  Here a
 request loop keeps plans (mu-map rotations, attenuation factors, PSF taps,
 rotation-adjoint lists) resident on the device, pooled per configuration, and
 admits requests by HBM footprint.  Every plan owns one HIP stream, so admitted
-requests run concurrently on the device (one host thread per admitted request
-drives its plan), and a request that arrives while others run is admitted as
-soon as a plan of its configuration and the memory for it are free.  The
-kernels of one plan do not depend on other plans, so a result is the same
-alone and in the mix.
+requests run concurrently on the device (each queued asynchronously on its
+plan's stream from the one loop thread), and a request that arrives while
+others run is admitted as soon as a plan of its configuration and the memory
+for it are free.  The kernels of one plan do not depend on other plans, so a
+result is the same alone and in the mix.  `Batch` runs many instances from one
+library call (sr_osem_batch).
 
 The queue and admission logic is backend-independent (`Server(backend=...)`);
 `DeviceBackend` drives the library.
@@ -20,6 +21,8 @@ import dataclasses
 import time
 
 import numpy as np
+
+from ._lib import DimensionMismatch, SPECTError
 
 
 @dataclasses.dataclass(frozen=True)
@@ -67,31 +70,49 @@ class Request:
     start: float | None = None
     finish: float | None = None
     rejected: bool = False
+    error: str | None = None  # why a rejected request was rejected
 
     @property
     def latency(self):
         return None if self.finish is None else self.finish - self.arrival
 
 
-class DeviceBackend:
-    """Plans and OS-EM on the device through the C-ABI.  An admitted request
-    runs on a host thread of its own that calls `sr_osem_strided` with host
-    arrays: the call stages them through its plan's buffers on the plan's
-    stream and releases the interpreter lock while the device works, so the
-    admitted requests' streams run concurrently (no device-wide allocation,
-    free or null-stream copy is made per request)."""
+class _Staging:
+    """A plan's request buffers, made once per plan: pinned host arrays (the
+    request is copied in and the result out with plain numpy copies) and
+    device arrays (the reconstruction's inputs and output)."""
 
-    def __init__(self, autotune: bool = False):
-        from .api import SPECTplan, mumap_phantom, osem_strided, psf_gauss
-        self._SPECTplan, self._mumap, self._psf, self._osem = SPECTplan, mumap_phantom, psf_gauss, osem_strided
+    def __init__(self, cfg: Config):
+        from .device import DeviceArray, PinnedArray
+        img, views = cfg.image_shape(), cfg.views_shape()
+        self.h = {k: PinnedArray(img if k in ("x0", "out") else views) for k in ("x0", "y", "b", "out")}
+        self.d = {k: DeviceArray(img if k in ("x0", "out") else views) for k in ("x0", "y", "b", "out")}
+
+
+class DeviceBackend:
+    """Plans and OS-EM on the device through the C-ABI, with no host thread
+    per request.  An admitted request's arrays go through its plan's pinned
+    buffers and asynchronous copies on the plan's stream; the reconstruction
+    is queued there with SR_ASYNC (the background was checked on the host,
+    SR_NO_CHECK) and its result copied back behind it, so `start` returns at
+    once and `done` polls the stream (sr_plan_query).  Plans replay their EM
+    kernel sequence from a HIP graph (sr_plan_set_graph), captured on a
+    plan's second request of a given subset count.  Admitted requests run on
+    different plans, so their streams overlap on the device."""
+
+    def __init__(self, autotune: bool = False, graphs: bool = True):
+        from .api import SPECTplan, mumap_phantom, psf_gauss
+        self._SPECTplan, self._mumap, self._psf = SPECTplan, mumap_phantom, psf_gauss
         # autotune: kernel choices are timed once per (config, subset size),
         # on one plan of the config (in Server.prepare, before concurrent
         # work, or else on first use), and set on whichever plan of the
         # config serves a request of that size, so that a request's result
         # does not depend on the plan it lands on
         self.autotune = autotune
+        self.graphs = graphs
         self.tuned = {}     # (config, nsub) -> kernel choices
         self.tune_log = []  # {"config", "nsub", "choice", "ms", "us"} per tuning
+        self.staging = {}   # id(plan) -> _Staging
 
     def tune(self, plan, cfg: Config, nsub: int):
         key = (cfg, nsub)
@@ -109,10 +130,19 @@ class DeviceBackend:
         mumap = self._mumap(cfg.nx, cfg.nx, cfg.nz)
         psf = self._psf(ny=cfg.nx, px=cfg.psf, fwhm_start=cfg.fwhm_start, fwhm_end=cfg.fwhm_end)
         psfs = np.asfortranarray(np.repeat(psf[:, :, :, None], cfg.nview, 3))
-        return self._SPECTplan(mumap, psfs, cfg.dy)
+        plan = self._SPECTplan(mumap, psfs, cfg.dy)
+        if self.graphs:
+            plan.set_graph(True)
+        return plan
+
+    def _stage(self, plan, cfg):
+        st = self.staging.get(id(plan))
+        if st is None:
+            st = self.staging[id(plan)] = _Staging(cfg)
+        return st
 
     def start(self, plan, req: Request):
-        import threading
+        from . import _lib as L
         if self.autotune:
             nsub = -(-req.config.nview // req.nblocks)
             choice = self.tuned.get((req.config, nsub))
@@ -120,31 +150,94 @@ class DeviceBackend:
                 self.tune(plan, req.config, nsub)
             elif plan.choice() != choice:
                 plan.set_choice(**choice)
-        box = {}
-
-        def work():
-            try:
-                box["out"] = self._osem(req.x0, req.y, req.b, plan, req.nblocks, req.niter)
-            except Exception as e:  # reported by result()
-                box["err"] = e
-        t = threading.Thread(target=work, daemon=True)
-        t.start()
-        return (t, box)
+        if not np.all(req.b > 0):  # src/ml-os-em.jl:106, before any device work
+            raise SPECTError("need background > 0")
+        st = self._stage(plan, req.config)
+        lib = L.lib()
+        h = plan.handle
+        for k, a in (("x0", req.x0), ("y", req.y), ("b", req.b)):
+            st.h[k].array[...] = a
+            L.check(lib.sr_plan_memcpy_async(h, st.d[k].ptr, st.h[k].ptr, st.d[k].nbytes, L.SR_COPY_H2D))
+        L.check(lib.sr_osem_strided(h, st.d["out"].ptr, st.d["x0"].ptr, st.d["y"].ptr, st.d["b"].ptr,
+                                    int(req.nblocks), int(req.niter),
+                                    L.SR_MEM_DEVICE | L.SR_ASYNC | L.SR_NO_CHECK))
+        L.check(lib.sr_plan_memcpy_async(h, st.h["out"].ptr, st.d["out"].ptr, st.d["out"].nbytes, L.SR_COPY_D2H))
+        return st
 
     def done(self, plan, token) -> bool:
-        return not token[0].is_alive()
+        import ctypes as C
+
+        from . import _lib as L
+        d = C.c_int32()
+        L.check(L.lib().sr_plan_query(plan.handle, C.byref(d)))
+        return bool(d.value)
 
     def result(self, plan, token):
-        token[0].join()
-        if "err" in token[1]:
-            raise token[1]["err"]
-        return token[1]["out"]
+        plan.synchronize()
+        return np.array(token.h["out"].array, order="F")
 
     def release(self, plan, token):
         pass
 
     def close_plan(self, plan):
+        st = self.staging.pop(id(plan), None)
         plan.close()
+        if st is not None:
+            for a in list(st.h.values()) + list(st.d.values()):
+                a.free()
+
+
+class Batch:
+    """Batched OS-EM on resident plans: request i on plans[i] (distinct
+    plans), its inputs uploaded to device arrays once.  `run()` runs all
+    reconstructions from one sr_osem_batch call (the instances' streams run
+    concurrently); `run(sequential=True)` one call per instance, each waited
+    for.  The device buffers stay between runs, so plans with graphs on
+    capture their EM sequence on the second run and replay it after that."""
+
+    def __init__(self, plans, requests):
+        import ctypes as C
+
+        from .device import DeviceArray
+        self.plans, self.requests = list(plans), list(requests)
+        n = len(self.requests)
+        self.dev = [{k: DeviceArray.from_host(getattr(r, k)) for k in ("x0", "y", "b")} for r in self.requests]
+        self.outs = [DeviceArray(r.config.image_shape()) for r in self.requests]
+        self._P = (C.c_void_p * n)(*[p.handle for p in self.plans])
+        self._arr = {k: (C.c_void_p * n)(*[d[k].ptr for d in self.dev]) for k in ("x0", "y", "b")}
+        self._out = (C.c_void_p * n)(*[o.ptr for o in self.outs])
+        self._nb = (C.c_int32 * n)(*[int(r.nblocks) for r in self.requests])
+        self._ni = (C.c_int32 * n)(*[int(r.niter) for r in self.requests])
+
+    def run(self, sequential=False):
+        """Seconds for all reconstructions (returns after they finished)."""
+        import ctypes as C
+
+        from . import _lib as L
+        lib = L.lib()
+        L.check(lib.sr_device_synchronize())
+        t0 = time.perf_counter()
+        if sequential:
+            V = C.c_void_p
+            for i in range(len(self.requests)):
+                L.check(lib.sr_osem_batch(1, C.byref(V(self._P[i])), C.byref(V(self._out[i])),
+                                          C.byref(V(self._arr["x0"][i])), C.byref(V(self._arr["y"][i])),
+                                          C.byref(V(self._arr["b"][i])), C.byref(C.c_int32(self._nb[i])),
+                                          C.byref(C.c_int32(self._ni[i])), L.SR_MEM_DEVICE))
+        else:
+            L.check(lib.sr_osem_batch(len(self.requests), self._P, self._out, self._arr["x0"], self._arr["y"],
+                                      self._arr["b"], self._nb, self._ni, L.SR_MEM_DEVICE))
+        return time.perf_counter() - t0
+
+    def results(self):
+        return [o.to_host() for o in self.outs]
+
+    def free(self):
+        for d in self.dev:
+            for a in d.values():
+                a.free()
+        for o in self.outs:
+            o.free()
 
 
 class Server:
@@ -229,12 +322,19 @@ class Server:
                 pending.append(req)
                 continue
             req.start = now
-            self.busy[id(plan)] = (plan, req, self.backend.start(plan, req))
+            try:
+                token = self.backend.start(plan, req)
+            except (SPECTError, DimensionMismatch) as e:  # the request's argument checks
+                req.rejected, req.error = True, str(e)
+                self.rejected.append(req)
+                continue
+            self.busy[id(plan)] = (plan, req, token)
         self.queue = pending
         return len(self.busy) + len(self.queue)
 
-    def run(self, requests, poll_interval=2e-4):
-        """Replay requests at their `arrival` offsets (seconds from now)."""
+    def run(self, requests, poll_interval=2e-4, progress=None):
+        """Replay requests at their `arrival` offsets (seconds from now);
+        `progress(submitted)` is called after every 250th submission."""
         t0 = self.clock()
         order = sorted(requests, key=lambda r: r.arrival)
         for r in order:
@@ -245,6 +345,8 @@ class Server:
             while i < len(order) and order[i].arrival <= now:
                 self.submit(order[i])
                 i += 1
+                if progress and i % 250 == 0:
+                    progress(i)
             if not self.poll() and i < len(order):
                 time.sleep(max(0.0, min(poll_interval, order[i].arrival - self.clock())))
             elif self.busy:
@@ -282,6 +384,21 @@ class Server:
                 self.backend.close_plan(plan)
         self.pools.clear()
         self.resident = 0.0
+
+
+def batched_instances(seed: int = 7, n: int = 32, dy: float = 0.48):
+    """The batched uneven workload's instances, drawn from a seed: nx in
+    {64, 96, 128}, nz in {32, 64, 80}, views in {60, 120, 128}, square
+    Gaussian PSF side in {15, 25, 37} (FWHM 1 -> 4 px), subsets in {4, 8};
+    [(config, nblocks)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx = int(rng.choice([64, 96, 128]))
+        cfg = Config(nx, int(rng.choice([32, 64, 80])), int(rng.choice([60, 120, 128])), int(rng.choice([15, 25, 37])),
+                     dy, 1.0, 4.0)
+        out.append((cfg, int(rng.choice([4, 8]))))
+    return out
 
 
 def synthetic_request(cfg: Config, seed: int, nblocks: int, niter: int, arrival: float = 0.0, rid: int = 0,

@@ -1020,3 +1020,95 @@ def test_fft_psf_osem_and_methods(gpu, O, method):
     if zf.info()["layout"] == "z-fastest":
         with pytest.raises(gpu.DeviceError):
             zf.set_choice(psf_method=2)
+
+
+def _serve_oracle(O, cfg, req):
+    """The FP64 oracle's OS-EM of a served / batched request (the plan's
+    phantom mu-map and Gaussian PSFs, strided subsets as sr_osem_strided)."""
+    import spectrecon_amd as S
+    mumap = S.mumap_phantom(cfg.nx, cfg.nx, cfg.nz)
+    psf = S.psf_gauss(ny=cfg.nx, px=cfg.psf, fwhm_start=cfg.fwhm_start, fwhm_end=cfg.fwhm_end)
+    psfs = np.asfortranarray(np.repeat(psf[:, :, :, None], cfg.nview, 3))
+    P64 = O.Problem(mumap, psfs, cfg.dy, S.default_viewangle(cfg.nview).astype(np.float64), "two", np.float64,
+                    "fft", nthread=16)
+    return P64.osem(req.x0.astype(np.float64), req.y.astype(np.float64), req.b.astype(np.float64), req.nblocks,
+                    req.niter)
+
+
+def test_batched_instances_vs_alone_and_oracle(gpu, O):
+    """The batched uneven workload: the 32 seeded instances (nx 64/96/128, nz
+    32/64/80, 60/120/128 views, PSF 15/25/37, 4/8 subsets) plus one at the
+    clinical geometry (128x128x80, 128 views, 37x37, 8 subsets), each on its
+    own resident plan with HIP graphs on, OS-EM x 4 from one sr_osem_batch
+    call: every result equals the same instance run alone (one call each),
+    on the eager call and on graph replays; the clinical instance and an
+    uneven 60-view / 8-subset instance match the FP64 oracle to 1e-4."""
+    from spectrecon_amd import serve as V
+    inst = V.batched_instances() + [(V.Config(128, 80, 128, 37, 0.48, 1.0, 4.0), 8)]
+    backend = V.DeviceBackend()
+    plans, reqs = [], []
+    for k, (cfg, nb) in enumerate(inst):
+        plan = backend.make_plan(cfg)
+        plans.append(plan)
+        reqs.append(V.synthetic_request(cfg, seed=1 + k, nblocks=nb, niter=4, rid=k, plans={cfg: plan}))
+    print(f"batched: {len(reqs)} instances prepared", flush=True)
+    batch = V.Batch(plans, reqs)
+    batch.run(sequential=True)  # (each plan's first call: eager)
+    alone = batch.results()
+    for rnd in range(3):  # graph capture, then replays
+        batch.run()
+        res = batch.results()
+        for k, (a, b) in enumerate(zip(alone, res)):
+            np.testing.assert_array_equal(a, b, err_msg=f"instance {k}, round {rnd}")
+    assert all(p.graph_stats() == (1, 2) for p in plans), [p.graph_stats() for p in plans]
+    batch.free()
+    uneven = next(k for k, (c, nb) in enumerate(inst) if c.nview == 60 and nb == 8)
+    for k in (len(inst) - 1, uneven):
+        xr = _serve_oracle(O, inst[k][0], reqs[k])
+        rel = float(np.linalg.norm(res[k] - xr) / np.linalg.norm(xr))
+        print(f"batched instance {k} {inst[k]}: vs FP64 {rel:.2e}", flush=True)
+        assert rel <= 1e-4, (k, rel)
+    for p in plans:
+        backend.close_plan(p)
+
+
+def test_served_stream_vs_alone_and_oracle(gpu, O):
+    """The served stream at 20 requests/s, Poisson arrivals, 70% clinical
+    (128x128x80, 128 views, 37x37 PSF FWHM 1.5 -> 6) and 30% small (64x64x32,
+    60 views, 15x15, FWHM 1 -> 4), OS-EM 8 x 4, through Server with the
+    device backend (pinned staging, asynchronous copies and reconstructions,
+    no host thread per request, graphs on): all 40 served, none rejected,
+    every result equal to the same request reconstructed alone, and one
+    request of each size within 1e-4 of the FP64 oracle."""
+    from spectrecon_amd import serve as V
+    big = V.Config(128, 80, 128, 37, 0.48, 1.5, 6.0)
+    small = V.Config(64, 32, 60, 15, 0.48, 1.0, 4.0)
+    rng = np.random.default_rng(11)
+    plans, reqs, t = {}, [], 0.0
+    for k in range(40):
+        t += float(rng.exponential(1 / 20.0))
+        cfg = big if rng.random() < 0.7 else small
+        reqs.append(V.synthetic_request(cfg, seed=1000 + k, nblocks=8, niter=4, arrival=t, rid=k, plans=plans))
+    for p in plans.values():
+        p.close()
+    server = V.Server(hbm_budget=100e9, max_active=8)
+    try:
+        st = server.run(reqs)
+    finally:
+        server.close()
+    print(f"served: {st['served']} served, p50 {st['latency_ms']['p50']:.1f} ms, "
+          f"p99 {st['latency_ms']['p99']:.1f} ms", flush=True)
+    assert st["served"] == 40 and st["rejected"] == 0
+    backend = V.DeviceBackend(graphs=False)
+    alone_plans = {cfg: backend.make_plan(cfg) for cfg in (big, small)}
+    for r in reqs:
+        np.testing.assert_array_equal(r.result, gpu.osem_strided(r.x0, r.y, r.b, alone_plans[r.config], r.nblocks,
+                                                                 r.niter), err_msg=f"request {r.rid}")
+    for cfg in (big, small):
+        r = next(q for q in reqs if q.config == cfg)
+        xr = _serve_oracle(O, cfg, r)
+        rel = float(np.linalg.norm(r.result - xr) / np.linalg.norm(xr))
+        print(f"served request {r.rid} ({cfg.nx}x{cfg.nx}x{cfg.nz}): vs FP64 {rel:.2e}", flush=True)
+        assert rel <= 1e-4, rel
+    for p in alone_plans.values():
+        p.close()

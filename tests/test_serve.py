@@ -174,3 +174,25 @@ def test_prepare_tunes_once_per_config_and_subset_size():
     plain = Server(hbm_budget=1e12, backend=FakeBackend(clock, {A: 1.0}), clock=clock)
     plain.prepare([A], nsubs={A: {4}})  # a backend without tuning: plans only
     assert plain.stats(1.0)["autotune"] == []
+
+
+def test_request_failing_its_checks_is_rejected_not_fatal():
+    """A request whose backend start raises the reference's argument error
+    (e.g. "need background > 0") is rejected with that message; the loop
+    goes on with the others."""
+    from spectrecon_amd._lib import SPECTError
+
+    clock = Clock()
+
+    class Picky(FakeBackend):
+        def start(self, plan, req):
+            if req.rid == 1:
+                raise SPECTError("need background > 0")
+            return super().start(plan, req)
+
+    be = Picky(clock, {A: 1.0})
+    server = Server(hbm_budget=1e12, max_active=4, backend=be, clock=clock)
+    reqs = [req(A, k) for k in range(3)]
+    drive(server, clock, reqs)
+    assert [r.rid for r in server.rejected] == [1] and reqs[1].error == "need background > 0"
+    assert sorted(r.rid for r in server.finished) == [0, 2]
