@@ -44,10 +44,15 @@ def workload():
 
 def algorithmic(cls, nb, g):
     """(bytes, flops) of one launch of a kernel class over nb views, for the
-    z-fastest :two path: the PSF forward rotates and attenuates while it
-    stages (per view: read the attenuation volume and the image, write the
-    view), the PSF adjoint writes the attenuated volumes the rotation
-    adjoint gathers."""
+    z-fastest :two path, counted one way everywhere (the bench line and the
+    profiles): per view, the projection reads the attenuation volume and the
+    image and writes the view (2 V + P floats), the back-projection reads the
+    view and the attenuation volume and writes the attenuated volume the
+    rotation adjoint gathers (P + 2 V); the rotation adjoint with the EM update
+    reads the nb volumes, x and the normaliser and writes x; the ratio reads
+    A x, y and b and writes y / (A x + b); a rotation or an attenuation scan
+    reads and writes one volume per view.  The PSF's flops are the separable
+    clamped convolution's, 2 passes x p MACs per voxel."""
     nx, ny, nz, p = g
     V, P = nx * ny * nz, nx * nz
     sep_flops = 2 * 2 * p * nx * nz * ny * nb  # separable clamped conv: 2 passes x p MACs per voxel
@@ -59,8 +64,8 @@ def algorithmic(cls, nb, g):
         return 4 * 2 * V, 0
     if cls == "rot_adj":  # read the volumes, x and the normaliser; write x (the EM update is fused)
         return 4 * (V * nb + 3 * V), 0
-    if cls == "view_reduce":
-        return 4 * 2 * P * nb, 0
+    if cls == "view_reduce":  # the ratio y / (A x + b): read A x, y, b, write the ratio
+        return 4 * 4 * P * nb, 0
     if cls == "img_reduce":  # the z-fastest image to the caller's layout
         return 4 * 2 * V, 0
     if cls == "atten":  # running sums along y: read the rotated mu volume, write the factors

@@ -21,6 +21,7 @@ _CLASSES = (
 
 
 def short_name(name):
+    name = name.replace("(anonymous namespace)::", "")
     return re.sub(r"\(.*", "", name).replace("void ", "").strip()
 
 

@@ -71,3 +71,11 @@ def test_traffic_keeps_the_kernel_that_ran(tmp_path):
     k = profiling.traffic(str(f), str(w), mib << 20)["kernels"]["psf_fwd"]
     assert k["kernel"] == "sr::k_psf_fwd_ws<5, 8>" and k["dispatches"] == 4
     assert k["fetch_bytes"] == 1024 * 1000 and k["write_bytes"] == 1024 * 10
+
+
+def test_short_name_keeps_kernels_in_anonymous_namespaces():
+    from spectrecon_amd.profiling import kernel_class, short_name
+    n = "void sr::(anonymous namespace)::k_psf_fwd_fft<256, 128>(float const*, float const*, int)"
+    assert short_name(n) == "sr::k_psf_fwd_fft<256, 128>"
+    assert kernel_class(n) == "psf_fwd"
+    assert kernel_class("void sr::k_atten_scan<1>(float*, sr::Geo, float, int)") == "atten"
