@@ -276,6 +276,20 @@ int sr_imrotate(float* out, const float* img, int32_t n, int32_t nz, float theta
  * replicate-padded convolution of an nx x nz image with a px x pz kernel. */
 int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const float* ker, int32_t px, int32_t pz,
                 int32_t adj, int32_t mem);
+/* sr_fft_conv3 replaces the 3-D forms of src/fft_convolve.jl:160-239 on
+ * [nx, ny, nz] arrays with kernels ker [px, pz, ny], every plane in one launch:
+ *   SR_CONV3_FWD:  out[:, iy, :] = fft_conv(img[:, iy, :], ker[:, :, iy])
+ *   SR_CONV3_ADJ:  out[:, iy, :] = fft_conv_adj(img[:, iy, :], ker[:, :, iy])
+ *   SR_CONV3_ADJ2: out[:, iy, :] = fft_conv_adj(img2, ker[:, :, iy]) for one
+ *                  [nx, nz] image img2 (fft_conv_adj2!, :217-239).
+ * Like sr_imrotate / sr_fft_conv it keeps its device buffers per device
+ * between calls (no allocation after the first call of a size) and returns
+ * after the device finished. */
+#define SR_CONV3_FWD 0
+#define SR_CONV3_ADJ 1
+#define SR_CONV3_ADJ2 2
+int sr_fft_conv3(float* out, const float* img, int32_t nx, int32_t ny, int32_t nz, const float* ker, int32_t px,
+                 int32_t pz, int32_t form, int32_t mem);
 /* The adjoints under their own names: imrotate_adj! (src/rotatez.jl:390-416,
  * :497-558) and fft_conv_adj! (src/fft_convolve.jl:80-130); the same as
  * sr_imrotate / sr_fft_conv with adj = 1. */

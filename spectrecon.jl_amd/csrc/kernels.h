@@ -123,6 +123,8 @@ hipError_t launch_psf_adj_fft(const float* views, const float* spec, int vstride
 // ---- elementwise --------------------------------------------------------
 hipError_t launch_fill(float* p, float v, long n, hipStream_t s);
 hipError_t launch_zero_to_inf(float* p, long n, hipStream_t s);
+// rows of nx floats: [z][y][x] -> [y][z][x] (mode 0) or back (mode 1)
+hipError_t launch_permute_rows(const float* src, float* dst, int nx, int ny, int nz, int mode, hipStream_t s);
 // dst[i] = src[i] for n 16-byte vectors (grid-stride streaming copy)
 hipError_t launch_copy16(void* dst, const void* src, long n, hipStream_t s);
 // z-fastest work layout ([y][x][z] volumes): the rotations as z-vector gathers

@@ -3792,6 +3792,23 @@ hipError_t launch_fill(float* p, float v, long n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Rows of nx floats permuted between [z][y][x] (mode 0 source) and
+// [y][z][x] (mode 0 target); mode 1 the other way.  grid (ny, nz), block 256.
+__global__ void __launch_bounds__(256) k_permute_rows(const float* __restrict__ src, float* __restrict__ dst, int nx,
+                                                      int ny, int nz, int mode) {
+  const int y = blockIdx.x, z = blockIdx.y;
+  const long zy = ((long)z * ny + y) * nx, yz = ((long)y * nz + z) * nx;
+  const float* s = src + (mode == 0 ? zy : yz);
+  float* d = dst + (mode == 0 ? yz : zy);
+  for (int x = threadIdx.x; x < nx; x += 256) d[x] = s[x];
+}
+
+hipError_t launch_permute_rows(const float* src, float* dst, int nx, int ny, int nz, int mode, hipStream_t s) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+  k_permute_rows<<<dim3(ny, nz), 256, 0, s>>>(src, dst, nx, ny, nz, mode);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy16(void* dst, const void* src, long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   k_copy16<<<std::min(cdiv(n, 256), 8192u), 256, 0, s>>>(static_cast<uint4*>(dst), static_cast<const uint4*>(src), n);

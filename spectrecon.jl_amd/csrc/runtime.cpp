@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -17,7 +19,9 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/spectrecon_amd.h"
@@ -51,6 +55,43 @@ long jmod(long x, long y) {
   long r = x % y;
   return r < 0 ? r + y : r;
 }
+
+// Plan-time host loops over independent items (PSF planes, views), split
+// over up to 16 host threads.
+template <class F>
+void host_parallel(long n, F&& f) {
+  const long hw = std::max(1L, (long)std::thread::hardware_concurrency());
+  const long nth = std::min({16L, hw, n / 16});
+  if (nth <= 1) {
+    for (long i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (long w = 0; w < nth; ++w)
+    th.emplace_back([&, w] {
+      for (long i = w; i < n; i += nth) f(i);
+    });
+  for (auto& t : th) t.join();
+}
+
+// Plan-creation phase times, printed to stderr when SR_PLAN_TIMING is set
+// (a diagnostic: the phases run the same either way).
+struct PhaseClock {
+  bool on = std::getenv("SR_PLAN_TIMING") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  std::string line;
+  void mark(const char* phase) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    char b[96];
+    std::snprintf(b, sizeof b, " %s %.2f", phase, std::chrono::duration<double, std::milli>(now - t).count());
+    line += b;
+    t = now;
+  }
+  ~PhaseClock() {
+    if (on && !line.empty()) std::fprintf(stderr, "sr_plan_create ms:%s\n", line.c_str());
+  }
+};
 
 // src/plan-rotate.jl:56
 int rot_padsize(int nx) { return (int)std::ceil(1.0 + nx * std::sqrt(2.0) / 2.0 - nx / 2.0); }
@@ -641,13 +682,22 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   if (d.nx % 2 != 0 || d.ny % 2 != 0) return fail(SR_ERR_ARG, "nx odd");
   if (d.px % 2 == 0 || d.pz % 2 == 0) return fail(SR_ERR_ARG, "non-odd size psfs");
   if (d.nx <= 0 || d.nz <= 0 || d.px <= 0 || d.pz <= 0 || d.nview <= 0) return fail(SR_ERR_ARG, "size");
+  PhaseClock clk;
   const long pxz = (long)d.px * d.pz;
   const long nplane = (long)d.ny * d.nview;
-  for (long pl = 0; pl < nplane; ++pl) {
-    const float* h = psfs + pl * pxz;
-    for (long k = 0; k < pxz; ++k)
-      if (!(h[k] == h[pxz - 1 - k])) return fail(SR_ERR_ARG, "asym. psf");
+  {  // every plane point-symmetric (src/spectplan.jl:83), planes checked in parallel
+    std::atomic<bool> asym{false};
+    host_parallel(nplane, [&](long pl) {
+      const float* h = psfs + pl * pxz;
+      for (long k = 0; k < pxz / 2 + 1; ++k)
+        if (!(h[k] == h[pxz - 1 - k])) {
+          asym.store(true, std::memory_order_relaxed);
+          return;
+        }
+    });
+    if (asym.load()) return fail(SR_ERR_ARG, "asym. psf");
   }
+  clk.mark("checks");
   if (d.interpmeth != SR_INTERP_ONE && d.interpmeth != SR_INTERP_TWO) return fail(SR_ERR_ARG, "bad interpmeth");
   if (d.mode != SR_MODE_FAST && d.mode != SR_MODE_MEM) return fail(SR_ERR_ARG, "bad mode");
   if (d.px > sr::kMaxTaps || d.pz > sr::kMaxTaps)
@@ -681,9 +731,14 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   }
 
   // --- PSF taps: dedupe views, choose separable vs direct ---
-  bool shared = true;
-  for (int v = 1; v < d.nview && shared; ++v)
-    shared = std::memcmp(psfs, psfs + (long)v * d.ny * pxz, sizeof(float) * d.ny * pxz) == 0;
+  std::atomic<bool> differs{false};
+  host_parallel(d.nview - 1, [&](long v) {
+    if (!differs.load(std::memory_order_relaxed) &&
+        std::memcmp(psfs, psfs + (v + 1) * d.ny * pxz, sizeof(float) * d.ny * pxz) != 0)
+      differs.store(true, std::memory_order_relaxed);
+  });
+  const bool shared = !differs.load();
+  clk.mark("dedupe");
   p->vstride = shared ? 0 : d.ny;
   const int nstore = shared ? d.ny : d.ny * d.nview;
   p->nt = round_taps(std::max(d.px, d.pz));
@@ -724,6 +779,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
     HIP_TRY(hipMemcpy(p->taps_h.p, hh.data(), sizeof(float) * hh.size(), hipMemcpyHostToDevice));
   }
 
+  clk.mark("taps");
   // --- rotation parameters ---
   const int Np = p->g.Np;
   std::vector<sr::RotView> rv(d.nview);
@@ -739,6 +795,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   HIP_TRY(hipMemcpy(p->tab_b.p, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->tab_w.p, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice));
 
+  clk.mark("rotation");
   // --- work layout: z-fastest for :two with the matrix-core PSF passes
   // (SR_LAYOUT=zyx in the environment keeps the x-fastest layout) ---
   {
@@ -757,6 +814,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
     }
   }
 
+  clk.mark("lists");
   // --- PSF spectra for the plane FFTs (x-fastest plans whose planes fit):
   // from the PSF planes on the device, in double, once ---
   if (!p->zfast && sr::psf_fft_plan(p->g, &p->fft)) {
@@ -781,12 +839,15 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
                      d.px, d.pz));
   }
 
+  clk.mark("spectra");
   // --- mu-map and attenuation factors for every view ---
   if ((rc = p->mumap.ensure(sizeof(float) * p->V))) return bail(rc);
   HIP_TRY(hipMemcpy(p->mumap.p, mumap, sizeof(float) * p->V, hipMemcpyHostToDevice));
   if ((rc = p->atten.ensure(sizeof(float) * p->V * d.nview))) return bail(rc);
+  clk.mark("alloc");
   if ((rc = build_atten(p))) return bail(rc);
   HIP_TRY(hipStreamSynchronize(p->stream));
+  clk.mark("atten");
   p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->taps_c.n + p->rotv.n + p->tab_b.n +
              p->tab_w.n + p->zimg.n + p->zent.n + p->fspec.n + p->ftw.n;  // (zh is a per-call buffer)
   *out = guard.release();
@@ -1657,6 +1718,94 @@ int sr_plan_query(sr_plan p, int32_t* done) {
 // Unit operators
 // ---------------------------------------------------------------------------
 
+}  // extern "C"
+
+namespace {
+
+// Device buffers of the unit operators (sr_imrotate, sr_fft_conv,
+// sr_fft_conv3), kept per device and grown as needed, so repeated calls
+// allocate nothing after the first of a size; calls are serialised per
+// device (the buffers are shared) and run on the null stream.
+struct UnitWs {
+  std::mutex mu;
+  DBuf rv, tb, tw, vids, u, w, c, h, in, out, part;
+};
+UnitWs* unit_ws() {
+  static UnitWs ws[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    dev = 0;
+  }
+  return &ws[dev];
+}
+
+// 0, 1, ..., n-1 into the workspace's view ids
+int unit_vids(UnitWs* ws, int n) {
+  int rc;
+  if ((rc = ws->vids.ensure(sizeof(int) * std::max(1, n)))) return rc;
+  std::vector<int> v(std::max(1, n));
+  for (int i = 0; i < (int)v.size(); ++i) v[i] = i;
+  HIP_TRY(hipMemcpy(ws->vids.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
+  return SR_OK;
+}
+
+// The tap tables of nplanes PSF planes ker[plane][pz][px] (host) into the
+// workspace: separable (u, w and the tail/head sums) when every plane is rank
+// 1 and the geometry takes the separable kernels, else the 2-D table.
+int unit_taps(UnitWs* ws, const float* ker, int nplanes, int px, int pz, const sr::Geo& g, sr::Taps* t, int* sep) {
+  const long pxz = (long)px * pz;
+  const int nt = round_taps(std::max(px, pz));
+  std::vector<double> u, w;
+  bool s = sr::psf_sep_supported(g);
+  for (int pl = 0; pl < nplanes && s; ++pl) s = factor_rank1(ker + pl * pxz, px, pz, u, w);
+  *t = sr::Taps{};
+  t->ntx = t->ntz = nt;
+  t->vstride = 1;  // plane of "view" v, row 0: v
+  int rc;
+  if (s) {
+    std::vector<float> hu((size_t)nplanes * nt, 0.f), hw((size_t)nplanes * nt, 0.f);
+    for (int pl = 0; pl < nplanes; ++pl) {
+      factor_rank1(ker + pl * pxz, px, pz, u, w);
+      for (int a = 0; a < px; ++a) hu[(size_t)pl * nt + a] = (float)u[a];
+      for (int b = 0; b < pz; ++b) hw[(size_t)pl * nt + b] = (float)w[b];
+    }
+    const std::vector<float> hc = cum_sums(hu, hw, nplanes, nt);
+    if ((rc = ws->u.ensure(sizeof(float) * hu.size())) || (rc = ws->w.ensure(sizeof(float) * hw.size())) ||
+        (rc = ws->c.ensure(sizeof(float) * hc.size())))
+      return rc;
+    HIP_TRY(hipMemcpy(ws->u.p, hu.data(), sizeof(float) * hu.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ws->w.p, hw.data(), sizeof(float) * hw.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ws->c.p, hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice));
+    t->u = ws->u.as<float>();
+    t->w = ws->w.as<float>();
+    t->cum = ws->c.as<float>();
+  } else {
+    std::vector<float> hh((size_t)nplanes * pz * nt, 0.f);
+    for (int pl = 0; pl < nplanes; ++pl)
+      for (int b = 0; b < pz; ++b)
+        for (int a = 0; a < px; ++a) hh[((size_t)pl * pz + b) * nt + a] = ker[pl * pxz + a + (long)px * b];
+    if ((rc = ws->h.ensure(sizeof(float) * hh.size()))) return rc;
+    HIP_TRY(hipMemcpy(ws->h.p, hh.data(), sizeof(float) * hh.size(), hipMemcpyHostToDevice));
+    t->h = ws->h.as<float>();
+  }
+  *sep = s ? 1 : 0;
+  return SR_OK;
+}
+
+int unit_conv_checks(const float* out, const float* img, const float* ker, int nx, int nz, int px, int pz) {
+  if (!out || !img || !ker) return fail(SR_ERR_ARG, "null argument");
+  if (nx <= 0 || nz <= 0 || px <= 0 || pz <= 0) return fail(SR_ERR_ARG, "size");
+  if (px % 2 == 0 || pz % 2 == 0) return fail(SR_ERR_ARG, "non-odd size psfs");
+  if (px > sr::kMaxTaps || pz > sr::kMaxTaps) return fail(SR_ERR_UNSUPPORTED, "kernel larger than %d taps", sr::kMaxTaps);
+  if ((long)nx * nz > sr::kMaxPlane) return fail(SR_ERR_UNSUPPORTED, "plane too large");
+  return check_device();
+}
+
+}  // namespace
+
+extern "C" {
+
 // src/rotatez.jl:240-416 and the 3-D forms :472-495
 int sr_imrotate(float* out, const float* img, int32_t n, int32_t nz, float theta, int32_t interpmeth, int32_t adj,
                 int32_t mem) {
@@ -1674,30 +1823,29 @@ int sr_imrotate(float* out, const float* img, int32_t n, int32_t nz, float theta
   std::vector<float> tw(4 * g.Np);
   rot_params((double)theta, interpmeth, g.Np, rv, tb.data(), tw.data());
   const long V = (long)n * n * nz;
-  DBuf d_rv, d_tb, d_tw, d_vid, d_in, d_out;
-  if ((rc = d_rv.ensure(sizeof rv)) || (rc = d_tb.ensure(sizeof(int) * tb.size())) ||
-      (rc = d_tw.ensure(sizeof(float) * tw.size())) || (rc = d_vid.ensure(sizeof(int))))
+  UnitWs* ws = unit_ws();
+  std::lock_guard<std::mutex> lock(ws->mu);
+  if ((rc = ws->rv.ensure(sizeof rv)) || (rc = ws->tb.ensure(sizeof(int) * tb.size())) ||
+      (rc = ws->tw.ensure(sizeof(float) * tw.size())) || (rc = unit_vids(ws, 1)))
     return rc;
-  const int zero = 0;
-  HIP_TRY(hipMemcpy(d_rv.p, &rv, sizeof rv, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(d_tb.p, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(d_tw.p, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(d_vid.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ws->rv.p, &rv, sizeof rv, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ws->tb.p, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ws->tw.p, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice));
   const bool host = (mem & 0xff) == SR_MEM_HOST;
   const float* din = img;
   float* dout = out;
   if (host) {
-    if ((rc = d_in.ensure(sizeof(float) * V)) || (rc = d_out.ensure(sizeof(float) * V))) return rc;
-    HIP_TRY(hipMemcpy(d_in.p, img, sizeof(float) * V, hipMemcpyHostToDevice));
-    din = d_in.as<float>();
-    dout = d_out.as<float>();
+    if ((rc = ws->in.ensure(sizeof(float) * V)) || (rc = ws->out.ensure(sizeof(float) * V))) return rc;
+    HIP_TRY(hipMemcpy(ws->in.p, img, sizeof(float) * V, hipMemcpyHostToDevice));
+    din = ws->in.as<float>();
+    dout = ws->out.as<float>();
   }
   if (adj)
-    HIP_TRY(sr::launch_rot_adj(interpmeth, din, dout, d_rv.as<sr::RotView>(), d_vid.as<int>(), 1, 1, d_tb.as<int>(),
-                               d_tw.as<float>(), g, nullptr));
+    HIP_TRY(sr::launch_rot_adj(interpmeth, din, dout, ws->rv.as<sr::RotView>(), ws->vids.as<int>(), 1, 1,
+                               ws->tb.as<int>(), ws->tw.as<float>(), g, nullptr));
   else
-    HIP_TRY(sr::launch_rot_fwd(interpmeth, din, dout, nullptr, 0, d_rv.as<sr::RotView>(), d_vid.as<int>(), 1,
-                               d_tb.as<int>(), d_tw.as<float>(), g, nullptr));
+    HIP_TRY(sr::launch_rot_fwd(interpmeth, din, dout, nullptr, 0, ws->rv.as<sr::RotView>(), ws->vids.as<int>(), 1,
+                               ws->tb.as<int>(), ws->tw.as<float>(), g, nullptr));
   HIP_TRY(hipStreamSynchronize(nullptr));
   if (host) HIP_TRY(hipMemcpy(out, dout, sizeof(float) * V, hipMemcpyDeviceToHost));
   return SR_OK;
@@ -1706,70 +1854,69 @@ int sr_imrotate(float* out, const float* img, int32_t n, int32_t nz, float theta
 // src/fft_convolve.jl:33-130 (fft_conv!, fft_conv_adj!), one plane
 int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const float* ker, int32_t px, int32_t pz,
                 int32_t adj, int32_t mem) {
-  if (!out || !img || !ker) return fail(SR_ERR_ARG, "null argument");
-  if (nx <= 0 || nz <= 0 || px <= 0 || pz <= 0) return fail(SR_ERR_ARG, "size");
-  if (px % 2 == 0 || pz % 2 == 0) return fail(SR_ERR_ARG, "non-odd size psfs");
-  if (px > sr::kMaxTaps || pz > sr::kMaxTaps) return fail(SR_ERR_UNSUPPORTED, "kernel larger than %d taps", sr::kMaxTaps);
-  if ((long)nx * nz > sr::kMaxPlane) return fail(SR_ERR_UNSUPPORTED, "plane too large");
   int rc;
-  if ((rc = check_device())) return rc;
+  if ((rc = unit_conv_checks(out, img, ker, nx, nz, px, pz))) return rc;
+  return sr_fft_conv3(out, img, nx, 1, nz, ker, px, pz, adj ? SR_CONV3_ADJ : SR_CONV3_FWD, mem);
+}
+
+// src/fft_convolve.jl:160-239: the 3-D forms over the planes iy of [nx, ny, nz]
+// arrays with kernels ker[:, :, iy]; every plane in one launch
+int sr_fft_conv3(float* out, const float* img, int32_t nx, int32_t ny, int32_t nz, const float* ker, int32_t px,
+                 int32_t pz, int32_t form, int32_t mem) {
+  int rc;
+  if ((rc = unit_conv_checks(out, img, ker, nx, nz, px, pz))) return rc;
+  if (ny <= 0) return fail(SR_ERR_ARG, "size");
+  if (form != SR_CONV3_FWD && form != SR_CONV3_ADJ && form != SR_CONV3_ADJ2) return fail(SR_ERR_ARG, "bad form");
   const bool host = (mem & 0xff) == SR_MEM_HOST;
-  std::vector<float> hk((size_t)px * pz);
+  const long P = (long)nx * nz, V = P * ny, nin = form == SR_CONV3_ADJ2 ? P : V;
+  const long pxz = (long)px * pz;
+  std::vector<float> hk((size_t)pxz * ny);
   if (host) std::memcpy(hk.data(), ker, sizeof(float) * hk.size());
   else HIP_TRY(hipMemcpy(hk.data(), ker, sizeof(float) * hk.size(), hipMemcpyDeviceToHost));
-  std::vector<double> u, w;
-  sr::Geo g{nx, 1, nz, 0, 0, px, pz};
-  const bool sep = sr::psf_sep_supported(g) && factor_rank1(hk.data(), px, pz, u, w);
-  const int nt = round_taps(std::max(px, pz));
-  DBuf d_u, d_w, d_h, d_c, d_vid, d_in, d_out, d_part;
-  sr::Taps t{};
-  t.ntx = t.ntz = nt;
-  t.vstride = 0;
-  if (sep) {
-    std::vector<float> hu(nt, 0.f), hw(nt, 0.f);
-    for (int a = 0; a < px; ++a) hu[a] = (float)u[a];
-    for (int b = 0; b < pz; ++b) hw[b] = (float)w[b];
-    const std::vector<float> hc = cum_sums(hu, hw, 1, nt);
-    if ((rc = d_u.ensure(sizeof(float) * nt)) || (rc = d_w.ensure(sizeof(float) * nt)) ||
-        (rc = d_c.ensure(sizeof(float) * hc.size())))
-      return rc;
-    HIP_TRY(hipMemcpy(d_u.p, hu.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_w.p, hw.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_c.p, hc.data(), sizeof(float) * hc.size(), hipMemcpyHostToDevice));
-  } else {
-    std::vector<float> hh((size_t)pz * nt, 0.f);
-    for (int b = 0; b < pz; ++b)
-      for (int a = 0; a < px; ++a) hh[(size_t)b * nt + a] = hk[a + (size_t)px * b];
-    if ((rc = d_h.ensure(sizeof(float) * hh.size()))) return rc;
-    HIP_TRY(hipMemcpy(d_h.p, hh.data(), sizeof(float) * hh.size(), hipMemcpyHostToDevice));
-  }
-  t.u = d_u.as<float>();
-  t.w = d_w.as<float>();
-  t.h = d_h.as<float>();
-  t.cum = d_c.as<float>();
-  const int zero = 0;
-  if ((rc = d_vid.ensure(sizeof(int)))) return rc;
-  HIP_TRY(hipMemcpy(d_vid.p, &zero, sizeof(int), hipMemcpyHostToDevice));
-  const long P = (long)nx * nz;
-  const float* din = img;
-  float* dout = out;
+  UnitWs* ws = unit_ws();
+  std::lock_guard<std::mutex> lock(ws->mu);
+  // the planes as a batch of one-row "views": plane iy is view iy with PSF
+  // plane iy (Geo ny = 1, taps vstride 1), [view][z][x] contiguous
+  sr::Geo gp{nx, 1, nz, 0, 0, px, pz};
+  sr::Taps t;
+  int sep = 0;
+  if ((rc = unit_taps(ws, hk.data(), ny, px, pz, gp, &t, &sep)) || (rc = unit_vids(ws, ny)) ||
+      (rc = ws->in.ensure(sizeof(float) * V)) || (rc = ws->out.ensure(sizeof(float) * V)) ||
+      (rc = ws->part.ensure(sizeof(float) * V)))
+    return rc;
+  float* din = ws->in.as<float>();
+  float* dout = ws->out.as<float>();
+  const float* src = img;
   if (host) {
-    if ((rc = d_in.ensure(sizeof(float) * P)) || (rc = d_out.ensure(sizeof(float) * P))) return rc;
-    HIP_TRY(hipMemcpy(d_in.p, img, sizeof(float) * P, hipMemcpyHostToDevice));
-    din = d_in.as<float>();
-    dout = d_out.as<float>();
+    HIP_TRY(hipMemcpy(din, img, sizeof(float) * nin, hipMemcpyHostToDevice));
+    src = din;
   }
-  if (adj) {
-    HIP_TRY(sr::launch_psf_adj(sep, din, t, d_vid.as<int>(), 1, dout, nullptr, 0, 1, g, nullptr));
+  const int* vids = ws->vids.as<int>();
+  if (form == SR_CONV3_ADJ2) {
+    // one 2-D image, every plane's adjoint: the PSF adjoint of one view over
+    // ny planes writes [z][y][x], the caller's [nx, ny, nz] itself
+    sr::Geo g{nx, ny, nz, 0, 0, px, pz};
+    sr::Taps t3 = t;
+    t3.vstride = 0;
+    float* dst = host ? dout : out;
+    HIP_TRY(sr::launch_psf_adj(sep, src, t3, vids, 1, dst, nullptr, 0, ny, g, nullptr));  // a block per plane
   } else {
-    if ((rc = d_part.ensure(sizeof(float) * P))) return rc;
-    int nchunk = 0;
-    HIP_TRY(sr::launch_psf_fwd(sep, din, t, d_vid.as<int>(), 1, d_part.as<float>(), 1, &nchunk, g, nullptr));
-    HIP_TRY(sr::launch_view_reduce(d_part.as<float>(), nchunk, d_vid.as<int>(), 1, dout, nullptr, nullptr,
-                                   sr::EPI_VIEWS, g, nullptr));
+    // the caller's [x][y][z] -> planes [y][z][x]
+    float* planes = ws->part.as<float>();
+    HIP_TRY(sr::launch_permute_rows(src, planes, nx, ny, nz, 0, nullptr));
+    float* res = host ? din : out;  // (the staged input is free now)
+    float* res_planes = dout;
+    if (form == SR_CONV3_FWD) {
+      int used = 0;
+      HIP_TRY(sr::launch_psf_fwd(sep, planes, t, vids, ny, res_planes, 1, &used, gp, nullptr));
+    } else {
+      HIP_TRY(sr::launch_psf_adj(sep, planes, t, vids, ny, res_planes, nullptr, 0, 1, gp, nullptr));
+    }
+    HIP_TRY(sr::launch_permute_rows(res_planes, res, nx, ny, nz, 1, nullptr));
+    dout = res;
   }
   HIP_TRY(hipStreamSynchronize(nullptr));
-  if (host) HIP_TRY(hipMemcpy(out, dout, sizeof(float) * P, hipMemcpyDeviceToHost));
+  if (host) HIP_TRY(hipMemcpy(out, dout, sizeof(float) * V, hipMemcpyDeviceToHost));
   return SR_OK;
 }
 

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(_PKG), "lib", "libspectrecon_amd.so")
 SR_OK, SR_ERR_ARG, SR_ERR_DIM, SR_ERR_HIP, SR_ERR_ALLOC, SR_ERR_UNSUPPORTED = range(6)
 SR_MEM_HOST, SR_MEM_DEVICE, SR_ASYNC, SR_NO_CHECK = 0, 1, 0x100, 0x200
 SR_COPY_H2D, SR_COPY_D2H, SR_COPY_D2D = 1, 2, 3
+SR_CONV3_FWD, SR_CONV3_ADJ, SR_CONV3_ADJ2 = 0, 1, 2
 SR_INTERP_ONE, SR_INTERP_TWO = 1, 2
 SR_MODE_FAST, SR_MODE_MEM = 0, 1
 SR_CONV_AUTO, SR_CONV_DIRECT, SR_CONV_SEPARABLE, SR_CONV_FFT = 0, 1, 2, 3
@@ -83,6 +84,8 @@ SIGNATURES = {
     "sr_osem": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_int32]),
     "sr_imrotate": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_int32]),
     "sr_fft_conv": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "sr_fft_conv3": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32,
+                               C.c_int32]),
     "sr_fft_conv_noalloc": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32,
                                       C.c_int32]),
     "sr_imrotate_adj": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32]),

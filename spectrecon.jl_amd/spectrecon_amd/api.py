@@ -725,14 +725,38 @@ def _conv2(out, img, ker, adj):
     return out
 
 
+def _conv3(out, img, ker, form):
+    """All planes of a 3-D form in one sr_fft_conv3 call (out: [nx, ny, nz])."""
+    img = _f32(img)
+    ker = _f32(ker)
+    nx, ny, nz = out.shape
+    px, pz = ker.shape[:2]
+    res = out if (out.dtype == np.float32 and out.flags.f_contiguous) else np.zeros((nx, ny, nz), np.float32, "F")
+    check(lib().sr_fft_conv3(res.ctypes.data, img.ctypes.data, nx, ny, nz, ker.ctypes.data, px, pz, form,
+                             L.SR_MEM_HOST))
+    if res is not out:
+        out[...] = res
+    return out
+
+
+def _plane_checks(p, nx, nz, ker):
+    """The size checks each plane's 2-D call makes (src/fft_convolve.jl:39-42)."""
+    if (nx, nz) != (p.nx, p.nz):
+        raise SPECTError("size img")
+    if tuple(np.shape(ker)[:2]) != (p.px, p.pz):
+        raise SPECTError(f"size ker {tuple(np.shape(ker)[:2])} {p.px} {p.pz}")
+
+
 def _conv(output, img, ker, plan, adj):
     p = plan[0] if isinstance(plan, (list, tuple)) else plan
     if np.ndim(img) == 3:  # 3-D form: plane iy of image3 with ker3[:,:,iy] (src/fft_convolve.jl:160-209)
         if np.shape(output) != np.shape(img):
             raise DimensionMismatch()
-        for iy in range(np.shape(img)[1]):
-            _conv2(output[:, iy, :], img[:, iy, :], np.asarray(ker)[:, :, iy], adj)
-        return output
+        nx, ny, nz = np.shape(img)
+        _plane_checks(p, nx, nz, ker)
+        if np.ndim(ker) != 3 or np.shape(ker)[2] < ny:
+            raise IndexError("ker3 has fewer planes than image3")
+        return _conv3(output, img, np.asarray(ker)[:, :, :ny], L.SR_CONV3_ADJ if adj else L.SR_CONV3_FWD)
     if np.shape(output) != np.shape(img):
         raise SPECTError("size output")
     if tuple(np.shape(img)) != (p.nx, p.nz):
@@ -752,13 +776,27 @@ def fft_conv_adj_(output, img, ker, plan):
 
 
 def fft_conv_adj2_(output, image2, ker3, plans):
-    """fft_conv_adj2!(output, image2, ker3, plans): adjoint of each plane's kernel on one 2-D image."""
+    """fft_conv_adj2!(output, image2, ker3, plans): adjoint of each plane's kernel on one 2-D image.
+
+    Like src/fft_convolve.jl:217-239 it writes the planes iy = 1..size(image2, 2)
+    of output (all of them when ny == nz, leaving the rest as they are when
+    nz < ny, and raising IndexError, Julia's BoundsError, when nz > ny).
+    """
     if np.shape(output)[0] != np.shape(image2)[0]:
         raise SPECTError("size 1")
     if np.shape(output)[2] != np.shape(image2)[1]:
         raise SPECTError("size 2")
-    for iy in range(np.shape(output)[1]):
-        _conv2(output[:, iy, :], image2, np.asarray(ker3)[:, :, iy], True)
+    p = plans[0] if isinstance(plans, (list, tuple)) else plans
+    nx, ny, nz = np.shape(output)
+    _plane_checks(p, nx, nz, ker3)
+    if nz > ny or np.ndim(ker3) != 3 or np.shape(ker3)[2] < nz:
+        raise IndexError("fft_conv_adj2!: plane index beyond output or ker3")
+    ker = np.asarray(ker3)[:, :, :nz]
+    if nz == ny:
+        return _conv3(output, image2, ker, L.SR_CONV3_ADJ2)
+    part = np.zeros((nx, nz, nz), np.float32, order="F")
+    _conv3(part, image2, ker, L.SR_CONV3_ADJ2)
+    output[:, :nz, :] = part
     return output
 
 

@@ -132,6 +132,56 @@ def test_fft_conv_bounds(gpu):
     assert result.max() <= 1.5
 
 
+def _ker3(rng, px, pz, n, kind):
+    ks = []
+    for i in range(n):
+        if kind == "separable" or (kind == "mixed" and i % 2 == 0):
+            u = rng.random(px) + 0.1
+            w = rng.random(pz) + 0.1
+            k = np.outer(u + u[::-1], w + w[::-1])
+        else:
+            k = rng.random((px, pz))
+            k = k + k[::-1, ::-1]
+        ks.append(k / k.sum())
+    return np.asfortranarray(np.stack(ks, axis=2).astype(np.float32))
+
+
+@pytest.mark.parametrize("nx,ny,nz,px,pz,kind", [(12, 12, 10, 5, 3, "separable"), (20, 7, 14, 5, 5, "general"),
+                                                  (64, 32, 32, 15, 15, "mixed"), (33, 3, 1, 9, 1, "separable"),
+                                                  (128, 4, 80, 37, 37, "general")])
+def test_fft_conv3_forms_vs_oracle(gpu, O, nx, ny, nz, px, pz, kind):
+    """The 3-D forms (src/fft_convolve.jl:160-239) in one launch vs the oracle plane by plane (FP64)."""
+    rng = np.random.default_rng(nx + 7 * ny + 31 * nz)
+    plan = gpu.plan_psf(nx=nx, nz=nz, px=px, pz=pz)
+    image3 = np.asfortranarray(rng.standard_normal((nx, ny, nz)).astype(np.float32))
+    ker3 = _ker3(rng, px, pz, ny + 1, kind)  # one plane more than needed: only the first ny are read
+    for adj in (False, True):
+        out = np.full_like(image3, np.nan)
+        (gpu.fft_conv_adj_ if adj else gpu.fft_conv_)(out, image3, ker3, plan)
+        for iy in range(ny):
+            r = O.fft_conv(image3[:, iy, :].astype(np.float64), ker3[:, :, iy].astype(np.float64), adj, "direct",
+                           np.float64)
+            assert maxrel(out[:, iy, :], r) <= ROT_TOL, (adj, iy, maxrel(out[:, iy, :], r))
+        # in place (output aliases image3) gives the same result
+        inplace = image3.copy(order="F")
+        (gpu.fft_conv_adj_ if adj else gpu.fft_conv_)(inplace, inplace, ker3, plan)
+        np.testing.assert_array_equal(inplace, out)
+    # fft_conv_adj2!: planes 1..size(image2, 2) of output from one image
+    image2 = np.asfortranarray(rng.standard_normal((nx, nz)).astype(np.float32))
+    if nz > ny:
+        with pytest.raises(IndexError):
+            gpu.fft_conv_adj2_(np.zeros_like(image3), image2, ker3, plan)
+        return
+    out = np.full_like(image3, 7.0)
+    gpu.fft_conv_adj2_(out, image2, ker3, plan)
+    for iy in range(ny):
+        if iy >= nz:
+            assert np.all(out[:, iy, :] == 7.0), iy  # not written, as in the reference
+            continue
+        r = O.fft_conv(image2.astype(np.float64), ker3[:, :, iy].astype(np.float64), True, "direct", np.float64)
+        assert maxrel(out[:, iy, :], r) <= ROT_TOL, ("adj2", iy, maxrel(out[:, iy, :], r))
+
+
 # ---------------------------------------------------------------------------
 # projector / back-projector
 # ---------------------------------------------------------------------------
@@ -191,7 +241,7 @@ def test_general_psf_large_planes(gpu, O, nx, nz, p):
     rng = np.random.default_rng(nx + nz + p)
     mumap = np.asfortranarray((0.05 * rng.random((nx, nx, nz))).astype(np.float32))
     psfs = rand_sym_psfs(rng, p, p, nx, nview)
-    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    plan = gpu.SPECTplan(mumap, psfs, 0.48, conv="direct")  # (the default takes the FFT path for 37 x 37)
     assert plan.info()["conv"] == "direct"
     ang = angles32(nview).astype(np.float64)
     P32 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float32, "direct")
