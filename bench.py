@@ -437,6 +437,17 @@ def units_workload(reps=20):
         rows.append({"op": f"fft_conv{'_adj' if adj else ''}! 200x64, 5x5", "published_ms": pub,
                      "gpu_ms": med(lambda: L.check(lib.sr_fft_conv(do.ptr, di.ptr, 200, 64, dk.ptr, 5, 5, adj,
                                                                     L.SR_MEM_DEVICE)))})
+    # the 3-D fft_conv! / fft_conv_adj! forms (src/fft_convolve.jl:160-209) over 128 planes of 128x80 with
+    # 37x37 Gaussian kernels: no published timing; every plane in one call and launch
+    nx3, ny3, nz3, p3 = 128, 128, 80, 37
+    img3 = S.DeviceArray.from_host(np.asfortranarray(rng.random((nx3, ny3, nz3), dtype=np.float32)))
+    out3 = S.DeviceArray((nx3, ny3, nz3))
+    ker3 = S.DeviceArray.from_host(np.asfortranarray(S.psf_gauss(ny=ny3, px=p3, pz=p3, fwhm_start=1.5,
+                                                                  fwhm_end=6.0)))
+    for form, name in ((L.SR_CONV3_FWD, "fft_conv!"), (L.SR_CONV3_ADJ, "fft_conv_adj!")):
+        rows.append({"op": f"3-D {name} 128x128x80, 37x37 per plane", "published_ms": None,
+                     "gpu_ms": med(lambda: L.check(lib.sr_fft_conv3(out3.ptr, img3.ptr, nx3, ny3, nz3, ker3.ptr, p3,
+                                                                     p3, form, L.SR_MEM_DEVICE)))})
     # SPECTplan, project!, backproject! on the timing problem (time/SPECTplan.jl, time/project.jl,
     # time/backproject.jl): 64x64x40, 60 views, random 19x19 PSFs symmetrised, dy = 4.7952
     nx, nz, nview, p = 64, 40, 60, 19
@@ -463,7 +474,7 @@ def units_workload(reps=20):
                                                                        L.SR_MEM_DEVICE)))})
         plan.close()
     for r in rows:
-        r["speedup"] = r["published_ms"] / r["gpu_ms"]
+        r["speedup"] = r["published_ms"] / r["gpu_ms"] if r["published_ms"] else None
     print(json.dumps({"workload": "units", "note": "published: the reference's own timings on an unstated CPU "
                       "(time/*.jl); gpu: this library on one MI355X", "rows": rows}), flush=True)
 

@@ -268,6 +268,10 @@ int sr_project_vjp_mu(sr_plan plan, float* grad_mu, const float* image, const fl
                       int32_t nindex, int32_t mem);
 
 /* ---- unit operators ----------------------------------------------------------
+ * The unit operators keep their device buffers and their tap and rotation
+ * tables per device between calls (a repeated size, kernel or angle allocates
+ * and uploads nothing), serialise on a per-device lock, run on the null
+ * stream and return after the device finished.
  * sr_imrotate replaces imrotate!/imrotate_adj! on a stack of nz square n x n
  * slices (src/rotatez.jl:240-416, 3-D forms :472-495). */
 int sr_imrotate(float* out, const float* img, int32_t n, int32_t nz, float theta, int32_t interpmeth, int32_t adj,
@@ -282,9 +286,8 @@ int sr_fft_conv(float* out, const float* img, int32_t nx, int32_t nz, const floa
  *   SR_CONV3_ADJ:  out[:, iy, :] = fft_conv_adj(img[:, iy, :], ker[:, :, iy])
  *   SR_CONV3_ADJ2: out[:, iy, :] = fft_conv_adj(img2, ker[:, :, iy]) for one
  *                  [nx, nz] image img2 (fft_conv_adj2!, :217-239).
- * Like sr_imrotate / sr_fft_conv it keeps its device buffers per device
- * between calls (no allocation after the first call of a size) and returns
- * after the device finished. */
+ * The same kernels as sr_fft_conv; the arrays are [nx, ny, nz] column-major
+ * like the reference's (ker: [px, pz, ny]). */
 #define SR_CONV3_FWD 0
 #define SR_CONV3_ADJ 1
 #define SR_CONV3_ADJ2 2

@@ -1729,6 +1729,14 @@ namespace {
 struct UnitWs {
   std::mutex mu;
   DBuf rv, tb, tw, vids, u, w, c, h, in, out, part;
+  int nvids = 0;  // vids holds 0, 1, ..., nvids - 1
+  // what the tap and rotation tables hold now (a call with the same kernels
+  // and geometry, or the same rotation, uploads nothing)
+  std::vector<float> ker;
+  int knx = -1, knz = -1, kpx = -1, kpz = -1, kplanes = -1, ksep = 0;
+  sr::Taps ktaps{};
+  float theta = 0.f;
+  int rn = -1, rinterp = -1;
 };
 UnitWs* unit_ws() {
   static UnitWs ws[64];
@@ -1742,11 +1750,15 @@ UnitWs* unit_ws() {
 
 // 0, 1, ..., n-1 into the workspace's view ids
 int unit_vids(UnitWs* ws, int n) {
+  n = std::max(1, n);
+  if (n <= ws->nvids) return SR_OK;
   int rc;
-  if ((rc = ws->vids.ensure(sizeof(int) * std::max(1, n)))) return rc;
-  std::vector<int> v(std::max(1, n));
-  for (int i = 0; i < (int)v.size(); ++i) v[i] = i;
-  HIP_TRY(hipMemcpy(ws->vids.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
+  ws->nvids = 0;
+  if ((rc = ws->vids.ensure(sizeof(int) * n))) return rc;
+  std::vector<int> v(n);
+  for (int i = 0; i < n; ++i) v[i] = i;
+  HIP_TRY(hipMemcpy(ws->vids.p, v.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  ws->nvids = n;
   return SR_OK;
 }
 
@@ -1755,6 +1767,13 @@ int unit_vids(UnitWs* ws, int n) {
 // 1 and the geometry takes the separable kernels, else the 2-D table.
 int unit_taps(UnitWs* ws, const float* ker, int nplanes, int px, int pz, const sr::Geo& g, sr::Taps* t, int* sep) {
   const long pxz = (long)px * pz;
+  if (ws->kplanes == nplanes && ws->kpx == px && ws->kpz == pz && ws->knx == g.nx && ws->knz == g.nz &&
+      std::memcmp(ws->ker.data(), ker, sizeof(float) * pxz * nplanes) == 0) {
+    *t = ws->ktaps;
+    *sep = ws->ksep;
+    return SR_OK;
+  }
+  ws->kplanes = -1;  // (valid again only once the new tables are up)
   const int nt = round_taps(std::max(px, pz));
   std::vector<double> u, w;
   bool s = sr::psf_sep_supported(g);
@@ -1790,6 +1809,14 @@ int unit_taps(UnitWs* ws, const float* ker, int nplanes, int px, int pz, const s
     t->h = ws->h.as<float>();
   }
   *sep = s ? 1 : 0;
+  ws->ker.assign(ker, ker + pxz * nplanes);
+  ws->knx = g.nx;
+  ws->knz = g.nz;
+  ws->kpx = px;
+  ws->kpz = pz;
+  ws->ksep = *sep;
+  ws->ktaps = *t;
+  ws->kplanes = nplanes;
   return SR_OK;
 }
 
@@ -1818,19 +1845,26 @@ int sr_imrotate(float* out, const float* img, int32_t n, int32_t nz, float theta
   sr::Geo g{n, n, nz, pad, n + 2 * pad, 1, 1};
   if (interpmeth == SR_INTERP_ONE && g.Np > sr::kMaxNpOne)
     return fail(SR_ERR_UNSUPPORTED, ":one rotation supports n up to 140");
-  sr::RotView rv;
-  std::vector<int> tb(2 * g.Np);
-  std::vector<float> tw(4 * g.Np);
-  rot_params((double)theta, interpmeth, g.Np, rv, tb.data(), tw.data());
   const long V = (long)n * n * nz;
   UnitWs* ws = unit_ws();
   std::lock_guard<std::mutex> lock(ws->mu);
-  if ((rc = ws->rv.ensure(sizeof rv)) || (rc = ws->tb.ensure(sizeof(int) * tb.size())) ||
-      (rc = ws->tw.ensure(sizeof(float) * tw.size())) || (rc = unit_vids(ws, 1)))
-    return rc;
-  HIP_TRY(hipMemcpy(ws->rv.p, &rv, sizeof rv, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ws->tb.p, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ws->tw.p, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice));
+  if ((rc = unit_vids(ws, 1))) return rc;
+  if (!(ws->rn == n && ws->rinterp == interpmeth && std::memcmp(&ws->theta, &theta, sizeof theta) == 0)) {
+    ws->rn = -1;
+    sr::RotView rv;
+    std::vector<int> tb(2 * g.Np);
+    std::vector<float> tw(4 * g.Np);
+    rot_params((double)theta, interpmeth, g.Np, rv, tb.data(), tw.data());
+    if ((rc = ws->rv.ensure(sizeof rv)) || (rc = ws->tb.ensure(sizeof(int) * tb.size())) ||
+        (rc = ws->tw.ensure(sizeof(float) * tw.size())))
+      return rc;
+    HIP_TRY(hipMemcpy(ws->rv.p, &rv, sizeof rv, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ws->tb.p, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ws->tw.p, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice));
+    ws->rn = n;
+    ws->rinterp = interpmeth;
+    ws->theta = theta;
+  }
   const bool host = (mem & 0xff) == SR_MEM_HOST;
   const float* din = img;
   float* dout = out;
@@ -1901,19 +1935,24 @@ int sr_fft_conv3(float* out, const float* img, int32_t nx, int32_t ny, int32_t n
     float* dst = host ? dout : out;
     HIP_TRY(sr::launch_psf_adj(sep, src, t3, vids, 1, dst, nullptr, 0, ny, g, nullptr));  // a block per plane
   } else {
-    // the caller's [x][y][z] -> planes [y][z][x]
-    float* planes = ws->part.as<float>();
-    HIP_TRY(sr::launch_permute_rows(src, planes, nx, ny, nz, 0, nullptr));
-    float* res = host ? din : out;  // (the staged input is free now)
-    float* res_planes = dout;
+    // the caller's [x][y][z] -> planes [y][z][x] (one plane: the same layout,
+    // read and written in place unless the output aliases the input)
+    float* res = host ? din : out;  // (the staged input is free after the permutation)
+    const bool direct = ny == 1 && (host || src != res);
+    const float* planes = src;
+    if (!direct) {
+      HIP_TRY(sr::launch_permute_rows(src, ws->part.as<float>(), nx, ny, nz, 0, nullptr));
+      planes = ws->part.as<float>();
+    }
+    float* res_planes = direct && !host ? res : dout;
     if (form == SR_CONV3_FWD) {
       int used = 0;
       HIP_TRY(sr::launch_psf_fwd(sep, planes, t, vids, ny, res_planes, 1, &used, gp, nullptr));
     } else {
       HIP_TRY(sr::launch_psf_adj(sep, planes, t, vids, ny, res_planes, nullptr, 0, 1, gp, nullptr));
     }
-    HIP_TRY(sr::launch_permute_rows(res_planes, res, nx, ny, nz, 1, nullptr));
-    dout = res;
+    if (!direct) HIP_TRY(sr::launch_permute_rows(res_planes, res, nx, ny, nz, 1, nullptr));
+    dout = direct ? res_planes : res;
   }
   HIP_TRY(hipStreamSynchronize(nullptr));
   if (host) HIP_TRY(hipMemcpy(out, dout, sizeof(float) * V, hipMemcpyDeviceToHost));

@@ -17,7 +17,7 @@ module SPECTreconAMD
 using LinearMapsAA: LinearMapAA, LinearMapAO
 
 export SPECTplanAMD, AblockAMD, operator, osem_strided!, gradient, project_vjp_mu, project_mu, set_mumap!,
-       autotune!, set_graph!
+       autotune!, set_graph!, PlanRotateAMD, PlanPSFAMD
 
 const libspect = joinpath(@__DIR__, "..", "lib", "libspectrecon_amd.so")
 
@@ -124,6 +124,85 @@ function dev_backproject!(image::AbstractArray{<:Real,3}, views::AbstractArray{<
         end
     end
     return image
+end
+
+# ---- unit operators ----------------------------------------------------------------
+# Device counterparts of PlanRotate (plan_rotate(nx; method), src/plan-rotate.jl:81-103)
+# and PlanPSF (plan_psf(; nx, nz, px, pz), src/plan-psf.jl:88-109): the library
+# keeps the buffers and tables, so a plan holds only what the checks need.
+struct PlanRotateAMD
+    nx::Int
+    method::Symbol
+end
+PlanRotateAMD(nx::Int; method::Symbol = :two) =
+    method in (:one, :two) ? PlanRotateAMD(nx, method) : throw("unknown rotation method $method")
+
+struct PlanPSFAMD
+    nx::Int
+    nz::Int
+    px::Int
+    pz::Int
+end
+PlanPSFAMD(; nx::Int = 128, nz::Int = nx, px::Int = 1, pz::Int = px) = PlanPSFAMD(nx, nz, px, pz)
+
+# imrotate! / imrotate_adj! of every [:, :, k] slice (src/rotatez.jl:240-416, :472-558)
+function dev_imrotate!(output::AbstractArray{<:Real}, img::AbstractArray{<:Real}, θ::Real, plan::PlanRotateAMD,
+                       adj::Bool)
+    size(output) == size(img) || throw(DimensionMismatch())
+    n = size(img, 1)
+    (size(img, 2) == n && n == plan.nx) || throw(DimensionMismatch("square slices of the plan's size"))
+    nz = ndims(img) == 3 ? size(img, 3) : 1
+    interp = Int32(plan.method === :one ? 1 : 2)
+    dense(output; out = true) do o
+        dense(img; out = false) do x
+            check(ccall((:sr_imrotate, libspect), Cint,
+                        (Ptr{Float32}, Ptr{Float32}, Int32, Int32, Float32, Int32, Int32, Int32),
+                        o, x, n, nz, Float32(θ), interp, Int32(adj), SR_MEM_HOST))
+        end
+    end
+    return output
+end
+
+# fft_conv! / fft_conv_adj! of one plane (src/fft_convolve.jl:33-130) with its
+# checks and messages
+function dev_fft_conv!(output::AbstractMatrix{<:Real}, img::AbstractMatrix{<:Real}, ker::AbstractMatrix{<:Real},
+                       plan::PlanPSFAMD, adj::Bool)
+    size(output) == size(img) || throw("size output")
+    size(img) == (plan.nx, plan.nz) || throw("size img")
+    size(ker) == (plan.px, plan.pz) || throw("size ker $(size(ker)) $(plan.px) $(plan.pz)")
+    dense(output; out = true) do o
+        dense(img; out = false) do x
+            dense(ker; out = false) do k
+                check(ccall((:sr_fft_conv, libspect), Cint,
+                            (Ptr{Float32}, Ptr{Float32}, Int32, Int32, Ptr{Float32}, Int32, Int32, Int32, Int32),
+                            o, x, plan.nx, plan.nz, k, plan.px, plan.pz, Int32(adj), SR_MEM_HOST))
+            end
+        end
+    end
+    return output
+end
+
+# the 3-D forms (src/fft_convolve.jl:160-239), every plane in one call:
+# form 0 fft_conv!, 1 fft_conv_adj! (image is [nx, ny, nz]), 2 fft_conv_adj2!
+# (image is one [nx, nz] plane; output planes iy = 1:size(output, 2))
+function dev_fft_conv3!(output::AbstractArray{<:Real,3}, image::AbstractArray{<:Real}, ker3::AbstractArray{<:Real,3},
+                        plan::PlanPSFAMD, form::Int)
+    nx, ny, nz = size(output)
+    (nx, nz) == (plan.nx, plan.nz) || throw("size img")
+    (size(ker3, 1), size(ker3, 2)) == (plan.px, plan.pz) || throw("size ker $(size(ker3)[1:2]) $(plan.px) $(plan.pz)")
+    size(ker3, 3) >= ny || throw(BoundsError(ker3, (1, 1, ny)))
+    k3 = size(ker3, 3) == ny ? ker3 : ker3[:, :, 1:ny]
+    dense(output; out = true) do o
+        dense(image; out = false) do x
+            dense(k3; out = false) do k
+                check(ccall((:sr_fft_conv3, libspect), Cint,
+                            (Ptr{Float32}, Ptr{Float32}, Int32, Int32, Int32, Ptr{Float32}, Int32, Int32, Int32,
+                             Int32),
+                            o, x, nx, ny, nz, k, plan.px, plan.pz, Int32(form), SR_MEM_HOST))
+            end
+        end
+    end
+    return output
 end
 
 # x0/out: the plan's image size; y/b: [nx, nz, nview] (the library reads and

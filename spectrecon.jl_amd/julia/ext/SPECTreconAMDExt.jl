@@ -1,7 +1,9 @@
 # SPECTreconAMDExt.jl -- the package extension that makes SPECTrecon.jl's own
 # functions run on a device plan: methods of SPECTrecon.project!,
 # backproject!, project, backproject, mlem!, osem! and Ablock for
-# SPECTreconAMD.SPECTplanAMD.  In SPECTrecon's Project.toml:
+# SPECTreconAMD.SPECTplanAMD, and of imrotate!, imrotate_adj!, fft_conv!,
+# fft_conv_adj! and fft_conv_adj2! for PlanRotateAMD / PlanPSFAMD.
+# In SPECTrecon's Project.toml:
 #   [weakdeps]    SPECTreconAMD = "<uuid>"
 #   [extensions]  SPECTreconAMDExt = "SPECTreconAMD"
 # Callers keep SPECTrecon's names and signatures; operators that are not
@@ -9,7 +11,8 @@
 module SPECTreconAMDExt
 
 using SPECTrecon: SPECTrecon, RealU
-using SPECTreconAMD: SPECTreconAMD, SPECTplanAMD, AblockAMD, dev_project!, dev_backproject!, dev_em!, dev_em_chat!
+using SPECTreconAMD: SPECTreconAMD, SPECTplanAMD, AblockAMD, dev_project!, dev_backproject!, dev_em!, dev_em_chat!,
+                     PlanRotateAMD, PlanPSFAMD, dev_imrotate!, dev_fft_conv!, dev_fft_conv3!
 using LinearMapsAA: LinearMapAA, LinearMapAO
 
 # ---- project! (src/project.jl) ----------------------------------------------------
@@ -99,5 +102,49 @@ function SPECTrecon.mlem!(out::AbstractArray, x0::AbstractArray, ynoisy::Abstrac
 end
 
 # (mlem(x0, ...) and osem(x0, ...), src/ml-os-em.jl:61, :143, call these mlem! / osem!)
+
+# ---- unit operators for device plans ------------------------------------------------
+# imrotate! / imrotate_adj! (src/rotatez.jl:240-416): one slice with a plan, a
+# 3-D stack with a vector of plans (:472-558; the device takes every slice at once)
+SPECTrecon.imrotate!(output::AbstractMatrix{<:RealU}, img::AbstractMatrix{<:RealU}, θ::RealU,
+                     plan::PlanRotateAMD) = dev_imrotate!(output, img, θ, plan, false)
+SPECTrecon.imrotate_adj!(output::AbstractMatrix{<:RealU}, img::AbstractMatrix{<:RealU}, θ::RealU,
+                         plan::PlanRotateAMD) = dev_imrotate!(output, img, θ, plan, true)
+SPECTrecon.imrotate!(output::AbstractArray{<:RealU,3}, image3::AbstractArray{<:RealU,3}, θ::RealU,
+                     plans::Vector{PlanRotateAMD}) = dev_imrotate!(output, image3, θ, plans[1], false)
+SPECTrecon.imrotate_adj!(output::AbstractArray{<:RealU,3}, image3::AbstractArray{<:RealU,3}, θ::RealU,
+                         plans::Vector{PlanRotateAMD}) = dev_imrotate!(output, image3, θ, plans[1], true)
+
+# fft_conv! / fft_conv_adj! (src/fft_convolve.jl:33-130) and the 3-D forms (:160-209)
+SPECTrecon.fft_conv!(output::AbstractMatrix{<:RealU}, img::AbstractMatrix{<:RealU}, ker::AbstractMatrix{<:RealU},
+                     plan::PlanPSFAMD) = dev_fft_conv!(output, img, ker, plan, false)
+SPECTrecon.fft_conv_adj!(output::AbstractMatrix{<:RealU}, img::AbstractMatrix{<:RealU},
+                         ker::AbstractMatrix{<:RealU}, plan::PlanPSFAMD) = dev_fft_conv!(output, img, ker, plan, true)
+function SPECTrecon.fft_conv!(output::AbstractArray{<:RealU,3}, image3::AbstractArray{<:RealU,3},
+                              ker3::AbstractArray{<:RealU,3}, plans::Vector{PlanPSFAMD})
+    size(output) == size(image3) || throw(DimensionMismatch())
+    return dev_fft_conv3!(output, image3, ker3, plans[1], 0)
+end
+function SPECTrecon.fft_conv_adj!(output::AbstractArray{<:RealU,3}, image3::AbstractArray{<:RealU,3},
+                                  ker3::AbstractArray{<:RealU,3}, plans::Vector{PlanPSFAMD})
+    size(output) == size(image3) || throw(DimensionMismatch())
+    return dev_fft_conv3!(output, image3, ker3, plans[1], 1)
+end
+
+# fft_conv_adj2!(output, image2, ker3, plans)  (src/fft_convolve.jl:217-239):
+# like the reference it writes planes iy = 1:size(image2, 2) of output (all of
+# them when nz == ny; a BoundsError when nz > ny)
+function SPECTrecon.fft_conv_adj2!(output::AbstractArray{<:RealU,3}, image2::AbstractMatrix{<:RealU},
+                                   ker3::AbstractArray{<:RealU,3}, plans::Vector{PlanPSFAMD})
+    size(output, 1) == size(image2, 1) || throw("size 1")
+    size(output, 3) == size(image2, 2) || throw("size 2")
+    nz = size(image2, 2)
+    nz <= size(output, 2) || throw(BoundsError(output, (1, nz, 1)))
+    nz == size(output, 2) && return dev_fft_conv3!(output, image2, ker3, plans[1], 2)
+    part = Array{Float32}(undef, size(output, 1), nz, nz)
+    dev_fft_conv3!(part, image2, ker3, plans[1], 2)
+    output[:, 1:nz, :] .= part
+    return output
+end
 
 end # module

@@ -146,6 +146,46 @@ def _ker3(rng, px, pz, n, kind):
     return np.asfortranarray(np.stack(ks, axis=2).astype(np.float32))
 
 
+def test_unit_workspace_reuse(gpu, O):
+    """The unit operators keep their tables and buffers between calls: a
+    sequence that switches kernels, plane sizes, angles and methods back and
+    forth (so cached tables are reused, replaced and grown) matches the
+    oracle at every call."""
+    rng = np.random.default_rng(21)
+    kers = {k: _ker3(rng, *k[:2], 1, k[2])[:, :, 0] for k in [(5, 5, "separable"), (5, 5, "general"),
+                                                              (9, 3, "separable")]}
+    imgs = {s: np.asfortranarray(rng.standard_normal(s).astype(np.float32)) for s in [(20, 14), (40, 24), (16, 16)]}
+    seq = [((20, 14), (5, 5, "separable")), ((20, 14), (5, 5, "general")), ((40, 24), (5, 5, "general")),
+           ((20, 14), (5, 5, "separable")), ((16, 16), (9, 3, "separable")), ((20, 14), (5, 5, "separable")),
+           ((40, 24), (9, 3, "separable"))]
+    for shape, kk in seq:
+        img, ker = imgs[shape], kers[kk]
+        for adj in (False, True):
+            r = O.fft_conv(img.astype(np.float64), ker.astype(np.float64), adj, "direct", np.float64)
+            g = gpu.fft_conv_adj(img, ker) if adj else gpu.fft_conv(img, ker)
+            assert maxrel(g, r) <= ROT_TOL, (shape, kk, adj)
+    vols = {n: np.asfortranarray(rng.standard_normal((n, n, 3)).astype(np.float32)) for n in (24, 40)}
+    for n, th, method in [(24, 0.3, "two"), (24, 2.0, "two"), (40, 2.0, "two"), (24, 0.3, "one"), (24, 0.3, "two"),
+                          (40, 0.3, "one")]:
+        th = float(np.float32(th))
+        for adj in (False, True):
+            g = (gpu.imrotate_adj if adj else gpu.imrotate)(vols[n], th, method=method)
+            r = O.imrotate(vols[n], th, method, adj=adj)
+            assert maxrel(g, r) <= ROT_TOL, (n, th, method, adj)
+    # device arrays, one plane: output apart from the input, and in place
+    from spectrecon_amd import _lib as L
+    lib = gpu.lib()
+    img, ker = imgs[(20, 14)], kers[(5, 5, "general")]
+    dk = gpu.DeviceArray.from_host(ker)
+    for adj in (0, 1):
+        di, do = gpu.DeviceArray.from_host(img), gpu.DeviceArray((20, 14))
+        L.check(lib.sr_fft_conv(do.ptr, di.ptr, 20, 14, dk.ptr, 5, 5, adj, L.SR_MEM_DEVICE))
+        L.check(lib.sr_fft_conv(di.ptr, di.ptr, 20, 14, dk.ptr, 5, 5, adj, L.SR_MEM_DEVICE))
+        np.testing.assert_array_equal(di.to_host(), do.to_host())
+        r = O.fft_conv(img.astype(np.float64), ker.astype(np.float64), bool(adj), "direct", np.float64)
+        assert maxrel(do.to_host(), r) <= ROT_TOL, ("device", adj)
+
+
 @pytest.mark.parametrize("nx,ny,nz,px,pz,kind", [(12, 12, 10, 5, 3, "separable"), (20, 7, 14, 5, 5, "general"),
                                                   (64, 32, 32, 15, 15, "mixed"), (33, 3, 1, 9, 1, "separable"),
                                                   (128, 4, 80, 37, 37, "general")])
