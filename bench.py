@@ -491,6 +491,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--latency-samples", type=int, default=200,
+                    help="reconstructions after the timed region for the latency percentiles")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in kernel choices")
     ap.add_argument("--choices", default=None,
                     help="pin the kernel choices (six comma-separated SR_CHOICE_* values) instead of autotuning")
@@ -590,7 +592,7 @@ def main():
         raise SystemExit("non-finite reconstruction")
     # latency percentiles from more samples than the timed steps (p99 of 20
     # samples is their maximum): the same reconstruction, outside the timed region
-    lat_n = max(200, args.steps)
+    lat_n = max(args.latency_samples, args.steps)
     lat_s = []
     for _ in range(lat_n):
         t1 = time.perf_counter()

@@ -932,6 +932,15 @@ __device__ __forceinline__ mf4 mfma4(float a, float b, mf4 c) { return __builtin
 // tail sum Tail(k) (k >= 0) and head sum Head(k) of nt-entry tables
 __device__ __forceinline__ float tail_at(const float* tail, int nt, int k) { return k < nt ? tail[max(k, 0)] : 0.f; }
 __device__ __forceinline__ float head_at(const float* head, int nt, int k) { return k < 0 ? 0.f : head[min(k, nt - 1)]; }
+// The A operand of a z-pass tile's replicate-boundary step (output row zm):
+// lane q4 == 0 Tail(zm + hz + 1) of the w tail sums swt, q4 == 1
+// Head(zm + hz - nz) of the head sums swh, other lanes 0.  One LDS load with
+// no branch: an LDS tap table holds zeros at entries -1 and nt (and beyond),
+// which stand for the zero cases of tail_at / head_at.
+__device__ __forceinline__ float bnd_tap(const float* swt, const float* swh, int nt, int q4, int zm, int hz, int nz) {
+  const float* p = q4 == 0 ? swt + min(zm + hz + 1, nt) : q4 == 1 ? swh + min(max(zm + hz - nz, -1), nt - 1) : swt + nt;
+  return *p;
+}
 
 // tap tables of one plane in LDS: u, w and the tail/head sums of u and w,
 // each with OFF zeros before and at least as many after (the Toeplitz
@@ -941,6 +950,7 @@ struct MfTaps {
   __host__ __device__ explicit MfTaps(int nt_) : nt(nt_) {
     off = 4 * mf_ksteps(nt_);
     tp = (off + 4 * mf_ksteps(nt_) + 16 + 3) / 4 * 4;
+    tp += (76 - tp % 64) % 64;  // tp = 12 (mod 64): lanes reading different tables of one plane hit distinct banks
     size = 6 * tp;  // u, w, u tail, u head, w tail, w head
   }
 };
@@ -1238,12 +1248,16 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
       rot_half(rb, MF_RA, yn);
     }
     // z pass: accumulate the view sum, the replicate boundary first
+    {
+      // (lanes q4 >= 2: zero weight on a real value; the loads first, then the products)
+      float ae[TPW], be[TPW];
 #pragma unroll
-    for (int i = 0; i < TPW; ++i) {
-      const int zm = z0s[i] + m16;
-      const float ae = q4 == 0 ? tail_at(swt, NT, zm + hz + 1) : q4 == 1 ? head_at(swh, NT, zm + hz - nz) : 0.f;
-      const float be = q4 == 0 ? Qz[hz * WQ + x0s[i] + m16] : q4 == 1 ? Qz[(hz + nz - 1) * WQ + x0s[i] + m16] : 0.f;
-      dv[i] = mfma4(ae, be, dv[i]);
+      for (int i = 0; i < TPW; ++i) {
+        ae[i] = bnd_tap(swt, swh, NT, q4, z0s[i] + m16, hz, nz);
+        be[i] = Qz[(q4 == 1 ? hz + nz - 1 : hz) * WQ + x0s[i] + m16];
+      }
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) dv[i] = mfma4(ae[i], be[i], dv[i]);
     }
     {
       float bq0[TPW], bq1[TPW], a0, a1;
@@ -1296,7 +1310,9 @@ __global__ void __launch_bounds__(NW * 64) k_psf_fwd_mfma(const float* __restric
 //   x pass  Q[z][x0+n] = sum_k P[clamp(x0-hx+k)][z] u[n+2hx-k]
 //   z pass  R[z0+m][x] = sum_k w[m+2hz-k] Q[z0-hz+k][x] + in[0]Tail + in[nz-1]Head
 // LDS: two planes (rows x, stride WZ = 16 mod 64 floats), per wave Qw (rows
-// z + hz, 16 columns), TBP tap planes.  grid (nchunk, nb), block 512.
+// z + hz, 16 columns in four 4-float chunks, chunk c of row r stored at
+// c ^ (((r - hz) / 2) & 3): the x pass's float4 stores then spread over all
+// banks), TBP tap planes.  grid (nchunk, nb), block 512.
 __host__ __device__ inline size_t mf_fcol_lds_floats(const Geo& g, int nt, int tbp) {
   return 2 * (size_t)g.nx * mf_wq(g.nz) + (size_t)MF_WAVES * mf_rq(g.nz, nt) * 16 + (size_t)tbp * MfTaps(nt).size +
          16 * (size_t)g.nx;
@@ -1491,7 +1507,7 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
     }
 #pragma unroll
     for (int i = 0; i < TPW; ++i)
-      *reinterpret_cast<float4*>(Qw + (hz + 16 * i + m16) * 16 + 4 * q4) =
+      *reinterpret_cast<float4*>(Qw + (hz + 16 * i + m16) * 16 + 4 * (q4 ^ ((m16 >> 1) & 3))) =
           make_float4(dx[i][0], dx[i][1], dx[i][2], dx[i][3]);
     // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -1502,20 +1518,25 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
     const float* swt = tb + 4 * TT.tp + TT.off;
     const float* swh = swt + TT.tp;
     {
-      const float be = q4 == 0 ? Qw[hz * 16 + m16] : q4 == 1 ? Qw[(hz + nz - 1) * 16 + m16] : 0.f;
+      // (lanes q4 >= 2: zero weight on a real value; the loads first, then the products)
+      const float be = Qw[q4 == 1 ? (hz + nz - 1) * 16 + (m16 ^ ((((nz - 1) >> 1) & 3) << 2)) : hz * 16 + m16];
+      float ae[TPW];
 #pragma unroll
-      for (int i = 0; i < TPW; ++i) {
-        const int zm = 16 * i + m16;
-        const float ae = q4 == 0 ? tail_at(swt, NT, zm + hz + 1) : q4 == 1 ? head_at(swh, NT, zm + hz - nz) : 0.f;
-        dv[i] = mfma4(ae, be, dv[i]);
-      }
+      for (int i = 0; i < TPW; ++i) ae[i] = bnd_tap(swt, swh, NT, q4, 16 * i + m16, hz, nz);
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) dv[i] = mfma4(ae[i], be, dv[i]);
     }
     {
+      // (Qw rows are swizzled in 4-float chunks: row r, column x at chunk
+      // (x / 4) ^ (((r - hz) / 2) & 3), which spreads the float4 stores of the
+      // x pass over all banks; a load's 4 rows x 16 columns stay on distinct banks)
+      const int hq = ((q4 - hz + 64) >> 1) & 3;
       float bq0[TPW], bq1[TPW], a0, a1;
       auto ld = [&](float (&bq)[TPW], float& aa, int s) {
         aa = wz[-4 * s];
+        const float* qr = Qw + (q4 + 4 * s) * 16 + (m16 ^ (((hq + 2 * s) & 3) << 2));
 #pragma unroll
-        for (int i = 0; i < TPW; ++i) bq[i] = Qw[(16 * i + q4 + 4 * s) * 16 + m16];
+        for (int i = 0; i < TPW; ++i) bq[i] = qr[256 * i];
       };
       ld(bq0, a0, 0);
       int s = 0;
@@ -1749,7 +1770,7 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
     }
 #pragma unroll
     for (int i = 0; i < TPW; ++i)
-      *reinterpret_cast<float4*>(Qw + (hz + 16 * i + m16) * 16 + 4 * q4) =
+      *reinterpret_cast<float4*>(Qw + (hz + 16 * i + m16) * 16 + 4 * (q4 ^ ((m16 >> 1) & 3))) =
           make_float4(dx[i][0], dx[i][1], dx[i][2], dx[i][3]);
     // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -1758,20 +1779,25 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
     const float* swt = tb + 4 * TT.tp + TT.off;
     const float* swh = swt + TT.tp;
     {
-      const float be = q4 == 0 ? Qw[hz * 16 + m16] : q4 == 1 ? Qw[(hz + nz - 1) * 16 + m16] : 0.f;
+      // (lanes q4 >= 2: zero weight on a real value; the loads first, then the products)
+      const float be = Qw[q4 == 1 ? (hz + nz - 1) * 16 + (m16 ^ ((((nz - 1) >> 1) & 3) << 2)) : hz * 16 + m16];
+      float ae[TPW];
 #pragma unroll
-      for (int i = 0; i < TPW; ++i) {
-        const int zm = 16 * i + m16;
-        const float ae = q4 == 0 ? tail_at(swt, NT, zm + hz + 1) : q4 == 1 ? head_at(swh, NT, zm + hz - nz) : 0.f;
-        dv[i] = mfma4(ae, be, dv[i]);
-      }
+      for (int i = 0; i < TPW; ++i) ae[i] = bnd_tap(swt, swh, NT, q4, 16 * i + m16, hz, nz);
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) dv[i] = mfma4(ae[i], be, dv[i]);
     }
     {
+      // (Qw rows are swizzled in 4-float chunks: row r, column x at chunk
+      // (x / 4) ^ (((r - hz) / 2) & 3), which spreads the float4 stores of the
+      // x pass over all banks; a load's 4 rows x 16 columns stay on distinct banks)
+      const int hq = ((q4 - hz + 64) >> 1) & 3;
       float bq0[TPW], bq1[TPW], a0, a1;
       auto ld = [&](float (&bq)[TPW], float& aa, int s) {
         aa = wz[-4 * s];
+        const float* qr = Qw + (q4 + 4 * s) * 16 + (m16 ^ (((hq + 2 * s) & 3) << 2));
 #pragma unroll
-        for (int i = 0; i < TPW; ++i) bq[i] = Qw[(16 * i + q4 + 4 * s) * 16 + m16];
+        for (int i = 0; i < TPW; ++i) bq[i] = qr[256 * i];
       };
       ld(bq0, a0, 0);
       int s = 0;
@@ -2499,7 +2525,8 @@ __global__ void __launch_bounds__(NW * 64) k_psf_adj_mfma(const float* __restric
 // float4 into the wave buffer.  LDS: Vx (rows z, column XO + x, stride WX =
 // 4 odd mod 64: a fragment's 16 rows x 4 columns hit distinct banks), per wave
 // Hw (rows z + hz with zero pads, 16 columns: a fragment's 4 rows x 16 columns
-// hit distinct banks), tap groups.  grid (nchunk, nb), block 512.
+// hit distinct banks; chunks swizzled as Qw in k_psf_fwd_col), tap groups.
+// grid (nchunk, nb), block 512.
 __host__ __device__ inline size_t mf_col_lds_floats(const Geo& g, int nt, int tbp, int nwv = MF_WAVES) {
   return (size_t)g.nz * mf_wx(g.nx, (g.px - 1) / 2, nt) + (size_t)nwv * mf_rq(g.nz, nt) * 16 +
          (size_t)tbp * MfTaps(nt).size;
@@ -2629,7 +2656,7 @@ __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restric
         }
 #pragma unroll
         for (int i = 0; i < TPW; ++i)
-          *reinterpret_cast<float4*>(Hw + (hz + 16 * i + m16) * 16 + 4 * q4) =
+          *reinterpret_cast<float4*>(Hw + (hz + 16 * i + m16) * 16 + 4 * (q4 ^ ((m16 >> 1) & 3))) =
               make_float4(dh[i][0], dh[i][1], dh[i][2], dh[i][3]);
         // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -2638,13 +2665,16 @@ __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restric
 #pragma unroll
         for (int i = 0; i < TPW; ++i) dz[i] = mf4{0.f, 0.f, 0.f, 0.f};
         {
+          // (Hw rows swizzled in 4-float chunks as the forward's Qw rows)
+          const int hq = ((q4 - hz + 64) >> 1) & 3;
           float bq0[TPW], bq1[TPW], a0[TPW], a1[TPW];
           auto ld = [&](float (&bq)[TPW], float (&aa)[TPW], int s) {
             const float am = sw[4 * s];
+            const float* hr = Hw + (q4 + 4 * s) * 16 + (m16 ^ (((hq + 2 * s) & 3) << 2));
 #pragma unroll
             for (int i = 0; i < TPW; ++i) {
               aa[i] = i == 0 ? ap0[4 * s] : i == TPW - 1 ? apl[4 * s] : am;
-              bq[i] = Hw[(16 * i + q4 + 4 * s) * 16 + m16];
+              bq[i] = hr[256 * i];
             }
           };
           ld(bq0, a0, 0);
