@@ -117,7 +117,7 @@ def test_fft_conv_vs_oracle(gpu, O, nx, nz, px, pz, kind):
 
 
 def test_fft_conv_bounds(gpu):
-    """test/fftconv.jl:198-214: box kernel keeps rand <= 1, adjoint <= 1.5."""
+    """test/fftconv.jl:32-48: box kernel keeps rand <= 1, adjoint <= 1.5."""
     rng = np.random.default_rng(3)
     nx, nz, px, pz = 12, 10, 5, 3
     plan = gpu.plan_psf(nx=nx, nz=nz, px=px, pz=pz)

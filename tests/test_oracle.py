@@ -206,7 +206,7 @@ def test_fft_conv_dense_adjoint(O):
 
 
 def test_fft_conv_bounds(O):
-    """test/fftconv.jl:198-214: box kernel of rand <= 1, adjoint <= 1.5."""
+    """test/fftconv.jl:32-48: box kernel of rand <= 1, adjoint <= 1.5."""
     rng = np.random.default_rng(6)
     img = rng.random((12, 10)).astype(np.float32)
     ker = np.ones((5, 3), np.float32) / 15
