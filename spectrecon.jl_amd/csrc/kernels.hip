@@ -929,6 +929,30 @@ __host__ __device__ __forceinline__ int mf_rq(int nz, int nt) { return nz - 16 +
 
 __device__ __forceinline__ mf4 mfma4(float a, float b, mf4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
+// Transpose of the 4 x 4 blocks (lane group q = lane / 16, register r) of an
+// accumulator: register g of group q receives register q of group g.  Applied
+// to an output tile with rows m = 4 q + r, register g becomes the B operand of
+// the K-step over rows 4 g .. 4 g + 3 (group q holds row 4 g + q), so one
+// pass's result feeds the other pass's products without leaving registers.
+// Two lane swaps across halves (v_permlane32_swap), two across 16-lane rows
+// (v_permlane16_swap).
+__device__ __forceinline__ mf4 xpose_groups(mf4 d) {
+  const auto p0 = __builtin_amdgcn_permlane32_swap(__float_as_uint(d[0]), __float_as_uint(d[2]), false, false);
+  const auto p1 = __builtin_amdgcn_permlane32_swap(__float_as_uint(d[1]), __float_as_uint(d[3]), false, false);
+  const auto s0 = __builtin_amdgcn_permlane16_swap(p0[0], p1[0], false, false);
+  const auto s1 = __builtin_amdgcn_permlane16_swap(p0[1], p1[1], false, false);
+  return mf4{__uint_as_float(s0[0]), __uint_as_float(s0[1]), __uint_as_float(s1[0]), __uint_as_float(s1[1])};
+}
+// Row groups of the register-resident z passes: output tile j (rows 16 j ..
+// 16 j + 15) takes source row groups G = 4 j + d (rows 4 G .. 4 G + 3) for
+// d in [zp_dlo(hz), zp_dhi(hz)], the groups that meet its band of 2 hz + 1
+// rows; groups outside [0, nz / 4) hold no rows and are skipped, not
+// multiplied as zeros.  ZP_DMAX bounds -zp_dlo for taps up to 64 (hz <= 31).
+constexpr int ZP_DMAX = 8;
+__host__ __device__ constexpr int zp_dlo(int hz) { return -((hz + 3) >> 2); }
+__host__ __device__ constexpr int zp_dhi(int hz) { return (15 + hz) >> 2; }
+__host__ __device__ constexpr int floor4(int d) { return d >= 0 ? d / 4 : -((3 - d) / 4); }
+
 // tail sum Tail(k) (k >= 0) and head sum Head(k) of nt-entry tables
 __device__ __forceinline__ float tail_at(const float* tail, int nt, int k) { return k < nt ? tail[max(k, 0)] : 0.f; }
 __device__ __forceinline__ float head_at(const float* head, int nt, int k) { return k < 0 ? 0.f : head[min(k, nt - 1)]; }
@@ -1571,19 +1595,20 @@ __global__ void __launch_bounds__(MF_THREADS) k_psf_fwd_col(const float* __restr
 
 // --- forward, z-fastest, producer and consumer waves -----------------------
 // The passes of k_psf_fwd_col, with the staging moved to waves of their own:
-// waves 0..7 (consumers) own an x column each and run plane y's x and z
-// passes from LDS, while the NPW producer waves stage plane y + 1 (the
-// gathers of the :two rotation through its tap table, attenuation, bilinear
-// combination), load plane y + 1's PSF taps and build plane y + 2's rotation
-// table.  The gathers and their arithmetic then overlap the matrix products
-// instead of alternating with them; one barrier per plane.
-// LDS: two planes (rows x, stride WZ), per consumer wave Qw, two tap planes
-// (planes of even / odd offset from y0), two rotation tables.
+// waves 0..7 (consumers) own an x column each and run plane y's x pass from
+// the staged plane and its z pass from registers (the x pass's tiles
+// transposed in 4 x 4 blocks of lane groups), while the NPW producer waves
+// stage plane y + 1 (the gathers of the :two rotation through its tap table,
+// attenuation, bilinear combination), load plane y + 1's PSF taps and build
+// plane y + 2's rotation table.  The gathers and their arithmetic then
+// overlap the matrix products instead of alternating with them; one barrier
+// per plane.  nz = 16 TPW.
+// LDS: two planes (rows x, stride WZ), two tap planes (planes of even / odd
+// offset from y0), two rotation tables.
 // grid (nchunk, nb), block 512 + 64 NPW.
 constexpr int WS_RE = 5;  // element slots per producer thread and round
 __host__ __device__ inline size_t mf_fws_lds_floats(const Geo& g, int nt) {
-  return 2 * (size_t)g.nx * mf_wq(g.nz) + (size_t)MF_WAVES * mf_rq(g.nz, nt) * 16 + 2 * (size_t)MfTaps(nt).size +
-         16 * (size_t)g.nx;
+  return 2 * (size_t)g.nx * mf_wq(g.nz) + 2 * (size_t)MfTaps(nt).size + 16 * (size_t)g.nx;
 }
 template <int TPW, int NPW>
 __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const float* __restrict__ img_t, Taps t,
@@ -1599,9 +1624,9 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
   const MfTaps TT(NT);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
-  const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
+  const int nsx = (16 + g.px - 1 + 3) / 4;
   const int ntx = nx >> 4;
-  const int WZ = mf_wq(nz), RG = mf_rq(nz, NT), PS = nx * WZ;
+  const int WZ = mf_wq(nz), PS = nx * WZ;
   const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
   const int v = vids[b];
   const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
@@ -1610,10 +1635,9 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
   const bool producer = wave >= MF_WAVES;
   const int m16 = lane & 15, q4 = lane >> 4;
   float* Pz = L;  // two plane buffers
-  float* Qw = L + 2 * PS + min(wave, MF_WAVES - 1) * RG * 16;
-  float* s_taps = L + 2 * PS + MF_WAVES * RG * 16;  // two tap planes of TT.size
+  float* s_taps = L + 2 * PS;  // two tap planes of TT.size
   float4* s_rt = reinterpret_cast<float4*>(s_taps + 2 * TT.size);
-  for (int k = tid; k < MF_WAVES * RG * 16 + 2 * TT.size; k += MF_THREADS + PT) L[2 * PS + k] = 0.f;
+  for (int k = tid; k < 2 * TT.size; k += MF_THREADS + PT) L[2 * PS + k] = 0.f;
 
   if (producer) {
     const RotView rot = rv[v];
@@ -1728,6 +1752,7 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
   __syncthreads();  // zero fill and the first two rotation tables done
   const bool own = wave < ntx;  // (waves past the last column recompute it and store nothing)
   const int x0 = 16 * min(wave, ntx - 1);
+  const int dlo = zp_dlo(hz), dhi = zp_dhi(hz);
   mf4 dv[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; ++i) dv[i] = mf4{0.f, 0.f, 0.f, 0.f};
@@ -1737,7 +1762,7 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
     lds_barrier();  // plane y and its taps staged; plane y - 1's buffers free
     const float* tb = s_taps + cur * TT.size;
     const float* Pc = Pz + cur * PS;
-    // x pass (operands swapped: lane (k, n) register r = Q[z0 + n][x0 + 4k + r])
+    // x pass: lane (k, n) register r = Q[16 i + 4 k + r][x0 + n]
     const float* ub = tb + TT.off + m16 - q4 + 2 * hx;
     mf4 dx[TPW];
 #pragma unroll
@@ -1756,68 +1781,44 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
         ld(a1, b1, s + 1);
         SR_SCHED_FENCE();
 #pragma unroll
-        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b0, a0[i], dx[i]);
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(a0[i], b0, dx[i]);
         SR_SCHED_FENCE();
         ld(a0, b0, min(s + 2, nsx - 1));
         SR_SCHED_FENCE();
 #pragma unroll
-        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b1, a1[i], dx[i]);
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(a1[i], b1, dx[i]);
       }
       if (s < nsx) {
 #pragma unroll
-        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(b0, a0[i], dx[i]);
+        for (int i = 0; i < TPW; ++i) dx[i] = mfma4(a0[i], b0, dx[i]);
       }
     }
+    // register g of tile i: lane (k, n) = Q[16 i + 4 g + k][x0 + n], the B
+    // operand of the z pass's step over source row group 4 i + g
 #pragma unroll
-    for (int i = 0; i < TPW; ++i)
-      *reinterpret_cast<float4*>(Qw + (hz + 16 * i + m16) * 16 + 4 * (q4 ^ ((m16 >> 1) & 3))) =
-          make_float4(dx[i][0], dx[i][1], dx[i][2], dx[i][3]);
-    // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    // z pass: accumulate the view sum, the replicate boundary first
-    const float* wz = tb + TT.tp + TT.off + m16 - q4 + 2 * hz;
+    for (int i = 0; i < TPW; ++i) dx[i] = xpose_groups(dx[i]);
+    // z pass: accumulate the view sum.  Output tile j, source group G = 4 j + d:
+    // A (m, k) = w[m - k + hz - 4 d]; the replicate boundary rides on the
+    // groups holding rows 0 and nz - 1 (k = 0 of G = 0 adds Tail(z + hz + 1),
+    // k = 3 of the last group Head(z + hz - nz), z = 16 j + m).
+    const float* wz = tb + TT.tp + TT.off + m16 - q4 + hz;
     const float* swt = tb + 4 * TT.tp + TT.off;
     const float* swh = swt + TT.tp;
-    {
-      // (lanes q4 >= 2: zero weight on a real value; the loads first, then the products)
-      const float be = Qw[q4 == 1 ? (hz + nz - 1) * 16 + (m16 ^ ((((nz - 1) >> 1) & 3) << 2)) : hz * 16 + m16];
-      float ae[TPW];
 #pragma unroll
-      for (int i = 0; i < TPW; ++i) ae[i] = bnd_tap(swt, swh, NT, q4, 16 * i + m16, hz, nz);
+    for (int d = -ZP_DMAX; d <= ZP_DMAX + 3; ++d) {
+      if (d < dlo || d > dhi) continue;  // (wave-uniform)
+      const int e = floor4(d), gg = d - 4 * e;
+      const float a = wz[-4 * d];
 #pragma unroll
-      for (int i = 0; i < TPW; ++i) dv[i] = mfma4(ae[i], be, dv[i]);
-    }
-    {
-      // (Qw rows are swizzled in 4-float chunks: row r, column x at chunk
-      // (x / 4) ^ (((r - hz) / 2) & 3), which spreads the float4 stores of the
-      // x pass over all banks; a load's 4 rows x 16 columns stay on distinct banks)
-      const int hq = ((q4 - hz + 64) >> 1) & 3;
-      float bq0[TPW], bq1[TPW], a0, a1;
-      auto ld = [&](float (&bq)[TPW], float& aa, int s) {
-        aa = wz[-4 * s];
-        const float* qr = Qw + (q4 + 4 * s) * 16 + (m16 ^ (((hq + 2 * s) & 3) << 2));
-#pragma unroll
-        for (int i = 0; i < TPW; ++i) bq[i] = qr[256 * i];
-      };
-      ld(bq0, a0, 0);
-      int s = 0;
-      for (; s + 1 < nsz; s += 2) {
-        ld(bq1, a1, s + 1);
-        SR_SCHED_FENCE();
-#pragma unroll
-        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a0, bq0[i], dv[i]);
-        SR_SCHED_FENCE();
-        ld(bq0, a0, min(s + 2, nsz - 1));
-        SR_SCHED_FENCE();
-#pragma unroll
-        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a1, bq1[i], dv[i]);
-      }
-      if (s < nsz) {
-#pragma unroll
-        for (int i = 0; i < TPW; ++i) dv[i] = mfma4(a0, bq0[i], dv[i]);
+      for (int j = 0; j < TPW; ++j) {
+        const int i = j + e;
+        if (i < 0 || i >= TPW) continue;
+        float aa = a;
+        if (i == 0 && gg == 0) aa += q4 == 0 ? swt[min(16 * j + m16 + hz + 1, NT)] : 0.f;
+        if (i == TPW - 1 && gg == 3) aa += q4 == 3 ? swh[min(max(16 * j + m16 + hz - nz, -1), NT - 1)] : 0.f;
+        dv[j] = mfma4(aa, dx[i][gg], dv[j]);
       }
     }
-    asm volatile("" ::: "memory");  // the next plane's Qw stores stay after these reads
   }
   if (own) {
     float* pb = part + ((long)b * nchunk + c) * nx * nz;
@@ -2513,27 +2514,24 @@ __global__ void __launch_bounds__(NW * 64) k_psf_adj_mfma(const float* __restric
 // --- adjoint, separable, matrix cores, column-owned (z-fastest) --------------
 // The adjoint in the other pass order (the two 1-D passes act on different
 // axes and commute): the x adjoint first, from the block's view staged once in
-// LDS and only read afterwards, then the z adjoint from a wave-private LDS
-// buffer.  Wave w owns the 16-wide x columns w, w + 8, ... with all nz/16 z
-// tiles of each, so no block barrier separates the passes or the planes (two
-// per tap group only):
+// LDS and only read afterwards, then the z adjoint from registers (the x
+// adjoint's tiles transposed in 4 x 4 blocks of lane groups, source row groups
+// outside the plane skipped).  Each wave takes (plane, 16-wide x column) items
+// with all nz/16 z tiles of the column, so no block barrier separates the
+// passes or the planes (two per tap group only):
 //   x adjoint  H[z][x0+n] = sum_k V[z][x0-hx+k] u'[k-n]
 //   z adjoint  R[z0+m][x] = sum_k w'[k-m] H[z0-hz+k][x]
 // (u', w': the taps, or their tail / head sums for output column / row 0 and
-// n-1, the transposed replicate boundary).  The x adjoint takes its operands
-// swapped, so a lane holds four consecutive x of one z and stores them as a
-// float4 into the wave buffer.  LDS: Vx (rows z, column XO + x, stride WX =
-// 4 odd mod 64: a fragment's 16 rows x 4 columns hit distinct banks), per wave
-// Hw (rows z + hz with zero pads, 16 columns: a fragment's 4 rows x 16 columns
-// hit distinct banks; chunks swizzled as Qw in k_psf_fwd_col), tap groups.
-// grid (nchunk, nb), block 512.
-__host__ __device__ inline size_t mf_col_lds_floats(const Geo& g, int nt, int tbp, int nwv = MF_WAVES) {
-  return (size_t)g.nz * mf_wx(g.nx, (g.px - 1) / 2, nt) + (size_t)nwv * mf_rq(g.nz, nt) * 16 +
-         (size_t)tbp * MfTaps(nt).size;
+// n-1, the transposed replicate boundary).  LDS: Vx (rows z, column XO + x,
+// stride WX = 4 odd mod 64: a fragment's 16 rows x 4 columns hit distinct
+// banks), tap groups.  nz = 16 TPW.
+// grid (nchunk, nb), block 64 NWV.
+__host__ __device__ inline size_t mf_col_lds_floats(const Geo& g, int nt, int tbp) {
+  return (size_t)g.nz * mf_wx(g.nx, (g.px - 1) / 2, nt) + (size_t)tbp * MfTaps(nt).size;
 }
-// NWV waves per block (8, or 12 where the LDS holds their buffers: three
-// waves per SIMD hide more of each other's LDS and memory waits); the block's
-// (plane, column) items of a tap group are dealt round robin to the waves.
+// NWV waves per block (16: four waves per SIMD hide more of each other's LDS
+// and memory waits); the block's (plane, column) items of a tap group are
+// dealt round robin to the waves.
 template <int TPW, int NWV>
 __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restrict__ views, Taps t,
                                                             const int* __restrict__ vids, int Yc,
@@ -2545,9 +2543,10 @@ __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restric
   const MfTaps TT(NT);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int hx = (g.px - 1) / 2, hz = (g.pz - 1) / 2;
-  const int nsx = (16 + g.px - 1 + 3) / 4, nsz = (16 + g.pz - 1 + 3) / 4;
+  const int nsx = (16 + g.px - 1 + 3) / 4;
+  const int dlo = zp_dlo(hz), dhi = zp_dhi(hz);
   const int ntx = nx >> 4;
-  const int WX = mf_wx(nx, hx, NT), XO = mf_xo(hx), RG = mf_rq(nz, NT);
+  const int WX = mf_wx(nx, hx, NT), XO = mf_xo(hx);
   const int b = blockIdx.y, c = blockIdx.x;
   const int v = vids[b];
   const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
@@ -2555,16 +2554,15 @@ __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restric
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int m16 = lane & 15, q4 = lane >> 4;
   float* Vx = L;
-  float* Hw = Vx + nz * WX + wave * RG * 16;
-  float* s_taps = Vx + nz * WX + NWV * RG * 16;  // TBP planes of TT.size
-  // zeros where no data lands: Vx's pad columns, the wave buffers, the tap tables
+  float* s_taps = Vx + nz * WX;  // TBP planes of TT.size
+  // zeros where no data lands: Vx's pad columns, the tap tables
   {
     const int padc = WX - nx;  // columns [0, XO) and [XO + nx, WX) of every row
     for (int k = tid; k < nz * padc; k += NTH) {
       const int z = k / padc, cc = k - z * padc;
       Vx[z * WX + (cc < XO ? cc : cc + nx)] = 0.f;
     }
-    for (int k = tid; k < NWV * RG * 16 + TBP * TT.size; k += NTH) L[nz * WX + k] = 0.f;
+    for (int k = tid; k < TBP * TT.size; k += NTH) L[nz * WX + k] = 0.f;
   }
   const float* vin = views + (long)b * nx * nz;
   const int nq = nx >> 2;
@@ -2613,8 +2611,9 @@ __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restric
       // z taps: the first tile's row 0 takes the tail sums, the last tile's
       // row 15 the head sums (nz = 16 TPW), every other row the plain taps, so
       // a step reads three tap words: plain, first tile, last tile
-      const float* ap0 = m16 == 0 ? swt : (TPW == 1 && m16 == 15) ? swh : sw;
-      const float* apl = m16 == 15 ? swh : sw;
+      const float* apm = sw + hz;
+      const float* ap0 = (m16 == 0 ? swt : (TPW == 1 && m16 == 15) ? swh : sw) + hz;
+      const float* apl = (m16 == 15 ? swh : sw) + hz;
       {
         const int x0 = 16 * col;
         const long ob = ((long)y * nx + x0 + m16) * nz + 4 * q4;  // + 16 i: lane's 4 z of tile i
@@ -2623,7 +2622,7 @@ __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restric
 #pragma unroll
         for (int i = 0; i < TPW; ++i)
           af[i] = av ? *reinterpret_cast<const float4*>(av + ob + 16 * i) : make_float4(1.f, 1.f, 1.f, 1.f);
-        // x adjoint (operands swapped: lane (k, n) register r = H[16i + n][x0 + 4k + r])
+        // x adjoint: lane (k, n) register r = H[16 i + 4 k + r][x0 + n]
         const float* bp = (x0 == 0 && m16 == 0) ? sut : (x0 + 16 == nx && m16 == 15) ? suh : su;
         const float* va = Vx + m16 * WX + XO - hx + x0 + q4;
         mf4 dh[TPW];
@@ -2642,60 +2641,38 @@ __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restric
             ld(a1, b1, s + 1);
             SR_SCHED_FENCE();
 #pragma unroll
-            for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b0, a0[i], dh[i]);
+            for (int i = 0; i < TPW; ++i) dh[i] = mfma4(a0[i], b0, dh[i]);
             SR_SCHED_FENCE();
             ld(a0, b0, min(s + 2, nsx - 1));
             SR_SCHED_FENCE();
 #pragma unroll
-            for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b1, a1[i], dh[i]);
+            for (int i = 0; i < TPW; ++i) dh[i] = mfma4(a1[i], b1, dh[i]);
           }
           if (s < nsx) {
 #pragma unroll
-            for (int i = 0; i < TPW; ++i) dh[i] = mfma4(b0, a0[i], dh[i]);
+            for (int i = 0; i < TPW; ++i) dh[i] = mfma4(a0[i], b0, dh[i]);
           }
         }
+        // register g of tile i: lane (k, n) = H[16 i + 4 g + k][x0 + n]
 #pragma unroll
-        for (int i = 0; i < TPW; ++i)
-          *reinterpret_cast<float4*>(Hw + (hz + 16 * i + m16) * 16 + 4 * (q4 ^ ((m16 >> 1) & 3))) =
-              make_float4(dh[i][0], dh[i][1], dh[i][2], dh[i][3]);
-        // the wave reads back what its own lanes wrote (LDS keeps a wave's order)
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        // z adjoint: lane (k, n) register r = R[16i + 4k + r][x0 + n]
+        for (int i = 0; i < TPW; ++i) dh[i] = xpose_groups(dh[i]);
+        // z adjoint from registers: output tile j, source group G = 4 j + d,
+        // A (m, k) = w'[4 d + k - m + hz]; lane register r = R[16 j + 4 k + r][x0 + n]
         mf4 dz[TPW];
 #pragma unroll
         for (int i = 0; i < TPW; ++i) dz[i] = mf4{0.f, 0.f, 0.f, 0.f};
-        {
-          // (Hw rows swizzled in 4-float chunks as the forward's Qw rows)
-          const int hq = ((q4 - hz + 64) >> 1) & 3;
-          float bq0[TPW], bq1[TPW], a0[TPW], a1[TPW];
-          auto ld = [&](float (&bq)[TPW], float (&aa)[TPW], int s) {
-            const float am = sw[4 * s];
-            const float* hr = Hw + (q4 + 4 * s) * 16 + (m16 ^ (((hq + 2 * s) & 3) << 2));
 #pragma unroll
-            for (int i = 0; i < TPW; ++i) {
-              aa[i] = i == 0 ? ap0[4 * s] : i == TPW - 1 ? apl[4 * s] : am;
-              bq[i] = hr[256 * i];
-            }
-          };
-          ld(bq0, a0, 0);
-          int s = 0;
-          for (; s + 1 < nsz; s += 2) {
-            ld(bq1, a1, s + 1);
-            SR_SCHED_FENCE();
+        for (int d = -ZP_DMAX; d <= ZP_DMAX + 3; ++d) {
+          if (d < dlo || d > dhi) continue;  // (wave-uniform)
+          const int e = floor4(d), gg = d - 4 * e;
+          const float am = apm[4 * d], a0 = ap0[4 * d], al = apl[4 * d];
 #pragma unroll
-            for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a0[i], bq0[i], dz[i]);
-            SR_SCHED_FENCE();
-            ld(bq0, a0, min(s + 2, nsz - 1));
-            SR_SCHED_FENCE();
-#pragma unroll
-            for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a1[i], bq1[i], dz[i]);
-          }
-          if (s < nsz) {
-#pragma unroll
-            for (int i = 0; i < TPW; ++i) dz[i] = mfma4(a0[i], bq0[i], dz[i]);
+          for (int j = 0; j < TPW; ++j) {
+            const int i = j + e;
+            if (i < 0 || i >= TPW) continue;
+            dz[j] = mfma4(j == 0 ? a0 : j == TPW - 1 ? al : am, dh[i][gg], dz[j]);
           }
         }
-        asm volatile("" ::: "memory");  // the next column's Hw stores stay after these reads
 #pragma unroll
         for (int i = 0; i < TPW; ++i)
           *reinterpret_cast<float4*>(wout + ob + 16 * i) = make_float4(dz[i][0] * af[i].x, dz[i][1] * af[i].y,
@@ -3573,23 +3550,16 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
   nchunk = (g.ny + Yc - 1) / Yc;
   if (separable && zfast && family != PSF_TILE && psf_adj_col_ok(g, t.ntx)) {
     const long per = MfTaps(t.ntx).size;
-    // tap planes that fit beside the view and nwv wave buffers
-    auto room = [&](int nwv) {
-      return ((long)(kMaxLds / sizeof(float)) - (long)mf_col_lds_floats(g, t.ntx, 0, nwv)) / per;
-    };
-    const int nwv = room(12) >= std::min(4, Yc) ? 12 : MF_WAVES;
-    const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), room(nwv)));
-    const size_t lds = sizeof(float) * mf_col_lds_floats(g, t.ntx, tbp, nwv);
+    // tap planes that fit beside the view
+    const long room = ((long)(kMaxLds / sizeof(float)) - (long)mf_col_lds_floats(g, t.ntx, 0)) / per;
+    const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), room));
+    const size_t lds = sizeof(float) * mf_col_lds_floats(g, t.ntx, tbp);
+    // (16 waves hold at most 128 registers each: the 8-tile columns take 8)
     SR_TPW_SWITCH(g.nz / 16, {
-      if (nwv == 12) {
-        allow_lds(k_psf_adj_col<TPW, 12>, lds);
-        k_psf_adj_col<TPW, 12><<<dim3(nchunk, nb), 64 * 12, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride,
+      constexpr int NWV = TPW >= 8 ? 8 : 16;
+      allow_lds(k_psf_adj_col<TPW, NWV>, lds);
+      k_psf_adj_col<TPW, NWV><<<dim3(nchunk, nb), 64 * NWV, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride,
                                                                       g, tbp);
-      } else {
-        allow_lds(k_psf_adj_col<TPW, MF_WAVES>, lds);
-        k_psf_adj_col<TPW, MF_WAVES><<<dim3(nchunk, nb), MF_THREADS, lds, s>>>(views, t, vids, Yc, work, atten,
-                                                                               atten_vstride, g, tbp);
-      }
     });
     return hipGetLastError();
   }
