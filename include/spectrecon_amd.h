@@ -226,8 +226,12 @@ int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
  *   PSF_METHOD: 0 the plan's built-in method, 1 the direct taps (separable
  *     when every PSF plane is rank 1, else the 2-D clamped convolution), 2
  *     the plane FFTs (plans with PSF spectra: x-fastest layout and padded
- *     planes the FFT kernels take; SR_ERR_UNSUPPORTED otherwise).
- * Defaults {0, 100, 0, 100, 0, 0}.  Every choice computes the same operator;
+ *     planes the FFT kernels take; SR_ERR_UNSUPPORTED otherwise);
+ *   EM_NORM: where OS-EM on a z-fastest plan with one PSF stack for all views
+ *     sums its subset normalisers A_b' 1: 0 built-in (1), 1 in the gather of
+ *     each subset's first iteration (the same pass as its back-projection),
+ *     2 all subsets in one gather launch before the first iteration.
+ * Defaults {0, 100, 0, 100, 0, 0, 0}.  Every choice computes the same operator;
  * the gather's slice count changes the order of its sums (last-bit
  * differences).  sr_plan_autotune times the candidates of each choice on
  * batches of nsub views (the OS-EM subset size) on the plan's stream, keeps
@@ -241,7 +245,8 @@ int sr_plan_graph_stats(sr_plan plan, int64_t* captures, int64_t* replays);
 #define SR_CHOICE_ADJ_CHUNKS 3
 #define SR_CHOICE_GATHER_SLICES 4
 #define SR_CHOICE_PSF_METHOD 5
-#define SR_NCHOICE 6
+#define SR_CHOICE_EM_NORM 6
+#define SR_NCHOICE 7
 int sr_plan_set_choice(sr_plan plan, const int32_t* choice);
 int sr_plan_get_choice(sr_plan plan, int32_t* choice);
 int sr_plan_autotune(sr_plan plan, int32_t nsub, char* log, size_t log_bytes);

@@ -139,6 +139,13 @@ bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::v
 hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* ent, int K, const int* vids, int nb,
                                      const Geo& g, hipStream_t s, const float* hvol, const float* atten,
                                      long atten_vstride, float* asum, int vs = 0);
+// every OS-EM normaliser of a reconstruction in one launch: asum[k] =
+// sum over view list k of rot2^T(hvol .* atten[v]) (zeros as Inf), for
+// nsubsets consecutive lists in vids of nb views, the first nb_rem of them
+// nb + 1 (the strided blocks laid out block after block); asum [k][y][x][z]
+hipError_t launch_rot2_adj_z_norms(const float* hvol, float* asum, const int2* ent, int K, const int* vids, int nb,
+                                   int nb_rem, int nsubsets, const Geo& g, hipStream_t s, const float* atten,
+                                   long atten_vstride, int vs = 0);
 // img [z][y][x] -> out [y][x][z]
 hipError_t launch_to_zfast(const float* img, float* out, const Geo& g, hipStream_t s);
 // out[b] = atten[v] .* rot2(img_t) per view, all [y][x][z]; atten may be null

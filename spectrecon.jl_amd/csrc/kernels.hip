@@ -2963,11 +2963,16 @@ __global__ void __launch_bounds__(256) k_rot2_fwd_z(const float* __restrict__ im
 // in place, x .*= back ./ asum (src/ml-os-em.jl:46), asum z-fastest too.
 // zero_inf (MODE 0/1): the sums are OS-EM normalisers, zeros written as Inf
 // (src/ml-os-em.jl:110).
-// grid 8 * xcd_chunk(ceil(NN / PW)), block 64 * VS
+// Subsets (gridDim.y > 1): grid row k sums the k-th of consecutive view
+// lists of nb views, the first nb_rem of them one view longer (the strided
+// blocks of src/ml-os-em.jl:76 laid out block after block), into volume k of
+// part (and of em_asum): every OS-EM normaliser from one launch.
+// grid (8 * xcd_chunk(ceil(NN / PW)), subsets), block 64 * VS
 template <int K, int VS, int Q, int MODE>
 __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restrict__ work, float* __restrict__ part,
                                                           const int2* __restrict__ ent, const int* __restrict__ vids,
-                                                          int nb, int C, Geo g, const float* __restrict__ hvol,
+                                                          int nb, int nb_rem, int C, Geo g,
+                                                          const float* __restrict__ hvol,
                                                           const float* __restrict__ atten, long atten_vstride,
                                                           float* __restrict__ em_asum, int zero_inf) {
   constexpr int NS = MODE == 2 ? 2 : 1;  // sums carried
@@ -2981,6 +2986,13 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
   const bool act = sub < pw && j < NN;
   const int jj = min(j, NN - 1), zo = 4 * Q * (act ? zl : 0);
   const long V = (long)NN * nz;
+  {
+    const int k = blockIdx.y;  // subset
+    vids += k * nb + min(k, nb_rem);
+    nb += k < nb_rem ? 1 : 0;
+    part += V * k;
+    if (em_asum) em_asum += V * k;
+  }
   float4 acc[Q * NS];
 #pragma unroll
   for (int q = 0; q < Q * NS; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -3712,34 +3724,41 @@ hipError_t launch_rot2_fwd_z(const float* img_t, float* out, const float* atten,
 }
 
 
+// nsubsets > 1: consecutive view lists of nb views (the first nb_rem of them
+// nb + 1) summed into consecutive volumes of part (em_asum), one grid row each
 static hipError_t launch_ell(int mode, const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
                              const Geo& g, hipStream_t s, const float* hvol, const float* atten, long atten_vstride,
-                             float* em_asum, bool zero_inf, int vs_req) {
-  if (nb <= 0) return hipSuccess;
+                             float* em_asum, bool zero_inf, int vs_req, int nsubsets = 1, int nb_rem = 0) {
+  if (nb <= 0 && nb_rem == 0) return hipSuccess;
+  if (nb < 0 || nsubsets <= 0 || nb_rem < 0 || nb_rem >= nsubsets) return hipErrorInvalidValue;
   const int q = (g.nz / 4) % 2 == 0 ? 2 : 1;  // z quads per lane
   const int pw = 64 / (g.nz / 4 / q);
   const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
   // view slices (waves) per block: each wave sums every vs-th view.
-  // Built-in: the fewest slices (2, 4, 8) that give the launch 4096 waves,
-  // at most the batch's views rounded down to a power of two (measured on
+  // Built-in: the fewest slices (2, 4, 8) that give the launch (all its
+  // subset rows) 4096 waves, at most the longest view list rounded down to a
+  // power of two (measured on
   // 4- to 64-view batches: 2 wins at 128x128 pixels, 8 at 64x64 from 8 views)
   int vs = 2;
   if (vs_req == 2 || vs_req == 4 || vs_req == 8)
     vs = vs_req;
   else
-    while (vs < 8 && 2 * vs <= nb && (long)kXcds * C * vs < 4096) vs *= 2;
+    while (vs < 8 && 2 * vs <= nb + (nb_rem > 0 ? 1 : 0) && (long)kXcds * C * vs * nsubsets < 4096) vs *= 2;
   const int zi = zero_inf ? 1 : 0;
 #define SR_ELL(KK, VV, QQ)                                                                                            \
   if (K == KK && vs == VV && q == QQ) {                                                                             \
     if (mode == 0)                                                                                                  \
-      k_rot2_adj_ell<KK, VV, QQ, 0><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, hvol, atten,      \
-                                                                  atten_vstride, em_asum, zi);                      \
+      k_rot2_adj_ell<KK, VV, QQ, 0><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(work, part, ent, vids, nb,       \
+                                                                  nb_rem, C, g, hvol, atten, atten_vstride, em_asum, \
+                                                                  zi);                                              \
     else if (mode == 1)                                                                                             \
-      k_rot2_adj_ell<KK, VV, QQ, 1><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, hvol, atten,      \
-                                                                  atten_vstride, em_asum, zi);                      \
+      k_rot2_adj_ell<KK, VV, QQ, 1><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(work, part, ent, vids, nb,       \
+                                                                  nb_rem, C, g, hvol, atten, atten_vstride, em_asum, \
+                                                                  zi);                                              \
     else                                                                                                            \
-      k_rot2_adj_ell<KK, VV, QQ, 2><<<kXcds * C, 64 * VV, 0, s>>>(work, part, ent, vids, nb, C, g, hvol, atten,      \
-                                                                  atten_vstride, em_asum, zi);                      \
+      k_rot2_adj_ell<KK, VV, QQ, 2><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(work, part, ent, vids, nb,       \
+                                                                  nb_rem, C, g, hvol, atten, atten_vstride, em_asum, \
+                                                                  zi);                                              \
     return hipGetLastError();                                                                                       \
   }
 #define SR_ELL_K(KK) SR_ELL(KK, 2, 1) SR_ELL(KK, 4, 1) SR_ELL(KK, 8, 1) SR_ELL(KK, 2, 2) SR_ELL(KK, 4, 2) SR_ELL(KK, 8, 2)
@@ -3757,6 +3776,13 @@ hipError_t launch_rot2_adj_z(const float* work, float* part, const int2* ent, in
                             vs)
                : launch_ell(0, work, part, ent, K, vids, nb, g, s, nullptr, nullptr, 0, const_cast<float*>(em_asum),
                             zero_inf, vs);
+}
+
+hipError_t launch_rot2_adj_z_norms(const float* hvol, float* asum, const int2* ent, int K, const int* vids, int nb,
+                                   int nb_rem, int nsubsets, const Geo& g, hipStream_t s, const float* atten,
+                                   long atten_vstride, int vs) {
+  return launch_ell(1, nullptr, asum, ent, K, vids, nb, g, s, hvol, atten, atten_vstride, nullptr, true, vs, nsubsets,
+                    nb_rem);
 }
 
 hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* ent, int K, const int* vids, int nb,

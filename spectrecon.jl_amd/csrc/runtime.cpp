@@ -279,8 +279,9 @@ struct sr_plan_s {
   // about 512 blocks otherwise
   // kernel choices (sr_plan_set_choice / sr_plan_autotune), SR_CHOICE_* order:
   // PSF forward family, its chunk count in percent of the heuristic one, the
-  // same two for the PSF adjoint, view slices of the rotation-adjoint gather
-  int32_t choice[SR_NCHOICE] = {0, 100, 0, 100, 0, 0};
+  // same two for the PSF adjoint, view slices of the rotation-adjoint gather,
+  // the PSF method, where OS-EM sums its normalisers
+  int32_t choice[SR_NCHOICE] = {0, 100, 0, 100, 0, 0, 0};
   // the PSF through the plane FFTs (else the direct / separable taps)
   bool use_fft() const {
     return fft_ok && (choice[SR_CHOICE_PSF_METHOD] == 2 || (choice[SR_CHOICE_PSF_METHOD] == 0 && fft_default));
@@ -318,6 +319,10 @@ struct sr_plan_s {
   int fam_fwd() const { return choice[SR_CHOICE_FWD_FAMILY]; }
   int fam_adj() const { return choice[SR_CHOICE_ADJ_FAMILY]; }
   int ell_vs() const { return choice[SR_CHOICE_GATHER_SLICES]; }
+  // OS-EM normalisers (one shared PSF stack, fused loop): each in its
+  // subset's first-iteration gather (built-in), else all subsets in one
+  // gather launch before the iterations
+  bool em_norm_upfront() const { return choice[SR_CHOICE_EM_NORM] == 2; }
   int groups_adj(int nb, int* vpg) const {
     // views per rotation-adjoint group: enough blocks (nz * ngroups) to fill
     // the device; the :one kernel takes one view per block
@@ -919,6 +924,7 @@ int sr_plan_set_choice(sr_plan p, const int32_t* c) {
   const int vs = c[SR_CHOICE_GATHER_SLICES];
   if (vs != 0 && vs != 2 && vs != 4 && vs != 8) return fail(SR_ERR_ARG, "gather slices must be 0, 2, 4 or 8");
   if (c[SR_CHOICE_PSF_METHOD] < 0 || c[SR_CHOICE_PSF_METHOD] > 2) return fail(SR_ERR_ARG, "PSF method must be 0..2");
+  if (c[SR_CHOICE_EM_NORM] < 0 || c[SR_CHOICE_EM_NORM] > 2) return fail(SR_ERR_ARG, "EM normaliser choice must be 0..2");
   if (c[SR_CHOICE_PSF_METHOD] == 2 && !p->fft_ok)
     return fail(SR_ERR_UNSUPPORTED, "the plan has no PSF spectra (z-fastest, or planes the FFT kernels do not take)");
   HIP_TRY(hipStreamSynchronize(p->stream));
@@ -944,7 +950,7 @@ int sr_plan_autotune(sr_plan p, int32_t nsub, char* log, size_t log_bytes) {
   if (!p) return fail(SR_ERR_ARG, "null plan");
   HIP_TRY(hipStreamSynchronize(p->stream));
   drop_graphs(p);
-  const int32_t defaults[SR_NCHOICE] = {0, 100, 0, 100, 0, 0};
+  const int32_t defaults[SR_NCHOICE] = {0, 100, 0, 100, 0, 0, 0};
   std::string out;
   if (nsub <= 0) {
     std::copy(defaults, defaults + SR_NCHOICE, p->choice);
@@ -1426,6 +1432,21 @@ int em_body(sr_plan p, const EmCall& c) {
       KTimer kt(p, sr::K_ROT_FWD);
       HIP_TRY(sr::launch_to_zfast(c.x0, p->zimg.as<float>(), p->g, p->stream));
     }
+    // every subset's normaliser from one gather launch (grid row k: block k,
+    // the lists laid out block after block, the first `rem` one view longer)
+    const bool norm_upfront = c.shared && c.niter > 0 && p->em_norm_upfront();
+    if (norm_upfront) {
+      const int base = c.cnt[c.nblocks - 1];
+      int rem = 0;
+      while (rem < c.nblocks && c.cnt[rem] == base + 1) ++rem;
+      for (int k = 0; k < c.nblocks; ++k)
+        if (c.cnt[k] != base + (k < rem ? 1 : 0) || c.off[k] != k * base + std::min(k, rem))
+          return fail(SR_ERR_ARG, "internal: subset lists are not strided blocks");
+      KTimer kt(p, sr::K_ROT_ADJ);
+      HIP_TRY(sr::launch_rot2_adj_z_norms(p->zh.as<float>(), p->em_asum.as<float>(), p->zent.as<int2>(), p->zk,
+                                          c.vids, base, rem, c.nblocks, p->g, p->stream, p->atten.as<float>(),
+                                          p->atten_vstride(), p->ell_vs()));
+    }
     const sr::Taps t = p->taps();
     for (int it = 0; it < c.niter; ++it)
       for (int nb = 0; nb < c.nblocks; ++nb) {
@@ -1440,7 +1461,7 @@ int em_body(sr_plan p, const EmCall& c) {
         }
         KTimer kt(p, sr::K_ROT_ADJ);
         float* as = p->em_asum.as<float>() + p->V * nb;
-        if (c.shared && it == 0)  // this subset's normaliser, summed in the same pass
+        if (c.shared && it == 0 && !norm_upfront)  // this subset's normaliser, summed in the same pass
           HIP_TRY(sr::launch_rot2_adj_z_em_norm(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk,
                                                 sv, ns, p->g, p->stream, p->zh.as<float>(), p->atten.as<float>(),
                                                 p->atten_vstride(), as, p->ell_vs()));

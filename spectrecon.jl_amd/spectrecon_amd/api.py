@@ -207,7 +207,7 @@ class SPECTplan:
         return cap.value, rep.value
 
     CHOICES = ("psf_fwd_family", "psf_fwd_chunk_percent", "psf_adj_family", "psf_adj_chunk_percent",
-               "gather_slices", "psf_method")
+               "gather_slices", "psf_method", "em_norm")
 
     def choice(self):
         """The plan's kernel choices: PSF families (forward: 0 built-in
@@ -215,7 +215,9 @@ class SPECTplan:
         with 8 / 4 producers, 5 producer/consumer with column-pair consumers;
         adjoint: 0 column-owned, 1 tile), chunk counts in
         percent of the built-in ones, gather view slices (0 built-in), PSF
-        method (0 built-in, 1 direct taps, 2 plane FFTs)."""
+        method (0 built-in, 1 direct taps, 2 plane FFTs), where OS-EM sums its
+        subset normalisers (0 built-in, 1 in each subset's first gather, 2 all
+        subsets in one launch before the iterations)."""
         c = (C.c_int32 * len(self.CHOICES))()
         check(lib().sr_plan_get_choice(self.handle, c))
         return dict(zip(self.CHOICES, list(c)))

@@ -802,6 +802,36 @@ def test_osem_strided_uneven_vs_oracle(gpu, O):
                                   gpu.osem(x0, y, b, gpu.Ablock(plan, 5), niter=2))
 
 
+@pytest.mark.parametrize("nx,nz,nview,nblocks", [(32, 16, 16, 4), (32, 16, 10, 4), (64, 32, 60, 8)])
+def test_em_normaliser_placements(gpu, O, nx, nz, nview, nblocks):
+    """OS-EM on a z-fastest plan with one PSF stack for all views sums its
+    subset normalisers A_b' 1 either in each subset's first-iteration gather
+    (em_norm 1, the built-in) or all in one gather launch before the
+    iterations (em_norm 2, grid row k = block k, the first nview % nblocks
+    blocks one view longer).
+    Both against the FP64 oracle, even and uneven strided blocks, and each
+    normaliser placement against the other."""
+    xtrue, mumap, psfs = em_problem(nx, nz, nview)
+    plan = gpu.SPECTplan(mumap, psfs, 0.8)
+    assert plan.info()["layout"] == "z-fastest"
+    ytrue = gpu.project(xtrue, plan)
+    b = np.full_like(ytrue, 0.1 * ytrue.mean())
+    y = gpu.poisson(20 * (ytrue + b), seed=7) / 20
+    x0 = np.ones_like(xtrue)
+    P64 = O.Problem(mumap, psfs, 0.8, angles32(nview).astype(np.float64), "two", np.float64, "direct")
+    xr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), nblocks, 2)
+    got = {}
+    for m in (1, 2, 0):
+        plan.set_choice(em_norm=m)
+        got[m] = gpu.osem_strided(x0, y, b, plan, nblocks, 2)
+        err = np.linalg.norm(got[m] - xr) / np.linalg.norm(xr)
+        assert err <= 1e-4, (m, err)
+    assert np.linalg.norm(got[1] - got[2]) <= 1e-6 * np.linalg.norm(got[2])
+    np.testing.assert_array_equal(got[0], got[1])
+    with pytest.raises(gpu.SPECTError, match="EM normaliser choice"):
+        plan.set_choice(em_norm=3)
+
+
 CHOICE_CASES = {
     "tile_kernels": dict(psf_fwd_family=1, psf_adj_family=1),
     "column_fwd": dict(psf_fwd_family=2, psf_adj_family=2, gather_slices=8),
@@ -881,7 +911,7 @@ def test_autotune_records_and_keeps_choices(gpu, O):
         with pytest.raises(gpu.SPECTError):
             plan.set_choice(**bad)
     assert plan.autotune(0) == {"reset": True}
-    assert list(plan.choice().values()) == [0, 100, 0, 100, 0, 0]
+    assert list(plan.choice().values()) == [0, 100, 0, 100, 0, 0, 0]
 
 
 def test_autotune_other_plans(gpu, O, monkeypatch):
