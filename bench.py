@@ -160,16 +160,18 @@ def calibration(O, S, cores):
     return out
 
 
-def stream_workload(kind, nreq, rate, seed=7, autotune=True):
+def stream_workload(kind, nreq, rate, seed=7, autotune=True, small_batch=0):
     """The batched and served patterns (one JSON line each; not the headline).
 
     batched: 32 seeded instances over nx in {64,96,128}, nz in {32,64,80},
       views in {60,120,128}, square PSF side in {15,25,37}, subsets in {4,8}
       (strided, uneven where the subsets do not divide the views), OS-EM x 4
       iterations, device-resident inputs on resident plans: all from one
-      sr_osem_batch call (concurrent streams, HIP-graph replays) against one
-      call per instance; every batched result must equal its one-at-a-time
-      result.
+      sr_osem_batch call (the subset steps of all instances as shared
+      launches, one HIP graph replay), against the same call with every
+      instance on its own stream, one call per instance, and the plain
+      drop-in call per instance; every batched result must equal its
+      one-at-a-time result.
     served: `nreq` requests with Poisson arrivals at `rate` per second, 70%
       128x128x80 / 128 views / 37x37 PSF and 30% 64x64x32 / 60 views / 15x15
       PSF, OS-EM 8 x 4 (strided subsets: 60 views over 8 blocks)."""
@@ -180,7 +182,12 @@ def stream_workload(kind, nreq, rate, seed=7, autotune=True):
     reqs = []
     inst = []
     if kind == "batched":
-        for k, (cfg, nblocks) in enumerate(V.batched_instances(seed)):
+        # the seeded uneven draw, or (small_batch) `small_batch` instances of
+        # the served stream's small configuration, 64x64x32 / 60 views / 15x15
+        # PSF, 8 subsets: launches that one instance alone does not fill
+        draw = (V.batched_instances(seed) if not small_batch else
+                [(V.Config(64, 32, 60, 15, DY, 1.0, 4.0), 8)] * small_batch)
+        for k, (cfg, nblocks) in enumerate(draw):
             inst.append((cfg, nblocks))
             reqs.append(V.synthetic_request(cfg, seed=1 + k, nblocks=nblocks, niter=4, rid=k, plans=plans))
     else:
@@ -204,8 +211,9 @@ def stream_workload(kind, nreq, rate, seed=7, autotune=True):
         # one resident plan per instance (built and, per configuration and
         # subset size, autotuned first: the batched pattern keeps them), the
         # inputs uploaded; then all instances from one sr_osem_batch call
-        # (their plans' streams run concurrently; each plan replays its EM
-        # sequence from a HIP graph after two calls), against one call per
+        # (shared launches per step and kernel instantiation, one graph
+        # replay after two calls), against every instance on its own stream
+        # (concurrent, each plan replaying its own graph) and one call per
         # instance, each waited for
         backend = V.DeviceBackend(autotune=autotune)
         t0 = time.perf_counter()
@@ -220,8 +228,12 @@ def stream_workload(kind, nreq, rate, seed=7, autotune=True):
         batch = V.Batch(bplans, reqs)
         for _ in range(2):  # (eager first call, graph capture on the second)
             batch.run()
-        t_one = float(np.median([batch.run(sequential=True) for _ in range(3)]))
+            batch.run(streams=True)
+        t_one = float(np.median([batch.run(sequential=True, streams=True) for _ in range(3)]))
         alone = batch.results()
+        t_streams = float(np.median([batch.run(streams=True) for _ in range(5)]))
+        same_streams = all(np.array_equal(a, b) for a, b in zip(alone, batch.results()))
+        s0 = V.batch_stats()
         # the plain path: the drop-in call per request (host arrays, staged by
         # the library), one request after the other
         import spectrecon_amd as S
@@ -229,23 +241,37 @@ def stream_workload(kind, nreq, rate, seed=7, autotune=True):
         plain = [S.osem_strided(r.x0, r.y, r.b, p, r.nblocks, r.niter) for p, r in zip(bplans, reqs)]
         t_plain = time.perf_counter() - t0
         t_all = float(np.median([batch.run() for _ in range(5)]))
+        s1 = V.batch_stats()
         res = batch.results()
-        same = all(np.array_equal(a, b) for a, b in zip(alone, res)) and all(
-            np.array_equal(a, b) for a, b in zip(plain, res))
+        same = same_streams and all(np.array_equal(a, b) for a, b in zip(alone, plain))
+        rel_shared = max(float(np.linalg.norm(a - b) / np.linalg.norm(a)) for a, b in zip(alone, res))
         caps = [p.graph_stats() for p in bplans]
         batch.free()
         for p in bplans:
             backend.close_plan(p)
-        out.update({"metric": f"batched uneven OS-EM recons/s ({len(reqs)} instances, one sr_osem_batch call)",
+        out.update({"metric": (f"batched uneven OS-EM recons/s ({len(reqs)} instances, one sr_osem_batch call)"
+                               if not small_batch else
+                               f"batched OS-EM recons/s ({len(reqs)} instances of 64x64x32 / 60 views / 15x15 / "
+                               "8 subsets, one sr_osem_batch call)"),
                     "unit": "recons/s", "value": len(reqs) / t_all,
                     "plain_one_at_a_time": len(reqs) / t_plain, "speedup_vs_plain": t_plain / t_all,
                     "resident_one_at_a_time": len(reqs) / t_one, "speedup_vs_resident": t_one / t_all,
-                    "results_equal_alone": bool(same), "batch_ms": 1e3 * t_all, "plain_ms": 1e3 * t_plain,
-                    "resident_sequential_ms": 1e3 * t_one, "prepare_s": prep_s,
+                    "streams": len(reqs) / t_streams, "speedup_vs_streams": t_streams / t_all,
+                    "results_equal_alone": bool(same),
+                    "shared_vs_alone_max_rel_l2": rel_shared,
+                    "results_note": "streams and plain results equal the one-at-a-time results bit for bit; the "
+                                    "shared launches' forwards take the group's plane chunks, so their view sums "
+                                    "round differently (max relative L2 difference reported)",
+                    "batch_ms": 1e3 * t_all, "plain_ms": 1e3 * t_plain,
+                    "resident_sequential_ms": 1e3 * t_one, "streams_ms": 1e3 * t_streams, "prepare_s": prep_s,
+                    "shared_launches": {"instances_per_run": (s1["instances"] - s0["instances"]) // 5,
+                                        "grouped_launches_per_run": (s1["launches"] - s0["launches"]) // 5,
+                                        "graph_replays": s1["replays"] - s0["replays"]},
                     "baselines": "plain: one drop-in call per instance with host arrays (sr_osem_strided, "
                                  "SR_MEM_HOST), each returning before the next; resident: one sr_osem_batch call "
-                                 "per instance on the same device-resident inputs and graph replays, each waited "
-                                 "for",
+                                 "per instance on its plan's stream (SR_BATCH_STREAMS) on the same device-resident "
+                                 "inputs with graph replays, each waited for; streams: one sr_osem_batch call with "
+                                 "every instance on its own stream, concurrently",
                     "graphs": {"captured": sum(c[0] for c in caps), "replays": sum(c[1] for c in caps)},
                     "instances": [{"nx": c.nx, "nz": c.nz, "nview": c.nview, "psf": c.psf, "nblocks": nb}
                                   for c, nb in inst],
@@ -486,6 +512,9 @@ def main():
                          "gradient: the differentiable projector; units: the reference's published timings; "
                          "psf: direct vs FFT PSF methods on general PSFs")
     ap.add_argument("--requests", type=int, default=100, help="served: number of requests")
+    ap.add_argument("--small-batch", type=int, default=0,
+                    help="batched: this many instances of 64x64x32 / 60 views / 15x15 / 8 subsets instead of the "
+                         "seeded uneven draw")
     ap.add_argument("--rate", type=float, default=20.0, help="served: Poisson arrival rate per second")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -527,7 +556,8 @@ def main():
     if args.workload == "psf":
         return psf_workload()
     if args.workload != "osem":
-        return stream_workload(args.workload, args.requests, args.rate, autotune=not args.no_autotune)
+        return stream_workload(args.workload, args.requests, args.rate, autotune=not args.no_autotune,
+                               small_batch=args.small_batch)
     if S.device_count() == 0:
         raise SystemExit("no HIP device")
     lib = S.lib()

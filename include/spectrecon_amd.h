@@ -41,6 +41,8 @@ extern "C" {
  * the call skips its device scan of b (which waits for the stream) and an
  * SR_ASYNC call returns without any host wait */
 #define SR_NO_CHECK 0x200
+/* sr_osem_batch: every instance on its own plan's stream, no shared launches */
+#define SR_BATCH_STREAMS 0x400
 
 #define SR_INTERP_ONE 1 /* :one, 3-pass 1-D linear interpolation          */
 #define SR_INTERP_TWO 2 /* :two, single-pass bilinear (reference default) */
@@ -191,12 +193,26 @@ int sr_host_free(void* ptr);
  * first (the reference's messages; nothing runs if one fails; b > 0 unless
  * SR_NO_CHECK), then every instance's kernel sequence is queued on its plan's
  * stream (replayed from the plan's HIP graph when sr_plan_set_graph is on),
- * so the device runs the instances concurrently.  Returns after all finished,
- * or at once with SR_ASYNC (then sr_plan_query / sr_plan_synchronize per
- * plan).  `mem` must be SR_MEM_DEVICE (| SR_ASYNC | SR_NO_CHECK). */
+ * so the device runs the instances concurrently.  Instances on z-fastest
+ * plans with one separable PSF stack whose subsets fit one launch batch run
+ * as shared launches instead: one launch sequence on a batch stream where,
+ * at step s, subset s % nblocks[i] of iteration s / nblocks[i] of every
+ * instance with more than s steps runs, and the PSF forwards, view
+ * reductions, PSF adjoints and rotation-adjoint gathers of the step that use
+ * one kernel instantiation share one grid (per-block descriptors of each
+ * instance's launch), captured into a HIP graph and replayed when every
+ * such plan has sr_plan_set_graph on.  Each instance's result equals its
+ * own sr_osem_strided call bit for bit.  SR_BATCH_STREAMS runs every
+ * instance on its own stream.  Returns after all finished, or at once with
+ * SR_ASYNC (then sr_plan_query / sr_plan_synchronize per plan).  `mem` must
+ * be SR_MEM_DEVICE (| SR_ASYNC | SR_NO_CHECK | SR_BATCH_STREAMS). */
 int sr_osem_batch(int32_t n, const sr_plan* plans, float* const* outs, const float* const* x0s,
                   const float* const* ys, const float* const* bs, const int32_t* nblocks, const int32_t* niters,
                   int32_t mem);
+/* Counters of sr_osem_batch's shared launches on the current device:
+ * instances run as shared launches, grouped launches issued, batch graphs
+ * captured and replayed (any pointer may be NULL). */
+int sr_batch_stats(int64_t* instances, int64_t* launches, int64_t* captures, int64_t* replays);
 /* HIP-graph replay of the EM kernel sequence (no reference counterpart; for
  * callers that run many small reconstructions, whose time is in launches).
  * With enable != 0, sr_mlem / sr_osem / sr_osem_strided run their first call

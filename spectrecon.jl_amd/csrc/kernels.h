@@ -174,6 +174,88 @@ hipError_t launch_mu_grad_scan(float* g, const float* w, int nb, const Geo& geo,
 // *flag = 0, then 1 if some p[k] is not > 0
 hipError_t launch_flag_nonpositive(const float* p, long n, int* flag, hipStream_t s);
 
+// ---- grouped launches -----------------------------------------------------
+// One grid over the blocks of several launches of one kernel, each with its
+// own plan's geometry, tables and buffers (the OS-EM steps of many instances
+// in shared launches).  Descriptor i owns grid blocks [blk0, blk0 + nblk);
+// its local block l runs as block (l % gx, l / gx) of its own launch.
+// Descriptors of one grouped launch are sorted by blk0, blk0 of the first is
+// 0, and each range starts where the previous one ends.
+struct GrpHdr {
+  int blk0, nblk, gx, pad;
+};
+struct PsfFwdGrp {  // k_psf_fwd_ws
+  GrpHdr h;
+  const float* img_t;
+  Taps t;
+  const int* vids;
+  int Yc;
+  float* part;
+  Geo g;
+  const float* atten;
+  long atten_vstride;
+  const RotView* rv;
+};
+struct ViewReduceGrp {  // k_view_reduce
+  GrpHdr h;
+  const float* part;
+  int nchunk, P;
+  const int* vids;
+  float* out;
+  const float* yobs;
+  const float* bkg;
+  int epi;
+};
+struct PsfAdjGrp {  // k_psf_adj_col
+  GrpHdr h;
+  const float* views;
+  Taps t;
+  const int* vids;
+  int Yc;
+  float* work;
+  const float* atten;
+  long atten_vstride;
+  Geo g;
+  int TBP;
+};
+struct EllGrp {  // k_rot2_adj_ell, one subset (gx = 1; blocks = 8 C, XCD-ordered)
+  GrpHdr h;
+  const float* work;
+  float* part;
+  const int2* ent;
+  const int* vids;
+  int nb, C;
+  Geo g;
+  const float* hvol;
+  const float* atten;
+  long atten_vstride;
+  float* em_asum;
+  int zero_inf;
+};
+// The kernel that launch_psf_fwd / launch_view_reduce / launch_psf_adj /
+// launch_rot2_adj_z(_em_norm) would run for the same arguments, as a
+// descriptor (its blk0 left 0) and a key naming the kernel instantiation
+// (0: that launch has no grouped form); *lds: its dynamic LDS bytes.
+// Launches of one key may share a grouped launch.
+int psf_fwd_grp_desc(const float* work, const Taps& t, const int* vids, int nb, float* part, int nchunk,
+                     int* nchunk_used, const Geo& g, const float* atten, long atten_vstride, const RotView* rv,
+                     int family, PsfFwdGrp* d, size_t* lds);
+int view_reduce_grp_desc(const float* part, int nchunk, const int* vids, int nb, float* out, const float* yobs,
+                         const float* bkg, int epi, const Geo& g, ViewReduceGrp* d);
+int psf_adj_grp_desc(const float* views, const Taps& t, const int* vids, int nb, float* work, const float* atten,
+                     long atten_vstride, int nchunk, const Geo& g, int family, PsfAdjGrp* d, size_t* lds);
+// mode 0: with em_asum, the OS-EM update of part (the z-fastest image) in
+// place; mode 2: the update with the normaliser summed in the same pass
+int ell_grp_desc(int mode, const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
+                 const Geo& g, const float* hvol, const float* atten, long atten_vstride, float* em_asum, int vs,
+                 EllGrp* d);
+// one grouped launch of key `key` over n descriptors (device memory) that
+// cover `blocks` grid blocks; lds: the largest of the descriptors' LDS
+hipError_t launch_psf_fwd_grp(int key, const PsfFwdGrp* d, int n, int blocks, size_t lds, hipStream_t s);
+hipError_t launch_view_reduce_grp(const ViewReduceGrp* d, int n, int blocks, hipStream_t s);
+hipError_t launch_psf_adj_grp(int key, const PsfAdjGrp* d, int n, int blocks, size_t lds, hipStream_t s);
+hipError_t launch_ell_grp(int key, const EllGrp* d, int n, int blocks, hipStream_t s);
+
 // psf kernel limits: the fast separable / direct kernels handle tap counts
 // up to this many (padded to a multiple of 8)
 constexpr int kMaxTaps = 64;

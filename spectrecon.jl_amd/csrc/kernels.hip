@@ -15,6 +15,35 @@ namespace sr {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// A pointer known to address global memory: pointers read from memory (not
+// kernel arguments) are otherwise generic, and generic (flat) accesses also
+// count against the LDS wait counter, which the LDS-only barriers rely on.
+template <class T>
+__device__ __forceinline__ T* glob(T* p) {
+  using G = __attribute__((address_space(1))) T;
+  return (T*)(G*)(uintptr_t)p;
+}
+__device__ __forceinline__ Taps glob(Taps t) {
+  t.u = glob(t.u), t.w = glob(t.w), t.h = glob(t.h), t.cum = glob(t.cum);
+  return t;
+}
+
+// descriptor of a grouped launch that owns this block: the last i with
+// d[i].h.blk0 <= blockIdx.x (sorted ranges; block-uniform, scalar loads)
+template <class D>
+__device__ __forceinline__ int grp_find(const D* __restrict__ d, int n) {
+  int lo = 0, hi = n - 1;
+  const int bx = (int)blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].h.blk0 <= bx)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 // Walks the elements k = start, start+step, ... of a row-major (rows x ncol)
 // array as (row, col) without a division per element.
 struct RowIter {
@@ -1610,16 +1639,29 @@ constexpr int WS_RE = 5;  // element slots per producer thread and round
 __host__ __device__ inline size_t mf_fws_lds_floats(const Geo& g, int nt) {
   return 2 * (size_t)g.nx * mf_wq(g.nz) + 2 * (size_t)MfTaps(nt).size + 16 * (size_t)g.nx;
 }
-template <int TPW, int NPW>
+// GRP: the grid holds the blocks of the ngrp launches that grp describes
+// (the arguments before grp are unused).
+template <int TPW, int NPW, bool GRP>
 __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const float* __restrict__ img_t, Taps t,
                                                                     const int* __restrict__ vids, int Yc,
                                                                     float* __restrict__ part, Geo g,
                                                                     const float* __restrict__ atten,
                                                                     long atten_vstride,
-                                                                    const RotView* __restrict__ rv) {
+                                                                    const RotView* __restrict__ rv,
+                                                                    const PsfFwdGrp* __restrict__ grp, int ngrp) {
   constexpr int PT = 64 * NPW;                       // producer threads
   constexpr int RR = (10240 / 4 + PT * WS_RE - 1) / (PT * WS_RE);  // rounds (nx nz <= 10240)
   extern __shared__ float L[];
+  int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
+  if constexpr (GRP) {
+    const PsfFwdGrp& a = grp[grp_find(grp, ngrp)];
+    const int l = (int)blockIdx.x - a.h.blk0;
+    img_t = glob(a.img_t), t = glob(a.t), vids = glob(a.vids), Yc = a.Yc, part = glob(a.part), g = a.g;
+    atten = glob(a.atten), atten_vstride = a.atten_vstride, rv = glob(a.rv);
+    b = __builtin_amdgcn_readfirstlane(l / a.h.gx);  // (uniform: kept in scalar registers)
+    c = __builtin_amdgcn_readfirstlane(l - b * a.h.gx);
+    nchunk = a.h.gx;
+  }
   const int NT = t.ntx;
   const MfTaps TT(NT);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -1627,7 +1669,6 @@ __global__ void __launch_bounds__(MF_THREADS + 64 * NPW) k_psf_fwd_ws(const floa
   const int nsx = (16 + g.px - 1 + 3) / 4;
   const int ntx = nx >> 4;
   const int WZ = mf_wq(nz), PS = nx * WZ;
-  const int b = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
   const int v = vids[b];
   const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
   const int tid = threadIdx.x, lane = tid & 63;
@@ -2125,12 +2166,23 @@ __global__ void __launch_bounds__(BLOCK) k_psf_fwd_2d(const float* __restrict__ 
 }
 
 // grid (ceil(nx*nz/256), nb): views = sum of chunk partials (+ ML-EM ratio)
+// GRP: the grid holds the blocks of the ngrp launches that grp describes
+// (the arguments before grp are unused).
+template <bool GRP>
 __global__ void __launch_bounds__(256) k_view_reduce(const float* __restrict__ part, int nchunk, int P,
                                                      const int* __restrict__ vids, float* __restrict__ out,
                                                      const float* __restrict__ yobs, const float* __restrict__ bkg,
-                                                     int epi) {
-  const int b = blockIdx.y;
-  const int k = blockIdx.x * 256 + threadIdx.x;
+                                                     int epi, const ViewReduceGrp* __restrict__ grp, int ngrp) {
+  int b = blockIdx.y, bx = blockIdx.x;
+  if constexpr (GRP) {
+    const ViewReduceGrp& a = grp[grp_find(grp, ngrp)];
+    const int l = (int)blockIdx.x - a.h.blk0;
+    part = glob(a.part), nchunk = a.nchunk, P = a.P, vids = glob(a.vids), out = glob(a.out);
+    yobs = glob(a.yobs), bkg = glob(a.bkg), epi = a.epi;
+    b = __builtin_amdgcn_readfirstlane(l / a.h.gx);
+    bx = __builtin_amdgcn_readfirstlane(l - b * a.h.gx);
+  }
+  const int k = bx * 256 + threadIdx.x;
   if (k >= P) return;
   const float* p = part + (long)b * nchunk * P + k;
   float yo = 0.f, bk = 0.f;
@@ -2532,12 +2584,24 @@ __host__ __device__ inline size_t mf_col_lds_floats(const Geo& g, int nt, int tb
 // NWV waves per block (16: four waves per SIMD hide more of each other's LDS
 // and memory waits); the block's (plane, column) items of a tap group are
 // dealt round robin to the waves.
-template <int TPW, int NWV>
+// GRP: the grid holds the blocks of the ngrp launches that grp describes
+// (the arguments before grp are unused).
+template <int TPW, int NWV, bool GRP>
 __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restrict__ views, Taps t,
                                                             const int* __restrict__ vids, int Yc,
                                                             float* __restrict__ work, const float* __restrict__ atten,
-                                                            long atten_vstride, Geo g, int TBP) {
+                                                            long atten_vstride, Geo g, int TBP,
+                                                            const PsfAdjGrp* __restrict__ grp, int ngrp) {
   extern __shared__ float L[];
+  int b = blockIdx.y, c = blockIdx.x;
+  if constexpr (GRP) {
+    const PsfAdjGrp& a = grp[grp_find(grp, ngrp)];
+    const int l = (int)blockIdx.x - a.h.blk0;
+    views = glob(a.views), t = glob(a.t), vids = glob(a.vids), Yc = a.Yc, work = glob(a.work);
+    atten = glob(a.atten), atten_vstride = a.atten_vstride, g = a.g, TBP = a.TBP;
+    b = __builtin_amdgcn_readfirstlane(l / a.h.gx);  // (uniform: kept in scalar registers)
+    c = __builtin_amdgcn_readfirstlane(l - b * a.h.gx);
+  }
   constexpr int NTH = 64 * NWV;
   const int NT = t.ntx;
   const MfTaps TT(NT);
@@ -2547,7 +2611,6 @@ __global__ void __launch_bounds__(64 * NWV) k_psf_adj_col(const float* __restric
   const int dlo = zp_dlo(hz), dhi = zp_dhi(hz);
   const int ntx = nx >> 4;
   const int WX = mf_wx(nx, hx, NT), XO = mf_xo(hx);
-  const int b = blockIdx.y, c = blockIdx.x;
   const int v = vids[b];
   const int y0 = c * Yc, y1 = min(ny, y0 + Yc);
   const int tid = threadIdx.x, lane = tid & 63;
@@ -2968,30 +3031,44 @@ __global__ void __launch_bounds__(256) k_rot2_fwd_z(const float* __restrict__ im
 // blocks of src/ml-os-em.jl:76 laid out block after block), into volume k of
 // part (and of em_asum): every OS-EM normaliser from one launch.
 // grid (8 * xcd_chunk(ceil(NN / PW)), subsets), block 64 * VS
-template <int K, int VS, int Q, int MODE>
+// GRP: the grid holds the blocks of the ngrp one-subset launches that grp
+// describes (the arguments before grp are unused); every launch's range
+// starts at a multiple of 8, so its blocks land on the XCDs as they would
+// alone.
+template <int K, int VS, int Q, int MODE, bool GRP>
 __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restrict__ work, float* __restrict__ part,
                                                           const int2* __restrict__ ent, const int* __restrict__ vids,
                                                           int nb, int nb_rem, int C, Geo g,
                                                           const float* __restrict__ hvol,
                                                           const float* __restrict__ atten, long atten_vstride,
-                                                          float* __restrict__ em_asum, int zero_inf) {
+                                                          float* __restrict__ em_asum, int zero_inf,
+                                                          const EllGrp* __restrict__ grp, int ngrp) {
   constexpr int NS = MODE == 2 ? 2 : 1;  // sums carried
   __shared__ float4 red[VS][64][Q * NS];
+  int lb, sk = blockIdx.y;
+  if constexpr (GRP) {
+    const EllGrp& a = grp[grp_find(grp, ngrp)];
+    const int l = (int)blockIdx.x - a.h.blk0;
+    work = glob(a.work), part = glob(a.part), ent = glob(a.ent), vids = glob(a.vids), nb = a.nb, nb_rem = 0;
+    C = a.C, g = a.g, hvol = glob(a.hvol), atten = glob(a.atten), atten_vstride = a.atten_vstride;
+    em_asum = glob(a.em_asum), zero_inf = a.zero_inf;
+    lb = __builtin_amdgcn_readfirstlane((l % kXcds) * C + l / kXcds), sk = 0;
+  } else {
+    lb = xcd_block(C);
+  }
   const int N = g.nx, NN = N * N, nz = g.nz, nl = (nz >> 2) / Q, pw = 64 / nl;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int sub = lane / nl, zl = lane - sub * nl;
-  const int lb = xcd_block(C);
   if (lb * pw >= NN) return;  // whole block past the end (uniform)
   const int j = lb * pw + sub;
   const bool act = sub < pw && j < NN;
   const int jj = min(j, NN - 1), zo = 4 * Q * (act ? zl : 0);
   const long V = (long)NN * nz;
-  {
-    const int k = blockIdx.y;  // subset
-    vids += k * nb + min(k, nb_rem);
-    nb += k < nb_rem ? 1 : 0;
-    part += V * k;
-    if (em_asum) em_asum += V * k;
+  {  // subset sk
+    vids += sk * nb + min(sk, nb_rem);
+    nb += sk < nb_rem ? 1 : 0;
+    part += V * sk;
+    if (em_asum) em_asum += V * sk;
   }
   float4 acc[Q * NS];
 #pragma unroll
@@ -3476,13 +3553,13 @@ hipError_t launch_psf_fwd(int separable, const float* work, const Taps& t, const
     const size_t lds = sizeof(float) * mf_fws_lds_floats(g, t.ntx);
     SR_TPW_SWITCH(g.nz / 16, {
       if (npw == 8) {
-        allow_lds(k_psf_fwd_ws<TPW, 8>, lds);
-        k_psf_fwd_ws<TPW, 8><<<dim3(nchunk, nb), MF_THREADS + 512, lds, s>>>(work, t, vids, Yc, part, g, atten,
-                                                                             atten_vstride, rv);
+        allow_lds(k_psf_fwd_ws<TPW, 8, false>, lds);
+        k_psf_fwd_ws<TPW, 8, false><<<dim3(nchunk, nb), MF_THREADS + 512, lds, s>>>(work, t, vids, Yc, part, g, atten,
+                                                                                    atten_vstride, rv, nullptr, 0);
       } else {
-        allow_lds(k_psf_fwd_ws<TPW, 4>, lds);
-        k_psf_fwd_ws<TPW, 4><<<dim3(nchunk, nb), MF_THREADS + 256, lds, s>>>(work, t, vids, Yc, part, g, atten,
-                                                                             atten_vstride, rv);
+        allow_lds(k_psf_fwd_ws<TPW, 4, false>, lds);
+        k_psf_fwd_ws<TPW, 4, false><<<dim3(nchunk, nb), MF_THREADS + 256, lds, s>>>(work, t, vids, Yc, part, g, atten,
+                                                                                    atten_vstride, rv, nullptr, 0);
       }
     });
     return hipGetLastError();
@@ -3550,7 +3627,7 @@ hipError_t launch_view_reduce(const float* part, int nchunk, const int* vids, in
                               const float* bkg, int epi, const Geo& g, hipStream_t s) {
   if (nb <= 0) return hipSuccess;
   const int P = g.nx * g.nz;
-  k_view_reduce<<<dim3(cdiv(P, 256), nb), 256, 0, s>>>(part, nchunk, P, vids, out, yobs, bkg, epi);
+  k_view_reduce<false><<<dim3(cdiv(P, 256), nb), 256, 0, s>>>(part, nchunk, P, vids, out, yobs, bkg, epi, nullptr, 0);
   return hipGetLastError();
 }
 
@@ -3569,9 +3646,9 @@ hipError_t launch_psf_adj(int separable, const float* views, const Taps& t, cons
     // (16 waves hold at most 128 registers each: the 8-tile columns take 8)
     SR_TPW_SWITCH(g.nz / 16, {
       constexpr int NWV = TPW >= 8 ? 8 : 16;
-      allow_lds(k_psf_adj_col<TPW, NWV>, lds);
-      k_psf_adj_col<TPW, NWV><<<dim3(nchunk, nb), 64 * NWV, lds, s>>>(views, t, vids, Yc, work, atten, atten_vstride,
-                                                                      g, tbp);
+      allow_lds(k_psf_adj_col<TPW, NWV, false>, lds);
+      k_psf_adj_col<TPW, NWV, false><<<dim3(nchunk, nb), 64 * NWV, lds, s>>>(views, t, vids, Yc, work, atten,
+                                                                             atten_vstride, g, tbp, nullptr, 0);
     });
     return hipGetLastError();
   }
@@ -3748,23 +3825,147 @@ static hipError_t launch_ell(int mode, const float* work, float* part, const int
 #define SR_ELL(KK, VV, QQ)                                                                                            \
   if (K == KK && vs == VV && q == QQ) {                                                                             \
     if (mode == 0)                                                                                                  \
-      k_rot2_adj_ell<KK, VV, QQ, 0><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(work, part, ent, vids, nb,       \
-                                                                  nb_rem, C, g, hvol, atten, atten_vstride, em_asum, \
-                                                                  zi);                                              \
+      k_rot2_adj_ell<KK, VV, QQ, 0, false><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(                       \
+          work, part, ent, vids, nb, nb_rem, C, g, hvol, atten, atten_vstride, em_asum, zi, nullptr, 0);          \
     else if (mode == 1)                                                                                             \
-      k_rot2_adj_ell<KK, VV, QQ, 1><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(work, part, ent, vids, nb,       \
-                                                                  nb_rem, C, g, hvol, atten, atten_vstride, em_asum, \
-                                                                  zi);                                              \
+      k_rot2_adj_ell<KK, VV, QQ, 1, false><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(                       \
+          work, part, ent, vids, nb, nb_rem, C, g, hvol, atten, atten_vstride, em_asum, zi, nullptr, 0);          \
     else                                                                                                            \
-      k_rot2_adj_ell<KK, VV, QQ, 2><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(work, part, ent, vids, nb,       \
-                                                                  nb_rem, C, g, hvol, atten, atten_vstride, em_asum, \
-                                                                  zi);                                              \
+      k_rot2_adj_ell<KK, VV, QQ, 2, false><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(                       \
+          work, part, ent, vids, nb, nb_rem, C, g, hvol, atten, atten_vstride, em_asum, zi, nullptr, 0);          \
     return hipGetLastError();                                                                                       \
   }
 #define SR_ELL_K(KK) SR_ELL(KK, 2, 1) SR_ELL(KK, 4, 1) SR_ELL(KK, 8, 1) SR_ELL(KK, 2, 2) SR_ELL(KK, 4, 2) SR_ELL(KK, 8, 2)
   SR_ELL_K(4) SR_ELL_K(5) SR_ELL_K(6) SR_ELL_K(7) SR_ELL_K(8)
 #undef SR_ELL_K
 #undef SR_ELL
+  return hipErrorInvalidConfiguration;
+}
+
+// ---- grouped launches: descriptors mirror the single launches above ----------
+int psf_fwd_grp_desc(const float* work, const Taps& t, const int* vids, int nb, float* part, int nchunk,
+                     int* nchunk_used, const Geo& g, const float* atten, long atten_vstride, const RotView* rv,
+                     int family, PsfFwdGrp* d, size_t* lds) {
+  // launch_psf_fwd on a z-fastest separable plan: its producer/consumer kernel
+  const int Yc = (g.ny + nchunk - 1) / nchunk;
+  nchunk = (g.ny + Yc - 1) / Yc;
+  if (nchunk_used) *nchunk_used = nchunk;
+  if (nb <= 0) return 0;
+  if (family == PSF_WS2 && psf_fwd_ws2_ok(g, t.ntx)) return 0;
+  const int npw = family == PSF_WS8 ? 8 : family == PSF_WS4 ? 4 : psf_fwd_auto_npw(g, t.ntx);
+  if (!((family == PSF_AUTO || family == PSF_WS8 || family == PSF_WS4) && (npw == 4 || npw == 8) &&
+        psf_fwd_ws_ok(g, t.ntx, npw)))
+    return 0;
+  const int tpw = g.nz / 16;
+  if (g.nz % 16 != 0 || tpw < 1 || tpw > 8) return 0;
+  *d = PsfFwdGrp{GrpHdr{0, nchunk * nb, nchunk, 0}, work, t, vids, Yc, part, g, atten, atten_vstride, rv};
+  *lds = sizeof(float) * mf_fws_lds_floats(g, t.ntx);
+  return 1 << 16 | tpw << 8 | npw;
+}
+
+hipError_t launch_psf_fwd_grp(int key, const PsfFwdGrp* d, int n, int blocks, size_t lds, hipStream_t s) {
+  if (n <= 0 || blocks <= 0) return hipSuccess;
+  if ((key >> 16) != 1) return hipErrorInvalidValue;
+  const int npw = key & 255;
+  SR_TPW_SWITCH((key >> 8) & 255, {
+    if (npw == 8) {
+      allow_lds(k_psf_fwd_ws<TPW, 8, true>, lds);
+      k_psf_fwd_ws<TPW, 8, true><<<blocks, MF_THREADS + 512, lds, s>>>(nullptr, Taps{}, nullptr, 0, nullptr, Geo{},
+                                                                       nullptr, 0, nullptr, d, n);
+    } else if (npw == 4) {
+      allow_lds(k_psf_fwd_ws<TPW, 4, true>, lds);
+      k_psf_fwd_ws<TPW, 4, true><<<blocks, MF_THREADS + 256, lds, s>>>(nullptr, Taps{}, nullptr, 0, nullptr, Geo{},
+                                                                       nullptr, 0, nullptr, d, n);
+    } else {
+      return hipErrorInvalidValue;
+    }
+  });
+  return hipGetLastError();
+}
+
+int view_reduce_grp_desc(const float* part, int nchunk, const int* vids, int nb, float* out, const float* yobs,
+                         const float* bkg, int epi, const Geo& g, ViewReduceGrp* d) {
+  if (nb <= 0) return 0;
+  const int P = g.nx * g.nz, gx = (int)cdiv(P, 256);
+  *d = ViewReduceGrp{GrpHdr{0, gx * nb, gx, 0}, part, nchunk, P, vids, out, yobs, bkg, epi};
+  return 1;
+}
+
+hipError_t launch_view_reduce_grp(const ViewReduceGrp* d, int n, int blocks, hipStream_t s) {
+  if (n <= 0 || blocks <= 0) return hipSuccess;
+  k_view_reduce<true><<<blocks, 256, 0, s>>>(nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 0, d, n);
+  return hipGetLastError();
+}
+
+int psf_adj_grp_desc(const float* views, const Taps& t, const int* vids, int nb, float* work, const float* atten,
+                     long atten_vstride, int nchunk, const Geo& g, int family, PsfAdjGrp* d, size_t* lds) {
+  // launch_psf_adj on a z-fastest separable plan: its column-owned kernel
+  if (nb <= 0) return 0;
+  const int Yc = (g.ny + nchunk - 1) / nchunk;
+  nchunk = (g.ny + Yc - 1) / Yc;
+  if (family == PSF_TILE || !psf_adj_col_ok(g, t.ntx)) return 0;
+  const int tpw = g.nz / 16;
+  if (g.nz % 16 != 0 || tpw < 1 || tpw > 8) return 0;
+  const long per = MfTaps(t.ntx).size;
+  const long room = ((long)(kMaxLds / sizeof(float)) - (long)mf_col_lds_floats(g, t.ntx, 0)) / per;
+  const int tbp = (int)std::max(1L, std::min<long>(std::min(8, Yc), room));
+  *lds = sizeof(float) * mf_col_lds_floats(g, t.ntx, tbp);
+  *d = PsfAdjGrp{GrpHdr{0, nchunk * nb, nchunk, 0}, views, t, vids, Yc, work, atten, atten_vstride, g, tbp};
+  return 2 << 16 | tpw << 8 | (tpw >= 8 ? 8 : 16);
+}
+
+hipError_t launch_psf_adj_grp(int key, const PsfAdjGrp* d, int n, int blocks, size_t lds, hipStream_t s) {
+  if (n <= 0 || blocks <= 0) return hipSuccess;
+  if ((key >> 16) != 2) return hipErrorInvalidValue;
+  SR_TPW_SWITCH((key >> 8) & 255, {
+    constexpr int NWV = TPW >= 8 ? 8 : 16;
+    if ((key & 255) != NWV) return hipErrorInvalidValue;
+    allow_lds(k_psf_adj_col<TPW, NWV, true>, lds);
+    k_psf_adj_col<TPW, NWV, true><<<blocks, 64 * NWV, lds, s>>>(nullptr, Taps{}, nullptr, 0, nullptr, nullptr, 0, Geo{},
+                                                                0, d, n);
+  });
+  return hipGetLastError();
+}
+
+int ell_grp_desc(int mode, const float* work, float* part, const int2* ent, int K, const int* vids, int nb,
+                 const Geo& g, const float* hvol, const float* atten, long atten_vstride, float* em_asum, int vs_req,
+                 EllGrp* d) {
+  // launch_ell of one subset, as launch_rot2_adj_z (mode 0) and
+  // launch_rot2_adj_z_em_norm (mode 2) call it
+  if (nb <= 0 || (mode != 0 && mode != 2) || K < 4 || K > 8) return 0;
+  const int q = (g.nz / 4) % 2 == 0 ? 2 : 1;
+  const int pw = 64 / (g.nz / 4 / q);
+  const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
+  int vs = 2;
+  if (vs_req == 2 || vs_req == 4 || vs_req == 8)
+    vs = vs_req;
+  else
+    while (vs < 8 && 2 * vs <= nb && (long)kXcds * C * vs < 4096) vs *= 2;
+  *d = EllGrp{GrpHdr{0, kXcds * C, 1, 0}, work, part, ent, vids, nb, C, g, hvol, atten, atten_vstride, em_asum,
+              mode == 2 ? 1 : 0};
+  return 3 << 16 | K << 12 | vs << 8 | q << 4 | mode;
+}
+
+hipError_t launch_ell_grp(int key, const EllGrp* d, int n, int blocks, hipStream_t s) {
+  if (n <= 0 || blocks <= 0) return hipSuccess;
+  if ((key >> 16) != 3) return hipErrorInvalidValue;
+  const int K = (key >> 12) & 15, vs = (key >> 8) & 15, q = (key >> 4) & 15, mode = key & 15;
+#define SR_ELLG(KK, VV, QQ)                                                       \
+  if (K == KK && vs == VV && q == QQ) {                                           \
+    if (mode == 0)                                                                \
+      k_rot2_adj_ell<KK, VV, QQ, 0, true><<<blocks, 64 * VV, 0, s>>>(                \
+          nullptr, nullptr, nullptr, nullptr, 0, 0, 0, Geo{}, nullptr, nullptr, 0, nullptr, 0, d, n);    \
+    else if (mode == 2)                                                           \
+      k_rot2_adj_ell<KK, VV, QQ, 2, true><<<blocks, 64 * VV, 0, s>>>(                \
+          nullptr, nullptr, nullptr, nullptr, 0, 0, 0, Geo{}, nullptr, nullptr, 0, nullptr, 0, d, n);    \
+    else                                                                          \
+      return hipErrorInvalidValue;                                                \
+    return hipGetLastError();                                                     \
+  }
+#define SR_ELLG_K(KK) SR_ELLG(KK, 2, 1) SR_ELLG(KK, 4, 1) SR_ELLG(KK, 8, 1) SR_ELLG(KK, 2, 2) SR_ELLG(KK, 4, 2) SR_ELLG(KK, 8, 2)
+  SR_ELLG_K(4) SR_ELLG_K(5) SR_ELLG_K(6) SR_ELLG_K(7) SR_ELLG_K(8)
+#undef SR_ELLG_K
+#undef SR_ELLG
   return hipErrorInvalidConfiguration;
 }
 

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -1392,7 +1393,96 @@ struct EmCall {
   bool fused, shared;
 };
 
+// where a fused EM call with one PSF stack for all views sums its subset
+// normalisers: all in one launch before the iterations, or each in its
+// subset's first-iteration gather
+bool em_norm_upfront(sr_plan p, const EmCall& c) { return c.shared && c.niter > 0 && p->em_norm_upfront(); }
+bool em_norm_in_step(sr_plan p, const EmCall& c, int it) { return c.shared && it == 0 && !p->em_norm_upfront(); }
+
+// The EM kernel sequence of a z-fastest plan whose subsets fit one launch
+// batch (c.fused) in three parts, so that the subset steps of many plans can
+// also run as shared launches (sr_osem_batch): the normalisers and the
+// image's z-fastest copy; one subset step; the result in the caller's layout.
+int em_fused_begin(sr_plan p, const EmCall& c) {
+  const int nsub = c.cnt[0];
+  HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P * nsub, p->stream));
+  int rc;
+  if (c.shared) {
+    const sr::Taps t = p->taps();
+    KTimer kt(p, sr::K_PSF_ADJ);
+    HIP_TRY(sr::launch_psf_adj(p->separable, p->ones.as<float>(), t, c.vids, 1, p->zh.as<float>(), nullptr, 0,
+                               p->nchunk_adj(1), p->g, p->stream, true, p->fam_adj()));
+  } else {
+    for (int nb = 0; nb < c.nblocks; ++nb) {
+      float* as = p->em_asum.as<float>() + p->V * nb;
+      if ((rc = run_backproject(p, p->em_back.as<float>(), p->ones.as<float>(), c.vids + c.off[nb], c.cnt[nb],
+                                sr::IMG_WRITE, nullptr)))
+        return rc;
+      HIP_TRY(sr::launch_to_zfast(p->em_back.as<float>(), as, p->g, p->stream));
+      HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
+    }
+  }
+  {
+    KTimer kt(p, sr::K_ROT_FWD);
+    HIP_TRY(sr::launch_to_zfast(c.x0, p->zimg.as<float>(), p->g, p->stream));
+  }
+  // every subset's normaliser from one gather launch (grid row k: block k,
+  // the lists laid out block after block, the first `rem` one view longer)
+  if (em_norm_upfront(p, c)) {
+    const int base = c.cnt[c.nblocks - 1];
+    int rem = 0;
+    while (rem < c.nblocks && c.cnt[rem] == base + 1) ++rem;
+    for (int k = 0; k < c.nblocks; ++k)
+      if (c.cnt[k] != base + (k < rem ? 1 : 0) || c.off[k] != k * base + std::min(k, rem))
+        return fail(SR_ERR_ARG, "internal: subset lists are not strided blocks");
+    KTimer kt(p, sr::K_ROT_ADJ);
+    HIP_TRY(sr::launch_rot2_adj_z_norms(p->zh.as<float>(), p->em_asum.as<float>(), p->zent.as<int2>(), p->zk,
+                                        c.vids, base, rem, c.nblocks, p->g, p->stream, p->atten.as<float>(),
+                                        p->atten_vstride(), p->ell_vs()));
+  }
+  return SR_OK;
+}
+
+// subset nb of iteration it: project (ratio y ./ (A x + b)), PSF adjoint,
+// and the gather that applies the update to the z-fastest image in place
+int em_fused_step(sr_plan p, const EmCall& c, int it, int nb) {
+  const int* sv = c.vids + c.off[nb];
+  const int ns = c.cnt[nb];
+  int rc;
+  if ((rc = run_project(p, p->em_ratio.as<float>(), nullptr, sv, ns, c.y, c.b, true))) return rc;
+  {
+    KTimer kt(p, sr::K_PSF_ADJ);
+    HIP_TRY(sr::launch_psf_adj(p->separable, p->em_ratio.as<float>(), p->taps(), sv, ns, p->work.as<float>(),
+                               p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(ns), p->g, p->stream, true,
+                               p->fam_adj()));
+  }
+  KTimer kt(p, sr::K_ROT_ADJ);
+  float* as = p->em_asum.as<float>() + p->V * nb;
+  if (em_norm_in_step(p, c, it))  // this subset's normaliser, summed in the same pass
+    HIP_TRY(sr::launch_rot2_adj_z_em_norm(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv,
+                                          ns, p->g, p->stream, p->zh.as<float>(), p->atten.as<float>(),
+                                          p->atten_vstride(), as, p->ell_vs()));
+  else
+    HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv, ns,
+                                  p->g, p->stream, nullptr, 0, as, false, p->ell_vs()));
+  return SR_OK;
+}
+
+int em_fused_end(sr_plan p, const EmCall& c) {
+  KTimer kt(p, sr::K_IMG_REDUCE);
+  HIP_TRY(sr::launch_img_reduce_z(p->zimg.as<float>(), 1, c.out, nullptr, sr::IMG_WRITE, p->g, p->stream));
+  return SR_OK;
+}
+
 int em_body(sr_plan p, const EmCall& c) {
+  int rc;
+  if (c.fused) {
+    if ((rc = em_fused_begin(p, c))) return rc;
+    for (int it = 0; it < c.niter; ++it)
+      for (int nb = 0; nb < c.nblocks; ++nb)
+        if ((rc = em_fused_step(p, c, it, nb))) return rc;
+    return em_fused_end(p, c);
+  }
   const int nsub = c.cnt[0];
   HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P * nsub, p->stream));
   if (c.shared) {
@@ -1400,9 +1490,6 @@ int em_body(sr_plan p, const EmCall& c) {
     KTimer kt(p, sr::K_PSF_ADJ);
     HIP_TRY(sr::launch_psf_adj(p->separable, p->ones.as<float>(), t, c.vids, 1, p->zh.as<float>(), nullptr, 0,
                                p->nchunk_adj(1), p->g, p->stream, true, p->fam_adj()));
-  }
-  int rc;
-  if (c.shared && !c.fused) {
     for (int nb = 0; nb < c.nblocks; ++nb) {
       float* as = p->em_asum.as<float>() + p->V * nb;
       {
@@ -1417,61 +1504,13 @@ int em_body(sr_plan p, const EmCall& c) {
       }
       HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
     }
-  } else if (!c.shared) {
+  } else {
     for (int nb = 0; nb < c.nblocks; ++nb) {
       float* as = p->em_asum.as<float>() + p->V * nb;
-      if ((rc = run_backproject(p, c.fused ? p->em_back.as<float>() : as, p->ones.as<float>(), c.vids + c.off[nb],
-                                c.cnt[nb], sr::IMG_WRITE, nullptr)))
+      if ((rc = run_backproject(p, as, p->ones.as<float>(), c.vids + c.off[nb], c.cnt[nb], sr::IMG_WRITE, nullptr)))
         return rc;
-      if (c.fused) HIP_TRY(sr::launch_to_zfast(p->em_back.as<float>(), as, p->g, p->stream));
       HIP_TRY(sr::launch_zero_to_inf(as, p->V, p->stream));
     }
-  }
-  if (c.fused) {
-    {
-      KTimer kt(p, sr::K_ROT_FWD);
-      HIP_TRY(sr::launch_to_zfast(c.x0, p->zimg.as<float>(), p->g, p->stream));
-    }
-    // every subset's normaliser from one gather launch (grid row k: block k,
-    // the lists laid out block after block, the first `rem` one view longer)
-    const bool norm_upfront = c.shared && c.niter > 0 && p->em_norm_upfront();
-    if (norm_upfront) {
-      const int base = c.cnt[c.nblocks - 1];
-      int rem = 0;
-      while (rem < c.nblocks && c.cnt[rem] == base + 1) ++rem;
-      for (int k = 0; k < c.nblocks; ++k)
-        if (c.cnt[k] != base + (k < rem ? 1 : 0) || c.off[k] != k * base + std::min(k, rem))
-          return fail(SR_ERR_ARG, "internal: subset lists are not strided blocks");
-      KTimer kt(p, sr::K_ROT_ADJ);
-      HIP_TRY(sr::launch_rot2_adj_z_norms(p->zh.as<float>(), p->em_asum.as<float>(), p->zent.as<int2>(), p->zk,
-                                          c.vids, base, rem, c.nblocks, p->g, p->stream, p->atten.as<float>(),
-                                          p->atten_vstride(), p->ell_vs()));
-    }
-    const sr::Taps t = p->taps();
-    for (int it = 0; it < c.niter; ++it)
-      for (int nb = 0; nb < c.nblocks; ++nb) {
-        const int* sv = c.vids + c.off[nb];
-        const int ns = c.cnt[nb];
-        if ((rc = run_project(p, p->em_ratio.as<float>(), nullptr, sv, ns, c.y, c.b, true))) return rc;
-        {
-          KTimer kt(p, sr::K_PSF_ADJ);
-          HIP_TRY(sr::launch_psf_adj(p->separable, p->em_ratio.as<float>(), t, sv, ns, p->work.as<float>(),
-                                     p->atten.as<float>(), p->atten_vstride(), p->nchunk_adj(ns), p->g, p->stream,
-                                     true, p->fam_adj()));
-        }
-        KTimer kt(p, sr::K_ROT_ADJ);
-        float* as = p->em_asum.as<float>() + p->V * nb;
-        if (c.shared && it == 0 && !norm_upfront)  // this subset's normaliser, summed in the same pass
-          HIP_TRY(sr::launch_rot2_adj_z_em_norm(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk,
-                                                sv, ns, p->g, p->stream, p->zh.as<float>(), p->atten.as<float>(),
-                                                p->atten_vstride(), as, p->ell_vs()));
-        else
-          HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv,
-                                        ns, p->g, p->stream, nullptr, 0, as, false, p->ell_vs()));
-      }
-    KTimer kt(p, sr::K_IMG_REDUCE);
-    HIP_TRY(sr::launch_img_reduce_z(p->zimg.as<float>(), 1, c.out, nullptr, sr::IMG_WRITE, p->g, p->stream));
-    return SR_OK;
   }
   // out = x0 (a copy kernel: the sequence holds kernels only)
   HIP_TRY(sr::launch_img_reduce(c.x0, 1, c.out, nullptr, p->V, sr::IMG_WRITE, p->stream));
@@ -1535,8 +1574,10 @@ int em_graph(sr_plan p, const EmCall& c) {
 
 // shared body of mlem!/osem!: blocks of views nb:nblocks:nview
 // (src/ml-os-em.jl:21-49 with nblocks == 1; :97-131 otherwise)
-int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* b, int nblocks, int niter,
-           int32_t mem) {
+// checks, host staging, the subset view lists and every buffer of one EM
+// call, leaving *c ready for its kernel sequence
+int em_setup(sr_plan p, EmCall& c, float* out, const float* x0, const float* y, const float* b, int nblocks,
+             int niter, int32_t mem) {
   const int nview = p->d.nview;
   const long M = p->P * nview;
   const bool host = (mem & 0xff) == SR_MEM_HOST;
@@ -1555,7 +1596,6 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
     HIP_TRY(hipStreamSynchronize(p->stream));
     if (bad) return fail(SR_ERR_ARG, "need background > 0");
   }
-  EmCall c;
   c.x0 = x0;
   c.y = y;
   c.b = b;
@@ -1609,6 +1649,15 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
       (((c.fused && !c.shared) || (!c.fused && nsub > p->max_batch)) &&
        (rc = p->em_back.ensure(sizeof(float) * p->V))))
     return rc;
+  return SR_OK;
+}
+
+int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* b, int nblocks, int niter,
+           int32_t mem) {
+  const bool host = (mem & 0xff) == SR_MEM_HOST;
+  EmCall c;
+  int rc;
+  if ((rc = em_setup(p, c, out, x0, y, b, nblocks, niter, mem))) return rc;
   if ((rc = (p->graph && !p->timing) ? em_graph(p, c) : em_body(p, c))) return rc;
   if (host) {
     HIP_TRY(hipMemcpyAsync(out, c.out, sizeof(float) * p->V, hipMemcpyDeviceToHost, p->stream));
@@ -1618,6 +1667,459 @@ int run_em(sr_plan p, float* out, const float* x0, const float* y, const float* 
   }
   return finish(p, mem);
 }
+
+// ---------------------------------------------------------------------------
+// Shared launches across plans (sr_osem_batch)
+// ---------------------------------------------------------------------------
+// The EM steps of many instances, each on its own plan, as one launch
+// sequence on a batch stream: at global step s every instance with more
+// than s steps runs subset s % nblocks of iteration s / nblocks, and the
+// forwards, view reductions, adjoints and gathers of the step that use one
+// kernel instantiation run as one grouped launch (one grid over all their
+// blocks; descriptors in device memory).  Each block computes what it
+// computes in the instance's own launch, so results equal the instance's
+// own call bit for bit.
+
+extern "C++" {  // (templates and C++ return types inside the C-ABI file)
+// a plan's own launches (the begin and end parts of its EM call) queued on
+// the batch stream: the plan's stream is swapped for the guard's lifetime
+struct StreamSwap {
+  sr_plan p;
+  hipStream_t old;
+  StreamSwap(sr_plan q, hipStream_t s) : p(q), old(q->stream) { p->stream = s; }
+  ~StreamSwap() { p->stream = old; }
+};
+
+enum { G_FWD = 0, G_REDUCE = 1, G_ADJ = 2, G_ELL = 3 };
+// one grouped launch: class, kernel key, byte offset of its descriptors in
+// the batch's descriptor image, their count, grid blocks, dynamic LDS
+struct GLaunch {
+  int cls, key;
+  size_t off;
+  int n, blocks;
+  size_t lds;
+};
+// the instances of one lane (the batch's launches run in lanes, one stream
+// each: instances whose kernels share instantiations share a lane), their
+// grouped launches, descriptors in the batch's descriptor image
+struct Lane {
+  std::vector<sr_plan> ps;
+  std::vector<const EmCall*> cs;
+  std::vector<GLaunch> seq;
+};
+
+template <class D>
+void put_group(std::vector<unsigned char>& desc, std::vector<GLaunch>& seq, int cls, int key, std::vector<D>& ds,
+               size_t lds) {
+  if (ds.empty()) return;
+  int blk = 0;
+  for (auto& d : ds) {
+    d.h.blk0 = blk;
+    blk += d.h.nblk;
+  }
+  const size_t off = (desc.size() + 255) & ~(size_t)255;
+  desc.resize(off + sizeof(D) * ds.size());
+  std::memcpy(desc.data() + off, ds.data(), sizeof(D) * ds.size());
+  seq.push_back({cls, key, off, (int)ds.size(), blk, lds});
+}
+
+// the four descriptors of instance (p, c)'s step (it, nb); false when one of
+// its launches has no grouped form
+struct StepDesc {
+  sr::PsfFwdGrp f;
+  sr::ViewReduceGrp r;
+  sr::PsfAdjGrp a;
+  sr::EllGrp e;
+  int kf, kr, ka, ke;
+  size_t lf, la;
+};
+bool step_desc(sr_plan p, const EmCall& c, int it, int nb, StepDesc& d) {
+  const int* sv = c.vids + c.off[nb];
+  const int ns = c.cnt[nb];
+  const sr::Taps t = p->taps();
+  int nchunk = 0;
+  d.lf = d.la = 0;
+  d.kf = sr::psf_fwd_grp_desc(p->zimg.as<float>(), t, sv, ns, p->fpart.as<float>(), p->nchunk_fwd(ns), &nchunk, p->g,
+                              p->atten.as<float>(), p->atten_vstride(), p->rotv.as<sr::RotView>(), p->fam_fwd(), &d.f,
+                              &d.lf);
+  d.kr = sr::view_reduce_grp_desc(p->fpart.as<float>(), nchunk, sv, ns, p->em_ratio.as<float>(), c.y, c.b,
+                                  sr::EPI_RATIO, p->g, &d.r);
+  d.ka = sr::psf_adj_grp_desc(p->em_ratio.as<float>(), t, sv, ns, p->work.as<float>(), p->atten.as<float>(),
+                              p->atten_vstride(), p->nchunk_adj(ns), p->g, p->fam_adj(), &d.a, &d.la);
+  float* as = p->em_asum.as<float>() + p->V * nb;
+  d.ke = em_norm_in_step(p, c, it)
+             ? sr::ell_grp_desc(2, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv, ns, p->g,
+                                p->zh.as<float>(), p->atten.as<float>(), p->atten_vstride(), as, p->ell_vs(), &d.e)
+             : sr::ell_grp_desc(0, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv, ns, p->g,
+                                nullptr, nullptr, 0, as, p->ell_vs(), &d.e);
+  return d.kf && d.kr && d.ka && d.ke;
+}
+
+// whether an instance's EM call can join the shared launches: the fused
+// z-fastest loop with one PSF stack and the kernels that have grouped forms
+bool batch_eligible(sr_plan p, const EmCall& c) {
+  if (!(c.fused && c.shared && p->zfast && p->separable && p->mfma) || p->timing) return false;
+  StepDesc d;
+  for (int it = 0; it < std::min(c.niter, 2); ++it)
+    for (int nb = 0; nb < c.nblocks; ++nb)
+      if (!step_desc(p, c, it, nb, d)) return false;
+  return true;
+}
+
+
+// plane chunks per view for a grouped PSF launch over views of planes of up
+// to ny_max planes (one block per CU): the cheapest c of ceil(c nviews / ncu)
+// rounds of (planes per chunk + 2) plane times, fewest on ties -- the plan's
+// own rule (chunks_base) applied to the whole group, so that instances that
+// share a launch take fewer, longer chunks than each would alone
+int group_chunks(long nviews, int ny_max, int ncu) {
+  int best = 1;
+  long best_cost = -1;
+  for (int c = 1; c <= ny_max; ++c) {
+    const int yc = (ny_max + c - 1) / c;
+    if ((ny_max + yc - 1) / yc != c) continue;
+    const long cost = ((long)c * std::max(1L, nviews) + ncu - 1) / ncu * (yc + 2);
+    if (best_cost < 0 || cost < best_cost) {
+      best = c;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
+int batch_build(Lane& ln, std::vector<unsigned char>& desc) {
+  const std::vector<sr_plan>& ps = ln.ps;
+  const std::vector<const EmCall*>& cs = ln.cs;
+  int steps = 0;
+  for (const EmCall* c : cs) steps = std::max(steps, c->niter * c->nblocks);
+  struct Member {
+    size_t i;
+    int it, nb, ns;
+    const int* sv;
+    StepDesc d;
+    int nchunk;  // the forward's chunks, for the view reduction
+  };
+  // groups: kernel key -> members (in instance order)
+  using Groups = std::vector<std::pair<int, std::vector<Member*>>>;
+  auto slot = [](Groups& g, int key) -> std::vector<Member*>& {
+    for (auto& e : g)
+      if (e.first == key) return e.second;
+    g.push_back({key, {}});
+    return g.back().second;
+  };
+  for (int st = 0; st < steps; ++st) {
+    std::vector<Member> mem;
+    mem.reserve(ps.size());
+    for (size_t i = 0; i < ps.size(); ++i) {
+      const EmCall& c = *cs[i];
+      if (st >= c.niter * c.nblocks) continue;
+      Member m{i, st / c.nblocks, st % c.nblocks, 0, nullptr, {}, 0};
+      m.sv = c.vids + c.off[m.nb];
+      m.ns = c.cnt[m.nb];
+      if (!step_desc(ps[i], c, m.it, m.nb, m.d))
+        return fail(SR_ERR_UNSUPPORTED, "internal: instance %zu has no shared-launch form", i);
+      mem.push_back(m);
+    }
+    Groups gf, ga, ge;
+    for (Member& m : mem) {
+      slot(gf, m.d.kf).push_back(&m);
+      slot(ga, m.d.ka).push_back(&m);
+      // gathers grouped without their view-slice count, which the group sets
+      // (unless a plan pins it)
+      const int vs = ps[m.i]->ell_vs();
+      slot(ge, vs ? m.d.ke : (m.d.ke & ~(15 << 8))).push_back(&m);
+    }
+    // PSF forward: the group's chunk count
+    for (auto& [key, ms] : gf) {
+      long nv = 0;
+      int ny = 1;
+      for (Member* m : ms) nv += m->ns, ny = std::max(ny, ps[m->i]->d.ny);
+      const int cg = group_chunks(nv, ny, ps[ms[0]->i]->ncu);
+      std::vector<sr::PsfFwdGrp> ds;
+      size_t lds = 0;
+      for (Member* m : ms) {
+        const sr_plan p = ps[m->i];
+        sr::PsfFwdGrp d;
+        size_t l = 0;
+        const int k = sr::psf_fwd_grp_desc(p->zimg.as<float>(), p->taps(), m->sv, m->ns, p->fpart.as<float>(),
+                                           std::min(cg, p->d.ny), &m->nchunk, p->g, p->atten.as<float>(),
+                                           p->atten_vstride(), p->rotv.as<sr::RotView>(), p->fam_fwd(), &d, &l);
+        if (k != key) return fail(SR_ERR_UNSUPPORTED, "internal: forward key changed with the chunk count");
+        ds.push_back(d);
+        lds = std::max(lds, l);
+      }
+      put_group(desc, ln.seq, G_FWD, key, ds, lds);
+    }
+    // view reductions over each forward's chunks, with the EM ratio
+    {
+      std::vector<sr::ViewReduceGrp> ds;
+      for (Member& m : mem) {
+        const sr_plan p = ps[m.i];
+        const EmCall& c = *cs[m.i];
+        sr::ViewReduceGrp d;
+        sr::view_reduce_grp_desc(p->fpart.as<float>(), m.nchunk, m.sv, m.ns, p->em_ratio.as<float>(), c.y, c.b,
+                                 sr::EPI_RATIO, p->g, &d);
+        ds.push_back(d);
+      }
+      put_group(desc, ln.seq, G_REDUCE, 1, ds, 0);
+    }
+    // PSF adjoint: the group's chunk count (planes are independent here: any
+    // count gives the same result)
+    for (auto& [key, ms] : ga) {
+      long nv = 0;
+      int ny = 1;
+      for (Member* m : ms) nv += m->ns, ny = std::max(ny, ps[m->i]->d.ny);
+      const int cg = group_chunks(nv, ny, ps[ms[0]->i]->ncu);
+      std::vector<sr::PsfAdjGrp> ds;
+      size_t lds = 0;
+      for (Member* m : ms) {
+        const sr_plan p = ps[m->i];
+        sr::PsfAdjGrp d;
+        size_t l = 0;
+        const int k = sr::psf_adj_grp_desc(p->em_ratio.as<float>(), p->taps(), m->sv, m->ns, p->work.as<float>(),
+                                           p->atten.as<float>(), p->atten_vstride(), std::min(cg, p->d.ny), p->g,
+                                           p->fam_adj(), &d, &l);
+        if (k != key) return fail(SR_ERR_UNSUPPORTED, "internal: adjoint key changed with the chunk count");
+        ds.push_back(d);
+        lds = std::max(lds, l);
+      }
+      put_group(desc, ln.seq, G_ADJ, key, ds, lds);
+    }
+    // rotation-adjoint gathers: view slices from the group's wave count (the
+    // fewest of 2, 4, 8 giving 4096 waves, at most the shortest view list)
+    for (auto& [gkey, ms] : ge) {
+      int vs = (gkey >> 8) & 15;
+      if (!vs) {
+        long blocks = 0;
+        int nmin = 1 << 30;
+        for (Member* m : ms) blocks += m->d.e.h.nblk, nmin = std::min(nmin, m->ns);
+        vs = 2;
+        while (vs < 8 && 2 * vs <= nmin && blocks * vs < 4096) vs *= 2;
+      }
+      std::vector<sr::EllGrp> ds;
+      int key = 0;
+      for (Member* m : ms) {
+        const sr_plan p = ps[m->i];
+        const EmCall& c = *cs[m->i];
+        float* as = p->em_asum.as<float>() + p->V * m->nb;
+        sr::EllGrp d;
+        const int k =
+            em_norm_in_step(p, c, m->it)
+                ? sr::ell_grp_desc(2, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, m->sv, m->ns,
+                                   p->g, p->zh.as<float>(), p->atten.as<float>(), p->atten_vstride(), as, vs, &d)
+                : sr::ell_grp_desc(0, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, m->sv, m->ns,
+                                   p->g, nullptr, nullptr, 0, as, vs, &d);
+        if (key && k != key) return fail(SR_ERR_UNSUPPORTED, "internal: gather keys differ within a group");
+        key = k;
+        ds.push_back(d);
+      }
+      put_group(desc, ln.seq, G_ELL, key, ds, 0);
+    }
+  }
+  return SR_OK;
+}
+
+// the whole batch sequence on stream s: every instance's begin part, the
+// grouped steps (descriptors at ddesc), every instance's end part
+int batch_run(const Lane& ln, const unsigned char* ddesc, hipStream_t s) {
+  const std::vector<sr_plan>& ps = ln.ps;
+  const std::vector<const EmCall*>& cs = ln.cs;
+  int rc;
+  for (size_t i = 0; i < ps.size(); ++i) {
+    StreamSwap sw(ps[i], s);
+    if ((rc = em_fused_begin(ps[i], *cs[i]))) return rc;
+  }
+  for (const GLaunch& g : ln.seq) {
+    const void* d = ddesc + g.off;
+    switch (g.cls) {
+      case G_FWD:
+        HIP_TRY(sr::launch_psf_fwd_grp(g.key, static_cast<const sr::PsfFwdGrp*>(d), g.n, g.blocks, g.lds, s));
+        break;
+      case G_REDUCE:
+        HIP_TRY(sr::launch_view_reduce_grp(static_cast<const sr::ViewReduceGrp*>(d), g.n, g.blocks, s));
+        break;
+      case G_ADJ:
+        HIP_TRY(sr::launch_psf_adj_grp(g.key, static_cast<const sr::PsfAdjGrp*>(d), g.n, g.blocks, g.lds, s));
+        break;
+      default:
+        HIP_TRY(sr::launch_ell_grp(g.key, static_cast<const sr::EllGrp*>(d), g.n, g.blocks, s));
+    }
+  }
+  for (size_t i = 0; i < ps.size(); ++i) {
+    StreamSwap sw(ps[i], s);
+    if ((rc = em_fused_end(ps[i], *cs[i]))) return rc;
+  }
+  return SR_OK;
+}
+
+// per device: the batch stream, the descriptor buffer of eager runs, and
+// graphs of whole batch sequences keyed by everything they bake in (the
+// descriptor image and every instance's begin/end arguments); the first
+// call with a key runs eagerly, the second captures, later ones replay
+constexpr int kLanes = 8;
+// lanes a batch uses: HIP's default of 4 hardware queues per process, so
+// that every lane's stream has a queue of its own
+constexpr int kLaneQueues = 4;
+struct BatchCtx {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr, fork = nullptr;
+  hipStream_t lane[kLanes] = {};
+  hipEvent_t join[kLanes] = {};
+  DBuf desc;
+  struct Graph {
+    std::vector<uintptr_t> key;
+    std::unique_ptr<DBuf> desc;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Graph> graphs;
+  std::vector<std::vector<uintptr_t>> seen;
+  long instances = 0, captures = 0, replays = 0, launches = 0, lanes_last = 0;
+};
+constexpr size_t kMaxBatchGraphs = 4;
+
+BatchCtx* batch_ctx() {
+  static BatchCtx ctx[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    dev = 0;
+  }
+  return &ctx[dev];
+}
+
+std::vector<uintptr_t> batch_key(const std::vector<sr_plan>& ps, const std::vector<const EmCall*>& cs) {
+  // every device address, count and choice the sequence bakes in (the
+  // descriptors are derived from these)
+  std::vector<uintptr_t> key;
+  key.push_back(ps.size());
+  for (size_t i = 0; i < ps.size(); ++i) {
+    const sr_plan p = ps[i];
+    const EmCall& c = *cs[i];
+    for (uintptr_t v :
+         {(uintptr_t)p, (uintptr_t)c.out, (uintptr_t)c.x0, (uintptr_t)c.y, (uintptr_t)c.b, (uintptr_t)c.vids,
+          (uintptr_t)c.nblocks, (uintptr_t)c.niter, (uintptr_t)p->max_batch, (uintptr_t)p->work.p,
+          (uintptr_t)p->fpart.p, (uintptr_t)p->ipart.p, (uintptr_t)p->ones.p, (uintptr_t)p->em_asum.p,
+          (uintptr_t)p->em_ratio.p, (uintptr_t)p->em_back.p, (uintptr_t)p->zimg.p, (uintptr_t)p->zh.p,
+          (uintptr_t)p->zent.p, (uintptr_t)p->zk, (uintptr_t)p->atten.p, (uintptr_t)p->rotv.p,
+          (uintptr_t)p->taps_u.p, (uintptr_t)p->taps_w.p, (uintptr_t)p->taps_c.p})
+      key.push_back(v);
+    for (int k = 0; k < SR_NCHOICE; ++k) key.push_back((uintptr_t)(uint32_t)p->choice[k]);
+  }
+  return key;
+}
+
+// runs the grouped instances on the batch stream (graph or eager) and makes
+// every grouped plan's stream wait for it
+// runs the grouped instances (graph or eager): lanes fork from the batch
+// stream, each on a stream of its own, and join it again; every grouped
+// plan's stream then waits for the batch stream
+int batch_exec(BatchCtx& bx, const std::vector<sr_plan>& ps, const std::vector<const EmCall*>& cs, bool graph) {
+  if (!bx.stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&bx.stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&bx.done, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&bx.fork, hipEventDisableTiming));
+    for (int l = 0; l < kLanes; ++l) {
+      HIP_TRY(hipStreamCreateWithFlags(&bx.lane[l], hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&bx.join[l], hipEventDisableTiming));
+    }
+  }
+  // lanes: the instances ordered by z tiles per column (which fix their PSF
+  // kernels' instantiations), then split into up to kLaneQueues contiguous
+  // runs of about equal work (voxels x views x iterations), one stream each
+  std::vector<size_t> order(ps.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  auto work_of = [&](size_t i) { return (double)ps[i]->V * ps[i]->d.nview * std::max(1, cs[i]->niter); };
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    const int ta = ps[a]->g.nz / 16, tb = ps[b]->g.nz / 16;
+    return ta != tb ? ta < tb : work_of(a) > work_of(b);
+  });
+  double total = 0;
+  for (size_t i : order) total += work_of(i);
+  const int nl = (int)std::min<size_t>(kLaneQueues, ps.size());
+  std::vector<Lane> lanes(nl);
+  double acc = 0;
+  for (size_t i : order) {
+    const int l = std::min(nl - 1, (int)(acc / total * nl));
+    acc += work_of(i);
+    lanes[l].ps.push_back(ps[i]);
+    lanes[l].cs.push_back(cs[i]);
+  }
+  lanes.erase(std::remove_if(lanes.begin(), lanes.end(), [](const Lane& ln) { return ln.ps.empty(); }), lanes.end());
+  std::vector<unsigned char> desc;
+  int rc;
+  for (Lane& ln : lanes)
+    if ((rc = batch_build(ln, desc))) return rc;
+  // the plans' queued work (their setup copies) and any earlier batch land first
+  for (sr_plan p : ps) HIP_TRY(hipStreamSynchronize(p->stream));
+  HIP_TRY(hipStreamSynchronize(bx.stream));
+  auto run = [&](const unsigned char* dd) -> int {
+    HIP_TRY(hipEventRecord(bx.fork, bx.stream));
+    for (size_t l = 0; l < lanes.size(); ++l) {
+      HIP_TRY(hipStreamWaitEvent(bx.lane[l], bx.fork, 0));
+      int r;
+      if ((r = batch_run(lanes[l], dd, bx.lane[l]))) return r;
+      HIP_TRY(hipEventRecord(bx.join[l], bx.lane[l]));
+      HIP_TRY(hipStreamWaitEvent(bx.stream, bx.join[l], 0));
+    }
+    return SR_OK;
+  };
+  bool ran = false;
+  if (graph) {
+    std::vector<uintptr_t> key = batch_key(ps, cs);
+    for (auto& e : bx.graphs)
+      if (e.key == key) {
+        HIP_TRY(hipGraphLaunch(e.exec, bx.stream));
+        ++bx.replays;
+        ran = true;
+        break;
+      }
+    if (!ran) {
+      auto seen = std::find(bx.seen.begin(), bx.seen.end(), key);
+      if (seen == bx.seen.end()) {
+        bx.seen.push_back(key);
+        if (bx.seen.size() > kMaxBatchGraphs) bx.seen.erase(bx.seen.begin());
+      } else {
+        bx.seen.erase(seen);
+        auto buf = std::make_unique<DBuf>();
+        if ((rc = buf->ensure(std::max<size_t>(desc.size(), 256)))) return rc;
+        HIP_TRY(hipMemcpy(buf->p, desc.data(), desc.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipStreamBeginCapture(bx.stream, hipStreamCaptureModeThreadLocal));
+        rc = run(buf->as<unsigned char>());
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(bx.stream, &g);
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = (rc == SR_OK && ec == hipSuccess && g) ? hipGraphInstantiate(&x, g, nullptr, nullptr, 0)
+                                                                     : hipErrorUnknown;
+        if (g) (void)hipGraphDestroy(g);
+        if (ei == hipSuccess) {
+          if (bx.graphs.size() >= kMaxBatchGraphs) {
+            (void)hipGraphExecDestroy(bx.graphs.front().exec);
+            bx.graphs.erase(bx.graphs.begin());
+          }
+          bx.graphs.push_back({std::move(key), std::move(buf), x});
+          ++bx.captures;
+          HIP_TRY(hipGraphLaunch(x, bx.stream));
+          ran = true;
+        } else {
+          (void)hipGetLastError();
+        }
+      }
+    }
+  }
+  if (!ran) {
+    if ((rc = bx.desc.ensure(std::max<size_t>(desc.size(), 256)))) return rc;
+    HIP_TRY(hipMemcpy(bx.desc.p, desc.data(), desc.size(), hipMemcpyHostToDevice));
+    if ((rc = run(bx.desc.as<unsigned char>()))) return rc;
+  }
+  for (const Lane& ln : lanes) bx.launches += (long)ln.seq.size();
+  bx.instances += (long)ps.size();
+  bx.lanes_last = (long)lanes.size();
+  HIP_TRY(hipEventRecord(bx.done, bx.stream));
+  for (sr_plan p : ps) HIP_TRY(hipStreamWaitEvent(p->stream, bx.done, 0));
+  return SR_OK;
+}
+}  // extern "C++"
 
 }  // namespace
 
@@ -1706,16 +2208,48 @@ int sr_osem_batch(int32_t n, const sr_plan* plans, float* const* outs, const flo
       if (bad) return fail(SR_ERR_ARG, "need background > 0");
     }
   }
+  // every instance set up (subset lists, buffers); those whose EM call has
+  // grouped forms run as shared launches on the batch stream, the others
+  // on their plans' streams, concurrently
+  std::vector<EmCall> calls(n);
+  std::vector<sr_plan> gp;
+  std::vector<const EmCall*> gc;
+  bool graph = true;
   for (int i = 0; i < n; ++i) {
-    const int rc = run_em(plans[i], outs[i], x0s[i], ys[i], bs[i], nblocks[i], niters[i],
-                          SR_MEM_DEVICE | SR_ASYNC | SR_NO_CHECK);
-    if (rc) return rc;
+    sr_plan p = plans[i];
+    int rc;
+    if ((rc = em_setup(p, calls[i], outs[i], x0s[i], ys[i], bs[i], nblocks[i], niters[i],
+                       SR_MEM_DEVICE | SR_NO_CHECK)))
+      return rc;
+    if (!(mem & SR_BATCH_STREAMS) && batch_eligible(p, calls[i])) {
+      gp.push_back(p);
+      gc.push_back(&calls[i]);
+      graph = graph && p->graph;
+      continue;
+    }
+    if ((rc = (p->graph && !p->timing) ? em_graph(p, calls[i]) : em_body(p, calls[i]))) return rc;
+  }
+  if (!gp.empty()) {
+    BatchCtx* bx = batch_ctx();
+    std::lock_guard<std::mutex> lk(bx->mu);
+    int rc;
+    if ((rc = batch_exec(*bx, gp, gc, graph))) return rc;
   }
   if (mem & SR_ASYNC) return SR_OK;
   for (int i = 0; i < n; ++i) {
     HIP_TRY(hipStreamSynchronize(plans[i]->stream));
     resolve_timing(plans[i]);
   }
+  return SR_OK;
+}
+
+int sr_batch_stats(int64_t* instances, int64_t* launches, int64_t* captures, int64_t* replays) {
+  BatchCtx* bx = batch_ctx();
+  std::lock_guard<std::mutex> lk(bx->mu);
+  if (instances) *instances = bx->instances;
+  if (launches) *launches = bx->launches;
+  if (captures) *captures = bx->captures;
+  if (replays) *replays = bx->replays;
   return SR_OK;
 }
 

@@ -14,7 +14,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_PKG), "lib", "libspectrecon_amd.so")
 
 SR_OK, SR_ERR_ARG, SR_ERR_DIM, SR_ERR_HIP, SR_ERR_ALLOC, SR_ERR_UNSUPPORTED = range(6)
-SR_MEM_HOST, SR_MEM_DEVICE, SR_ASYNC, SR_NO_CHECK = 0, 1, 0x100, 0x200
+SR_MEM_HOST, SR_MEM_DEVICE, SR_ASYNC, SR_NO_CHECK, SR_BATCH_STREAMS = 0, 1, 0x100, 0x200, 0x400
 SR_COPY_H2D, SR_COPY_D2H, SR_COPY_D2D = 1, 2, 3
 SR_CONV3_FWD, SR_CONV3_ADJ, SR_CONV3_ADJ2 = 0, 1, 2
 SR_INTERP_ONE, SR_INTERP_TWO = 1, 2
@@ -69,6 +69,7 @@ SIGNATURES = {
     "sr_host_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     "sr_host_free": (C.c_int, [_vp]),
     "sr_osem_batch": (C.c_int, [C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int32]),
+    "sr_batch_stats": (C.c_int, [C.POINTER(C.c_int64)] * 4),
     "sr_plan_set_graph": (C.c_int, [_vp, C.c_int32]),
     "sr_plan_graph_stats": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "sr_plan_set_choice": (C.c_int, [_vp, _ip]),

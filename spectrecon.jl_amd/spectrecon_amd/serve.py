@@ -187,13 +187,28 @@ class DeviceBackend:
                 a.free()
 
 
+def batch_stats():
+    """sr_osem_batch's shared-launch counters on the current device:
+    instances run as shared launches, grouped launches, batch graphs captured
+    and replayed."""
+    import ctypes as C
+
+    from . import _lib as L
+    v = [C.c_int64() for _ in range(4)]
+    L.check(L.lib().sr_batch_stats(*[C.byref(x) for x in v]))
+    return dict(zip(("instances", "launches", "captures", "replays"), (x.value for x in v)))
+
+
 class Batch:
     """Batched OS-EM on resident plans: request i on plans[i] (distinct
     plans), its inputs uploaded to device arrays once.  `run()` runs all
-    reconstructions from one sr_osem_batch call (the instances' streams run
-    concurrently); `run(sequential=True)` one call per instance, each waited
-    for.  The device buffers stay between runs, so plans with graphs on
-    capture their EM sequence on the second run and replay it after that."""
+    reconstructions from one sr_osem_batch call: the subset steps of every
+    instance on a z-fastest plan as shared launches (one grid per kernel
+    instantiation and step), replayed from one HIP graph when the plans have
+    graphs on; `run(streams=True)` each instance on its plan's stream,
+    concurrently; `run(sequential=True)` one call per instance, each waited
+    for.  The device buffers stay between runs, so the launch sequence is
+    captured on the second run and replayed after that."""
 
     def __init__(self, plans, requests):
         import ctypes as C
@@ -209,12 +224,13 @@ class Batch:
         self._nb = (C.c_int32 * n)(*[int(r.nblocks) for r in self.requests])
         self._ni = (C.c_int32 * n)(*[int(r.niter) for r in self.requests])
 
-    def run(self, sequential=False):
+    def run(self, sequential=False, streams=False):
         """Seconds for all reconstructions (returns after they finished)."""
         import ctypes as C
 
         from . import _lib as L
         lib = L.lib()
+        mem = L.SR_MEM_DEVICE | (L.SR_BATCH_STREAMS if streams else 0)
         L.check(lib.sr_device_synchronize())
         t0 = time.perf_counter()
         if sequential:
@@ -223,10 +239,10 @@ class Batch:
                 L.check(lib.sr_osem_batch(1, C.byref(V(self._P[i])), C.byref(V(self._out[i])),
                                           C.byref(V(self._arr["x0"][i])), C.byref(V(self._arr["y"][i])),
                                           C.byref(V(self._arr["b"][i])), C.byref(C.c_int32(self._nb[i])),
-                                          C.byref(C.c_int32(self._ni[i])), L.SR_MEM_DEVICE))
+                                          C.byref(C.c_int32(self._ni[i])), mem))
         else:
             L.check(lib.sr_osem_batch(len(self.requests), self._P, self._out, self._arr["x0"], self._arr["y"],
-                                      self._arr["b"], self._nb, self._ni, L.SR_MEM_DEVICE))
+                                      self._arr["b"], self._nb, self._ni, mem))
         return time.perf_counter() - t0
 
     def results(self):
