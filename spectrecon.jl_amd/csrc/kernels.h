@@ -135,7 +135,8 @@ constexpr int kMaxEll = 8;
 bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::vector<int2>& ent);
 // OS-EM update of the z-fastest image x with the subset normaliser computed
 // in the same pass: asum = sum rot2^T(hvol .* atten[v]) (zeros as Inf, written
-// to asum), x .*= (sum rot2^T(work[b])) ./ asum
+// to asum), x .*= (sum rot2^T(work[b])) ./ asum; hvol null: atten holds the
+// products hvol .* atten[v] already (launch_mul_volumes)
 hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* ent, int K, const int* vids, int nb,
                                      const Geo& g, hipStream_t s, const float* hvol, const float* atten,
                                      long atten_vstride, float* asum, int vs = 0);
@@ -146,6 +147,8 @@ hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* en
 hipError_t launch_rot2_adj_z_norms(const float* hvol, float* asum, const int2* ent, int K, const int* vids, int nb,
                                    int nb_rem, int nsubsets, const Geo& g, hipStream_t s, const float* atten,
                                    long atten_vstride, int vs = 0);
+// out[v] = h .* atten[v] for nview volumes of V floats (V % 4 == 0)
+hipError_t launch_mul_volumes(const float* h, const float* atten, float* out, long V, int nview, hipStream_t s);
 // img [z][y][x] -> out [y][x][z]
 hipError_t launch_to_zfast(const float* img, float* out, const Geo& g, hipStream_t s);
 // out[b] = atten[v] .* rot2(img_t) per view, all [y][x][z]; atten may be null

@@ -3022,6 +3022,8 @@ __global__ void __launch_bounds__(256) k_rot2_fwd_z(const float* __restrict__ im
 // view-independent PSF adjoint result.  MODE 2: both sums in one pass (the
 // slots are loaded once): the back-projection of work and the normaliser;
 // the normaliser goes to em_asum (zeros as Inf) and drives the EM update.
+// MODE 3: MODE 2 with the products H .* atten[v] read from `atten` (the
+// plan's premultiplied volumes, the same roundings), one load per slot.
 // em_asum (MODE 0, optional): part holds the z-fastest image x and is updated
 // in place, x .*= back ./ asum (src/ml-os-em.jl:46), asum z-fastest too.
 // zero_inf (MODE 0/1): the sums are OS-EM normalisers, zeros written as Inf
@@ -3043,7 +3045,7 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
                                                           const float* __restrict__ atten, long atten_vstride,
                                                           float* __restrict__ em_asum, int zero_inf,
                                                           const EllGrp* __restrict__ grp, int ngrp) {
-  constexpr int NS = MODE == 2 ? 2 : 1;  // sums carried
+  constexpr int NS = MODE >= 2 ? 2 : 1;  // sums carried
   __shared__ float4 red[VS][64][Q * NS];
   int lb, sk = blockIdx.y;
   if constexpr (GRP) {
@@ -3100,7 +3102,23 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
           for (int q = 0; q < Q; ++q) fma4(acc[q], w, pa[k][q]);
       }
     }
-    if (MODE != 0) {  // normaliser: H .* atten[v]
+    if (MODE == 3) {  // normaliser: the premultiplied H .* atten[v]
+      const float* aa = atten + atten_vstride * vids[b] + zo;
+      float4 pa[K][Q];
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (__int_as_float(sa[k].y) != 0.f)
+#pragma unroll
+          for (int q = 0; q < Q; ++q) pa[k][q] = *reinterpret_cast<const float4*>(aa + (long)sa[k].x * nz + 4 * q);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float w = __int_as_float(sa[k].y);
+        if (w != 0.f)
+#pragma unroll
+          for (int q = 0; q < Q; ++q) fma4(acc[Q + q], w, pa[k][q]);
+      }
+    }
+    if (MODE == 1 || MODE == 2) {  // normaliser: H .* atten[v]
       const float* ha = hvol + zo;
       const float* aa = atten + atten_vstride * vids[b] + zo;
       float4 pa[K][Q];
@@ -3147,7 +3165,7 @@ __global__ void __launch_bounds__(64 * VS) k_rot2_adj_ell(const float* __restric
   for (int q = 0; q < Q; ++q) {
     const long o = (long)j * nz + zo + 4 * q;
     float4* po = reinterpret_cast<float4*>(part + o);
-    if (MODE == 2) {
+    if (MODE >= 2) {
       const float4 a = zinf(acc[Q + q]);
       *reinterpret_cast<float4*>(em_asum + o) = a;
       const float4 x = *po;
@@ -3830,8 +3848,11 @@ static hipError_t launch_ell(int mode, const float* work, float* part, const int
     else if (mode == 1)                                                                                             \
       k_rot2_adj_ell<KK, VV, QQ, 1, false><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(                       \
           work, part, ent, vids, nb, nb_rem, C, g, hvol, atten, atten_vstride, em_asum, zi, nullptr, 0);          \
-    else                                                                                                            \
+    else if (mode == 2)                                                                                             \
       k_rot2_adj_ell<KK, VV, QQ, 2, false><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(                       \
+          work, part, ent, vids, nb, nb_rem, C, g, hvol, atten, atten_vstride, em_asum, zi, nullptr, 0);          \
+    else                                                                                                            \
+      k_rot2_adj_ell<KK, VV, QQ, 3, false><<<dim3(kXcds * C, nsubsets), 64 * VV, 0, s>>>(                       \
           work, part, ent, vids, nb, nb_rem, C, g, hvol, atten, atten_vstride, em_asum, zi, nullptr, 0);          \
     return hipGetLastError();                                                                                       \
   }
@@ -3932,7 +3953,7 @@ int ell_grp_desc(int mode, const float* work, float* part, const int2* ent, int 
                  EllGrp* d) {
   // launch_ell of one subset, as launch_rot2_adj_z (mode 0) and
   // launch_rot2_adj_z_em_norm (mode 2) call it
-  if (nb <= 0 || (mode != 0 && mode != 2) || K < 4 || K > 8) return 0;
+  if (nb <= 0 || (mode != 0 && mode != 2 && mode != 3) || K < 4 || K > 8) return 0;
   const int q = (g.nz / 4) % 2 == 0 ? 2 : 1;
   const int pw = 64 / (g.nz / 4 / q);
   const int C = xcd_chunk(cdiv((long)g.nx * g.nx, pw));
@@ -3942,7 +3963,7 @@ int ell_grp_desc(int mode, const float* work, float* part, const int2* ent, int 
   else
     while (vs < 8 && 2 * vs <= nb && (long)kXcds * C * vs < 4096) vs *= 2;
   *d = EllGrp{GrpHdr{0, kXcds * C, 1, 0}, work, part, ent, vids, nb, C, g, hvol, atten, atten_vstride, em_asum,
-              mode == 2 ? 1 : 0};
+              mode >= 2 ? 1 : 0};
   return 3 << 16 | K << 12 | vs << 8 | q << 4 | mode;
 }
 
@@ -3957,6 +3978,9 @@ hipError_t launch_ell_grp(int key, const EllGrp* d, int n, int blocks, hipStream
           nullptr, nullptr, nullptr, nullptr, 0, 0, 0, Geo{}, nullptr, nullptr, 0, nullptr, 0, d, n);    \
     else if (mode == 2)                                                           \
       k_rot2_adj_ell<KK, VV, QQ, 2, true><<<blocks, 64 * VV, 0, s>>>(                \
+          nullptr, nullptr, nullptr, nullptr, 0, 0, 0, Geo{}, nullptr, nullptr, 0, nullptr, 0, d, n);    \
+    else if (mode == 3)                                                           \
+      k_rot2_adj_ell<KK, VV, QQ, 3, true><<<blocks, 64 * VV, 0, s>>>(                \
           nullptr, nullptr, nullptr, nullptr, 0, 0, 0, Geo{}, nullptr, nullptr, 0, nullptr, 0, d, n);    \
     else                                                                          \
       return hipErrorInvalidValue;                                                \
@@ -3989,7 +4013,28 @@ hipError_t launch_rot2_adj_z_norms(const float* hvol, float* asum, const int2* e
 hipError_t launch_rot2_adj_z_em_norm(const float* work, float* x, const int2* ent, int K, const int* vids, int nb,
                                      const Geo& g, hipStream_t s, const float* hvol, const float* atten,
                                      long atten_vstride, float* asum, int vs) {
-  return launch_ell(2, work, x, ent, K, vids, nb, g, s, hvol, atten, atten_vstride, asum, true, vs);
+  // (hvol null: atten holds the products H .* atten[v])
+  return launch_ell(hvol ? 2 : 3, work, x, ent, K, vids, nb, g, s, hvol, atten, atten_vstride, asum, true, vs);
+}
+
+// out[v][i] = h[i] * atten[v][i] over nview volumes of V floats (grid-stride)
+__global__ void __launch_bounds__(256) k_mul_volumes(const float* __restrict__ h, const float* __restrict__ atten,
+                                                     float* __restrict__ out, long V, int nview) {
+  const long n4 = V >> 2, total = n4 * nview;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < total; i += (long)gridDim.x * 256) {
+    const long v = i / n4, k = i - v * n4;
+    const float4 a = reinterpret_cast<const float4*>(atten)[i];
+    const float4 b = reinterpret_cast<const float4*>(h)[k];
+    reinterpret_cast<float4*>(out)[i] = make_float4(b.x * a.x, b.y * a.y, b.z * a.z, b.w * a.w);
+  }
+}
+
+hipError_t launch_mul_volumes(const float* h, const float* atten, float* out, long V, int nview, hipStream_t s) {
+  if (V <= 0 || nview <= 0) return hipSuccess;
+  if (V % 4 != 0) return hipErrorInvalidValue;
+  const long total = (V >> 2) * nview;
+  k_mul_volumes<<<(unsigned)std::min<long>(cdiv(total, 256), 8192), 256, 0, s>>>(h, atten, out, V, nview);
+  return hipGetLastError();
 }
 
 hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const float* asum, int mode, const Geo& g,

@@ -237,6 +237,11 @@ struct sr_plan_s {
   DBuf flag;  // one int: device-side argument checks
   DBuf zimg, zent;  // z-fastest image copy, per-target rotation lists
   DBuf zh;          // OS-EM: the PSF adjoint of a ones view (one PSF stack for all views)
+  // OS-EM with one PSF stack on the fused path: H .* atten[v] for every view
+  // (the normalisers' sources, [view][y][x][z]), valid while atten_h_fam is
+  // the PSF adjoint family H was computed with (-1: not built)
+  DBuf atten_h;
+  int atten_h_fam = -1;
   DBuf gview, gwork2, gpart, gloss;  // sr_grad: residual views, R(x) .* atten, loss partial sums
   int zk = 0;        // slots per target of zent
   std::vector<int> h_vids, h_order;
@@ -574,6 +579,7 @@ int stage_in(sr_plan p, DBuf& buf, const void* host, size_t bytes, size_t min_by
 // each view's rotated mu volume, then the running sums along y and exp, in
 // batches of 32 views on the plan's stream.
 int build_atten(sr_plan p) {
+  p->atten_h_fam = -1;  // (H .* atten[v] follows the factors)
   const sr_plan_desc& d = p->d;
   std::vector<int> all(d.nview);
   for (int v = 0; v < d.nview; ++v) all[v] = v;
@@ -855,7 +861,7 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
   HIP_TRY(hipStreamSynchronize(p->stream));
   clk.mark("atten");
   p->bytes = p->mumap.n + p->atten.n + p->taps_u.n + p->taps_w.n + p->taps_h.n + p->taps_c.n + p->rotv.n + p->tab_b.n +
-             p->tab_w.n + p->zimg.n + p->zent.n + p->fspec.n + p->ftw.n;  // (zh is a per-call buffer)
+             p->tab_w.n + p->zimg.n + p->zent.n + p->fspec.n + p->ftw.n;  // (zh, atten_h: the OS-EM calls')
   *out = guard.release();
   return SR_OK;
 }
@@ -1183,7 +1189,8 @@ int sr_plan_info(sr_plan p, size_t* device_bytes, int32_t* conv_method, int32_t*
   if (!p) return fail(SR_ERR_ARG, "null plan");
   if (device_bytes)
     *device_bytes = p->bytes + p->work.n + p->fpart.n + p->ipart.n + p->st_img.n + p->st_views.n + p->st_aux1.n +
-                    p->st_aux2.n + p->st_x0.n + p->em_ratio.n + p->em_back.n + p->em_asum.n + p->ones.n;
+                    p->st_aux2.n + p->st_x0.n + p->em_ratio.n + p->em_back.n + p->em_asum.n + p->ones.n +
+                    p->zh.n + p->atten_h.n;
   if (conv_method) *conv_method = p->use_fft() ? SR_CONV_FFT : p->separable ? SR_CONV_SEPARABLE : SR_CONV_DIRECT;
   if (max_batch) *max_batch = p->max_batch;
   return SR_OK;
@@ -1404,15 +1411,10 @@ bool em_norm_in_step(sr_plan p, const EmCall& c, int it) { return c.shared && it
 // also run as shared launches (sr_osem_batch): the normalisers and the
 // image's z-fastest copy; one subset step; the result in the caller's layout.
 int em_fused_begin(sr_plan p, const EmCall& c) {
-  const int nsub = c.cnt[0];
-  HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P * nsub, p->stream));
   int rc;
-  if (c.shared) {
-    const sr::Taps t = p->taps();
-    KTimer kt(p, sr::K_PSF_ADJ);
-    HIP_TRY(sr::launch_psf_adj(p->separable, p->ones.as<float>(), t, c.vids, 1, p->zh.as<float>(), nullptr, 0,
-                               p->nchunk_adj(1), p->g, p->stream, true, p->fam_adj()));
-  } else {
+  // (one PSF stack: H and H .* atten[v] are plan state, built by em_setup)
+  if (!c.shared) {
+    HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P * c.cnt[0], p->stream));
     for (int nb = 0; nb < c.nblocks; ++nb) {
       float* as = p->em_asum.as<float>() + p->V * nb;
       if ((rc = run_backproject(p, p->em_back.as<float>(), p->ones.as<float>(), c.vids + c.off[nb], c.cnt[nb],
@@ -1460,7 +1462,7 @@ int em_fused_step(sr_plan p, const EmCall& c, int it, int nb) {
   float* as = p->em_asum.as<float>() + p->V * nb;
   if (em_norm_in_step(p, c, it))  // this subset's normaliser, summed in the same pass
     HIP_TRY(sr::launch_rot2_adj_z_em_norm(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv,
-                                          ns, p->g, p->stream, p->zh.as<float>(), p->atten.as<float>(),
+                                          ns, p->g, p->stream, nullptr, p->atten_h.as<float>(),
                                           p->atten_vstride(), as, p->ell_vs()));
   else
     HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv, ns,
@@ -1536,7 +1538,8 @@ int em_graph(sr_plan p, const EmCall& c) {
       (uintptr_t)c.out, (uintptr_t)c.x0, (uintptr_t)c.y, (uintptr_t)c.b, (uintptr_t)c.vids,
       (uintptr_t)c.nblocks, (uintptr_t)c.niter, (uintptr_t)p->max_batch, (uintptr_t)p->work.p,
       (uintptr_t)p->fpart.p, (uintptr_t)p->ipart.p, (uintptr_t)p->ones.p, (uintptr_t)p->em_asum.p,
-      (uintptr_t)p->em_ratio.p, (uintptr_t)p->em_back.p, (uintptr_t)p->zimg.p, (uintptr_t)p->zh.p};
+      (uintptr_t)p->em_ratio.p, (uintptr_t)p->em_back.p, (uintptr_t)p->zimg.p, (uintptr_t)p->zh.p,
+      (uintptr_t)p->atten_h.p};
   for (auto& e : p->graphs)
     if (e.key == key) {
       HIP_TRY(hipGraphLaunch(e.exec, p->stream));
@@ -1574,6 +1577,30 @@ int em_graph(sr_plan p, const EmCall& c) {
 
 // shared body of mlem!/osem!: blocks of views nb:nblocks:nview
 // (src/ml-os-em.jl:21-49 with nblocks == 1; :97-131 otherwise)
+// H = the PSF adjoint of a ones view (one PSF stack for all views: the same
+// for every view) into zh, and H .* atten[v] for every view into atten_h,
+// rounded as the two-sum gather rounds them: the sources of the OS-EM
+// normalisers A_b' 1 = sum over b of rot2^T(H .* atten[v]).  Plan state,
+// rebuilt when the attenuation factors or the PSF adjoint family change.
+// vids: any device view id (the taps are those of every view).
+int ensure_atten_h(sr_plan p, const int* vids) {
+  if (p->atten_h_fam == p->fam_adj()) return SR_OK;
+  int rc;
+  if ((rc = p->zh.ensure(sizeof(float) * p->V)) || (rc = p->ones.ensure(sizeof(float) * p->P)) ||
+      (rc = p->atten_h.ensure(sizeof(float) * p->V * p->d.nview)))
+    return rc;
+  HIP_TRY(sr::launch_fill(p->ones.as<float>(), 1.f, p->P, p->stream));
+  {
+    KTimer kt(p, sr::K_PSF_ADJ);
+    HIP_TRY(sr::launch_psf_adj(p->separable, p->ones.as<float>(), p->taps(), vids, 1, p->zh.as<float>(), nullptr, 0,
+                               p->nchunk_adj(1), p->g, p->stream, true, p->fam_adj()));
+  }
+  HIP_TRY(sr::launch_mul_volumes(p->zh.as<float>(), p->atten.as<float>(), p->atten_h.as<float>(), p->V, p->d.nview,
+                                 p->stream));
+  p->atten_h_fam = p->fam_adj();
+  return SR_OK;
+}
+
 // checks, host staging, the subset view lists and every buffer of one EM
 // call, leaving *c ready for its kernel sequence
 int em_setup(sr_plan p, EmCall& c, float* out, const float* x0, const float* y, const float* b, int nblocks,
@@ -1649,6 +1676,7 @@ int em_setup(sr_plan p, EmCall& c, float* out, const float* x0, const float* y, 
       (((c.fused && !c.shared) || (!c.fused && nsub > p->max_batch)) &&
        (rc = p->em_back.ensure(sizeof(float) * p->V))))
     return rc;
+  if (c.fused && c.shared && (rc = ensure_atten_h(p, c.vids))) return rc;
   return SR_OK;
 }
 
@@ -1748,8 +1776,8 @@ bool step_desc(sr_plan p, const EmCall& c, int it, int nb, StepDesc& d) {
                               p->atten_vstride(), p->nchunk_adj(ns), p->g, p->fam_adj(), &d.a, &d.la);
   float* as = p->em_asum.as<float>() + p->V * nb;
   d.ke = em_norm_in_step(p, c, it)
-             ? sr::ell_grp_desc(2, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv, ns, p->g,
-                                p->zh.as<float>(), p->atten.as<float>(), p->atten_vstride(), as, p->ell_vs(), &d.e)
+             ? sr::ell_grp_desc(3, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv, ns, p->g,
+                                nullptr, p->atten_h.as<float>(), p->atten_vstride(), as, p->ell_vs(), &d.e)
              : sr::ell_grp_desc(0, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, sv, ns, p->g,
                                 nullptr, nullptr, 0, as, p->ell_vs(), &d.e);
   return d.kf && d.kr && d.ka && d.ke;
@@ -1905,8 +1933,8 @@ int batch_build(Lane& ln, std::vector<unsigned char>& desc) {
         sr::EllGrp d;
         const int k =
             em_norm_in_step(p, c, m->it)
-                ? sr::ell_grp_desc(2, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, m->sv, m->ns,
-                                   p->g, p->zh.as<float>(), p->atten.as<float>(), p->atten_vstride(), as, vs, &d)
+                ? sr::ell_grp_desc(3, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, m->sv, m->ns,
+                                   p->g, nullptr, p->atten_h.as<float>(), p->atten_vstride(), as, vs, &d)
                 : sr::ell_grp_desc(0, p->work.as<float>(), p->zimg.as<float>(), p->zent.as<int2>(), p->zk, m->sv, m->ns,
                                    p->g, nullptr, nullptr, 0, as, vs, &d);
         if (key && k != key) return fail(SR_ERR_UNSUPPORTED, "internal: gather keys differ within a group");
@@ -2001,7 +2029,7 @@ std::vector<uintptr_t> batch_key(const std::vector<sr_plan>& ps, const std::vect
           (uintptr_t)c.nblocks, (uintptr_t)c.niter, (uintptr_t)p->max_batch, (uintptr_t)p->work.p,
           (uintptr_t)p->fpart.p, (uintptr_t)p->ipart.p, (uintptr_t)p->ones.p, (uintptr_t)p->em_asum.p,
           (uintptr_t)p->em_ratio.p, (uintptr_t)p->em_back.p, (uintptr_t)p->zimg.p, (uintptr_t)p->zh.p,
-          (uintptr_t)p->zent.p, (uintptr_t)p->zk, (uintptr_t)p->atten.p, (uintptr_t)p->rotv.p,
+          (uintptr_t)p->zent.p, (uintptr_t)p->zk, (uintptr_t)p->atten.p, (uintptr_t)p->atten_h.p, (uintptr_t)p->rotv.p,
           (uintptr_t)p->taps_u.p, (uintptr_t)p->taps_w.p, (uintptr_t)p->taps_c.p})
       key.push_back(v);
     for (int k = 0; k < SR_NCHOICE; ++k) key.push_back((uintptr_t)(uint32_t)p->choice[k]);

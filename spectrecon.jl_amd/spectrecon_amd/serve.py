@@ -44,11 +44,12 @@ class Config:
 
     def footprint(self, max_batch=64):
         """Device bytes of one plan of this config and its per-call buffers
-        (attenuation factors, lists, work volumes, normalisers), an upper
-        estimate used for admission."""
+        (attenuation factors and their products with the normaliser's PSF
+        adjoint, lists, work volumes, normalisers), an upper estimate used
+        for admission."""
         V = self.nx * self.nx * self.nz
         batch = min(self.nview, max_batch)
-        atten = 4 * V * self.nview
+        atten = 2 * 4 * V * self.nview
         lists = 8 * 8 * self.nx * self.nx * self.nview
         work = 4 * V * (batch + 8)
         images = 4 * V * (2 + 8)
