@@ -987,6 +987,68 @@ def test_graph_replay_equals_eager(gpu, O, monkeypatch, variant):
     assert graphed.graph_stats() == (cap + 1, rep + 2)
 
 
+def test_shared_launch_batch_async_mixed(gpu, O):
+    """sr_osem_batch with SR_ASYNC over a mix: two z-fastest instances (even
+    and uneven strided subsets) run as shared launches, an x-fastest one
+    (non-separable PSFs) on its own stream; after sr_plan_synchronize per
+    plan, the shared results are within 1e-5 relative L2 of each instance
+    alone and identical on every run (eager, graph capture, replay), the
+    x-fastest one equals its own call bit for bit, and all match the FP64
+    oracle."""
+    import ctypes as C
+
+    from spectrecon_amd import _lib as L
+    from spectrecon_amd import serve as V
+    from spectrecon_amd.device import DeviceArray
+    rng = np.random.default_rng(11)
+    cases = []
+    for nx, nz, nview, nblocks, general in ((32, 16, 16, 4, False), (64, 32, 12, 5, False), (32, 16, 10, 3, True)):
+        xtrue, mumap, psfs = em_problem(nx, nz, nview)
+        if general:
+            psfs = rand_sym_psfs(rng, 5, 5, nx, nview)
+        plan = gpu.SPECTplan(mumap, psfs, 0.8)
+        assert plan.info()["layout"] == ("x-fastest" if general else "z-fastest")
+        plan.set_graph(True)
+        ytrue = gpu.project(xtrue, plan)
+        b = np.full_like(ytrue, 0.1 * ytrue.mean())
+        y = gpu.poisson(20 * (ytrue + b), seed=nx + nview) / 20
+        x0 = np.ones_like(xtrue)
+        alone = gpu.osem_strided(x0, y, b, plan, nblocks, 2)
+        P64 = O.Problem(mumap, psfs, 0.8, angles32(nview).astype(np.float64), "two", np.float64, "direct")
+        xr = P64.osem(x0.astype(np.float64), y.astype(np.float64), b.astype(np.float64), nblocks, 2)
+        dev = [DeviceArray.from_host(a) for a in (x0, y, b)] + [DeviceArray(x0.shape)]
+        cases.append(dict(plan=plan, nblocks=nblocks, alone=alone, xr=xr, general=general, dev=dev))
+    n = len(cases)
+    P = (C.c_void_p * n)(*[c["plan"].handle for c in cases])
+    arr = [(C.c_void_p * n)(*[c["dev"][k].ptr for c in cases]) for k in range(4)]
+    nb = (C.c_int32 * n)(*[c["nblocks"] for c in cases])
+    ni = (C.c_int32 * n)(*([2] * n))
+    lib = L.lib()
+    s0 = V.batch_stats()
+    runs = []
+    for rnd in range(3):
+        L.check(lib.sr_osem_batch(n, P, arr[3], arr[0], arr[1], arr[2], nb, ni, L.SR_MEM_DEVICE | L.SR_ASYNC))
+        for c in cases:
+            c["plan"].synchronize()
+        runs.append([c["dev"][3].to_host() for c in cases])
+    s1 = V.batch_stats()
+    assert s1["instances"] - s0["instances"] == 3 * 2
+    assert (s1["captures"] - s0["captures"], s1["replays"] - s0["replays"]) == (1, 1)
+    for k, c in enumerate(cases):
+        for rnd in range(1, 3):
+            np.testing.assert_array_equal(runs[rnd][k], runs[0][k], err_msg=f"instance {k}, run {rnd}")
+        got = runs[0][k]
+        if c["general"]:
+            np.testing.assert_array_equal(got, c["alone"])
+        else:
+            assert np.linalg.norm(got - c["alone"]) <= 1e-5 * np.linalg.norm(c["alone"]), k
+        assert np.linalg.norm(got - c["xr"]) <= 1e-4 * np.linalg.norm(c["xr"]), k
+    for c in cases:
+        for d in c["dev"]:
+            d.free()
+        c["plan"].close()
+
+
 def test_serve_mix_equals_alone(gpu):
     """Requests of three sizes admitted concurrently by the serving loop (each
     on its own resident plan and stream): every result equals the same
