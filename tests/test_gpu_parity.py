@@ -988,8 +988,10 @@ def test_graph_replay_equals_eager(gpu, O, monkeypatch, variant):
 
 
 def test_shared_launch_batch_async_mixed(gpu, O):
-    """sr_osem_batch with SR_ASYNC over a mix: two z-fastest instances (even
-    and uneven strided subsets) run as shared launches, an x-fastest one
+    """sr_osem_batch with SR_ASYNC over a mix: three z-fastest instances (even
+    and uneven strided subsets; one with the 4-producer forward and pinned
+    gather slices, so that it shares no launch with the same-size instance
+    on the built-in choices) run as shared launches, an x-fastest one
     (non-separable PSFs) on its own stream; after sr_plan_synchronize per
     plan, the shared results are within 1e-5 relative L2 of each instance
     alone and identical on every run (eager, graph capture, replay), the
@@ -1002,13 +1004,19 @@ def test_shared_launch_batch_async_mixed(gpu, O):
     from spectrecon_amd.device import DeviceArray
     rng = np.random.default_rng(11)
     cases = []
-    for nx, nz, nview, nblocks, general in ((32, 16, 16, 4, False), (64, 32, 12, 5, False), (32, 16, 10, 3, True)):
+    for nx, nz, nview, nblocks, general, choice in (
+            (32, 16, 16, 4, False, {}),
+            (64, 32, 12, 5, False, dict(psf_fwd_family=4, gather_slices=4)),  # pinned gather slices
+            (64, 32, 12, 3, False, {}),
+            (32, 16, 10, 3, True, {})):
         xtrue, mumap, psfs = em_problem(nx, nz, nview)
         if general:
             psfs = rand_sym_psfs(rng, 5, 5, nx, nview)
         plan = gpu.SPECTplan(mumap, psfs, 0.8)
         assert plan.info()["layout"] == ("x-fastest" if general else "z-fastest")
         plan.set_graph(True)
+        if choice:
+            plan.set_choice(**choice)
         ytrue = gpu.project(xtrue, plan)
         b = np.full_like(ytrue, 0.1 * ytrue.mean())
         y = gpu.poisson(20 * (ytrue + b), seed=nx + nview) / 20
@@ -1032,7 +1040,7 @@ def test_shared_launch_batch_async_mixed(gpu, O):
             c["plan"].synchronize()
         runs.append([c["dev"][3].to_host() for c in cases])
     s1 = V.batch_stats()
-    assert s1["instances"] - s0["instances"] == 3 * 2
+    assert s1["instances"] - s0["instances"] == 3 * 3
     assert (s1["captures"] - s0["captures"], s1["replays"] - s0["replays"]) == (1, 1)
     for k, c in enumerate(cases):
         for rnd in range(1, 3):
