@@ -50,9 +50,8 @@ namespace {
 
 constexpr double kPiD = 3.14159265358979323846;
 constexpr int kReadAll = 2;    // items per thread of the pass that reads all, then writes
-// threads per block: the forward 512 (8 waves, one block per CU, up to 256
-// VGPRs: the chunk's spectrum sums live in registers), the adjoint 1024 (its
-// view spectrum spread over twice the threads)
+// threads per block: 512 (8 waves, one block per CU, up to 256 VGPRs: the
+// forward's chunk sums and the adjoint's view spectrum live in registers)
 constexpr int kNTF = 512, kNTA = 512;
 
 template <int N>
@@ -96,6 +95,20 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// The thread's index for the passes' item loops.  OPQ (the adjoint): read
+// through an empty asm, so that the item indices and LDS offsets derived from
+// it are recomputed in every pass of the plane loop instead of being hoisted
+// out of it and held in registers for every plane.  At 256 x 128 that hoisting
+// spilled 67 VGPRs to scratch and 184 SGPRs; with OPQ in every adjoint pass
+// the adjoint holds 184 VGPRs and spills nothing, and a 64-view launch takes
+// 0.85 ms instead of 1.41.  The forward is as fast or faster without it.
+template <bool OPQ>
+__device__ __forceinline__ int item_tid() {
+  int t = threadIdx.x;
+  if constexpr (OPQ) asm volatile("" : "+v"(t));
+  return t;
+}
 
 // In-register DFT of R points, natural order in and out (radix-2 stages):
 // forward a[k] = sum_n a[n] e^{-2 pi i n k / R}; INV: e^{+...}, unnormalised.
@@ -142,11 +155,11 @@ __device__ __forceinline__ void dft(float2 (&a)[R]) {
 // rows q = n1 + R1z n2 of column f = h NFH + fl, the real row r = q - left
 // separated from its row pair in A (ZERO_PAD 0: r clamped, the replicate pad;
 // 1: zero outside the image), and writes Y to B at n1 + R1z k2.
-template <int NT, int L1, int L2, int ZERO_PAD>
+template <int NT, int L1, int L2, int ZERO_PAD, bool OPQ = false>
 __device__ __forceinline__ void z_pass1_fwd(const float2* A, float2* B, const float2* tw2, int NPS, int nz, int left,
                                             int h) {
   constexpr int R1z = Split<L2>::R1, R2z = Split<L2>::R2, NF = L1 / 2 + 1, NFH = (NF + 1) / 2;
-  for (int it = threadIdx.x; it < NFH * R1z; it += NT) {
+  for (int it = item_tid<OPQ>(); it < NFH * R1z; it += NT) {
     const int fl = it % NFH, n1 = it / NFH, f = h * NFH + fl;
     if (f >= NF) continue;
     const int pf = f, pm = (L1 - f) & (L1 - 1);
@@ -172,14 +185,14 @@ __device__ __forceinline__ void z_pass1_fwd(const float2* A, float2* B, const fl
 // The spectrum items of z pass 2 and 2': slot s of a thread is item
 // it = tid + s NT, in half h = it / (NFH R2z), column fl, k2 (the item's
 // R1z values are the frequencies k2 + R2z k1 at B positions k1 + R1z k2).
-template <int NT, int L1, int L2>
+template <int NT, int L1, int L2, bool OPQ = false>
 struct SpecItems {
   static constexpr int R2z = Split<L2>::R2, NF = L1 / 2 + 1, NFH = (NF + 1) / 2;
   static constexpr int PER_HALF = NFH * R2z;
   static constexpr int SLOTS = (2 * PER_HALF + NT - 1) / NT;
   // false when slot s is not an item of half h
   __device__ static __forceinline__ bool at(int s, int h, int* fl, int* k2) {
-    const int it = threadIdx.x + s * NT;
+    const int it = item_tid<OPQ>() + s * NT;
     if (it / PER_HALF != h) return false;
     const int rem = it - h * PER_HALF;
     *fl = rem % NFH;
@@ -193,10 +206,10 @@ struct SpecItems {
 // memory (LOAD(p, j) gives the pair's two values at padded position p), pass 2
 // completes the transform.  Items of pass 1 are (n1, j) with n1 fastest, so a
 // wave reads consecutive x.
-template <int NT, int L1, class Load>
+template <int NT, int L1, bool OPQ = false, class Load>
 __device__ __forceinline__ void x_passes_fwd(float2* A, const float2* tw1, int NPS, int NP, Load load) {
   constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2;
-  for (int it = threadIdx.x; it < NP * R1x; it += NT) {
+  for (int it = item_tid<OPQ>(); it < NP * R1x; it += NT) {
     const int n1 = it % R1x, j = it / R1x;
     float2 a[R2x];
 #pragma unroll
@@ -213,7 +226,7 @@ __device__ __forceinline__ void x_passes_fwd(float2* A, const float2* tw1, int N
   float2 xr[RA][R1x];
 #pragma unroll
   for (int s = 0; s < RA; ++s) {
-    const int it = threadIdx.x + s * NT;
+    const int it = item_tid<OPQ>() + s * NT;
     if (it < NP * R2x) {
       const int j = it % NP, k2 = it / NP;
 #pragma unroll
@@ -224,7 +237,7 @@ __device__ __forceinline__ void x_passes_fwd(float2* A, const float2* tw1, int N
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < RA; ++s) {
-    const int it = threadIdx.x + s * NT;
+    const int it = item_tid<OPQ>() + s * NT;
     if (it < NP * R2x) {
       const int j = it % NP, k2 = it / NP;
 #pragma unroll
@@ -237,10 +250,10 @@ __device__ __forceinline__ void x_passes_fwd(float2* A, const float2* tw1, int N
 // z pass 1' of column half h: item (fl, n1) transforms B back along z; rows
 // r = q - left inside the image go to I [r][f] in A; with PADS the rows above
 // and below the image are summed per item into P [lo / hi][n1][fl].
-template <int NT, int L1, int L2, bool PADS>
+template <int NT, int L1, int L2, bool PADS, bool OPQ = false>
 __device__ __forceinline__ void z_pass1_inv(float2* A, const float2* B, float2* P, int nz, int left, int h) {
   constexpr int R1z = Split<L2>::R1, R2z = Split<L2>::R2, NF = L1 / 2 + 1, NFH = (NF + 1) / 2;
-  for (int it = threadIdx.x; it < NFH * R1z; it += NT) {
+  for (int it = item_tid<OPQ>(); it < NFH * R1z; it += NT) {
     const int fl = it % NFH, n1 = it / NFH, f = h * NFH + fl;
     if (f >= NF) continue;
     float2 a[R2z];
@@ -264,10 +277,10 @@ __device__ __forceinline__ void z_pass1_inv(float2* A, const float2* B, float2* 
 
 // the z pads folded onto rows 0 and nz - 1 (src/fft_convolve.jl:100-110),
 // for the columns of half h, in a fixed order
-template <int NT, int L1, int L2>
+template <int NT, int L1, int L2, bool OPQ = false>
 __device__ __forceinline__ void z_fold(float2* A, const float2* P, int nz, int h) {
   constexpr int R1z = Split<L2>::R1, NF = L1 / 2 + 1, NFH = (NF + 1) / 2;
-  for (int fl = threadIdx.x; fl < NFH; fl += NT) {
+  for (int fl = item_tid<OPQ>(); fl < NFH; fl += NT) {
     const int f = h * NFH + fl;
     if (f >= NF) continue;
     float2 lo = make_float2(0.f, 0.f), hi = make_float2(0.f, 0.f);
@@ -285,14 +298,14 @@ __device__ __forceinline__ void z_fold(float2* A, const float2* P, int nz, int h
 // row pair j at frequency f is X_2j[f] + i X_2j+1[f] (X_r[f] = conj X_r[L1 - f]
 // above L1/2).  Reads all its items (at most kReadAll per thread), then after
 // a barrier writes the x-buffer over I.
-template <int NT, int L1>
+template <int NT, int L1, bool OPQ = false>
 __device__ __forceinline__ void x_pass2_inv(float2* A, const float2* tw1, int NPS, int NP, int nz) {
   constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, NF = L1 / 2 + 1;
   constexpr int RA = kReadAll * kNTF / NT;  // (the same item bound for every block size)
   float2 xr[RA][R1x];
 #pragma unroll
   for (int s = 0; s < RA; ++s) {
-    const int it = threadIdx.x + s * NT;
+    const int it = item_tid<OPQ>() + s * NT;
     if (it < NP * R2x) {
       const int j = it % NP, k2 = it / NP;
 #pragma unroll
@@ -316,7 +329,7 @@ __device__ __forceinline__ void x_pass2_inv(float2* A, const float2* tw1, int NP
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < RA; ++s) {
-    const int it = threadIdx.x + s * NT;
+    const int it = item_tid<OPQ>() + s * NT;
     if (it < NP * R2x) {
       const int j = it % NP, k2 = it / NP;
 #pragma unroll
@@ -430,7 +443,7 @@ __global__ void __launch_bounds__(kNTA) k_psf_adj_fft(const float* __restrict__ 
                                                      int poff, const float2* __restrict__ twg) {
   constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, R1z = Split<L2>::R1;
   constexpr int NF = L1 / 2 + 1, NFH = (NF + 1) / 2, NT = kNTA;
-  using SI = SpecItems<NT, L1, L2>;
+  using SI = SpecItems<NT, L1, L2, true>;
   extern __shared__ float2 smf[];
   float2* tw1 = smf;
   float2* tw2 = smf + L1;
@@ -448,14 +461,14 @@ __global__ void __launch_bounds__(kNTA) k_psf_adj_fft(const float* __restrict__ 
   for (int m = tid; m < L1 + L2; m += NT) smf[m] = twg[m];
   __syncthreads();
   // the zero-padded view's spectrum, kept in registers for the chunk
-  x_passes_fwd<NT, L1>(A, tw1, NPS, NP, [&](int p, int j) {
+  x_passes_fwd<NT, L1, true>(A, tw1, NPS, NP, [&](int p, int j) {
     const int x = p - up, z0 = 2 * j;
     if (x < 0 || x >= nx) return make_float2(0.f, 0.f);
     return make_float2(vin[x + nx * z0], z0 + 1 < nz ? vin[x + nx * (z0 + 1)] : 0.f);
   });
   float2 vs[SI::SLOTS][R1z];
   for (int h = 0; h < 2; ++h) {
-    z_pass1_fwd<NT, L1, L2, 1>(A, B, tw2, NPS, nz, left, h);
+    z_pass1_fwd<NT, L1, L2, 1, true>(A, B, tw2, NPS, nz, left, h);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < SI::SLOTS; ++s) {
@@ -488,17 +501,19 @@ __global__ void __launch_bounds__(kNTA) k_psf_adj_fft(const float* __restrict__ 
         for (int n1 = 0; n1 < R1z; ++n1) B[(n1 + R1z * k2) * NFH + fl] = n1 ? cmulc(a[n1], tw2[n1 * k2]) : a[n1];
       }
       __syncthreads();
-      z_pass1_inv<NT, L1, L2, true>(A, B, P, nz, left, h);
+      z_pass1_inv<NT, L1, L2, true, true>(A, B, P, nz, left, h);
       __syncthreads();
-      z_fold<NT, L1, L2>(A, P, nz, h);
+      z_fold<NT, L1, L2, true>(A, P, nz, h);
       __syncthreads();
     }
-    x_pass2_inv<NT, L1>(A, tw1, NPS, NP, nz);
+    x_pass2_inv<NT, L1, true>(A, tw1, NPS, NP, nz);
     // x pass 1', the x pads folded onto x = 0 and nx - 1 (sums over the R1x
     // lanes of a row pair, src/fft_convolve.jl:112-122), times the
-    // attenuation, into the work volume
+    // attenuation, into the work volume (the item index opaque as in the
+    // passes: otherwise the per-position edge and crop masks of both rounds
+    // are hoisted out of the plane loop, about 130 SGPRs spilled)
     for (int rd = 0; rd < nx1; ++rd) {
-      const int it = tid + rd * NT;
+      const int it = item_tid<true>() + rd * NT;
       const bool ok = it < NP * R1x;
       const int n1 = it % R1x, j = ok ? it / R1x : 0;
       float2 a[R2x];
@@ -523,15 +538,23 @@ __global__ void __launch_bounds__(kNTA) k_psf_adj_fft(const float* __restrict__ 
         const int z0 = 2 * j;
         const bool two = z0 + 1 < nz;
         const int ob = nx * y + S * z0 + n1 - up;  // + R1x n2: position n1 + R1x n2
+        // the folds and the attenuation as selects and unconditional loads
+        // (pad positions read a clamped in-volume element and are not
+        // stored; a single last row reads its own row twice), so that all
+        // the loads issue together: 0.84 -> 0.70 ms per 64-view launch at
+        // 256 x 128 against per-element branches
 #pragma unroll
         for (int n2 = 0; n2 < R2x; ++n2) {
           const int x = n1 + R1x * n2 - up;
-          if (x < 0 || x >= nx) continue;  // (positions in the pads: no image element, no load)
-          if (x == 0) a[n2] = cadd(a[n2], lo);
-          if (x == nx - 1) a[n2] = cadd(a[n2], hi);
-          if (av) {
-            a[n2].x *= av[ob + R1x * n2];
-            if (two) a[n2].y *= av[ob + R1x * n2 + S];
+          a[n2] = cadd(a[n2], x == 0 ? lo : x == nx - 1 ? hi : make_float2(0.f, 0.f));
+        }
+        if (av) {
+          const int ol = nx * y + S * z0, oh = two ? ol + S : ol;
+#pragma unroll
+          for (int n2 = 0; n2 < R2x; ++n2) {
+            const int x = min(max(n1 + R1x * n2 - up, 0), nx - 1);
+            a[n2].x *= av[ol + x];
+            a[n2].y *= av[oh + x];
           }
         }
 #pragma unroll
