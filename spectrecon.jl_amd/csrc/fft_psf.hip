@@ -8,7 +8,10 @@
 // 2j + 1 travel as the real and imaginary parts of one complex row ("row
 // pair" j): the x transform of a pair holds both rows' spectra, separated on
 // the fly as A = (Z[f] + conj Z[-f]) / 2, B = -i (Z[f] - conj Z[-f]) / 2.  The z
-// transform then runs over the half spectrum f = 0..L1/2 only.  Each 1-D
+// transform then runs over the half spectrum f = 0..L1/2 only, as L1/2
+// complex columns: f = 0 and f = L1/2 are real per row, so they travel as the
+// real and imaginary parts of one packed column 0 (its spectral product is
+// taken apart with the column's mirror frequencies, below).  Each 1-D
 // transform is two radix-R passes in LDS (L = R1 R2, R <= 16, in-register
 // DFTs): the forward passes are decimation in frequency, the inverse passes
 // decimation in time.  The x passes leave the x spectrum in natural order
@@ -23,7 +26,16 @@
 // fftshift2! amounts to, src/fft_convolve.jl:17-23), is real: H[f][g] =
 // sum_{a,b} h[a,b] cos(2 pi (f (a - cx) / L1 + g (b - cz) / L2)), computed in
 // double at plan time and stored as float with the 1 / (L1 L2) of the
-// inverse folded in: [plane][L2 positions][L1/2 + 1] floats.
+// inverse folded in: [plane][L2 positions][L1/2 + 1] floats, where column 0
+// holds (H[0] + H[L1/2]) / 2 and column L1/2 holds (H[0] - H[L1/2]) / 2.
+//
+// The packed column: with P = C0 + i CN (C0, CN the real rows' f = 0 and
+// f = L1/2 columns) and its z transform P^, the wanted product is
+// C0^ H[0] + i CN^ H[L1/2] = P^[g] Hp[g] + conj(P^[-g]) Hm[g], Hp and Hm the
+// stored half sum and half difference (H[0][g] and H[L1/2][g] are even in g).
+// The forward sums P^ Hp in registers and W = P^ Hm in LDS over the chunk and
+// adds conj(W[-g]) once at its end; the adjoint keeps the view's P^ in LDS and
+// adds conj(P^[-g]) Hm per plane.
 //
 // Forward (project): per block one view and a chunk of planes.  Each plane's
 // rows are read with clamped x (the replicate pad, padrepl!) and its z
@@ -144,11 +156,14 @@ __device__ __forceinline__ void dft(float2 (&a)[R]) {
   }
 }
 
-// LDS regions (float2): the twiddles W_L1^m, W_L2^m; region A, the x-buffer
-// [pos][pair] (stride NPS) and in turn the rows' half spectra I [r][f]
-// (stride NF); region B, the z-buffer of one half of the columns [pos][fl]
-// (stride NFH = ceil(NF / 2)); in the adjoint, region P, the z pads' partial
-// sums [2][R1z][NFH].  The z transforms run over the two column halves in
+// LDS regions (float2): the twiddles W_L1^m, W_L2^m; region W, the packed
+// column's mirror terms [g] (L2); region A, the x-buffer [pos][pair] (stride
+// NPS) and in turn the rows' half spectra I [r][f] (L1 / 2 columns at the
+// odd stride NIS = L1 / 2 + 1: x pass 2' reads rows 2j of neighbouring pairs
+// on different banks);
+// region B, the z-buffer of one half of the columns [pos][fl] (stride
+// NFH = L1 / 4); in the adjoint, region P, the z pads' partial sums
+// [2][R1z][NFH].  The z transforms run over the two column halves in
 // turn, so a z pass never overwrites what another thread has yet to read.
 
 // z pass 1 of the forward transform of column half h: item (fl, n1) reads
@@ -158,11 +173,10 @@ __device__ __forceinline__ void dft(float2 (&a)[R]) {
 template <int NT, int L1, int L2, int ZERO_PAD, bool OPQ = false>
 __device__ __forceinline__ void z_pass1_fwd(const float2* A, float2* B, const float2* tw2, int NPS, int nz, int left,
                                             int h) {
-  constexpr int R1z = Split<L2>::R1, R2z = Split<L2>::R2, NF = L1 / 2 + 1, NFH = (NF + 1) / 2;
+  constexpr int R1z = Split<L2>::R1, R2z = Split<L2>::R2, NFH = L1 / 4;
   for (int it = item_tid<OPQ>(); it < NFH * R1z; it += NT) {
     const int fl = it % NFH, n1 = it / NFH, f = h * NFH + fl;
-    if (f >= NF) continue;
-    const int pf = f, pm = (L1 - f) & (L1 - 1);
+    const int pf = f, pm = f ? L1 - f : L1 / 2;  // (column 0: f = 0 and f = L1/2 packed)
     float2 a[R2z];
 #pragma unroll
     for (int n2 = 0; n2 < R2z; ++n2) {
@@ -171,8 +185,9 @@ __device__ __forceinline__ void z_pass1_fwd(const float2* A, float2* B, const fl
       if (!ZERO_PAD || (r >= 0 && r < nz)) {
         r = min(max(r, 0), nz - 1);
         const float2 zf = A[pf * NPS + (r >> 1)], zm = A[pm * NPS + (r >> 1)];
-        val = (r & 1) ? make_float2(0.5f * (zf.y + zm.y), -0.5f * (zf.x - zm.x))
-                      : make_float2(0.5f * (zf.x + zm.x), 0.5f * (zf.y - zm.y));
+        val = f ? ((r & 1) ? make_float2(0.5f * (zf.y + zm.y), -0.5f * (zf.x - zm.x))
+                           : make_float2(0.5f * (zf.x + zm.x), 0.5f * (zf.y - zm.y)))
+                : ((r & 1) ? make_float2(zf.y, zm.y) : make_float2(zf.x, zm.x));
       }
       a[n2] = val;
     }
@@ -187,7 +202,7 @@ __device__ __forceinline__ void z_pass1_fwd(const float2* A, float2* B, const fl
 // R1z values are the frequencies k2 + R2z k1 at B positions k1 + R1z k2).
 template <int NT, int L1, int L2, bool OPQ = false>
 struct SpecItems {
-  static constexpr int R2z = Split<L2>::R2, NF = L1 / 2 + 1, NFH = (NF + 1) / 2;
+  static constexpr int R2z = Split<L2>::R2, NFH = L1 / 4;
   static constexpr int PER_HALF = NFH * R2z;
   static constexpr int SLOTS = (2 * PER_HALF + NT - 1) / NT;
   // false when slot s is not an item of half h
@@ -197,7 +212,7 @@ struct SpecItems {
     const int rem = it - h * PER_HALF;
     *fl = rem % NFH;
     *k2 = rem / NFH;
-    return h * NFH + *fl < NF;
+    return true;
   }
 };
 
@@ -252,10 +267,9 @@ __device__ __forceinline__ void x_passes_fwd(float2* A, const float2* tw1, int N
 // and below the image are summed per item into P [lo / hi][n1][fl].
 template <int NT, int L1, int L2, bool PADS, bool OPQ = false>
 __device__ __forceinline__ void z_pass1_inv(float2* A, const float2* B, float2* P, int nz, int left, int h) {
-  constexpr int R1z = Split<L2>::R1, R2z = Split<L2>::R2, NF = L1 / 2 + 1, NFH = (NF + 1) / 2;
+  constexpr int R1z = Split<L2>::R1, R2z = Split<L2>::R2, NIS = L1 / 2 + 1, NFH = L1 / 4;
   for (int it = item_tid<OPQ>(); it < NFH * R1z; it += NT) {
     const int fl = it % NFH, n1 = it / NFH, f = h * NFH + fl;
-    if (f >= NF) continue;
     float2 a[R2z];
 #pragma unroll
     for (int k2 = 0; k2 < R2z; ++k2) a[k2] = B[(n1 + R1z * k2) * NFH + fl];
@@ -266,7 +280,7 @@ __device__ __forceinline__ void z_pass1_inv(float2* A, const float2* B, float2* 
       const int r = n1 + R1z * n2 - left;
       if (r < 0) lo = cadd(lo, a[n2]);
       else if (r >= nz) hi = cadd(hi, a[n2]);
-      else A[r * NF + f] = a[n2];
+      else A[r * NIS + f] = a[n2];
     }
     if (PADS) {
       P[n1 * NFH + fl] = lo;
@@ -279,10 +293,9 @@ __device__ __forceinline__ void z_pass1_inv(float2* A, const float2* B, float2* 
 // for the columns of half h, in a fixed order
 template <int NT, int L1, int L2, bool OPQ = false>
 __device__ __forceinline__ void z_fold(float2* A, const float2* P, int nz, int h) {
-  constexpr int R1z = Split<L2>::R1, NF = L1 / 2 + 1, NFH = (NF + 1) / 2;
+  constexpr int R1z = Split<L2>::R1, NIS = L1 / 2 + 1, NFH = L1 / 4;
   for (int fl = item_tid<OPQ>(); fl < NFH; fl += NT) {
     const int f = h * NFH + fl;
-    if (f >= NF) continue;
     float2 lo = make_float2(0.f, 0.f), hi = make_float2(0.f, 0.f);
 #pragma unroll
     for (int m = 0; m < R1z; ++m) {
@@ -290,17 +303,17 @@ __device__ __forceinline__ void z_fold(float2* A, const float2* P, int nz, int h
       hi = cadd(hi, P[(R1z + m) * NFH + fl]);
     }
     A[f] = cadd(A[f], lo);
-    A[(nz - 1) * NF + f] = cadd(A[(nz - 1) * NF + f], hi);
+    A[(nz - 1) * NIS + f] = cadd(A[(nz - 1) * NIS + f], hi);
   }
 }
 
 // x pass 2' of the inverse transform from the half spectra I [r][f] in A:
 // row pair j at frequency f is X_2j[f] + i X_2j+1[f] (X_r[f] = conj X_r[L1 - f]
-// above L1/2).  Reads all its items (at most kReadAll per thread), then after
+// above L1/2; X_r[0] and X_r[L1/2] the real and imaginary parts of I[r][0]).  Reads all its items (at most kReadAll per thread), then after
 // a barrier writes the x-buffer over I.
 template <int NT, int L1, bool OPQ = false>
 __device__ __forceinline__ void x_pass2_inv(float2* A, const float2* tw1, int NPS, int NP, int nz) {
-  constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, NF = L1 / 2 + 1;
+  constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, NIS = L1 / 2 + 1;
   constexpr int RA = kReadAll * kNTF / NT;  // (the same item bound for every block size)
   float2 xr[RA][R1x];
 #pragma unroll
@@ -311,13 +324,19 @@ __device__ __forceinline__ void x_pass2_inv(float2* A, const float2* tw1, int NP
 #pragma unroll
       for (int k1 = 0; k1 < R1x; ++k1) {
         const int f = k2 + R2x * k1;
-        const bool upper = f > L1 / 2;
-        const int fh = upper ? L1 - f : f;
-        float2 x0 = A[(2 * j) * NF + fh];
-        float2 x1 = 2 * j + 1 < nz ? A[(2 * j + 1) * NF + fh] : make_float2(0.f, 0.f);
+        const bool upper = f > L1 / 2, nyq = f == L1 / 2;
+        const int fh = upper ? L1 - f : nyq ? 0 : f;
+        float2 x0 = A[(2 * j) * NIS + fh];
+        float2 x1 = 2 * j + 1 < nz ? A[(2 * j + 1) * NIS + fh] : make_float2(0.f, 0.f);
         if (upper) {
           x0 = cconj(x0);
           x1 = cconj(x1);
+        } else if (f == 0) {
+          x0.y = 0.f;
+          x1.y = 0.f;
+        } else if (nyq) {
+          x0 = make_float2(x0.y, 0.f);
+          x1 = make_float2(x1.y, 0.f);
         }
         xr[s][k1] = make_float2(x0.x - x1.y, x0.y + x1.x);
       }
@@ -340,19 +359,20 @@ __device__ __forceinline__ void x_pass2_inv(float2* A, const float2* tw1, int NP
 }
 
 // --- forward -----------------------------------------------------------------
-// grid (nchunk, nb), block NT; LDS: twiddles, A, B (boff float2 after A).
+// grid (nchunk, nb), block NT; LDS: twiddles, W, A, B (boff float2 after A).
 template <int L1, int L2>
 __global__ void __launch_bounds__(kNTF) k_psf_fwd_fft(const float* __restrict__ work, const float* __restrict__ spec,
                                                      int vstride, const int* __restrict__ vids, int Yc,
                                                      float* __restrict__ part, Geo g, int up, int left, int NPS,
                                                      int boff, const float2* __restrict__ twg) {
-  constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, R1z = Split<L2>::R1;
-  constexpr int NF = L1 / 2 + 1, NFH = (NF + 1) / 2, NT = kNTF;
+  constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, R1z = Split<L2>::R1, R2z = Split<L2>::R2;
+  constexpr int NF = L1 / 2 + 1, NFH = L1 / 4, NT = kNTF;
   using SI = SpecItems<NT, L1, L2>;
   extern __shared__ float2 smf[];
   float2* tw1 = smf;
   float2* tw2 = smf + L1;
-  float2* A = smf + L1 + L2;
+  float2* W = smf + L1 + L2;
+  float2* A = W + L2;
   float2* B = A + boff;
   const int tid = threadIdx.x;
   const int nx = g.nx, ny = g.ny, nz = g.nz, NP = (nz + 1) / 2;
@@ -361,6 +381,7 @@ __global__ void __launch_bounds__(kNTF) k_psf_fwd_fft(const float* __restrict__ 
   const int S = nx * ny;  // (offsets within one volume fit an int)
   const float* wv = work + (long)S * nz * b;
   for (int m = tid; m < L1 + L2; m += NT) smf[m] = twg[m];
+  for (int m = tid; m < L2; m += NT) W[m] = make_float2(0.f, 0.f);
   float2 acc[SI::SLOTS][R1z];
 #pragma unroll
   for (int s = 0; s < SI::SLOTS; ++s)
@@ -388,14 +409,33 @@ __global__ void __launch_bounds__(kNTF) k_psf_fwd_fft(const float* __restrict__ 
         for (int n1 = 0; n1 < R1z; ++n1) a[n1] = B[(n1 + R1z * k2) * NFH + fl];
         dft<R1z, false>(a);
         const int f = h * NFH + fl;
+        // (Hm loaded by every item, outside the packed column's branch: the
+        // items of a wave share k2, so the loads share one address)
+        float hm[R1z];
 #pragma unroll
         for (int k1 = 0; k1 < R1z; ++k1) {
           const float hv = hy[(k1 + R1z * k2) * NF + f];
+          hm[k1] = hy[(k1 + R1z * k2) * NF + L1 / 2];
           acc[s][k1] = make_float2(fmaf(a[k1].x, hv, acc[s][k1].x), fmaf(a[k1].y, hv, acc[s][k1].y));
+        }
+        if (f == 0) {  // the packed column: W[g] += P^[g] Hm[g]
+#pragma unroll
+          for (int k1 = 0; k1 < R1z; ++k1) {
+            float2& w = W[k2 + R2z * k1];
+            w = make_float2(fmaf(a[k1].x, hm[k1], w.x), fmaf(a[k1].y, hm[k1], w.y));
+          }
         }
       }
       __syncthreads();  // (B is rewritten next)
     }
+  }
+  // the packed column's sums: + conj(W[-g]) (W complete after the last barrier)
+#pragma unroll
+  for (int s = 0; s < SI::SLOTS; ++s) {
+    int fl, k2;
+    if (!SI::at(s, 0, &fl, &k2) || fl != 0) continue;
+#pragma unroll
+    for (int k1 = 0; k1 < R1z; ++k1) acc[s][k1] = cadd(acc[s][k1], cconj(W[(L2 - k2 - R2z * k1) & (L2 - 1)]));
   }
   // inverse, column half by column half: z pass 2' from the sums, then z
   // pass 1' into the rows' half spectra
@@ -434,20 +474,21 @@ __global__ void __launch_bounds__(kNTF) k_psf_fwd_fft(const float* __restrict__ 
 }
 
 // --- adjoint -----------------------------------------------------------------
-// grid (nchunk, nb), block NT; LDS: twiddles, A, B, P (boff, poff after A).
+// grid (nchunk, nb), block NT; LDS: twiddles, W, A, B, P (boff, poff after A).
 template <int L1, int L2>
 __global__ void __launch_bounds__(kNTA) k_psf_adj_fft(const float* __restrict__ views, const float* __restrict__ spec,
                                                      int vstride, const int* __restrict__ vids, int Yc,
                                                      float* __restrict__ work, const float* __restrict__ atten,
                                                      long atten_vstride, Geo g, int up, int left, int NPS, int boff,
                                                      int poff, const float2* __restrict__ twg) {
-  constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, R1z = Split<L2>::R1;
-  constexpr int NF = L1 / 2 + 1, NFH = (NF + 1) / 2, NT = kNTA;
+  constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, R1z = Split<L2>::R1, R2z = Split<L2>::R2;
+  constexpr int NF = L1 / 2 + 1, NFH = L1 / 4, NT = kNTA;
   using SI = SpecItems<NT, L1, L2, true>;
   extern __shared__ float2 smf[];
   float2* tw1 = smf;
   float2* tw2 = smf + L1;
-  float2* A = smf + L1 + L2;
+  float2* W = smf + L1 + L2;
+  float2* A = W + L2;
   float2* B = A + boff;
   float2* P = A + poff;
   const int tid = threadIdx.x;
@@ -477,6 +518,9 @@ __global__ void __launch_bounds__(kNTA) k_psf_adj_fft(const float* __restrict__ 
 #pragma unroll
       for (int n1 = 0; n1 < R1z; ++n1) vs[s][n1] = B[(n1 + R1z * k2) * NFH + fl];
       dft<R1z, false>(vs[s]);
+      if (h == 0 && fl == 0)  // the packed column's P^ for its mirror terms
+#pragma unroll
+        for (int k1 = 0; k1 < R1z; ++k1) W[k2 + R2z * k1] = vs[s][k1];
     }
     __syncthreads();
   }
@@ -491,11 +535,19 @@ __global__ void __launch_bounds__(kNTA) k_psf_adj_fft(const float* __restrict__ 
         if (!SI::at(s, h, &fl, &k2)) continue;
         const int f = h * NFH + fl;
         float2 a[R1z];
+        float hm[R1z];  // (loaded by every item, as in the forward)
 #pragma unroll
         for (int k1 = 0; k1 < R1z; ++k1) {
           const float hv = hy[(k1 + R1z * k2) * NF + f];
+          hm[k1] = hy[(k1 + R1z * k2) * NF + L1 / 2];
           a[k1] = make_float2(vs[s][k1].x * hv, vs[s][k1].y * hv);
         }
+        if (f == 0)  // the packed column: + conj(P^[-g]) Hm[g]
+#pragma unroll
+          for (int k1 = 0; k1 < R1z; ++k1) {
+            const float2 m = W[(L2 - k2 - R2z * k1) & (L2 - 1)];
+            a[k1] = make_float2(fmaf(m.x, hm[k1], a[k1].x), fmaf(-m.y, hm[k1], a[k1].y));
+          }
         dft<R1z, true>(a);
 #pragma unroll
         for (int n1 = 0; n1 < R1z; ++n1) B[(n1 + R1z * k2) * NFH + fl] = n1 ? cmulc(a[n1], tw2[n1 * k2]) : a[n1];
@@ -572,6 +624,7 @@ __global__ void __launch_bounds__(kNTA) k_psf_adj_fft(const float* __restrict__ 
 
 // --- kernel spectra ------------------------------------------------------------
 // spec[plane][pos][f] = H[f][g(pos)] / (L1 L2), g(pos) = pos / R1z + R2z (pos % R1z),
+// except columns 0 and L1/2: (H[0] + H[L1/2]) / 2 and (H[0] - H[L1/2]) / 2,
 // for psfs [plane][pz][px] (each plane [px, pz] column-major), in double:
 // C[f][b] = sum_a h[a,b] cos(2 pi f (a - cx) / L1), S likewise with sin, then
 // H = sum_b C cos(2 pi g (b - cz) / L2) - S sin(...).  grid (planes), block 256.
@@ -607,13 +660,20 @@ __global__ void __launch_bounds__(256) k_psf_spectrum(const float* __restrict__ 
   float* out = spec + (long)blockIdx.x * L2 * NF;
   for (int k = threadIdx.x; k < L2 * NF; k += 256) {
     const int f = k % NF, pos = k / NF;
+    if (f == L1 / 2) continue;  // (written with column 0)
     const int gz = pos / R1z + R2z * (pos % R1z);
-    double acc = 0;
+    double acc = 0, accn = 0;
     for (int bb = 0; bb < pz; ++bb) {
       const int m = ((gz * (bb - hz)) % L2 + L2) % L2;
       acc += C[bb * NF + f] * cz[m] - Sn[bb * NF + f] * sz[m];
+      if (f == 0) accn += C[bb * NF + L1 / 2] * cz[m] - Sn[bb * NF + L1 / 2] * sz[m];
     }
-    out[k] = (float)(acc * scale);
+    if (f == 0) {
+      out[k] = (float)(0.5 * (acc + accn) * scale);
+      out[k + L1 / 2] = (float)(0.5 * (acc - accn) * scale);
+    } else {
+      out[k] = (float)(acc * scale);
+    }
   }
 }
 
@@ -674,13 +734,13 @@ bool psf_fft_plan(const Geo& g, FftPlan* fp) {
   f.left = (f.L2 - g.nz + 1) / 2;
   const int NP = (g.nz + 1) / 2;
   f.NPS = NP | 1;  // (odd: rows of A start on different LDS banks)
-  const int NF = f.L1 / 2 + 1, NFH = (NF + 1) / 2;
+  const int NIS = f.L1 / 2 + 1, NFH = f.L1 / 4;
   const int R1z = split_r1(f.L2), R2x = f.L1 / split_r1(f.L1);
-  const long a = std::max((long)f.L1 * f.NPS, (long)g.nz * NF);  // x-buffer / rows' half spectra
+  const long a = std::max((long)f.L1 * f.NPS, (long)g.nz * NIS);  // x-buffer / rows' half spectra
   const long bz = (long)f.L2 * NFH;                              // z-buffer of a column half
   f.boff = (int)a;
   f.poff = (int)(a + bz);
-  f.lds_fwd = sizeof(float2) * (f.L1 + f.L2 + a + bz);
+  f.lds_fwd = sizeof(float2) * (f.L1 + 2L * f.L2 + a + bz);  // twiddles, W, A, B
   f.lds_adj = f.lds_fwd + sizeof(float2) * 2 * R1z * NFH;
   // the x pass 2' reads at most kReadAll items per thread before writing
   if ((long)NP * R2x > (long)kReadAll * kNTF) return false;
