@@ -205,11 +205,18 @@ struct SpecItems {
   static constexpr int R2z = Split<L2>::R2, NFH = L1 / 4;
   static constexpr int PER_HALF = NFH * R2z;
   static constexpr int SLOTS = (2 * PER_HALF + NT - 1) / NT;
-  // false when slot s is not an item of half h
+  // false when slot s is not an item of half h (decided at compile time
+  // when a half is whole slots)
   __device__ static __forceinline__ bool at(int s, int h, int* fl, int* k2) {
-    const int it = item_tid<OPQ>() + s * NT;
-    if (it / PER_HALF != h) return false;
-    const int rem = it - h * PER_HALF;
+    int rem;
+    if constexpr (PER_HALF % NT == 0) {
+      if (s / (PER_HALF / NT) != h) return false;
+      rem = item_tid<OPQ>() + (s - h * (PER_HALF / NT)) * NT;
+    } else {
+      const int it = item_tid<OPQ>() + s * NT;
+      if (it / PER_HALF != h) return false;
+      rem = it - h * PER_HALF;
+    }
     *fl = rem % NFH;
     *k2 = rem / NFH;
     return true;
@@ -367,7 +374,12 @@ __global__ void __launch_bounds__(kNTF) k_psf_fwd_fft(const float* __restrict__ 
                                                      int boff, const float2* __restrict__ twg) {
   constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, R1z = Split<L2>::R1, R2z = Split<L2>::R2;
   constexpr int NF = L1 / 2 + 1, NFH = L1 / 4, NT = kNTF;
-  using SI = SpecItems<NT, L1, L2>;
+  // the opaque item index (as in the adjoint) for the largest planes, where the
+  // hoisted per-slot offsets spill (69 VGPRs at 256 x 128: 0.81 ms per 64-view
+  // launch, 0.45 ms without); smaller planes do not spill and are faster
+  // without it (128 x 64: 92 against 102 us)
+  constexpr bool OPQ = L1 * L2 >= 32768;
+  using SI = SpecItems<NT, L1, L2, OPQ>;
   extern __shared__ float2 smf[];
   float2* tw1 = smf;
   float2* tw2 = smf + L1;
@@ -397,7 +409,7 @@ __global__ void __launch_bounds__(kNTF) k_psf_fwd_fft(const float* __restrict__ 
     });
     const float* hy = spec + (long)(v * vstride + y) * L2 * NF;
     for (int h = 0; h < 2; ++h) {
-      z_pass1_fwd<NT, L1, L2, 0>(A, B, tw2, NPS, nz, left, h);
+      z_pass1_fwd<NT, L1, L2, 0, OPQ>(A, B, tw2, NPS, nz, left, h);
       __syncthreads();
       // z pass 2, times the plane's kernel spectrum, summed over the chunk
 #pragma unroll
