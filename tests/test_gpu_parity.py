@@ -1132,6 +1132,8 @@ FFT_CASES = [
     (16, 24, 9, False),    # 32 x 32
     (96, 64, 25, False),   # 128 x 128
     (128, 80, 37, False),  # 256 x 128: the clinical size with a non-separable PSF
+    (8, 8, 5, False),      # 16 x 16, the smallest planes (a column half is 4 columns)
+    (48, 110, 25, True),   # 128 x 256: the forward's opaque-index form with 16-point z passes
 ]
 
 
