@@ -133,6 +133,12 @@ bool zfast_supported(const Geo& g);
 // padded to *K slots ([view][K][nx*nx]); false when a list exceeds kMaxEll
 constexpr int kMaxEll = 8;
 bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::vector<int2>& ent);
+// the same lists built on the device: the largest list per 256-target block
+// (rot2_list_blocks(g) blocks per view) into blockmax, then, with K their
+// maximum (at least 4), the slots into ent ([view][K][nx*nx])
+int rot2_list_blocks(const Geo& g);
+hipError_t launch_rot2_list_count(const RotView* rv, int nview, const Geo& g, int* blockmax, hipStream_t s);
+hipError_t launch_rot2_list_fill(const RotView* rv, int nview, const Geo& g, int K, int2* ent, hipStream_t s);
 // OS-EM update of the z-fastest image x with the subset normaliser computed
 // in the same pass: asum = sum rot2^T(hvol .* atten[v]) (zeros as Inf, written
 // to asum), x .*= (sum rot2^T(work[b])) ./ asum; hvol null: atten holds the

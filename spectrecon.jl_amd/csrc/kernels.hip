@@ -3804,6 +3804,68 @@ bool build_rot2_lists(const RotView* rv, int nview, const Geo& g, int* K, std::v
   return true;
 }
 
+// The same lists built on the device, target by target: target q of view v
+// takes the sources p whose forward footprint holds q, in increasing p.  The
+// forward samples at A f + b (f = (x + pad + 1, y + pad + 1), A the float32
+// rotation of RotView), and q = (qx, qy) is in the footprint of a sample in
+// [qx + pad, qx + pad + 2) x [qy + pad, qy + pad + 2).  That square mapped
+// back through A (a rotation) lies within sqrt(2) of the preimage of its
+// centre; the window searched is 2.5 pixels about it, and every candidate is
+// tested with bilinear_coef itself, so membership, weights and order are the
+// host builder's.  COUNT: the block's largest list (a block-wide max, no
+// atomics) to blockmax[v * gridDim.x + blockIdx.x]; else the K slots of each
+// target, unused ones (q, 0).  grid (ceil(NN / 256), nview), block 256.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_rot2_lists(const RotView* __restrict__ rv, Geo g, int K,
+                                                    int2* __restrict__ ent, int* __restrict__ blockmax) {
+  __shared__ int wmax[4];
+  const int N = g.nx, NN = N * N, v = blockIdx.y;
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  int n = 0;
+  if (q < NN) {
+    const RotView r = rv[v];
+    const int qx = q % N, qy = q / N;
+    const double sx = qx + g.pad + 1.0 - (double)r.x0, sy = qy + g.pad + 1.0 - (double)r.y0;
+    const double det = (double)r.xx * r.yy - (double)r.xy * r.yx;
+    const double ox = ((double)r.yy * sx - (double)r.xy * sy) / det - g.pad - 1.0;
+    const double oy = (-(double)r.yx * sx + (double)r.xx * sy) / det - g.pad - 1.0;
+    const int x0 = max(0, (int)floor(ox - 2.5)), x1 = min(N - 1, (int)ceil(ox + 2.5));
+    const int y0 = max(0, (int)floor(oy - 2.5)), y1 = min(N - 1, (int)ceil(oy + 2.5));
+    int2* eq = ent + (long)v * K * NN + q;
+    for (int y = y0; y <= y1; ++y)
+      for (int x = x0; x <= x1; ++x) {
+        const BiCoef c = bilinear_coef(r, g.pad, x, y);
+        const int dx = qx - c.j1, dy = qy - c.j2;
+        if ((unsigned)dx > 1u || (unsigned)dy > 1u) continue;
+        const float w = (dx ? c.w12 : c.w11) * (dy ? c.w22 : c.w21);
+        if (w == 0.f) continue;
+        if (!COUNT && n < K) eq[(long)n * NN] = make_int2(y * N + x, __float_as_int(w));
+        ++n;
+      }
+    if (!COUNT)
+      for (int k = n; k < K; ++k) eq[(long)k * NN] = make_int2(q, 0);
+  }
+  if (COUNT) {
+    for (int m = 32; m >= 1; m >>= 1) n = max(n, __shfl_xor(n, m, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      blockmax[v * gridDim.x + blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+  }
+}
+
+int rot2_list_blocks(const Geo& g) { return (int)cdiv((long)g.nx * g.nx, 256); }
+
+hipError_t launch_rot2_list_count(const RotView* rv, int nview, const Geo& g, int* blockmax, hipStream_t s) {
+  k_rot2_lists<true><<<dim3(rot2_list_blocks(g), nview), 256, 0, s>>>(rv, g, 0, nullptr, blockmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_rot2_list_fill(const RotView* rv, int nview, const Geo& g, int K, int2* ent, hipStream_t s) {
+  k_rot2_lists<false><<<dim3(rot2_list_blocks(g), nview), 256, 0, s>>>(rv, g, K, ent, nullptr);
+  return hipGetLastError();
+}
+
 hipError_t launch_to_zfast(const float* img, float* out, const Geo& g, hipStream_t s) {
   k_to_zfast<<<dim3(cdiv(g.nx, 32), g.ny), 256, 0, s>>>(img, out, g);
   return hipGetLastError();

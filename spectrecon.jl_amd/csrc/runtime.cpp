@@ -817,12 +817,35 @@ int sr_plan_create(const sr_plan_desc* desc, const float* mumap, const float* ps
                sr::zfast_supported(p->g);
   }
   if (p->zfast) {
-    std::vector<int2> ent;
-    p->zfast = sr::build_rot2_lists(rv.data(), d.nview, p->g, &p->zk, ent);
-    if (p->zfast) {
-      if ((rc = p->zent.ensure(sizeof(int2) * ent.size())) || (rc = p->zimg.ensure(sizeof(float) * p->V)))
-        return bail(rc);
-      HIP_TRY(hipMemcpy(p->zent.p, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+    const char* lists = std::getenv("SR_LISTS");
+    const long NN = (long)d.nx * d.nx;
+    if (lists && std::strcmp(lists, "host") == 0) {
+      // (the host builder: SR_LISTS=host, for A/B tests of the device one)
+      std::vector<int2> ent;
+      p->zfast = sr::build_rot2_lists(rv.data(), d.nview, p->g, &p->zk, ent);
+      if (p->zfast) {
+        if ((rc = p->zent.ensure(sizeof(int2) * ent.size())) || (rc = p->zimg.ensure(sizeof(float) * p->V)))
+          return bail(rc);
+        HIP_TRY(hipMemcpy(p->zent.p, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+      }
+    } else {
+      // on the device: the longest list (K, at least 4), then the slots
+      const int nblk = sr::rot2_list_blocks(p->g);
+      std::vector<int> bmax((size_t)nblk * d.nview);
+      DBuf dmax;
+      if ((rc = dmax.ensure(sizeof(int) * bmax.size()))) return bail(rc);
+      HIP_TRY(sr::launch_rot2_list_count(p->rotv.as<sr::RotView>(), d.nview, p->g, dmax.as<int>(), p->stream));
+      HIP_TRY(hipMemcpyAsync(bmax.data(), dmax.p, sizeof(int) * bmax.size(), hipMemcpyDeviceToHost, p->stream));
+      HIP_TRY(hipStreamSynchronize(p->stream));
+      p->zk = std::max(4, *std::max_element(bmax.begin(), bmax.end()));
+      p->zfast = p->zk <= sr::kMaxEll;
+      if (p->zfast) {
+        if ((rc = p->zent.ensure(sizeof(int2) * d.nview * p->zk * NN)) ||
+            (rc = p->zimg.ensure(sizeof(float) * p->V)))
+          return bail(rc);
+        HIP_TRY(sr::launch_rot2_list_fill(p->rotv.as<sr::RotView>(), d.nview, p->g, p->zk, p->zent.as<int2>(),
+                                          p->stream));
+      }
     }
   }
 
