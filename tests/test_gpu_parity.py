@@ -943,6 +943,8 @@ def test_autotune_other_plans(gpu, O, monkeypatch):
     (128, 80, 128, None),  # the benchmark geometry
     (96, 32, 60, None),
     (64, 16, 9, "mixed"),  # 0, quarter turns, their neighbours, arbitrary angles
+    (16, 16, 8, "mixed"),  # the smallest z-fastest planes
+    (160, 32, 6, None),    # the largest z-fastest planes in x (the tile-distributed PSF kernels)
 ])
 def test_device_lists_equal_host_lists(gpu, O, monkeypatch, nx, nz, nview, angles):
     """The rotation-adjoint lists built on the device at plan creation equal
