@@ -316,8 +316,9 @@ __device__ __forceinline__ void z_fold(float2* A, const float2* P, int nz, int h
 
 // x pass 2' of the inverse transform from the half spectra I [r][f] in A:
 // row pair j at frequency f is X_2j[f] + i X_2j+1[f] (X_r[f] = conj X_r[L1 - f]
-// above L1/2; X_r[0] and X_r[L1/2] the real and imaginary parts of I[r][0]).  Reads all its items (at most kReadAll per thread), then after
-// a barrier writes the x-buffer over I.
+// above L1/2; X_r[0] and X_r[L1/2] the real and imaginary parts of I[r][0]).
+// Reads all its items (at most kReadAll per thread), then after a barrier
+// writes the x-buffer over I.
 template <int NT, int L1, bool OPQ = false>
 __device__ __forceinline__ void x_pass2_inv(float2* A, const float2* tw1, int NPS, int NP, int nz) {
   constexpr int R1x = Split<L1>::R1, R2x = Split<L1>::R2, NIS = L1 / 2 + 1;
