@@ -6,6 +6,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -3332,6 +3333,24 @@ __global__ void __launch_bounds__(256) k_flag_nonpositive(const float* __restric
 // ===========================================================================
 static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// compute units of the current device (cached per device)
+static int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  int n = cache[dev].load(std::memory_order_relaxed);
+  if (n > 0) return n;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    n = 256;
+  }
+  cache[dev].store(n, std::memory_order_relaxed);
+  return n;
+}
+
 // Strip length of the separable kernels: one strip per thread per phase, at
 // most 1024 threads per block.
 static int sep_strip_len(const Geo& g) {
@@ -3387,8 +3406,19 @@ hipError_t launch_rot_fwd(int interp, const float* img, float* out, const float*
     while (vpg < 8 && (long)g.nz * cdiv(nb, vpg * 2) >= 1024) vpg *= 2;
     const size_t lds = sizeof(float) * S;
     if (g.nx % 4 == 0 && lds <= kMaxLds) {
-      int vpg4 = 1;
-      while (vpg4 < 8 && (long)g.nz * cdiv(nb, vpg4 * 2) >= 512) vpg4 *= 2;
+      // views per block: blocks run in rounds of (CUs x blocks per CU: LDS
+      // and 1024-thread blocks allow two), a block taking about half a view's
+      // time to stage its slice and one per view, so the time goes as
+      // rounds x (2 views + 1); the fewest views per block at that minimum
+      // (at 128x128x80, 64 views: 11 views, one round of 480 blocks, against
+      // 8 views in 1.25 rounds)
+      const long slots = (long)device_cus() * std::max(1L, std::min(2L, (long)(kMaxLds / lds)));
+      int vpg4 = nb;
+      long best = -1;
+      for (int v = 1; v <= nb; ++v) {
+        const long cost = (long)cdiv((long)g.nz * cdiv(nb, v), slots) * (2L * v + 1);
+        if (best < 0 || cost < best) best = cost, vpg4 = v;
+      }
       allow_lds(k_rot2_fwd4, lds);
       k_rot2_fwd4<<<dim3(g.nz, cdiv(nb, vpg4)), R2F_THREADS, lds, s>>>(img, out, atten, atten_vstride, rv, vids, nb,
                                                                        vpg4, g);
