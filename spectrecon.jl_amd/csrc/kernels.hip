@@ -3276,17 +3276,32 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__
 // E_y = g_y * w_y (g = conv^T(r), w = R(x) .* atten, z-fastest [b][y][x][z]),
 // dL/dmu_r[j] = -dy (sum_{y>j} E_y + E_j / 2), a reverse scan along y per
 // (x, z) column.  grid (ceil(nx*nz / 256), nb), block 256
-__global__ void __launch_bounds__(256) k_mu_grad_scan(float* __restrict__ g, const float* __restrict__ w, Geo geo,
-                                                      float dy) {
+// ATT: w holds the rotated image R(x) alone and the view's attenuation
+// factors atten[vids[b]] are read here: e = g (w a) (the same products as
+// with w = R(x) .* a), and g .* a, the attenuated PSF adjoint that the image
+// gradient gathers, is written over w.
+template <bool ATT>
+__global__ void __launch_bounds__(256) k_mu_grad_scan(float* __restrict__ g, float* __restrict__ w, Geo geo,
+                                                      float dy, const float* __restrict__ atten, long atten_vstride,
+                                                      const int* __restrict__ vids) {
   const long cols = (long)geo.nx * geo.nz, c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= cols) return;
   const long V = cols * geo.ny;
   float* gb = g + V * blockIdx.y + c;
-  const float* wb = w + V * blockIdx.y + c;
+  float* wb = w + V * blockIdx.y + c;
+  const float* ab = ATT ? atten + atten_vstride * vids[blockIdx.y] + c : nullptr;
   const float mdy = -dy;
   float acc = 0.f;
   for (int y = geo.ny - 1; y >= 0; --y) {
-    const float e = gb[y * cols] * wb[y * cols];
+    const float gv = gb[y * cols];
+    float e;
+    if constexpr (ATT) {
+      const float a = ab[y * cols];
+      e = gv * (wb[y * cols] * a);
+      wb[y * cols] = gv * a;
+    } else {
+      e = gv * wb[y * cols];
+    }
     gb[y * cols] = mdy * (acc + 0.5f * e);
     acc = acc + e;
   }
@@ -4146,7 +4161,16 @@ hipError_t launch_resid_loss(const float* p, const float* y, float* r, long n, d
 
 hipError_t launch_mu_grad_scan(float* g, const float* w, int nb, const Geo& geo, float dy, hipStream_t s) {
   if (nb <= 0) return hipSuccess;
-  k_mu_grad_scan<<<dim3(cdiv((long)geo.nx * geo.nz, 256), nb), 256, 0, s>>>(g, w, geo, dy);
+  k_mu_grad_scan<false><<<dim3(cdiv((long)geo.nx * geo.nz, 256), nb), 256, 0, s>>>(g, const_cast<float*>(w), geo, dy,
+                                                                                nullptr, 0, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_mu_grad_scan_att(float* g, float* w, const float* atten, long atten_vstride, const int* vids, int nb,
+                                   const Geo& geo, float dy, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  k_mu_grad_scan<true><<<dim3(cdiv((long)geo.nx * geo.nz, 256), nb), 256, 0, s>>>(g, w, geo, dy, atten, atten_vstride,
+                                                                               vids);
   return hipGetLastError();
 }
 

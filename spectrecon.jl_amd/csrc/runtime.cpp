@@ -1325,9 +1325,41 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
   // r = 2 (A x - y), L = sum (A x - y)^2; the projection leaves x's z-fastest copy in zimg
   if ((rc = run_project(p, r, d_img, d_vids, nindex, nullptr, nullptr))) return rc;
   HIP_TRY(sr::launch_resid_loss(r, d_y, r, p->P * nindex, p->gpart.as<double>(), p->gloss.as<double>(), p->stream));
-  // dL/dx = A' r
-  if ((rc = run_backproject(p, d_gx, r, d_vids, nindex, sr::IMG_WRITE, nullptr))) return rc;
-  if ((rc = run_mu_vjp(p, d_gmu, r, d_vids, nindex))) return rc;
+  if (nindex == 0) {
+    HIP_TRY(sr::launch_fill(d_gx, 0.f, p->V, p->stream));
+    HIP_TRY(sr::launch_fill(d_gmu, 0.f, p->V, p->stream));
+  }
+  // both gradients from one PSF adjoint per batch: G = conv^T(r) (no
+  // attenuation); the scan forms E = G .* R(x) .* a for dL/dmu and leaves
+  // G .* a, the back-projector's work volume, for dL/dx = A' r
+  const sr::Taps t = p->taps();
+  for (int s0 = 0; s0 < nindex; s0 += p->max_batch) {
+    const int nb = std::min(p->max_batch, nindex - s0);
+    if ((rc = ensure_batch_buffers(p, nb))) return rc;
+    const int m = s0 == 0 ? sr::IMG_WRITE : sr::IMG_ACCUM;
+    {
+      KTimer kt(p, sr::K_PSF_ADJ);
+      HIP_TRY(sr::launch_psf_adj(p->separable, r + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(), nullptr,
+                                 0, p->nchunk_adj(nb), p->g, p->stream, true, p->fam_adj()));
+    }
+    {
+      KTimer kt(p, sr::K_ROT_FWD);
+      HIP_TRY(sr::launch_rot2_fwd_z(p->zimg.as<float>(), p->gwork2.as<float>(), nullptr, 0, p->rotv.as<sr::RotView>(),
+                                    d_vids + s0, nb, p->g, p->stream));
+    }
+    HIP_TRY(sr::launch_mu_grad_scan_att(p->work.as<float>(), p->gwork2.as<float>(), p->atten.as<float>(),
+                                        p->atten_vstride(), d_vids + s0, nb, p->g, p->d.dy, p->stream));
+    for (int k = 0; k < 2; ++k) {  // dL/dx from G .* a, then dL/dmu from the scan
+      {
+        KTimer kt(p, sr::K_ROT_ADJ);
+        HIP_TRY(sr::launch_rot2_adj_z(k ? p->work.as<float>() : p->gwork2.as<float>(), p->ipart.as<float>(),
+                                      p->zent.as<int2>(), p->zk, d_vids + s0, nb, p->g, p->stream, nullptr, 0,
+                                      nullptr, false, p->ell_vs()));
+      }
+      KTimer kt(p, sr::K_IMG_REDUCE);
+      HIP_TRY(sr::launch_img_reduce_z(p->ipart.as<float>(), 1, k ? d_gmu : d_gx, nullptr, m, p->g, p->stream));
+    }
+  }
   // (no view: the residual kernel writes no loss, and the sum over no views is 0)
   if (nindex > 0) HIP_TRY(hipMemcpyAsync(loss, p->gloss.p, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   else *loss = 0.0;

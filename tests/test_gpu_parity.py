@@ -758,6 +758,30 @@ def test_named_adjoint_entry_points(gpu):
     assert np.abs(c).sum() > 0
 
 
+@pytest.mark.parametrize("nview", [40, 7])
+def test_gradient_equals_operators(gpu, O, nview):
+    """sr_grad takes both gradients from one PSF adjoint per launch batch (40
+    views: two batches); they equal the separate operators bit for bit:
+    dL/dx = backproject(r) and dL/dmu = project_vjp_mu(x, r) with
+    r = 2 (A x - y), for all views and a view subset."""
+    nx, nz, dy = 32, 16, 0.48
+    rng = np.random.default_rng(97)
+    mumap = np.asfortranarray((0.2 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = np.asfortranarray(np.repeat(np.asarray(O.psf_gauss(nx, 7, 7, 1.0, 3.0), np.float32)[:, :, :, None],
+                                       nview, 3))
+    plan = gpu.SPECTplan(mumap, psfs, dy)
+    assert plan.info()["layout"] == "z-fastest"
+    x = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    y = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    for idx in (None, [nview - 1, 0, 3, 2]):
+        yi = y if idx is None else np.asfortranarray(y[:, :, idx])
+        loss, gx, gmu = gpu.gradient(x, yi, plan, index=idx)
+        r = np.asfortranarray((2 * (gpu.project(x, plan, index=idx) - yi)).astype(np.float32))
+        np.testing.assert_array_equal(gx, gpu.backproject(r, plan, index=idx))
+        np.testing.assert_array_equal(gmu, gpu.project_vjp_mu(x, r, plan, index=idx))
+        assert abs(loss - 0.25 * float(np.sum(r.astype(np.float64) ** 2))) <= 1e-6 * loss
+
+
 def test_gradient_empty_index(gpu, O):
     """sr_grad over no view: the loss is 0 and both gradients are zero."""
     nx, nz, nview = 32, 16, 4
