@@ -3292,17 +3292,42 @@ __global__ void __launch_bounds__(256) k_mu_grad_scan(float* __restrict__ g, flo
   const float* ab = ATT ? atten + atten_vstride * vids[blockIdx.y] + c : nullptr;
   const float mdy = -dy;
   float acc = 0.f;
-  for (int y = geo.ny - 1; y >= 0; --y) {
-    const float gv = gb[y * cols];
+  // y from the far side in runs of U whose loads are issued together (the
+  // walk only carries acc); the sums in the order of a one-by-one walk
+  constexpr int U = 8;
+  int y = geo.ny - 1;
+  for (; y >= U - 1; y -= U) {
+    float gv[U], wv[U], av[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      gv[u] = gb[(long)(y - u) * cols];
+      wv[u] = wb[(long)(y - u) * cols];
+      if constexpr (ATT) av[u] = ab[(long)(y - u) * cols];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      float e;
+      if constexpr (ATT) {
+        e = gv[u] * (wv[u] * av[u]);
+        wb[(long)(y - u) * cols] = gv[u] * av[u];
+      } else {
+        e = gv[u] * wv[u];
+      }
+      gb[(long)(y - u) * cols] = mdy * (acc + 0.5f * e);
+      acc = acc + e;
+    }
+  }
+  for (; y >= 0; --y) {
+    const float gv = gb[(long)y * cols];
     float e;
     if constexpr (ATT) {
-      const float a = ab[y * cols];
-      e = gv * (wb[y * cols] * a);
-      wb[y * cols] = gv * a;
+      const float a = ab[(long)y * cols];
+      e = gv * (wb[(long)y * cols] * a);
+      wb[(long)y * cols] = gv * a;
     } else {
-      e = gv * wb[y * cols];
+      e = gv * wb[(long)y * cols];
     }
-    gb[y * cols] = mdy * (acc + 0.5f * e);
+    gb[(long)y * cols] = mdy * (acc + 0.5f * e);
     acc = acc + e;
   }
 }
