@@ -178,12 +178,12 @@ hipError_t launch_img_reduce_z(const float* part, int ngroups, float* out, const
 hipError_t launch_resid_loss(const float* p, const float* y, float* r, long n, double* partial, double* out,
                              hipStream_t s);
 // mu-gradient in the rotated frame, in place over g ([b][y][x][z]):
-// g[j] = -dy (sum_{y>j} g_y w_y + g_j w_j / 2)
-hipError_t launch_mu_grad_scan(float* g, const float* w, int nb, const Geo& geo, float dy, hipStream_t s);
-// the same with w = R(x) alone and the factors w_y = R(x)_y a_y formed here
-// from atten[vids[b]]; w then receives g .* a (the attenuated PSF adjoint)
-hipError_t launch_mu_grad_scan_att(float* g, float* w, const float* atten, long atten_vstride, const int* vids, int nb,
-                                   const Geo& geo, float dy, hipStream_t s);
+// g[j] = -dy (sum_{y>j} g_y w_y + g_j w_j / 2) with w_y = R(x)_y a_y formed
+// on the fly (R(x) the :two rotation of the z-fastest image img_t, as
+// launch_rot2_fwd_z; a = atten[vids[b]]); ga (optional) receives g .* a
+hipError_t launch_mu_grad_scan_rot(float* g, float* ga, const float* img_t, const float* atten, long atten_vstride,
+                                   const RotView* rv, const int* vids, int nb, const Geo& geo, float dy,
+                                   hipStream_t s);
 // *flag = 0, then 1 if some p[k] is not > 0
 hipError_t launch_flag_nonpositive(const float* p, long n, int* flag, hipStream_t s);
 

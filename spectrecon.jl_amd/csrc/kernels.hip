@@ -3276,24 +3276,51 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__
 // E_y = g_y * w_y (g = conv^T(r), w = R(x) .* atten, z-fastest [b][y][x][z]),
 // dL/dmu_r[j] = -dy (sum_{y>j} E_y + E_j / 2), a reverse scan along y per
 // (x, z) column.  grid (ceil(nx*nz / 256), nb), block 256
-// ATT: w holds the rotated image R(x) alone and the view's attenuation
-// factors atten[vids[b]] are read here: e = g (w a) (the same products as
-// with w = R(x) .* a), and g .* a, the attenuated PSF adjoint that the image
-// gradient gathers, is written over w.
-template <bool ATT>
-__global__ void __launch_bounds__(256) k_mu_grad_scan(float* __restrict__ g, float* __restrict__ w, Geo geo,
-                                                      float dy, const float* __restrict__ atten, long atten_vstride,
-                                                      const int* __restrict__ vids) {
-  const long cols = (long)geo.nx * geo.nz, c = (long)blockIdx.x * 256 + threadIdx.x;
+// The same scan with the rotated image formed on the fly instead of read:
+// w_y = R(x)_y a_y, R(x) the :two rotation of the z-fastest image img_t at
+// the view's coefficients (the expression, masks and clamps of k_rot2_fwd_z,
+// one z element per thread), a the view's attenuation factors.  GA: g .* a,
+// the attenuated PSF adjoint that the image gradient gathers, into ga.  y
+// from the far side in runs of U whose loads are issued together; the sums
+// in the order of a one-by-one walk.  grid (ceil(nx nz / 256), nb), block 256.
+template <bool GA>
+__global__ void __launch_bounds__(256) k_mu_grad_scan_rot(float* __restrict__ g, float* __restrict__ ga,
+                                                          const float* __restrict__ img_t,
+                                                          const float* __restrict__ atten, long atten_vstride,
+                                                          const RotView* __restrict__ rv,
+                                                          const int* __restrict__ vids, Geo geo, float dy) {
+  const int N = geo.nx, nz = geo.nz;
+  const long cols = (long)N * nz, c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= cols) return;
+  const int x = (int)(c / nz), z = (int)(c - (long)x * nz);
   const long V = cols * geo.ny;
+  const int v = vids[blockIdx.y];
+  const RotView r = rv[v];
   float* gb = g + V * blockIdx.y + c;
-  float* wb = w + V * blockIdx.y + c;
-  const float* ab = ATT ? atten + atten_vstride * vids[blockIdx.y] + c : nullptr;
+  float* gab = GA ? ga + V * blockIdx.y + c : nullptr;
+  const float* ab = atten + atten_vstride * v + c;
+  const float* im = img_t + z;  // element z of pixel p at im[p nz]
+  auto rot = [&](int y) -> float {
+    if (r.kind == 0) return im[(long)(y * N + x) * nz];
+    const BiCoef cf = bilinear_coef(r, geo.pad, x, y);
+    const bool x0 = (unsigned)cf.j1 < (unsigned)N, x1 = (unsigned)(cf.j1 + 1) < (unsigned)N;
+    const bool y0 = (unsigned)cf.j2 < (unsigned)N, y1 = (unsigned)(cf.j2 + 1) < (unsigned)N;
+    const long ja = clampi(cf.j1, 0, N - 1), jb = clampi(cf.j1 + 1, 0, N - 1);
+    const long ra = (long)clampi(cf.j2, 0, N - 1) * N, rb = (long)clampi(cf.j2 + 1, 0, N - 1) * N;
+    const float s11 = (x0 && y0) ? im[(ra + ja) * nz] : 0.f;
+    const float s12 = (x1 && y0) ? im[(ra + jb) * nz] : 0.f;
+    const float s21 = (x0 && y1) ? im[(rb + ja) * nz] : 0.f;
+    const float s22 = (x1 && y1) ? im[(rb + jb) * nz] : 0.f;
+    return (s11 * cf.w11 + s12 * cf.w12) * cf.w21 + (s21 * cf.w11 + s22 * cf.w12) * cf.w22;
+  };
   const float mdy = -dy;
   float acc = 0.f;
-  // y from the far side in runs of U whose loads are issued together (the
-  // walk only carries acc); the sums in the order of a one-by-one walk
+  auto step = [&](long o, float gv, float rv_, float av) {
+    const float e = gv * (rv_ * av);
+    if constexpr (GA) gab[o] = gv * av;
+    gb[o] = mdy * (acc + 0.5f * e);
+    acc = acc + e;
+  };
   constexpr int U = 8;
   int y = geo.ny - 1;
   for (; y >= U - 1; y -= U) {
@@ -3301,35 +3328,13 @@ __global__ void __launch_bounds__(256) k_mu_grad_scan(float* __restrict__ g, flo
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       gv[u] = gb[(long)(y - u) * cols];
-      wv[u] = wb[(long)(y - u) * cols];
-      if constexpr (ATT) av[u] = ab[(long)(y - u) * cols];
+      av[u] = ab[(long)(y - u) * cols];
+      wv[u] = rot(y - u);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      float e;
-      if constexpr (ATT) {
-        e = gv[u] * (wv[u] * av[u]);
-        wb[(long)(y - u) * cols] = gv[u] * av[u];
-      } else {
-        e = gv[u] * wv[u];
-      }
-      gb[(long)(y - u) * cols] = mdy * (acc + 0.5f * e);
-      acc = acc + e;
-    }
+    for (int u = 0; u < U; ++u) step((long)(y - u) * cols, gv[u], wv[u], av[u]);
   }
-  for (; y >= 0; --y) {
-    const float gv = gb[(long)y * cols];
-    float e;
-    if constexpr (ATT) {
-      const float a = ab[(long)y * cols];
-      e = gv * (wb[(long)y * cols] * a);
-      wb[(long)y * cols] = gv * a;
-    } else {
-      e = gv * wb[(long)y * cols];
-    }
-    gb[(long)y * cols] = mdy * (acc + 0.5f * e);
-    acc = acc + e;
-  }
+  for (; y >= 0; --y) step((long)y * cols, gb[(long)y * cols], rot(y), ab[(long)y * cols]);
 }
 
 // ===========================================================================
@@ -4184,18 +4189,15 @@ hipError_t launch_resid_loss(const float* p, const float* y, float* r, long n, d
   return hipGetLastError();
 }
 
-hipError_t launch_mu_grad_scan(float* g, const float* w, int nb, const Geo& geo, float dy, hipStream_t s) {
+hipError_t launch_mu_grad_scan_rot(float* g, float* ga, const float* img_t, const float* atten, long atten_vstride,
+                                   const RotView* rv, const int* vids, int nb, const Geo& geo, float dy,
+                                   hipStream_t s) {
   if (nb <= 0) return hipSuccess;
-  k_mu_grad_scan<false><<<dim3(cdiv((long)geo.nx * geo.nz, 256), nb), 256, 0, s>>>(g, const_cast<float*>(w), geo, dy,
-                                                                                nullptr, 0, nullptr);
-  return hipGetLastError();
-}
-
-hipError_t launch_mu_grad_scan_att(float* g, float* w, const float* atten, long atten_vstride, const int* vids, int nb,
-                                   const Geo& geo, float dy, hipStream_t s) {
-  if (nb <= 0) return hipSuccess;
-  k_mu_grad_scan<true><<<dim3(cdiv((long)geo.nx * geo.nz, 256), nb), 256, 0, s>>>(g, w, geo, dy, atten, atten_vstride,
-                                                                               vids);
+  const dim3 grid(cdiv((long)geo.nx * geo.nz, 256), nb);
+  if (ga)
+    k_mu_grad_scan_rot<true><<<grid, 256, 0, s>>>(g, ga, img_t, atten, atten_vstride, rv, vids, geo, dy);
+  else
+    k_mu_grad_scan_rot<false><<<grid, 256, 0, s>>>(g, nullptr, img_t, atten, atten_vstride, rv, vids, geo, dy);
   return hipGetLastError();
 }
 

@@ -242,7 +242,7 @@ struct sr_plan_s {
   // the PSF adjoint family H was computed with (-1: not built)
   DBuf atten_h;
   int atten_h_fam = -1;
-  DBuf gview, gwork2, gpart, gloss;  // sr_grad: residual views, R(x) .* atten, loss partial sums
+  DBuf gview, gwork2, gpart, gloss;  // sr_grad: residual views, G .* atten, loss partial sums
   int zk = 0;        // slots per target of zent
   std::vector<int> h_vids, h_order;
 
@@ -1330,8 +1330,9 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
     HIP_TRY(sr::launch_fill(d_gmu, 0.f, p->V, p->stream));
   }
   // both gradients from one PSF adjoint per batch: G = conv^T(r) (no
-  // attenuation); the scan forms E = G .* R(x) .* a for dL/dmu and leaves
-  // G .* a, the back-projector's work volume, for dL/dx = A' r
+  // attenuation); the scan forms E = G .* (R(x) .* a) for dL/dmu, rotating x
+  // on the fly, and leaves G .* a, the back-projector's work volume, for
+  // dL/dx = A' r
   const sr::Taps t = p->taps();
   for (int s0 = 0; s0 < nindex; s0 += p->max_batch) {
     const int nb = std::min(p->max_batch, nindex - s0);
@@ -1342,13 +1343,12 @@ int sr_grad(sr_plan p, double* loss, float* grad_x, float* grad_mu, const float*
       HIP_TRY(sr::launch_psf_adj(p->separable, r + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(), nullptr,
                                  0, p->nchunk_adj(nb), p->g, p->stream, true, p->fam_adj()));
     }
-    {
+    {  // (the rotation of x formed inside the scan, from zimg)
       KTimer kt(p, sr::K_ROT_FWD);
-      HIP_TRY(sr::launch_rot2_fwd_z(p->zimg.as<float>(), p->gwork2.as<float>(), nullptr, 0, p->rotv.as<sr::RotView>(),
-                                    d_vids + s0, nb, p->g, p->stream));
+      HIP_TRY(sr::launch_mu_grad_scan_rot(p->work.as<float>(), p->gwork2.as<float>(), p->zimg.as<float>(),
+                                          p->atten.as<float>(), p->atten_vstride(), p->rotv.as<sr::RotView>(),
+                                          d_vids + s0, nb, p->g, p->d.dy, p->stream));
     }
-    HIP_TRY(sr::launch_mu_grad_scan_att(p->work.as<float>(), p->gwork2.as<float>(), p->atten.as<float>(),
-                                        p->atten_vstride(), d_vids + s0, nb, p->g, p->d.dy, p->stream));
     for (int k = 0; k < 2; ++k) {  // dL/dx from G .* a, then dL/dmu from the scan
       {
         KTimer kt(p, sr::K_ROT_ADJ);
@@ -1397,7 +1397,6 @@ int sr_project_vjp_mu(sr_plan p, float* grad_mu, const float* image, const float
     d_dy = p->st_aux1.as<float>();
     d_gmu = p->st_aux2.as<float>();
   }
-  if ((rc = p->gwork2.ensure(sizeof(float) * p->V * std::min<long>(std::max(1, nindex), p->max_batch)))) return rc;
   {  // the image's z-fastest copy, which the rotation of x reads
     KTimer kt(p, sr::K_ROT_FWD);
     HIP_TRY(sr::launch_to_zfast(d_img, p->zimg.as<float>(), p->g, p->stream));
@@ -1422,12 +1421,12 @@ int run_mu_vjp(sr_plan p, float* d_gmu, const float* r, const int* d_vids, int n
       HIP_TRY(sr::launch_psf_adj(p->separable, r + (long)s0 * p->P, t, d_vids + s0, nb, p->work.as<float>(), nullptr,
                                  0, p->nchunk_adj(nb), p->g, p->stream, true, p->fam_adj()));
     }
-    {
+    {  // (the rotation of x formed inside the scan, from zimg)
       KTimer kt(p, sr::K_ROT_FWD);
-      HIP_TRY(sr::launch_rot2_fwd_z(p->zimg.as<float>(), p->gwork2.as<float>(), p->atten.as<float>(),
-                                    p->atten_vstride(), p->rotv.as<sr::RotView>(), d_vids + s0, nb, p->g, p->stream));
+      HIP_TRY(sr::launch_mu_grad_scan_rot(p->work.as<float>(), nullptr, p->zimg.as<float>(), p->atten.as<float>(),
+                                          p->atten_vstride(), p->rotv.as<sr::RotView>(), d_vids + s0, nb, p->g,
+                                          p->d.dy, p->stream));
     }
-    HIP_TRY(sr::launch_mu_grad_scan(p->work.as<float>(), p->gwork2.as<float>(), nb, p->g, p->d.dy, p->stream));
     {
       KTimer kt(p, sr::K_ROT_ADJ);
       HIP_TRY(sr::launch_rot2_adj_z(p->work.as<float>(), p->ipart.as<float>(), p->zent.as<int2>(), p->zk,
