@@ -273,9 +273,11 @@ int sr_plan_autotune(sr_plan plan, int32_t nsub, char* log, size_t log_bytes);
  * L = sum over the indexed views of ||A(x, mu) - y||^2 with mu the plan's
  * mu-map, *loss = L, grad_x = dL/dx = A'(2 (A x - y)) and grad_mu = dL/dmu
  * through the rotated exponential cumulative sum of the attenuation.  y holds
- * the indexed views in order ([nx,nz,nindex]).  Needs a z-fastest plan
- * (sr_plan_layout); SR_ERR_UNSUPPORTED otherwise.  `loss` is a host pointer;
- * the call returns after the stream finished. */
+ * the indexed views in order ([nx,nz,nindex]).  grad_x and grad_mu equal
+ * sr_backproject and sr_project_vjp_mu of the residual r = 2 (A x - y) bit
+ * for bit (both come from one PSF adjoint of r per launch batch).  Needs a
+ * z-fastest plan (sr_plan_layout); SR_ERR_UNSUPPORTED otherwise.  `loss` is a
+ * host pointer; the call returns after the stream finished. */
 int sr_grad(sr_plan plan, double* loss, float* grad_x, float* grad_mu, const float* image, const float* y,
             const int32_t* index, int32_t nindex, int32_t mem);
 /* The mu vector-Jacobian product of the projector (the suggested
