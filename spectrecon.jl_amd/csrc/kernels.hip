@@ -217,12 +217,16 @@ __global__ void __launch_bounds__(R2F_THREADS) k_rot2_fwd4(const float* __restri
 constexpr int RA_THREADS = 1024;
 constexpr int RA_MAXP = 16;  // pixels per thread and pass (one pass covers N <= 128)
 
+// kLds: the view's work slice staged in LDS (row stride N + 1, odd: rows fall
+// in different banks); else read where it lies (row stride N), for slices
+// larger than LDS (N >= 202); the sums are the same either way.
+template <bool kLds>
 __global__ void __launch_bounds__(RA_THREADS) k_rot2_adj(const float* __restrict__ work, float* __restrict__ part,
                                                          const RotView* __restrict__ rv,
                                                          const int* __restrict__ vids, int nb, int vpg, Geo g) {
-  extern __shared__ float sp[];  // the work slice, row stride N+1 (odd: rows fall in different banks)
+  extern __shared__ float sp[];
   const int N = g.nx, pad = g.pad, z = blockIdx.x, grp = blockIdx.y;
-  const int NN = N * N, SP = N + 1;
+  const int NN = N * N, SP = kLds ? N + 1 : N;
   const long S = (long)NN, V = S * g.nz;
   const int tid = threadIdx.x;
   const int b0 = grp * vpg, b1 = min(nb, b0 + vpg);
@@ -234,23 +238,26 @@ __global__ void __launch_bounds__(RA_THREADS) k_rot2_adj(const float* __restrict
     for (int b = b0; b < b1; ++b) {
       const RotView r = rv[vids[b]];
       const float* src = work + V * b + S * z;
-      __syncthreads();  // previous view's reads of sp are done
-      for (int k0 = 0; k0 < NN; k0 += RA_THREADS * 8) {
-        float tv[8];
-        int dst[8];
-        RowIter it(k0 + tid, RA_THREADS, N);
+      if constexpr (kLds) {
+        __syncthreads();  // previous view's reads of sp are done
+        for (int k0 = 0; k0 < NN; k0 += RA_THREADS * 8) {
+          float tv[8];
+          int dst[8];
+          RowIter it(k0 + tid, RA_THREADS, N);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int k = k0 + tid + e * RA_THREADS;
-          tv[e] = k < NN ? src[k] : 0.f;
-          dst[e] = k < NN ? it.row * SP + it.col : -1;
-          it.next();
+          for (int e = 0; e < 8; ++e) {
+            const int k = k0 + tid + e * RA_THREADS;
+            tv[e] = k < NN ? src[k] : 0.f;
+            dst[e] = k < NN ? it.row * SP + it.col : -1;
+            it.next();
+          }
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            if (dst[e] >= 0) sp[dst[e]] = tv[e];
         }
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-          if (dst[e] >= 0) sp[dst[e]] = tv[e];
+        __syncthreads();
       }
-      __syncthreads();
+      const float* sl = kLds ? sp : src;  // (the slice, row stride SP)
       RowIter it(p0 + tid, RA_THREADS, N);
 #pragma unroll
       for (int k = 0; k < RA_MAXP; ++k) {
@@ -288,7 +295,7 @@ __global__ void __launch_bounds__(RA_THREADS) k_rot2_adj(const float* __restrict
               const float x2 = (C[a] + D[bb]) + r.y0;
               const float w1 = fmaxf(0.f, 1.f - fabsf(x1 - J1));
               const float w2 = fmaxf(0.f, 1.f - fabsf(x2 - J2));
-              const float pv = (vi[a] && vj[bb]) ? sp[ci[a] + cj[bb]] : 0.f;
+              const float pv = (vi[a] && vj[bb]) ? sl[ci[a] + cj[bb]] : 0.f;
               sum = sum + (w1 * w2) * pv;
             }
           acc[k] = sum;
@@ -3504,9 +3511,12 @@ hipError_t launch_rot_adj(int interp, const float* work, float* part, const RotV
       k_rot2_adj_scatter<<<dim3(g.nz, ng), RS_THREADS, lds, s>>>(work, part, rv, vids, nb, vpg, g);
     } else {
       const size_t lds = sizeof(float) * g.nx * (g.nx + 1);
-      if (lds > kMaxLds) return hipErrorInvalidConfiguration;
-      allow_lds(k_rot2_adj, lds);
-      k_rot2_adj<<<dim3(g.nz, ng), RA_THREADS, lds, s>>>(work, part, rv, vids, nb, vpg, g);
+      if (lds <= kMaxLds) {
+        allow_lds(k_rot2_adj<true>, lds);
+        k_rot2_adj<true><<<dim3(g.nz, ng), RA_THREADS, lds, s>>>(work, part, rv, vids, nb, vpg, g);
+      } else {
+        k_rot2_adj<false><<<dim3(g.nz, ng), RA_THREADS, 0, s>>>(work, part, rv, vids, nb, vpg, g);
+      }
     }
   }
   return hipGetLastError();

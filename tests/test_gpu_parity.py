@@ -237,6 +237,30 @@ def small_problem(seed, nx=16, nz=8, nview=8, px=5, pz=3, separable=False):
     return mumap, psfs, image
 
 
+@pytest.mark.parametrize("nx", [144, 210])
+@pytest.mark.parametrize("separable", [False, True])
+def test_large_slices_vs_oracle(gpu, O, nx, separable):
+    """:two projection and back-projection of wide slices (x-fastest: the
+    gather adjoint for nx > 128; at 210 the slice no longer fits LDS, so the
+    forward rotation and the adjoint read it where it lies) against the FP32
+    oracle, and the adjoint identity."""
+    nz, nview = 4, 3
+    mumap, psfs, image = small_problem(17, nx, nz, nview, 5, 5 if separable else 3, separable)
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    assert plan.info()["layout"] == "x-fastest"
+    ang = angles32(nview).astype(np.float64)
+    P32 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float32, "fft")
+    g = gpu.project(image, plan)
+    assert per_view_maxrel(g, P32.project(image)) <= PROJ_TOL
+    rng = np.random.default_rng(nx)
+    views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    gb = gpu.backproject(views, plan)
+    assert maxrel(gb, P32.backproject(views)) <= PROJ_TOL
+    a = float(np.vdot(views.astype(np.float64), g.astype(np.float64)))
+    b = float(np.vdot(image.astype(np.float64), gb.astype(np.float64)))
+    assert abs(a - b) <= 1e-5 * abs(a)
+
+
 @pytest.mark.parametrize("method", ["one", "two"])
 @pytest.mark.parametrize("mode", ["fast", "mem"])
 @pytest.mark.parametrize("separable", [False, True])
