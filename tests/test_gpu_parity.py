@@ -261,6 +261,48 @@ def test_large_slices_vs_oracle(gpu, O, nx, separable):
     assert abs(a - b) <= 1e-5 * abs(a)
 
 
+@pytest.mark.parametrize("nx,nz,nview", [(128, 128, 4), (256, 64, 3), (192, 96, 3)])
+def test_large_volumes_vs_oracle(gpu, O, nx, nz, nview):
+    """Volumes beyond the z-fastest path's planes (x-fastest: the matrix-core
+    PSF kernels on the 128x128 and 192x96 planes, 256-wide slices beyond LDS)
+    with a 15x15 Gaussian PSF: projection and back-projection against the
+    oracle's FP32 direct convolution (its FP32 FFT restatement is itself up
+    to 8e-5 from FP64 at the edges of 128-deep planes) and the FP64 oracle,
+    the adjoint identity, and an OS-EM pass that stays finite and
+    non-negative."""
+    rng = np.random.default_rng(nx + nz)
+    mumap = np.asfortranarray((0.02 * rng.random((nx, nx, nz))).astype(np.float32))
+    psfs = np.asfortranarray(np.repeat(np.asarray(O.psf_gauss(nx, 15, 15, 1.0, 4.0), np.float32)[:, :, :, None],
+                                       nview, 3))
+    print(f"large: {nx}x{nx}x{nz}, {nview} views", flush=True)
+    plan = gpu.SPECTplan(mumap, psfs, 0.48)
+    assert plan.info()["layout"] == "x-fastest"
+    image = np.asfortranarray(rng.random((nx, nx, nz), dtype=np.float32))
+    ang = angles32(nview).astype(np.float64)
+    P32 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float32, "direct", nthread=16)
+    P64 = O.Problem(mumap, psfs, 0.48, ang, "two", np.float64, "direct", nthread=16)
+    # (against FP64 the bar is 1e-4: the :two sample coordinates are float32,
+    # about 3e-5 of a pixel apart near 256, so the bilinear weights of wide
+    # slices carry that error in the reference's arithmetic as in this one;
+    # the oracle's FP32 back-projection is itself 3.3e-5 from FP64 at
+    # 256x256x64 and 2.1e-5 at 192x192x96, with mu = 0 as well)
+    g = gpu.project(image, plan)
+    assert per_view_maxrel(g, P32.project(image)) <= PROJ_TOL
+    e64 = per_view_maxrel(g, P64.project(image.astype(np.float64)))
+    views = np.asfortranarray(rng.random((nx, nz, nview), dtype=np.float32))
+    gb = gpu.backproject(views, plan)
+    assert maxrel(gb, P32.backproject(views)) <= PROJ_TOL
+    e64b = maxrel(gb, P64.backproject(views.astype(np.float64)))
+    print(f"large: vs FP64 project {e64:.2e}, back-project {e64b:.2e}", flush=True)
+    assert e64 <= 1e-4 and e64b <= 1e-4
+    a = float(np.vdot(views.astype(np.float64), g.astype(np.float64)))
+    b = float(np.vdot(image.astype(np.float64), gb.astype(np.float64)))
+    assert abs(a - b) <= 1e-5 * abs(a)
+    y = g + 0.1
+    xh = gpu.osem(np.ones_like(image), y, np.full_like(y, 0.1), gpu.Ablock(plan, 1), niter=1)
+    assert np.all(np.isfinite(xh)) and xh.min() >= 0
+
+
 @pytest.mark.parametrize("method", ["one", "two"])
 @pytest.mark.parametrize("mode", ["fast", "mem"])
 @pytest.mark.parametrize("separable", [False, True])
